@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Benchmark of the footile fill path on one MI355X.
+
+Measured workload (one step = one launch of the device pipeline over the
+device-resident ops of two fills):
+    random walk: one self-intersecting path of 10,000 cubic_to (SplitMix64
+    seed 1, N(0, 40 px) steps clamped to the canvas), filled NonZero into a
+    4096x4096 Rgba8p raster A and EvenOdd into a 4096x4096 Rgba8p raster B,
+    colour Rgba8p(200, 120, 40, 255).
+value = filled Mpixels/s = sum over the two fills of W * (H - max(top_row, 0))
+per step / wall time per step.  The first step's rasters are compared byte for
+byte with the CPU oracle before timing ("parity": true).
+
+Also reported under "extra": the batched atlas (16,384 tiles in a 16384^2
+Matte8 atlas, one launch) and the plain 256x256 fill/stroke latency through
+the Plotter API.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+POINTS_PER_OP = {0: 0, 1: 1, 2: 1, 3: 2, 4: 3, 5: 0}
+
+
+def op_bytes(ops):
+    """Algorithmic op-input bytes: 8 B (x, y f32) per control or end point."""
+    return 8 * int(sum(POINTS_PER_OP[int(t)] for t in ops["tag"]))
+
+
+def cpu_random_walk(ops, budget_s):
+    """The oracle (C restatement of the reference, 1 core) on the same fills:
+    Mpix/s over whole steps (NonZero + EvenOdd fill of fresh rasters)."""
+    from oracle import pyoracle as po
+    steps, px, t0 = 0, 0, time.perf_counter()
+    outs = None
+    while True:
+        a = po.Plotter(4096, 4096, po.RGBA8P)
+        b = po.Plotter(4096, 4096, po.RGBA8P)
+        assert a.fill(po.NON_ZERO, ops, (200, 120, 40, 255)) == 0
+        assert b.fill(po.EVEN_ODD, ops, (200, 120, 40, 255)) == 0
+        if outs is None:
+            outs = (a.pixels(), b.pixels(), a.last_counts())
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return outs, steps, dt
+
+
+def bench_atlas(ft, steps, warmup):
+    from footile_amd import workloads
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    side = 16384
+    ops, tiles = workloads.atlas(2, n_paths=16384, atlas_side=side)
+    atlas = DeviceBuffer(side * side)
+    atlas.zero()
+    descs = [FillDesc(int(b), int(c), atlas.ptr + int(y) * side + int(x), int(s), int(s), side,
+                      fresh=True) for b, c, x, y, s in tiles]
+    batch = Batch(ops, descs)
+    batch.run()
+    for _ in range(warmup):
+        batch.launch()
+        batch.sync()
+    batch.set_timing(True)
+    batch.timing()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.launch()
+    batch.sync()
+    dt = (time.perf_counter() - t0) / steps
+    total_ms, raster_ms, runs = batch.timing()
+    tile_px = int((tiles[:, 4] ** 2).sum())
+    bytes_per = tile_px + op_bytes(ops)
+    offered, stored, edges = batch.counts()
+    # spot-check tiles against the oracle (fresh s x s Matte8 per tile)
+    from oracle import pyoracle as po
+    img = atlas.download().reshape(side, side)
+    rng = np.random.default_rng(0)
+    checked = 0
+    for t in rng.choice(len(tiles), size=64, replace=False):
+        b, c, x, y, s = (int(v) for v in tiles[t])
+        o = po.Plotter(s, s)
+        assert o.fill(po.NON_ZERO, ops[b:b + c]) == 0
+        if not np.array_equal(img[y:y + s, x:x + s].reshape(-1), o.pixels()):
+            raise SystemExit(f"atlas tile {t} differs from the oracle")
+        checked += 1
+    raster_avg = raster_ms / max(runs, 1)
+    return {
+        "workload": "atlas: 16,384 paths, lognormal(3.5,1) segments, 60/30/10 cubic/quad/line, "
+                    "tiles 32..256 in a 16384^2 Matte8 atlas, one launch",
+        "paths_per_s": len(tiles) / dt,
+        "mpix_per_s": tile_px / dt / 1e6,
+        "ms_per_step": dt * 1e3,
+        "device_ms_per_step": total_ms / max(runs, 1),
+        "raster_kernel_ms": raster_avg,
+        "raster_kernel_gbs": bytes_per / (raster_avg * 1e-3) / 1e9 if raster_avg > 0 else None,
+        "edges_per_pixel": float(edges.sum()) / tile_px,
+        "points_stored": int(stored.sum()),
+        "tiles_checked_vs_oracle": checked,
+    }
+
+
+def bench_plain(ft, reps=50):
+    from footile_amd import workloads
+    from oracle import pyoracle as po
+    fill_ops = workloads.plain_256(0)
+    stroke_ops = workloads.plain_256_stroke(0)
+    out = {}
+    for name, call in [("fill", lambda p: p.fill(0, fill_ops)), ("stroke", lambda p: p.stroke(stroke_ops))]:
+        p = ft.Plotter(ft.Raster.with_clear(256, 256))
+        call(p)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call(p)
+        out[f"gpu_{name}_us"] = (time.perf_counter() - t0) / reps * 1e6
+        o = po.Plotter(256, 256)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            if name == "fill":
+                o.fill(0, fill_ops)
+            else:
+                o.stroke(stroke_ops)
+        out[f"cpu_{name}_us"] = (time.perf_counter() - t0) / reps * 1e6
+    out["workload"] = "plain 256x256 Matte8, 48 line + 16 cubic (seed 0); fill NonZero, stroke w=4 " \
+                      "(Plotter API: host ops, upload + launch + sync per call)"
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the atlas and plain measurements")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(the rasteriser is measured on one GPU's share; see DESIGN.md for sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import footile_amd as ft
+    from footile_amd import workloads
+    from footile_amd.pixel import RGBA8P
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+
+    W = H = 4096
+    clr = (200, 120, 40, 255)
+    ops = workloads.random_walk(1)
+    print(f"[bench] random walk: {len(ops)} ops", file=sys.stderr, flush=True)
+    A, B = DeviceBuffer(W * H * 4), DeviceBuffer(W * H * 4)
+    A.zero()
+    B.zero()
+    both = np.concatenate([ops, ops])
+    descs = [FillDesc(0, len(ops), A.ptr, W, H, W * 4, RGBA8P, rule=0, clr=clr),
+             FillDesc(len(ops), len(ops), B.ptr, W, H, W * 4, RGBA8P, rule=1, clr=clr)]
+    batch = Batch(both, descs)
+    batch.run()  # first step on fresh rasters: the parity sample
+    img_a, img_b = A.download(), B.download()
+    offered, stored, edges, tops = batch.results()
+    px_per_step = int(sum(W * (H - max(int(t), 0)) for t in tops if t < H))
+    bytes_per_step = px_per_step * 8 + op_bytes(both)
+
+    print("[bench] CPU oracle baseline + parity", file=sys.stderr, flush=True)
+    (cpu_a, cpu_b, cpu_counts), cpu_steps, cpu_dt = cpu_random_walk(ops, args.cpu_seconds)
+    parity = bool(np.array_equal(img_a, cpu_a) and np.array_equal(img_b, cpu_b)
+                  and (int(offered[0]), int(stored[0])) == tuple(cpu_counts))
+    if not parity:
+        print("[bench] PARITY FAILURE: GPU rasters differ from the CPU oracle", file=sys.stderr)
+
+    print(f"[bench] warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        batch.launch()
+        batch.sync()
+    batch.set_timing(True)
+    batch.timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.launch()
+    batch.sync()
+    t1 = time.perf_counter()
+    total_ms, raster_ms, runs = batch.timing()
+    sec_per_step = (t1 - t0) / args.steps
+    raster_avg_ms = raster_ms / max(runs, 1)
+    achieved = bytes_per_step / (raster_avg_ms * 1e-3) / 1e9
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_random_walk.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    extra = {
+        "paths_per_s": 2 / sec_per_step,
+        "device_ms_per_step": total_ms / max(runs, 1),
+        "raster_kernel_ms": raster_avg_ms,
+        "points_offered_per_fill": int(offered[0]),
+        "points_stored_per_fill": int(stored[0]),
+        "edges_per_fill": int(edges[0]),
+        "top_rows": [int(t) for t in tops],
+    }
+    if not args.no_extra:
+        print("[bench] atlas", file=sys.stderr, flush=True)
+        extra["atlas"] = bench_atlas(ft, max(3, args.steps // 4), 1)
+        print("[bench] plain 256", file=sys.stderr, flush=True)
+        extra["plain_256"] = bench_plain(ft)
+
+    cpu_px = cpu_steps * 2 * W * (H - max(int(tops[0]), 0))
+    line = {
+        "metric": "filled Mpixels/s (bit-exact vs CPU) at 1 GPU",
+        "value": px_per_step / sec_per_step / 1e6,
+        "unit": "Mpixels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec_per_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i32 (16.16 Fixed) / u8",
+        "data": "synthetic",
+        "config": {"workload": "random_walk 4096x4096 Rgba8p, 10,000 cubics (seed 1), NonZero + EvenOdd",
+                   "width": W, "height": H, "n_cubics": 10000, "fills_per_step": 2},
+        "parity": parity,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_raster", "algorithmic_bytes_per_launch": bytes_per_step},
+        "cpu_baseline": {"value": cpu_px / cpu_dt / 1e6, "unit": "Mpixels/s", "cores": 1, "kind": "port",
+                         "sample": f"{cpu_steps} step(s) of the same two fills (oracle, C, 1 thread, "
+                                   f"SSSE3 accumulate), {cpu_dt:.1f} s"},
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+// ft_pipeline.h -- host-side interface of the device fill pipeline.
+//
+// One "run" renders a batch of independent fills (a single Plotter::fill is a
+// batch of one).  All stages run on the GPU from device-resident ops:
+//   count -> scan -> emit          (op interpretation + De Casteljau flatten)
+//   ingest -> kept -> scan -> compact   (Fig::add_point/close + 65,535 cap)
+//   edges -> scan -> bin           (Edge::new per segment, row-band binning)
+//   raster                         (signed-area cells in LDS, row scan,
+//                                   NonZero/EvenOdd map, Matte/Rgba store)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/footile_hip.h"
+
+namespace ft {
+
+enum : uint32_t { MODE_FILL = 0, MODE_STROKE_FLATTEN = 1 };
+
+// Per-path descriptor as the kernels see it.
+struct PathDev {
+    uint32_t op_begin, op_count;
+    uint32_t rule, fmt;
+    uint64_t dst;
+    uint32_t width, height, pitch;
+    uint32_t clr;  // packed bytes c0 | c1<<8 | c2<<16 | c3<<24
+    float tr[6];
+    float tol_sq;
+    float s_width0;
+    uint32_t band_base, band_rows, n_bands;
+    uint32_t flags;  // PATH_FRESH | PATH_RAW
+};
+
+enum : uint32_t {
+    PATH_FRESH = 1u,  // region starts cleared: rows above the figure are zeroed
+    PATH_RAW = 2u     // points go to Fig::add_point untransformed (Fig-level entry)
+};
+
+// Per-path results of a run (cleared at the start of each run).
+struct PathState {
+    unsigned long long topkey;  // min over points of (y, x) as a biased u64
+    uint32_t topidx;            // min local index among points at topkey
+    uint32_t offered;           // points offered to Fig::add_point
+    uint32_t repop;             // 1 if the cap re-pop dropped local point 65534
+    uint32_t pt_base;           // global index of the path's first point
+    uint32_t n_edges;           // non-horizontal segments
+    uint32_t stored;            // points stored in the Fig after the cap
+};
+
+struct DevStatus {
+    uint32_t err;        // first ft_status error raised by a kernel
+    uint32_t overflow;   // bit 0: events, bit 1: band pairs
+    uint32_t n_events, n_closes;
+    uint32_t n_kept, n_pairs;
+    uint32_t need_events, need_pairs;
+    unsigned long long op_total;  // packed (events << 32 | closes) of all ops
+    uint32_t spin_fail;
+    uint32_t tickets[4];
+    uint32_t pad[3];
+};
+
+struct EdgeRec {
+    int32_t y_upper, y_lower, x_bot0, inv, step;
+    uint32_t flags;  // bit0: edge present, bit1: Forward direction
+};
+
+struct Workspace {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // batch layout
+    size_t n_ops = 0, n_paths = 0, n_bands = 0;
+    size_t cap_ops = 0, cap_paths = 0, cap_bands = 0;
+    ft_path_op *d_ops = nullptr;
+    uint32_t *d_op_path = nullptr;
+    PathDev *d_paths = nullptr;
+    uint32_t *d_band_path = nullptr;
+    uint32_t lds_cells = 0;  // dynamic LDS cells needed by the raster kernel
+    // per-op scratch
+    unsigned long long *d_op_cnt = nullptr, *d_op_off = nullptr;
+    uint32_t *d_op_pw = nullptr, *d_op_lastpw = nullptr;
+    // events and everything sized by them
+    size_t cap_events = 0;
+    int2 *d_ev_xy = nullptr;
+    float4 *d_ev_f = nullptr;  // stroke-mode events
+    uint32_t *d_ev_meta = nullptr, *d_ev_kept = nullptr, *d_ev_idx = nullptr;
+    int32_t *d_run_first = nullptr, *d_run_last = nullptr;
+    uint32_t *d_run_path = nullptr, *d_run_start = nullptr, *d_run_end = nullptr;
+    int2 *d_pts = nullptr;
+    uint32_t *d_pt_run = nullptr;
+    EdgeRec *d_edges = nullptr;
+    // bands
+    uint32_t *d_band_cnt = nullptr, *d_band_off = nullptr, *d_band_cur = nullptr;
+    size_t cap_pairs = 0;
+    uint32_t *d_band_list = nullptr;
+    // scan look-back state (3 scans + stroke width scan)
+    size_t cap_tiles = 0;
+    uint32_t *d_scan_flags = nullptr;
+    unsigned long long *d_scan_agg = nullptr, *d_scan_incl = nullptr;
+    // results
+    PathState *d_pstate = nullptr;
+    DevStatus *d_status = nullptr;
+    DevStatus *h_status = nullptr;  // pinned
+    std::vector<PathState> h_pstate;
+    // optional event timing of whole runs and of the raster kernel alone:
+    // launch k records into ring slot k (up to TIMING_RING launches between
+    // two finishes); finish adds up every recorded slot.
+    static constexpr int TIMING_RING = 256;
+    bool timing = false;
+    int n_pending = 0;
+    hipEvent_t *ev = nullptr;  // TIMING_RING x 4 events, created on first use
+    double t_total_ms = 0.0, t_raster_ms = 0.0;
+    uint64_t n_timed = 0;
+};
+
+// Allocate the stream and status buffers.
+ft_status ws_init(Workspace &ws);
+void ws_free(Workspace &ws);
+// Upload a batch layout: ops + path descriptors (host arrays).
+// extra_flags (PATH_RAW, ...) are or-ed into every path's flags.
+ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const ft_fill_desc *paths,
+                       size_t n_paths, uint32_t extra_flags = 0);
+// Re-upload only the ops (same count and layout) -- for a plotter reusing ws.
+ft_status ws_upload_ops(Workspace &ws, const ft_path_op *ops, size_t n_ops);
+// Enqueue a run.  mode = MODE_FILL renders; MODE_STROKE_FLATTEN stops after
+// emitting (x, y, width, kind) events for the host stroker.
+ft_status ws_launch(Workspace &ws, uint32_t mode);
+// Wait and read the status block; returns the run's status.
+ft_status ws_finish(Workspace &ws, bool read_pstate);
+// launch + finish, growing capacities and relaunching on overflow.
+ft_status ws_run(Workspace &ws, uint32_t mode, bool read_pstate);
+
+// Band geometry for a raster: rows per band and LDS cells per band.
+void band_geometry(uint32_t width, uint32_t height, uint32_t *rows, uint32_t *n_bands, uint32_t *cells);
+
+}  // namespace ft

@@ -1,0 +1,175 @@
+/*
+ * footile_hip.h -- C ABI of the MI355X (gfx950) footile fill/stroke rasteriser.
+ *
+ * This is the drop-in boundary for footile's hot path: `Plotter::fill` and
+ * `Plotter::stroke` over a pix raster (Matte8 / Rgba8p / Graya8p).  Each entry
+ * point names the reference interface it replaces.  Plain pointers and sizes
+ * only; every function returns an ft_status and never aborts.  A handle is used
+ * by one host thread at a time and owns one HIP stream of default priority.
+ *
+ * Reference: footile 0.7.0 (Rust).  Paths below are relative to its tree.
+ */
+#ifndef FOOTILE_HIP_H
+#define FOOTILE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FT_OK = 0,
+    FT_EINVAL = 1,   /* null handle/pointer, zero size, bad tag, NaN tolerance */
+    FT_ENOMEM = 2,   /* host or device allocation failed */
+    FT_EDEVICE = 3,  /* HIP runtime error */
+    FT_ERANGE = 4,   /* curve flattening deeper than FT_FLATTEN_MAX_DEPTH or with
+                        more than FT_FLATTEN_MAX_LEAVES leaves (the reference
+                        recurses without bound, src/plotter.rs:248-332) */
+    FT_ENOTSUP = 5   /* raster wider than FT_MAX_WIDTH */
+} ft_status;
+
+/* Pixel formats of pix 0.14 usable with Plotter<P> (src/plotter.rs:89-92). */
+typedef enum {
+    FT_MATTE8 = 0,  /* 1 byte: alpha.  Fill overwrites rows (src/fig.rs:632-635) */
+    FT_RGBA8P = 1,  /* 4 bytes: premultiplied R,G,B,A; SrcOver composite */
+    FT_GRAYA8P = 2  /* 2 bytes: premultiplied value, alpha; SrcOver composite */
+} ft_format;
+
+/* FillRule, src/path.rs:8-14 */
+typedef enum { FT_NON_ZERO = 0, FT_EVEN_ODD = 1 } ft_fill_rule;
+
+/* JoinStyle, src/stroker.rs:13-20 */
+typedef enum { FT_JOIN_MITER = 0, FT_JOIN_BEVEL = 1, FT_JOIN_ROUND = 2 } ft_join;
+
+/* PathOp, src/path.rs:18-31.  28 bytes, absolute coordinates:
+ *   CLOSE      -
+ *   MOVE       v[0..1] = point
+ *   LINE       v[0..1] = end point
+ *   QUAD       v[0..1] = control, v[2..3] = end
+ *   CUBIC      v[0..1] = control 0, v[2..3] = control 1, v[4..5] = end
+ *   PEN_WIDTH  v[0]    = width                                            */
+typedef enum {
+    FT_OP_CLOSE = 0, FT_OP_MOVE = 1, FT_OP_LINE = 2,
+    FT_OP_QUAD = 3, FT_OP_CUBIC = 4, FT_OP_PEN_WIDTH = 5
+} ft_op_tag;
+
+typedef struct {
+    uint32_t tag;
+    float v[6];
+} ft_path_op;
+
+#define FT_FLATTEN_MAX_DEPTH 32
+#define FT_FLATTEN_MAX_LEAVES (1u << 20)
+#define FT_MAX_POINTS 65535u /* Vid::MAX, src/vid.rs:17 */
+#define FT_MAX_WIDTH 16384u
+
+/* ---------------------------------------------------------------------------
+ * Plotter -- replaces footile::Plotter<P> (src/plotter.rs:38-381).
+ * The raster lives in device memory; read/write copy it to/from the host.
+ * ------------------------------------------------------------------------- */
+typedef struct ft_plotter ft_plotter;
+
+/* Plotter::new(Raster::with_clear(w, h)) -- src/plotter.rs:96-115 */
+ft_status ft_plotter_new(uint32_t width, uint32_t height, ft_format fmt, ft_plotter **out);
+/* Plotter::new(Raster::with_pixels(w, h, px)) -- src/plotter.rs:96-115 */
+ft_status ft_plotter_new_with_pixels(uint32_t width, uint32_t height, ft_format fmt,
+                                     const void *pixels, ft_plotter **out);
+/* drop(Plotter) */
+void ft_plotter_free(ft_plotter *p);
+/* Plotter::width / height -- src/plotter.rs:118-125 */
+ft_status ft_plotter_size(const ft_plotter *p, uint32_t *width, uint32_t *height);
+/* Plotter::set_tolerance -- src/plotter.rs:133-137 */
+ft_status ft_plotter_set_tolerance(ft_plotter *p, float t);
+/* Plotter::set_transform -- src/plotter.rs:140-143; e = pointy Transform
+ * elements: x' = e0*x + e1*y + e2, y' = e3*x + e4*y + e5. */
+ft_status ft_plotter_set_transform(ft_plotter *p, const float e[6]);
+/* Plotter::set_join -- src/plotter.rs:158-161 (miter_limit used by MITER) */
+ft_status ft_plotter_set_join(ft_plotter *p, ft_join kind, float miter_limit);
+/* Plotter::fill -- src/plotter.rs:339-350.  clr: the pixel value (Matte8
+ * ignores it, as the reference does). */
+ft_status ft_plotter_fill(ft_plotter *p, ft_fill_rule rule, const ft_path_op *ops, size_t n,
+                          const uint8_t clr[4]);
+/* Plotter::stroke -- src/plotter.rs:356-365 */
+ft_status ft_plotter_stroke(ft_plotter *p, const ft_path_op *ops, size_t n, const uint8_t clr[4]);
+/* Plotter::raster / into_raster: copy the raster to host memory. */
+ft_status ft_plotter_read(const ft_plotter *p, void *dst, size_t bytes);
+/* Plotter::raster_mut: overwrite the raster from host memory. */
+ft_status ft_plotter_write(ft_plotter *p, const void *src, size_t bytes);
+/* Zero-copy access for device consumers: raster base and row pitch (bytes). */
+ft_status ft_plotter_device_ptr(ft_plotter *p, void **dptr, size_t *pitch);
+/* The stroke width state that persists across calls (src/plotter.rs:53,151). */
+ft_status ft_plotter_pen_width(const ft_plotter *p, float *width);
+/* Figure statistics of the last fill: points offered to Fig::add_point and
+ * points stored after the 65,535-point cap (src/fig.rs:428-442). */
+ft_status ft_plotter_last_counts(const ft_plotter *p, uint64_t *offered, uint64_t *stored);
+
+/* ---------------------------------------------------------------------------
+ * Fig -- debug entry for footile's crate-private Fig (src/fig.rs:339-502), so
+ * the reference's Fig-level known answers replay on the GPU.  Points are
+ * Fig::add_point calls; close_after[i] != 0 calls Fig::close after point i.
+ * The raster is host memory, updated in place.
+ * ------------------------------------------------------------------------- */
+ft_status ft_fig_fill(ft_fill_rule rule, const float *xy, const uint8_t *close_after, size_t n,
+                      ft_format fmt, uint32_t width, uint32_t height, void *pixels,
+                      const uint8_t clr[4]);
+
+/* ---------------------------------------------------------------------------
+ * Batch -- many independent fills in one device-resident launch sequence.
+ * No reference counterpart: it runs N Plotter::fill calls (src/plotter.rs:
+ * 339-350), each into its own raster region, with ops uploaded once.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    uint64_t op_begin;   /* first op of this path in the batch op array */
+    uint32_t op_count;
+    uint32_t rule;       /* ft_fill_rule */
+    uint64_t dst;        /* device address of the raster's first pixel */
+    uint32_t width, height;
+    uint32_t pitch;      /* bytes between rows */
+    uint32_t fmt;        /* ft_format */
+    uint8_t clr[4];
+    float transform[6];
+    float tolerance;     /* as given to set_tolerance (floored at 0.01) */
+    float pen_width;     /* s_width at the start of the fill */
+    uint32_t fresh;      /* 1: region starts cleared (rows above the path are
+                            written as zeros, as a fresh Raster::with_clear) */
+} ft_fill_desc;
+
+typedef struct ft_batch ft_batch;
+
+ft_status ft_batch_new(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *paths,
+                       size_t n_paths, ft_batch **out);
+void ft_batch_free(ft_batch *b);
+/* Enqueue the whole batch on the batch's stream (asynchronous). */
+ft_status ft_batch_launch(ft_batch *b);
+/* Wait for the last launch; returns its status (FT_ERANGE etc.). */
+ft_status ft_batch_sync(ft_batch *b);
+/* Launch + sync; grows workspace and relaunches once if a capacity ran out. */
+ft_status ft_batch_run(ft_batch *b);
+/* Per-path results of the last run (arrays of n_paths, any may be NULL):
+ * points offered to Fig::add_point, points stored, non-horizontal edges, and
+ * the figure's top row (row_of of the smallest y; INT32_MAX if empty). */
+ft_status ft_batch_counts(ft_batch *b, uint64_t *offered, uint64_t *stored, uint64_t *edges,
+                          int32_t *top_rows);
+/* The batch's HIP stream (hipStream_t) for event timing by the caller. */
+ft_status ft_batch_stream(ft_batch *b, void **stream);
+/* Event timing on the batch stream: when on, every launch records HIP events
+ * around the whole run and around the raster kernel; ft_batch_timing returns
+ * the sums (ms) and the number of timed runs, and resets them. */
+ft_status ft_batch_set_timing(ft_batch *b, int on);
+ft_status ft_batch_timing(ft_batch *b, double *total_ms, double *raster_ms, uint64_t *runs);
+/* Device allocation helpers for callers without a runtime of their own. */
+ft_status ft_device_alloc(size_t bytes, void **dptr);
+ft_status ft_device_free(void *dptr);
+ft_status ft_memcpy_h2d(void *dst, const void *src, size_t bytes);
+ft_status ft_memcpy_d2h(void *dst, const void *src, size_t bytes);
+ft_status ft_memset_d(void *dst, int value, size_t bytes);
+
+/* Library version string. */
+const char *ft_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FOOTILE_HIP_H */

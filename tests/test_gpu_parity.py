@@ -1,0 +1,246 @@
+"""GPU parity: libfootile_hip.so (through its C ABI) against the CPU oracle.
+
+Integer work (Fixed scan, accumulate, composite) must be bit-exact; the f32
+flattening must emit the same points, so rasters are compared byte-for-byte.
+"""
+import numpy as np
+import pytest
+
+from footile_amd import workloads
+from footile_amd.path import CLOSE, CUBIC, LINE, MOVE, PEN_WIDTH, QUAD, ops_array
+from footile_amd.pixel import BPP, GRAYA8P, MATTE8, RGBA8P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ft():
+    import footile_amd
+    footile_amd.lib()
+    return footile_amd
+
+
+def gpu_plotter(ft, w, h, fmt, init=None, transform=None, tol=None, join=None):
+    r = ft.Raster(w, h, fmt, init)
+    p = ft.Plotter(r)
+    if transform is not None:
+        p.set_transform(transform)
+    if tol is not None:
+        p.set_tolerance(tol)
+    if join is not None:
+        p.set_join(join)
+    return p
+
+
+def oracle_plotter(oracle, w, h, fmt, init=None, transform=None, tol=None, join=None):
+    p = oracle.Plotter(w, h, fmt)
+    if init is not None:
+        p.set_pixels(init)
+    if transform is not None:
+        p.set_transform(transform)
+    if tol is not None:
+        p.set_tolerance(tol)
+    if join is not None:
+        p.set_join(*join)
+    return p
+
+
+def compare_fill(ft, oracle, w, h, fmt, calls, init=None, transform=None, tol=None, join=None, tag=""):
+    """Run the same sequence of ("fill", rule, ops, clr) / ("stroke", ops, clr)
+    calls on both plotters and compare every byte after each call."""
+    g = gpu_plotter(ft, w, h, fmt, init, transform, tol, join)
+    o = oracle_plotter(oracle, w, h, fmt, init, transform, tol, join)
+    for k, c in enumerate(calls):
+        if c[0] == "fill":
+            _, rule, ops, clr = c
+            g.fill(rule, ops, clr)
+            assert o.fill(rule, ops, clr) == 0
+        else:
+            _, ops, clr = c
+            g.stroke(ops, clr)
+            assert o.stroke(ops, clr) == 0
+        got = g.raster().as_u8_slice()
+        want = o.pixels()
+        if not np.array_equal(got, want):
+            bad = np.flatnonzero(got != want)
+            raise AssertionError(f"{tag} call {k}: {bad.size} bytes differ, first at {bad[:8]} "
+                                 f"got {got[bad[:8]]} want {want[bad[:8]]}")
+    return g, o
+
+
+def test_fig_known_answers(ft, kats):
+    for k in kats["fig"]:
+        fmt = MATTE8 if k["fmt"] == "matte8" else RGBA8P
+        n = len(k["points"])
+        closes = [0] * (n - 1) + [1]
+        clr = list(k["clr"]) + [0] * (4 - len(k["clr"]))
+        px = ft.fig_fill(0, k["points"], closes, k["w"], k["h"], fmt, clr)
+        assert list(px) == k["expect"], k["name"]
+
+
+def test_overlapping_completes(ft, oracle, kats):
+    k = kats["plotter_overlapping"]
+    tags = {"move": MOVE, "line": LINE, "cubic": CUBIC}
+    ops = ops_array([(tags[o[0]], tuple(o[1:])) for o in k["ops"]])
+    compare_fill(ft, oracle, k["w"], k["h"], MATTE8, [("fill", 0, ops, (255,))], tag="overlapping")
+
+
+def _rect(x0, y0, x1, y1, ccw=False):
+    pts = [(x0, y0), (x1, y0), (x1, y1), (x0, y1)]
+    if ccw:
+        pts = pts[::-1]
+    return [(MOVE, pts[0])] + [(LINE, p) for p in pts[1:]] + [(CLOSE, ())]
+
+
+def test_rectangles_holes_and_clipping(ft, oracle):
+    cases = [_rect(3, 4, 17, 9), _rect(2.5, 0, 9, 8), _rect(-5, 3, 40, 11),
+             _rect(2, 2, 30, 30) + _rect(10, 10, 20, 20, ccw=True),
+             _rect(2, 2, 30, 30) + _rect(10, 10, 20, 20),
+             _rect(4, -6, 12, 5),            # figure above the raster top (shifted rows)
+             _rect(4, 50, 12, 60)]           # figure below the raster
+    for i, ops in enumerate(cases):
+        for rule in (0, 1):
+            compare_fill(ft, oracle, 32, 24, MATTE8, [("fill", rule, ops_array(ops), (255,))],
+                         tag=f"rect{i}/{rule}")
+
+
+def random_path(rng, w, h, n_sub=1, kinds=(LINE,), margin=10.0, close_prob=1.0):
+    ops = []
+    for _ in range(n_sub):
+        n = int(rng.integers(2, 24))
+        ops.append((MOVE, tuple(rng.uniform(-margin, [w + margin, h + margin]).astype(np.float32))))
+        for _ in range(n):
+            kind = kinds[int(rng.integers(0, len(kinds)))]
+            m = {LINE: 1, QUAD: 2, CUBIC: 3}[kind]
+            pts = rng.uniform(-margin, [w + margin, h + margin], size=(m, 2)).astype(np.float32)
+            ops.append((kind, tuple(pts.reshape(-1))))
+        if rng.uniform() < close_prob:
+            ops.append((CLOSE, ()))
+    return ops_array(ops)
+
+
+def test_random_polygons_matte8(ft, oracle):
+    rng = np.random.default_rng(1234)
+    for t in range(40):
+        w = int(rng.integers(1, 300))
+        h = int(rng.integers(1, 120))
+        ops = random_path(rng, w, h, n_sub=int(rng.integers(1, 4)))
+        compare_fill(ft, oracle, w, h, MATTE8, [("fill", t % 2, ops, (255,))], tag=f"poly{t} {w}x{h}")
+
+
+def test_random_curves_all_formats(ft, oracle):
+    rng = np.random.default_rng(99)
+    for t in range(30):
+        fmt = (MATTE8, RGBA8P, GRAYA8P)[t % 3]
+        w = int(rng.integers(8, 200))
+        h = int(rng.integers(8, 100))
+        init = rng.integers(0, 256, size=w * h * BPP[fmt]).astype(np.uint8) if t % 2 else None
+        clr = tuple(int(v) for v in rng.integers(0, 256, size=4))
+        if fmt == RGBA8P:
+            clr = (min(clr[0], clr[3]), min(clr[1], clr[3]), min(clr[2], clr[3]), clr[3])
+        ops = random_path(rng, w, h, n_sub=int(rng.integers(1, 3)), kinds=(LINE, QUAD, CUBIC))
+        compare_fill(ft, oracle, w, h, fmt, [("fill", t % 2, ops, clr)], init=init, tag=f"curve{t}")
+
+
+def test_plain_256_fill_and_stroke(ft, oracle):
+    compare_fill(ft, oracle, 256, 256, MATTE8, [("fill", 0, workloads.plain_256(0), (255,))],
+                 tag="plain fill")
+    compare_fill(ft, oracle, 256, 256, MATTE8, [("stroke", workloads.plain_256_stroke(0), (255,))],
+                 tag="plain stroke")
+
+
+def test_transform_tolerance_and_state(ft, oracle):
+    fish = ft.Path2D().relative().pen_width(3.0).move_to(112.0, 24.0).line_to(-32.0, 24.0) \
+        .cubic_to(-96.0, -48.0, -96.0, 80.0, 0.0, 32.0).line_to(32.0, 24.0) \
+        .line_to(-16.0, -40.0).close().finish()
+    compare_fill(ft, oracle, 256, 256, MATTE8,
+                 [("fill", 0, fish, (255,)), ("stroke", fish, (255,)), ("fill", 1, fish, (255,))],
+                 transform=(2, 0, 0, 0, 2, 0), tag="fishy scale 2")
+    compare_fill(ft, oracle, 128, 128, RGBA8P,
+                 [("fill", 0, fish, (127, 96, 96, 255)), ("stroke", fish, (255, 208, 208, 255))],
+                 tol=0.05, tag="fishyp")
+
+
+def test_stroke_joins(ft, oracle):
+    rng = np.random.default_rng(5)
+    for t in range(12):
+        ops = random_path(rng, 96, 96, n_sub=2, kinds=(LINE, QUAD, CUBIC), margin=-8.0,
+                          close_prob=0.5)
+        ops = np.concatenate([ops_array([(PEN_WIDTH, (float(rng.uniform(1, 9)),))]), ops])
+        join = [(0, 4.0), (1, 0.0), (2, 0.0), (0, 1.5)][t % 4]
+        compare_fill(ft, oracle, 96, 96, MATTE8, [("stroke", ops, (255,))], join=join, tag=f"stroke{t}")
+
+
+def test_sequential_fills_keep_state(ft, oracle):
+    rng = np.random.default_rng(77)
+    for fmt in (MATTE8, RGBA8P):
+        calls = [("fill", i % 2, random_path(rng, 64, 48, kinds=(LINE, CUBIC)), (40, 80, 120, 200))
+                 for i in range(4)]
+        compare_fill(ft, oracle, 64, 48, fmt, calls, tag=f"sequence fmt{fmt}")
+
+
+def test_random_walk_4096(ft, oracle):
+    ops = workloads.random_walk(1)
+    clr = (200, 120, 40, 255)
+    for rule in (0, 1):
+        g, o = compare_fill(ft, oracle, 4096, 4096, RGBA8P, [("fill", rule, ops, clr)],
+                            tag=f"random walk rule {rule}")
+        assert g.last_counts() == o.last_counts()
+        assert g.last_counts()[0] > 65535 and g.last_counts()[1] <= 65535
+
+
+def test_point_cap_close_pop(ft, oracle):
+    """The 65,535-point cap (src/vid.rs:17, src/fig.rs:429-431) and its
+    interplay with the closing pop (src/fig.rs:373-383)."""
+    rng = np.random.default_rng(8)
+    n = 66000
+    pts = np.stack([rng.uniform(0, 250, n), rng.uniform(0, 250, n)], axis=1).astype(np.float32)
+    pts[65534] = pts[0]  # the point stored last under the cap equals the run's first
+    base = [(MOVE, tuple(pts[0]))] + [(LINE, tuple(p)) for p in pts[1:]] + [(CLOSE, ())]
+    tail = [(MOVE, (5.0, 5.0)), (LINE, (60.0, 9.0)), (LINE, (30.0, 70.0)), (CLOSE, ())]
+    for ops in (base, base + tail, base[:65536] + [(CLOSE, ())] + tail,
+                base[:65535] + [(LINE, tuple(pts[0])), (CLOSE, ())] + tail):
+        a = ops_array(ops)
+        g, o = compare_fill(ft, oracle, 256, 256, MATTE8, [("fill", 0, a, (255,))], tag="cap")
+        assert g.last_counts() == o.last_counts()
+
+
+def test_nan_curve_is_erange_and_leaves_raster(ft, oracle):
+    ops = ops_array([(MOVE, (10.0, 20.0)), (CUBIC, (float("nan"), 24.0, 16.0, 28.0, 19.0, 32.0)),
+                     (CLOSE, ())])
+    init = np.arange(64 * 64, dtype=np.uint32).astype(np.uint8)
+    g = gpu_plotter(ft, 64, 64, MATTE8, init)
+    with pytest.raises(ft.FootileError) as e:
+        g.fill(0, ops)
+    assert e.value.code == 4
+    assert np.array_equal(g.raster().as_u8_slice(), init)
+
+
+def test_deterministic_repeat(ft):
+    ops = workloads.random_walk(3, n_cubics=3000)
+    out = []
+    for _ in range(2):
+        g = gpu_plotter(ft, 2048, 2048, MATTE8)
+        g.fill(1, ops)
+        out.append(g.raster().as_u8_slice().copy())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_batch_matches_individual_plotters(ft, oracle):
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    ops, tiles = workloads.atlas(2, n_paths=160, atlas_side=2048)
+    side = 2048
+    atlas = DeviceBuffer(side * side)
+    atlas.upload(np.full(side * side, 77, dtype=np.uint8))  # fresh tiles must not show it
+    descs = [FillDesc(int(b), int(c), atlas.ptr + int(y) * side + int(x), int(s), int(s), side,
+                      fresh=True) for b, c, x, y, s in tiles]
+    batch = Batch(ops, descs)
+    batch.run()
+    img = atlas.download().reshape(side, side)
+    offered, stored, edges = batch.counts()
+    for t, (b, c, x, y, s) in enumerate(tiles):
+        o = oracle.Plotter(int(s), int(s))
+        assert o.fill(0, ops[b:b + c]) == 0
+        want = o.pixels().reshape(s, s)
+        assert np.array_equal(img[y:y + s, x:x + s], want), f"tile {t}"
+        assert (offered[t], stored[t]) == o.last_counts()

@@ -1,0 +1,348 @@
+"""Pin the CPU oracle (oracle/footile_oracle.c) before trusting it.
+
+Checks, in order of strength:
+  * every known-answer vector of footile's own tests (src/fixed.rs:166-313,
+    src/fig.rs:691-794, src/imgbuf.rs:205-232, src/plotter.rs:389-403);
+  * the SSSE3 and scalar accumulate paths agree (src/imgbuf.rs);
+  * closed forms: integer-aligned rectangles, half-pixel edges, holes by
+    opposite winding, exact-integer translation invariance, and agreement
+    with FP64 exact-area coverage;
+  * flattening: every emitted point is the Bezier curve at an increasing
+    dyadic parameter, checked against an independent FP64 Bernstein
+    evaluation of the same subdivision rule.
+"""
+import numpy as np
+import pytest
+
+from footile_amd import workloads
+from footile_amd.path import CLOSE, CUBIC, LINE, MOVE, QUAD, Path2D, ops_array
+
+
+def fx(o, v):
+    L = o.lib()
+    if "i" in v:
+        return L.or_fx_from_i32(v["i"])
+    return L.or_fx_from_f32(v["f"])
+
+
+def test_fixed_known_answers(oracle, kats):
+    L = oracle.lib()
+    F = kats["fixed"]
+    for name, fn in [("add", L.or_fx_add), ("sub", L.or_fx_sub), ("mul", L.or_fx_mul),
+                     ("div", L.or_fx_div), ("avg", L.or_fx_avg)]:
+        for a, b, want in F[name]:
+            assert fn(fx(oracle, a), fx(oracle, b)) == fx(oracle, want), (name, a, b, want)
+    for name, fn in [("shl", L.or_fx_shl), ("shr", L.or_fx_shr)]:
+        for a, s, want in F[name]:
+            assert fn(fx(oracle, a), s) == fx(oracle, want), (name, a, s)
+    for name, fn in [("abs", L.or_fx_abs), ("floor", L.or_fx_floor), ("ceil", L.or_fx_ceil),
+                     ("round", L.or_fx_round), ("trunc", L.or_fx_trunc), ("fract", L.or_fx_fract)]:
+        for a, want in F[name]:
+            assert fn(fx(oracle, a)) == fx(oracle, want), (name, a, want)
+    for a, want in F["into_i32"]:
+        assert L.or_fx_to_i32(fx(oracle, a)) == want
+    for a, want in F["into_f32"]:
+        assert L.or_fx_to_f32(fx(oracle, a)) == want
+    for a, b in F["cmp_gt"]:
+        assert fx(oracle, a) > fx(oracle, b)
+
+
+def test_fixed_release_semantics(oracle):
+    L = oracle.lib()
+    # `as i32` from f32 saturates and maps NaN to 0 (Rust language semantics)
+    assert L.or_fx_from_f32(float("nan")) == 0
+    assert L.or_fx_from_f32(1e30) == 2**31 - 1
+    assert L.or_fx_from_f32(-1e30) == -(2**31)
+    assert L.or_fx_from_f32(-0.5) == -32768
+    assert L.or_fx_from_f32(-1.0 / 65536 / 2) == 0  # truncation toward zero, not floor
+    # i32 add wraps; mul/div truncate to i32
+    assert L.or_fx_add(2**31 - 1, 1) == -(2**31)
+    assert L.or_fx_mul(2**30, 2**30) == L.or_fx_mul(2**30, 2**30)  # defined, no trap
+    assert L.or_fx_mul(3 << 16, 5 << 16) == 15 << 16
+    assert L.or_fx_div(-7 << 16, 2 << 16) == -(7 << 15)
+    assert L.or_fx_div(1, 3 << 16) == 0  # (1 << 16) / (3 << 16) truncates
+    # pixel_cov spans 0..=256 and rounds half up
+    assert L.or_pixel_cov(0) == 0 and L.or_pixel_cov(65536) == 256
+    assert L.or_pixel_cov(128) == 1 and L.or_pixel_cov(127) == 0
+
+
+def test_fixed_pt_widdershins(oracle, kats):
+    L = oracle.lib()
+    k = kats["fixed_pt"]
+    a = [L.or_fx_from_f32(v) for v in k["a"]]
+    b = [L.or_fx_from_f32(v) for v in k["b"]]
+    c = [L.or_fx_from_f32(v) for v in k["c"]]
+    assert [b[0] - a[0], b[1] - a[1]] == [L.or_fx_from_f32(v) for v in k["b_minus_a"]]
+    assert bool(L.or_fx_widdershins(*a, *b)) is k["a_widdershins_b"]
+    assert bool(L.or_fx_widdershins(*b, *a)) is k["b_widdershins_a"]
+    assert bool(L.or_fx_widdershins(*b, *c)) is k["b_widdershins_c"]
+
+
+@pytest.mark.parametrize("ssse3", [True, False])
+def test_fig_known_answers(oracle, kats, ssse3):
+    for k in kats["fig"]:
+        f = oracle.Fig()
+        for x, y in k["points"]:
+            f.add_point(x, y)
+        f.close()
+        fmt = oracle.MATTE8 if k["fmt"] == "matte8" else oracle.RGBA8P
+        px = f.fill(oracle.NON_ZERO, k["w"], k["h"], fmt=fmt, clr=k["clr"], sgn_len=k["sgn_len"],
+                    ssse3=ssse3)
+        assert list(px) == k["expect"], k["name"]
+
+
+def test_imgbuf_known_answers(oracle, kats):
+    L = oracle.lib()
+    for name, variants in [("non_zero", (L.or_accumulate_non_zero_scalar, L.or_accumulate_non_zero_ssse3)),
+                           ("even_odd", (L.or_accumulate_even_odd_scalar, L.or_accumulate_even_odd_ssse3))]:
+        for case in kats["imgbuf"][name]:
+            for fn in variants:
+                dst = np.zeros(case["n"], dtype=np.uint8)
+                src = np.zeros(case["n"], dtype=np.int16)
+                src[0] = case["delta0"]
+                fn(oracle._ptr(dst), oracle._ptr(src), case["n"])
+                assert (dst == case["expect_all"]).all(), name
+                assert not src.any()
+
+
+def test_accumulate_ssse3_matches_scalar(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(7)
+    for n in [1, 7, 8, 9, 15, 16, 17, 63, 64, 65, 255, 1000]:
+        for scale in [4, 300, 30000]:
+            src = rng.integers(-scale, scale + 1, size=n).astype(np.int16)
+            for a, b in [(L.or_accumulate_non_zero_scalar, L.or_accumulate_non_zero_ssse3),
+                         (L.or_accumulate_even_odd_scalar, L.or_accumulate_even_odd_ssse3)]:
+                s1, s2 = src.copy(), src.copy()
+                d1, d2 = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+                a(oracle._ptr(d1), oracle._ptr(s1), n)
+                b(oracle._ptr(d2), oracle._ptr(s2), n)
+                assert (d1 == d2).all() and not s1.any() and not s2.any()
+                # both equal the i16-wrapped prefix sum mapped by the rule
+                ps = np.cumsum(src.astype(np.int64)).astype(np.int16).astype(np.int32)
+                if a is L.or_accumulate_non_zero_scalar:
+                    want = np.clip(ps, 0, 255)
+                else:
+                    want = np.minimum(np.abs((ps & 0xFF) - (ps & 0x100)), 255)
+                assert (d1 == want).all()
+
+
+def test_plotter_overlapping_completes(oracle, kats):
+    k = kats["plotter_overlapping"]
+    tags = {"move": MOVE, "line": LINE, "cubic": CUBIC}
+    ops = ops_array([(tags[o[0]], tuple(o[1:])) for o in k["ops"]])
+    p = oracle.Plotter(k["w"], k["h"])
+    assert p.fill(oracle.NON_ZERO, ops) == 0
+
+
+def _rect(x0, y0, x1, y1, ccw=False):
+    pts = [(x0, y0), (x1, y0), (x1, y1), (x0, y1)]
+    if ccw:
+        pts = pts[::-1]
+    ops = [(MOVE, pts[0])] + [(LINE, p) for p in pts[1:]] + [(CLOSE, ())]
+    return ops
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+def test_integer_rectangle_is_exact(oracle, rule):
+    for (x0, y0, x1, y1) in [(3, 4, 17, 9), (0, 0, 32, 32), (-5, -3, 7, 11), (10, 20, 40, 21)]:
+        p = oracle.Plotter(32, 24)
+        assert p.fill(rule, ops_array(_rect(x0, y0, x1, y1))) == 0
+        img = p.pixels().reshape(24, 32)
+        want = np.zeros((24, 32), np.uint8)
+        want[max(y0, 0):max(min(y1, 24), 0), max(x0, 0):max(min(x1, 32), 0)] = 255
+        if y0 >= 0:  # a figure above the raster top shifts rows (fig.rs:496-500)
+            assert (img == want).all(), (x0, y0, x1, y1)
+
+
+def test_half_pixel_edge_is_128(oracle):
+    p = oracle.Plotter(16, 8)
+    assert p.fill(0, ops_array(_rect(2.5, 0, 9, 8))) == 0
+    img = p.pixels().reshape(8, 16)
+    assert (img[:, 2] == 128).all() and (img[:, 3:9] == 255).all() and (img[:, 9:] == 0).all()
+
+
+def test_opposite_winding_hole(oracle):
+    outer = _rect(2, 2, 30, 30)
+    inner = _rect(10, 10, 20, 20, ccw=True)
+    same = _rect(10, 10, 20, 20)
+    p = oracle.Plotter(32, 32)
+    assert p.fill(0, ops_array(outer + inner)) == 0
+    img = p.pixels().reshape(32, 32)
+    assert (img[10:20, 10:20] == 0).all() and (img[3:9, 3:29] == 255).all()
+    p = oracle.Plotter(32, 32)
+    assert p.fill(0, ops_array(outer + same)) == 0
+    assert (p.pixels().reshape(32, 32)[10:20, 10:20] == 255).all()
+    p = oracle.Plotter(32, 32)
+    assert p.fill(1, ops_array(outer + same)) == 0
+    assert (p.pixels().reshape(32, 32)[10:20, 10:20] == 0).all()
+
+
+def random_polygon(rng, n, lo, hi):
+    return [(float(np.float32(x)), float(np.float32(y))) for x, y in rng.uniform(lo, hi, size=(n, 2))]
+
+
+def poly_ops(pts):
+    return ops_array([(MOVE, pts[0])] + [(LINE, p) for p in pts[1:]] + [(CLOSE, ())])
+
+
+def test_integer_translation_invariance(oracle):
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        pts = random_polygon(rng, 9, 4, 28)
+        dx, dy = (int(v) for v in rng.integers(0, 20, size=2))
+        a = oracle.Plotter(64, 64)
+        b = oracle.Plotter(64, 64)
+        assert a.fill(0, poly_ops(pts)) == 0
+        assert b.fill(0, poly_ops([(x + dx, y + dy) for x, y in pts])) == 0
+        ia = a.pixels().reshape(64, 64)
+        ib = b.pixels().reshape(64, 64)
+        assert (ib[dy:dy + 44, dx:dx + 44] == ia[:44, :44]).all()
+
+
+def _clip_area(poly, x0, y0, x1, y1):
+    """Area of a convex polygon clipped to a box (Sutherland-Hodgman, f64)."""
+    def clip(pts, inside, cut):
+        out = []
+        for i in range(len(pts)):
+            p, q = pts[i], pts[(i + 1) % len(pts)]
+            if inside(q):
+                if not inside(p):
+                    out.append(cut(p, q))
+                out.append(q)
+            elif inside(p):
+                out.append(cut(p, q))
+        return out
+
+    def cx(v):
+        return lambda p, q: (v, p[1] + (q[1] - p[1]) * (v - p[0]) / (q[0] - p[0]))
+
+    def cy(v):
+        return lambda p, q: (p[0] + (q[0] - p[0]) * (v - p[1]) / (q[1] - p[1]), v)
+
+    pts = poly
+    for inside, cut in [(lambda p: p[0] >= x0, cx(x0)), (lambda p: p[0] <= x1, cx(x1)),
+                        (lambda p: p[1] >= y0, cy(y0)), (lambda p: p[1] <= y1, cy(y1))]:
+        if not pts:
+            return 0.0
+        pts = clip(pts, inside, cut)
+    if len(pts) < 3:
+        return 0.0
+    s = 0.0
+    for i in range(len(pts)):
+        s += pts[i][0] * pts[(i + 1) % len(pts)][1] - pts[(i + 1) % len(pts)][0] * pts[i][1]
+    return abs(s) / 2
+
+
+def test_coverage_agrees_with_exact_area(oracle):
+    """Sanity bound against FP64 exact pixel areas.  footile's coverage model
+    is approximate near edges (scan_area ramps coverage by the edge's
+    slope: fig.rs:285-321), so the bound is: pixels whose 3x3 neighbourhood
+    is fully inside are 255, fully outside are 0, and the total coverage is
+    within 10% of the polygon's exact area."""
+    rng = np.random.default_rng(11)
+    for _ in range(10):
+        c = rng.uniform(14, 18, size=2)
+        ang = np.sort(rng.uniform(0, 2 * np.pi, size=5))
+        rad = rng.uniform(7, 12, size=5)
+        pts = [(float(np.float32(c[0] + r * np.cos(a))), float(np.float32(c[1] + r * np.sin(a))))
+               for a, r in zip(ang, rad)]
+        p = oracle.Plotter(32, 32)
+        assert p.fill(0, poly_ops(pts)) == 0
+        img = p.pixels().reshape(32, 32).astype(np.int64)
+        area = np.array([[_clip_area(pts, x, y, x + 1, y + 1) for x in range(32)] for y in range(32)])
+        pad = np.pad(area, 1, constant_values=0.0)
+        nb_min = np.min([pad[1 + dy:33 + dy, 1 + dx:33 + dx] for dy in (-1, 0, 1) for dx in (-1, 0, 1)], axis=0)
+        nb_max = np.max([pad[1 + dy:33 + dy, 1 + dx:33 + dx] for dy in (-1, 0, 1) for dx in (-1, 0, 1)], axis=0)
+        assert (img[nb_min >= 1.0] == 255).all()
+        assert (img[nb_max <= 0.0] == 0).all()
+        total = _clip_area(pts, -1, -1, 33, 33) * 255
+        assert abs(img.sum() - total) <= 0.10 * total
+
+
+def _bezier(ctrl, t):
+    n = len(ctrl) - 1
+    from math import comb
+    return sum(comb(n, i) * (1 - t) ** (n - i) * t ** i * np.asarray(ctrl[i], dtype=np.float64)
+               for i in range(n + 1))
+
+
+def _dyadic_leaves(ctrl, tol_sq, t0=0.0, t1=1.0, out=None, margin=None):
+    """Independent FP64 statement of the subdivision rule: split [t0, t1] until
+    the curve point at the mid-parameter is within tolerance of the chord
+    midpoint; leaves are emitted at t1 in increasing order."""
+    if out is None:
+        out, margin = [], [np.inf]
+    pm = _bezier(ctrl, (t0 + t1) / 2)
+    chord = (_bezier(ctrl, t0) + _bezier(ctrl, t1)) / 2
+    d = float(((pm - chord) ** 2).sum())
+    margin[0] = min(margin[0], abs(d - tol_sq) / tol_sq)
+    if d <= tol_sq:
+        out.append(t1)
+    else:
+        _dyadic_leaves(ctrl, tol_sq, t0, (t0 + t1) / 2, out, margin)
+        _dyadic_leaves(ctrl, tol_sq, (t0 + t1) / 2, t1, out, margin)
+    return out, margin[0]
+
+
+def test_flatten_points_lie_on_curve_at_dyadic_parameters(oracle):
+    rng = np.random.default_rng(5)
+    tol = 0.3
+    checked = 0
+    for trial in range(60):
+        kind = QUAD if trial % 3 == 0 else CUBIC
+        m = 2 if kind == QUAD else 3
+        pen = rng.uniform(0, 200, size=2).astype(np.float32)
+        ctrl_f32 = rng.uniform(0, 200, size=(m, 2)).astype(np.float32)
+        ops = ops_array([(MOVE, tuple(pen)), (kind, tuple(ctrl_f32.reshape(-1)))])
+        p = oracle.Plotter(256, 256)
+        rc, pts = p.flatten(ops)
+        assert rc == 0
+        pts = pts[2:]  # drop the Move's close marker and point
+        ctrl = [pen.astype(np.float64)] + [c.astype(np.float64) for c in ctrl_f32]
+        ts, margin = _dyadic_leaves(ctrl, np.float64(np.float32(tol) * np.float32(tol)))
+        if margin < 1e-3:
+            continue  # a decision within rounding of the threshold
+        assert len(ts) == len(pts), trial
+        want = np.array([_bezier(ctrl, t) for t in ts])
+        assert np.abs(pts[:, :2] - want).max() < 2e-3, trial
+        assert (pts[-1, :2] == ctrl_f32[-1]).all()  # the end point is emitted exactly
+        checked += 1
+    assert checked >= 50
+
+
+def test_flatten_collinear_cubic_is_one_segment(oracle):
+    ops = ops_array([(MOVE, (10.0, 20.0)), (CUBIC, (13.0, 24.0, 16.0, 28.0, 19.0, 32.0))])
+    rc, pts = oracle.Plotter(64, 64).flatten(ops)
+    assert rc == 0 and len(pts) == 3 and tuple(pts[-1, :2]) == (19.0, 32.0)
+
+
+def test_flatten_nan_is_erange(oracle):
+    ops = ops_array([(MOVE, (10.0, 20.0)), (CUBIC, (float("nan"), 24.0, 16.0, 28.0, 19.0, 32.0)),
+                     (CLOSE, ())])
+    p = oracle.Plotter(64, 64)
+    assert p.fill(0, ops) == 4  # OR_ERANGE
+    assert not p.pixels().any()
+
+
+def test_path2d_builder_relative_and_close():
+    ops = (Path2D().relative().pen_width(3.0).move_to(112.0, 24.0).line_to(-32.0, 24.0)
+           .cubic_to(-96.0, -48.0, -96.0, 80.0, 0.0, 32.0).close().line_to(5.0, 6.0).finish())
+    assert ops[1]["tag"] == MOVE and tuple(ops[1]["v"][:2]) == (112.0, 24.0)
+    assert tuple(ops[2]["v"][:2]) == (80.0, 48.0)
+    assert tuple(ops[3]["v"]) == (-16.0, 0.0, -16.0, 128.0, 80.0, 80.0)
+    assert tuple(ops[5]["v"][:2]) == (5.0, 6.0)  # close() moved the pen to the origin
+
+
+def test_workloads_are_deterministic():
+    a, b = workloads.plain_256(0), workloads.plain_256(0)
+    assert (a.tobytes() == b.tobytes()) and len(a) == 66
+    assert (a["tag"] == LINE).sum() == 48 and (a["tag"] == CUBIC).sum() == 16
+    w = workloads.random_walk(1, n_cubics=50)
+    assert len(w) == 52 and (w["v"][1:-1] >= 0).all() and (w["v"][1:-1] < 4096).all()
+    ops, tiles = workloads.atlas(2, n_paths=300, atlas_side=4096)
+    assert tiles[:, 1].sum() == len(ops)
+    occ = np.zeros((4096, 4096), bool)
+    for _, _, x, y, s in tiles:
+        assert not occ[y:y + s, x:x + s].any()
+        occ[y:y + s, x:x + s] = True
