@@ -149,8 +149,8 @@ ft_status ft_plotter_size(const ft_plotter *p, uint32_t *width, uint32_t *height
 }
 
 ft_status ft_plotter_set_tolerance(ft_plotter *p, float t) {
-    if (!p || isnan(t)) return FT_EINVAL;
-    p->tolerance = t > 0.01f ? t : 0.01f;  // plotter.rs:134
+    if (!p) return FT_EINVAL;
+    p->tolerance = t > 0.01f ? t : 0.01f;  // t.max(0.01), plotter.rs:134 (NaN -> 0.01)
     return FT_OK;
 }
 
@@ -184,13 +184,13 @@ ft_status ft_plotter_stroke(ft_plotter *p, const ft_path_op *ops, size_t n, cons
     s = ft::ws_run(p->ws, ft::MODE_STROKE_FLATTEN, false);
     if (s) return s;
     const uint32_t n_ev = p->ws.h_status->n_events;
-    std::vector<float4> ev(n_ev);
-    if (n_ev && hipMemcpy(ev.data(), p->ws.d_ev_f, n_ev * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess)
+    std::vector<ft::F4> ev(n_ev);
+    if (n_ev && hipMemcpy(ev.data(), p->ws.d_ev_f, n_ev * sizeof(ft::F4), hipMemcpyDeviceToHost) != hipSuccess)
         return FT_EDEVICE;
     // 2. host stroker: Stroke::add_point / close / path_ops
     const float tol = p->tolerance;
     ft::Stroker st(p->join, p->miter, tol * tol);
-    for (const float4 &e : ev) {
+    for (const ft::F4 &e : ev) {
         if (e.w == 0.0f)
             st.add_point(ft::WidePoint{e.x, e.y, e.z});
         else

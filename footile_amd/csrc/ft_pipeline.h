@@ -16,57 +16,9 @@
 #include <vector>
 
 #include "../../include/footile_hip.h"
+#include "ft_types.h"
 
 namespace ft {
-
-enum : uint32_t { MODE_FILL = 0, MODE_STROKE_FLATTEN = 1 };
-
-// Per-path descriptor as the kernels see it.
-struct PathDev {
-    uint32_t op_begin, op_count;
-    uint32_t rule, fmt;
-    uint64_t dst;
-    uint32_t width, height, pitch;
-    uint32_t clr;  // packed bytes c0 | c1<<8 | c2<<16 | c3<<24
-    float tr[6];
-    float tol_sq;
-    float s_width0;
-    uint32_t band_base, band_rows, n_bands;
-    uint32_t flags;  // PATH_FRESH | PATH_RAW
-};
-
-enum : uint32_t {
-    PATH_FRESH = 1u,  // region starts cleared: rows above the figure are zeroed
-    PATH_RAW = 2u     // points go to Fig::add_point untransformed (Fig-level entry)
-};
-
-// Per-path results of a run (cleared at the start of each run).
-struct PathState {
-    unsigned long long topkey;  // min over points of (y, x) as a biased u64
-    uint32_t topidx;            // min local index among points at topkey
-    uint32_t offered;           // points offered to Fig::add_point
-    uint32_t repop;             // 1 if the cap re-pop dropped local point 65534
-    uint32_t pt_base;           // global index of the path's first point
-    uint32_t n_edges;           // non-horizontal segments
-    uint32_t stored;            // points stored in the Fig after the cap
-};
-
-struct DevStatus {
-    uint32_t err;        // first ft_status error raised by a kernel
-    uint32_t overflow;   // bit 0: events, bit 1: band pairs
-    uint32_t n_events, n_closes;
-    uint32_t n_kept, n_pairs;
-    uint32_t need_events, need_pairs;
-    unsigned long long op_total;  // packed (events << 32 | closes) of all ops
-    uint32_t spin_fail;
-    uint32_t tickets[4];
-    uint32_t pad[3];
-};
-
-struct EdgeRec {
-    int32_t y_upper, y_lower, x_bot0, inv, step;
-    uint32_t flags;  // bit0: edge present, bit1: Forward direction
-};
 
 struct Workspace {
     hipStream_t stream = nullptr;
@@ -84,12 +36,12 @@ struct Workspace {
     uint32_t *d_op_pw = nullptr, *d_op_lastpw = nullptr;
     // events and everything sized by them
     size_t cap_events = 0;
-    int2 *d_ev_xy = nullptr;
-    float4 *d_ev_f = nullptr;  // stroke-mode events
+    I2 *d_ev_xy = nullptr;
+    F4 *d_ev_f = nullptr;  // stroke-mode events
     uint32_t *d_ev_meta = nullptr, *d_ev_kept = nullptr, *d_ev_idx = nullptr;
     int32_t *d_run_first = nullptr, *d_run_last = nullptr;
     uint32_t *d_run_path = nullptr, *d_run_start = nullptr, *d_run_end = nullptr;
-    int2 *d_pts = nullptr;
+    I2 *d_pts = nullptr;
     uint32_t *d_pt_run = nullptr;
     EdgeRec *d_edges = nullptr;
     // bands
@@ -132,8 +84,5 @@ ft_status ws_launch(Workspace &ws, uint32_t mode);
 ft_status ws_finish(Workspace &ws, bool read_pstate);
 // launch + finish, growing capacities and relaunching on overflow.
 ft_status ws_run(Workspace &ws, uint32_t mode, bool read_pstate);
-
-// Band geometry for a raster: rows per band and LDS cells per band.
-void band_geometry(uint32_t width, uint32_t height, uint32_t *rows, uint32_t *n_bands, uint32_t *cells);
 
 }  // namespace ft

@@ -17,30 +17,29 @@
 //  * cell deltas are accumulated in LDS with i32 atomics (integer adds commute,
 //    so the sums are exact; i16 wrap is applied at the row scan), then a
 //    wavefront scan per row maps coverage and writes the raster once.
-#include "ft_device.h"
 #include "ft_pipeline.h"
+#include "ft_stages.h"
 
 #include <algorithm>
 #include <cstring>
-#include <type_traits>
 
 namespace ft {
 
 constexpr int BLOCK = 256;
 constexpr uint32_t GRID_MAX = 2048;     // grid-stride kernels: 8 blocks per CU
-constexpr uint32_t LDS_CELLS = 8192;    // i32 signed-area cells per raster block
-constexpr uint32_t BAND_ROWS_MAX = 64;
-constexpr uint32_t CELL_ALIGN = 16;     // row stride of cells in LDS
-constexpr uint32_t CLOSE_BIT = 0x80000000u;
-constexpr uint32_t NO_IDX = 0xFFFFFFFFu;
-constexpr unsigned long long NO_KEY = ~0ull;
 constexpr uint32_t SCAN_ITEMS = 8;
 constexpr uint32_t SCAN_TILE = BLOCK * SCAN_ITEMS;
 constexpr uint32_t SCAN_SLOTS = 4;      // look-back state instances per run
 constexpr uint32_t SPIN_LIMIT = 1u << 24;
 
-enum : uint32_t { TAG_CLOSE = 0, TAG_MOVE, TAG_LINE, TAG_QUAD, TAG_CUBIC, TAG_PEN_WIDTH };
-enum : uint32_t { FMT_MATTE8 = 0, FMT_RGBA8P = 1, FMT_GRAYA8P = 2 };
+// Device atomics for the stage code.
+struct DevAtom {
+    __device__ static void max_i32(int32_t *p, int32_t v) { atomicMax(p, v); }
+    __device__ static void max_u32(uint32_t *p, uint32_t v) { atomicMax(p, v); }
+    __device__ static void min_u32(uint32_t *p, uint32_t v) { atomicMin(p, v); }
+    __device__ static uint32_t add_u32(uint32_t *p, uint32_t v) { return atomicAdd(p, v); }
+    __device__ static void add_cell(int32_t *p, int32_t v) { atomicAdd(p, v); }
+};
 
 __device__ inline void raise_err(DevStatus *st, uint32_t code) { atomicCAS(&st->err, 0u, code); }
 __device__ inline bool run_failed(const DevStatus *st) {
@@ -48,186 +47,47 @@ __device__ inline bool run_failed(const DevStatus *st) {
            __hip_atomic_load(&st->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
 }
 
-// ---------------------------------------------------------------------------
-// Flattening (src/plotter.rs:163-332, src/geom.rs:9-35)
-// ---------------------------------------------------------------------------
-struct WP {
-    float x, y, w;
-};
-
-// WidePt::midpoint: position midpoint (a+b)/2 per axis, width (w0+w1)/2.
-__device__ inline WP wmid(WP a, WP b) {
-    WP r;
-    r.x = (a.x + b.x) / 2.0f;
-    r.y = (a.y + b.y) / 2.0f;
-    r.w = (a.w + b.w) / 2.0f;
-    return r;
+// Add `v` to a per-path counter (u32 field at `base + p * stride_words`) for
+// every lane with `has`: one atomic per wavefront for the lanes sharing the
+// first active lane's path, per-lane atomics for the rest.
+__device__ inline void wave_count(uint32_t *base, size_t stride_words, uint32_t n_paths, bool has, uint32_t p,
+                                  uint32_t v) {
+    const int lane = threadIdx.x & 63;
+    has = has & (p < n_paths);
+    const unsigned long long hm = __ballot(has);
+    if (!hm) return;
+    const int fl = __ffsll((long long)hm) - 1;
+    const uint32_t lead = __shfl(p, fl, 64);
+    const bool same = has && p == lead;
+    uint32_t t = same ? v : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (lane == fl && t && lead < n_paths) atomicAdd(base + (size_t)lead * stride_words, t);
+    if (has && !same && v) atomicAdd(base + (size_t)p * stride_words, v);
 }
 
-// Plotter::is_within_tolerance2: distance_sq <= tol_sq (plotter.rs:273-276)
-__device__ inline bool within_tol(WP a, WP b, float tol_sq) {
-    float dx = a.x - b.x;
-    float dy = a.y - b.y;
-    return dx * dx + dy * dy <= tol_sq;
-}
-
-// Plotter::transform_point (plotter.rs:169-172): width is not transformed.
-__device__ inline WP xform(const PathDev &P, float x, float y, float w) {
-    WP r;
-    if (P.flags & PATH_RAW) {
-        r.x = x;
-        r.y = y;
-    } else {
-        r.x = P.tr[0] * x + P.tr[1] * y + P.tr[2];
-        r.y = P.tr[3] * x + P.tr[4] * y + P.tr[5];
+// Per-path minimum of a u64 key, reduced per wavefront like wave_count.
+__device__ inline void wave_min_key(PathState *ps, uint32_t n_paths, bool has, uint32_t p, unsigned long long key) {
+    const int lane = threadIdx.x & 63;
+    has = has & (p < n_paths);
+    const unsigned long long hm = __ballot(has);
+    if (!hm) return;
+    const int fl = __ffsll((long long)hm) - 1;
+    const uint32_t lead = __shfl(p, fl, 64);
+    unsigned long long k = (has && p == lead) ? key : NO_KEY;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long t = __shfl_xor(k, o, 64);
+        k = t < k ? t : k;
     }
-    r.w = w;
-    return r;
+    if (lane == fl && lead < n_paths) atomicMin(&ps[lead].topkey, k);
+    if (has && p != lead) atomicMin(&ps[p].topkey, key);
 }
 
-struct Quad {
-    WP a, b, c;
-};
-struct Cubic {
-    WP a, b, c, d;
-};
-
-// One subdivision step of quad_to_tran (plotter.rs:255-258).
-__device__ inline bool quad_split(const Quad &q, float tol_sq, Quad &L, Quad &R) {
-    WP ab = wmid(q.a, q.b);
-    WP bc = wmid(q.b, q.c);
-    WP ab_bc = wmid(ab, bc);
-    WP ac = wmid(q.a, q.c);
-    L.a = q.a; L.b = ab; L.c = ab_bc;
-    R.a = ab_bc; R.b = bc; R.c = q.c;
-    return within_tol(ab_bc, ac, tol_sq);
-}
-
-// One subdivision step of cubic_to_tran (plotter.rs:319-326).
-__device__ inline bool cubic_split(const Cubic &q, float tol_sq, Cubic &L, Cubic &R) {
-    WP ab = wmid(q.a, q.b);
-    WP bc = wmid(q.b, q.c);
-    WP cd = wmid(q.c, q.d);
-    WP ab_bc = wmid(ab, bc);
-    WP bc_cd = wmid(bc, cd);
-    WP pe = wmid(ab_bc, bc_cd);
-    WP ad = wmid(q.a, q.d);
-    L.a = q.a; L.b = ab; L.c = ab_bc; L.d = pe;
-    R.a = pe; R.b = bc_cd; R.c = cd; R.d = q.d;
-    return within_tol(pe, ad, tol_sq);
-}
-
-__device__ inline WP end_of(const Quad &q) { return q.c; }
-__device__ inline WP end_of(const Cubic &q) { return q.d; }
-
-// Depth-first walk of the subdivision tree in the recursion's order (left
-// subtree, then right).  `bits` holds the left/right choices from the root;
-// moving to a right sibling replays those choices from the root, so every
-// node is computed by exactly the float operations the recursion performs.
-// Returns 0, or FT_ERANGE when a node at FT_FLATTEN_MAX_DEPTH is still out of
-// tolerance or the curve yields more than FT_FLATTEN_MAX_LEAVES points.
-template <class Curve, class Emit>
-__device__ uint32_t flatten_walk(const Curve &root, float tol_sq, Emit &&emit) {
-    Curve cur = root;
-    uint32_t depth = 0, bits = 0, leaves = 0;
-    for (;;) {
-        Curve L, R;
-        bool leaf;
-        if constexpr (std::is_same_v<Curve, Quad>)
-            leaf = quad_split(cur, tol_sq, L, R);
-        else
-            leaf = cubic_split(cur, tol_sq, L, R);
-        if (leaf) {
-            if (++leaves > FT_FLATTEN_MAX_LEAVES) return FT_ERANGE;
-            emit(end_of(cur));
-            while (depth > 0 && (bits & 1u)) {
-                bits >>= 1;
-                --depth;
-            }
-            if (depth == 0) return 0;
-            bits |= 1u;
-            cur = root;
-            for (int lv = (int)depth - 1; lv >= 0; --lv) {
-                Curve l2, r2;
-                if constexpr (std::is_same_v<Curve, Quad>)
-                    quad_split(cur, tol_sq, l2, r2);
-                else
-                    cubic_split(cur, tol_sq, l2, r2);
-                cur = ((bits >> lv) & 1u) ? r2 : l2;
-            }
-        } else {
-            if (depth >= FT_FLATTEN_MAX_DEPTH) return FT_ERANGE;
-            cur = L;
-            bits <<= 1;
-            ++depth;
-        }
-    }
-}
-
-// Index of the op that last set the pen before op i (plotter.rs:128-130,
-// 164-166): PenWidth ops do not move the pen.  Returns < begin at path start.
-__device__ inline int64_t pen_event_before(const ft_path_op *ops, uint32_t i, uint32_t begin) {
-    int64_t j = (int64_t)i - 1;
-    while (j >= (int64_t)begin && ops[j].tag == TAG_PEN_WIDTH) --j;
-    return j;
-}
-
-// s_width in effect just before op k: the last PenWidth op of the path before
-// k, else the width the fill started with (plotter.rs:53,151-153).
-__device__ inline float width_before(const ft_path_op *ops, const uint32_t *lastpw, int64_t k,
-                                     const PathDev &P) {
-    if (!lastpw || k < (int64_t)P.op_begin) return P.s_width0;
-    uint32_t l = lastpw[k];
-    if (l == 0 || l - 1 < P.op_begin) return P.s_width0;
-    return ops[l - 1].v[0];
-}
-
-// Pen (untransformed WidePt) before op i.
-__device__ inline WP pen_before(const ft_path_op *ops, const uint32_t *lastpw, uint32_t i,
-                                const PathDev &P) {
-    int64_t j = pen_event_before(ops, i, P.op_begin);
-    WP pen;
-    pen.x = 0.0f;
-    pen.y = 0.0f;
-    if (j < (int64_t)P.op_begin) {
-        pen.w = P.s_width0;
-        return pen;
-    }
-    pen.w = width_before(ops, lastpw, j, P);
-    const ft_path_op &o = ops[j];
-    switch (o.tag) {
-    case TAG_MOVE:
-    case TAG_LINE: pen.x = o.v[0]; pen.y = o.v[1]; break;
-    case TAG_QUAD: pen.x = o.v[2]; pen.y = o.v[3]; break;
-    case TAG_CUBIC: pen.x = o.v[4]; pen.y = o.v[5]; break;
-    default: break;  // Close resets the pen to the origin
-    }
-    return pen;
-}
-
-__device__ inline Quad quad_of(const ft_path_op &o, WP pen, float sw, const PathDev &P) {
-    Quad q;  // plotter.rs:233-240
-    q.a = xform(P, pen.x, pen.y, pen.w);
-    q.b = xform(P, o.v[0], o.v[1], (pen.w + sw) / 2.0f);
-    q.c = xform(P, o.v[2], o.v[3], sw);
-    return q;
-}
-
-__device__ inline Cubic cubic_of(const ft_path_op &o, WP pen, float sw, const PathDev &P) {
-    // float_lerp(a, b, t) = b + (a - b) * t   (geom.rs:14-16, plotter.rs:294-295)
-    const float t1 = 1.0f / 3.0f, t2 = 2.0f / 3.0f;
-    float w0 = sw + (pen.w - sw) * t1;
-    float w1 = sw + (pen.w - sw) * t2;
-    Cubic q;
-    q.a = xform(P, pen.x, pen.y, pen.w);
-    q.b = xform(P, o.v[0], o.v[1], w0);
-    q.c = xform(P, o.v[2], o.v[3], w1);
-    q.d = xform(P, o.v[4], o.v[5], sw);
-    return q;
-}
+constexpr size_t PS_WORDS = sizeof(PathState) / 4;
 
 // ---------------------------------------------------------------------------
-// Stage 0: clear per-run state.
+// Stage 0: clear per-run state (a kernel, not hipMemsetAsync: graph-safe).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, uint32_t n_paths, uint32_t *band_cnt,
                                                  uint32_t *band_cur, uint32_t n_bands, uint32_t *flags,
@@ -252,8 +112,7 @@ __global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, uint32_t n_paths
         }
         if (i < n_flags) flags[i] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x < sizeof(DevStatus) / 4)
-        reinterpret_cast<uint32_t *>(st)[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < sizeof(DevStatus) / 4) reinterpret_cast<uint32_t *>(st)[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -344,8 +203,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint3
                             __hip_atomic_load(&incl[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), prefix);
                         break;
                     }
-                    prefix = scan_op<IS_MAX>(
-                        __hip_atomic_load(&agg[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), prefix);
+                    prefix = scan_op<IS_MAX>(__hip_atomic_load(&agg[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                                             prefix);
                     --j;
                 }
                 __hip_atomic_store(&incl[tile], scan_op<IS_MAX>(prefix, btot), __ATOMIC_RELAXED,
@@ -365,409 +224,147 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint3
 }
 
 // ---------------------------------------------------------------------------
-// Stage 1: per-op event counts (plotter.rs:175-224 + the final fig.close()).
-// An op yields: Close 1 close; Move 1 close + 1 point; Line 1 point; Quad and
-// Cubic one point per subdivision leaf; PenWidth nothing.
+// Stages 1-2: ops -> events (count, scan, emit).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_pen_width_mark(const ft_path_op *ops, uint32_t n_ops, uint32_t *op_pw) {
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n_ops; i += gridDim.x * BLOCK)
         op_pw[i] = ops[i].tag == TAG_PEN_WIDTH ? i + 1 : 0u;
 }
 
-struct CountLeaf {
-    __device__ void operator()(WP) const {}
-};
-
 __global__ __launch_bounds__(BLOCK) void k_count(const ft_path_op *ops, const uint32_t *op_path,
                                                  const PathDev *paths, uint32_t n_ops, uint32_t mode,
                                                  const uint32_t *lastpw, unsigned long long *op_cnt,
                                                  DevStatus *st) {
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n_ops; i += gridDim.x * BLOCK) {
-        uint32_t ne = 0, nc = 0;
-        const uint32_t p = op_path[i];
-        if (p != NO_IDX) {
-            const PathDev &P = paths[p];
-            const ft_path_op o = ops[i];
-            switch (o.tag) {
-            case TAG_CLOSE: ne = 1; nc = 1; break;
-            case TAG_MOVE: ne = 2; nc = 1; break;
-            case TAG_LINE: ne = 1; break;
-            case TAG_QUAD:
-            case TAG_CUBIC: {
-                WP pen = pen_before(ops, lastpw, i, P);
-                float sw = width_before(ops, lastpw, i, P);
-                uint32_t leaves = 0;
-                auto cnt = [&](WP) { ++leaves; };
-                uint32_t rc = o.tag == TAG_QUAD ? flatten_walk(quad_of(o, pen, sw, P), P.tol_sq, cnt)
-                                                : flatten_walk(cubic_of(o, pen, sw, P), P.tol_sq, cnt);
-                if (rc) raise_err(st, rc);
-                else ne = leaves;
-                break;
-            }
-            case TAG_PEN_WIDTH: break;
-            default: raise_err(st, FT_EINVAL); break;
-            }
-            if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count) {
-                ++ne;  // Plotter::fill closes the figure (plotter.rs:347)
-                ++nc;
-            }
-        }
-        op_cnt[i] = ((unsigned long long)ne << 32) | nc;
+        const OpCount c = count_op(ops, op_path, paths, i, mode, lastpw);
+        if (c.err) raise_err(st, c.err);
+        op_cnt[i] = pack_count(c);
     }
 }
-
-// ---------------------------------------------------------------------------
-// Stage 2: emit events.  Fill mode writes Fixed points (f32 -> Fixed as
-// Fig::add_point does, fig.rs:435-436) and the run (sub-figure) id of every
-// event; stroke mode writes (x, y, width, kind) for the host stroker.
-// ---------------------------------------------------------------------------
-struct Emitter {
-    uint32_t mode;
-    uint64_t e;
-    uint32_t run, p;
-    int2 *ev_xy;
-    float4 *ev_f;
-    uint32_t *ev_meta;
-    int32_t *run_first, *run_last;
-    uint32_t *run_path, *run_start, *run_end;
-    __device__ void point(WP q) {
-        if (mode == MODE_FILL) {
-            ev_xy[e] = make_int2(fx_from_f32(q.x), fx_from_f32(q.y));
-            ev_meta[e] = run;
-        } else {
-            ev_f[e] = make_float4(q.x, q.y, q.w, 0.0f);
-        }
-        ++e;
-    }
-    __device__ void close(bool joined) {
-        if (mode == MODE_FILL) {
-            ev_meta[e] = run | CLOSE_BIT;
-            run_first[run] = -1;
-            run_last[run] = -1;
-            run_path[run] = p;
-            run_start[run] = NO_IDX;
-            run_end[run] = 0;
-        } else {
-            ev_f[e] = make_float4(__builtin_nanf(""), __builtin_nanf(""), joined ? 1.0f : 0.0f,
-                                  joined ? 2.0f : 1.0f);
-        }
-        ++run;
-        ++e;
-    }
-};
 
 __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uint32_t *op_path,
                                                 const PathDev *paths, uint32_t n_ops, uint32_t mode,
                                                 const uint32_t *lastpw, const unsigned long long *op_off,
-                                                uint64_t cap_events, Emitter em_base, DevStatus *st) {
+                                                EventSink sink, DevStatus *st) {
     if (run_failed(st)) return;
     const unsigned long long tot = st->op_total;
     const uint32_t n_events = (uint32_t)(tot >> 32);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->n_events = n_events;
         st->n_closes = (uint32_t)tot;
-        if (n_events > cap_events) {
+        if (n_events > sink.cap_events) {
             atomicOr(&st->overflow, 1u);
             st->need_events = n_events;
         }
     }
-    if (n_events > cap_events) return;
-    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n_ops; i += gridDim.x * BLOCK) {
-        const uint32_t p = op_path[i];
-        if (p == NO_IDX) continue;
-        const PathDev &P = paths[p];
-        const ft_path_op o = ops[i];
-        Emitter em = em_base;
-        em.e = op_off[i] >> 32;
-        em.run = (uint32_t)op_off[i];
-        em.p = p;
-        switch (o.tag) {
-        case TAG_CLOSE: em.close(true); break;  // plotter.rs:200-203
-        case TAG_MOVE: {                         // plotter.rs:208-214
-            em.close(false);
-            em.point(xform(P, o.v[0], o.v[1], width_before(ops, lastpw, i, P)));
-            break;
-        }
-        case TAG_LINE:  // plotter.rs:219-224
-            em.point(xform(P, o.v[0], o.v[1], width_before(ops, lastpw, i, P)));
-            break;
-        case TAG_QUAD:
-        case TAG_CUBIC: {
-            WP pen = pen_before(ops, lastpw, i, P);
-            float sw = width_before(ops, lastpw, i, P);
-            auto put = [&](WP q) { em.point(q); };
-            if (o.tag == TAG_QUAD)
-                flatten_walk(quad_of(o, pen, sw, P), P.tol_sq, put);
-            else
-                flatten_walk(cubic_of(o, pen, sw, P), P.tol_sq, put);
-            break;
-        }
-        default: break;
-        }
-        if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count) em.close(true);
-    }
+    if (n_events > sink.cap_events) return;
+    uint32_t bad = 0;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n_ops; i += gridDim.x * BLOCK)
+        bad |= emit_op(ops, op_path, paths, i, mode, lastpw, op_off[i], sink);
+    if (bad) raise_err(st, FT_EDEVICE);
+}
+
+// Limits of the current run, from the counts the earlier stages produced.
+__device__ inline Limits run_limits(const DevStatus *st, uint32_t n_ops, uint32_t n_paths, uint32_t n_bands,
+                                    uint32_t cap_points) {
+    Limits L;
+    L.n_ops = n_ops;
+    L.n_paths = n_paths;
+    L.n_bands = n_bands;
+    L.n_events = st->n_events;
+    L.n_runs = st->n_closes;
+    L.cap_points = cap_points;
+    L.n_pairs = st->n_pairs;
+    return L;
 }
 
 // ---------------------------------------------------------------------------
-// Stage 3: Fig::add_point / Fig::close semantics (fig.rs:373-383,428-451).
-// Within a run (events between closes) a point is pushed if it is the run's
-// first or differs in Fixed from the previous point event; closing pops the
-// last pushed point when it equals the run's first point.
+// Stages 3-4: Fig ingest (runs, closing pop, cap, compaction).
 // ---------------------------------------------------------------------------
-__device__ inline bool int2_eq(int2 a, int2 b) { return a.x == b.x && a.y == b.y; }
-
-__device__ inline bool is_pushed(const uint32_t *meta, const int2 *xy, uint32_t e) {
-    if (e == 0 || (meta[e - 1] & CLOSE_BIT)) return true;
-    return !int2_eq(xy[e], xy[e - 1]);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_ingest(const uint32_t *ev_meta, const int2 *ev_xy, int32_t *run_first,
-                                                  int32_t *run_last, const uint32_t *run_path, PathState *ps,
-                                                  DevStatus *st) {
+__global__ __launch_bounds__(BLOCK) void k_ingest(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
+                                                  int32_t *run_first, int32_t *run_last, const uint32_t *run_path,
+                                                  PathState *ps, DevStatus *st) {
     if (run_failed(st)) return;
-    const uint32_t n = st->n_events;
-    const int lane = threadIdx.x & 63;
-    for (uint32_t b = blockIdx.x * BLOCK; b < n; b += gridDim.x * BLOCK) {
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t bad = 0;
+    for (uint32_t b = blockIdx.x * BLOCK; b < L.n_events; b += gridDim.x * BLOCK) {
         const uint32_t e = b + threadIdx.x;
-        bool is_pt = false;
         uint32_t p = NO_IDX;
-        if (e < n) {
-            const uint32_t m = ev_meta[e];
-            if (!(m & CLOSE_BIT)) {
-                is_pt = true;
-                const bool first = e == 0 || (ev_meta[e - 1] & CLOSE_BIT);
-                if (first) run_first[m] = (int32_t)e;
-                if (first || !int2_eq(ev_xy[e], ev_xy[e - 1])) atomicMax(&run_last[m], (int32_t)e);
-                p = run_path[m];
-            }
-        }
-        // offered points per path, aggregated per wavefront
-        const unsigned long long hm = __ballot(is_pt);
-        if (hm) {
-            const int fl = __ffsll((long long)hm) - 1;
-            const uint32_t lead = __shfl(p, fl, 64);
-            const unsigned long long same = __ballot(is_pt && p == lead);
-            if (lane == fl) atomicAdd(&ps[lead].offered, (uint32_t)__popcll(same));
-            if (is_pt && p != lead) atomicAdd(&ps[p].offered, 1u);
-        }
+        if (e < L.n_events) p = ingest_event<DevAtom>(e, L, ev_meta, ev_xy, run_first, run_last, run_path, bad);
+        wave_count(&ps[0].offered, PS_WORDS, L.n_paths, p != NO_IDX, p, 1u);
     }
+    if (bad) raise_err(st, FT_EDEVICE);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_kept(const uint32_t *ev_meta, const int2 *ev_xy, const int32_t *run_first,
-                                                const int32_t *run_last, uint32_t *ev_kept, DevStatus *st) {
+__global__ __launch_bounds__(BLOCK) void k_kept(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
+                                                const int32_t *run_first, const int32_t *run_last, uint32_t *ev_kept,
+                                                DevStatus *st) {
     if (run_failed(st)) return;
-    const uint32_t n = st->n_events;
-    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < n; e += gridDim.x * BLOCK) {
-        const uint32_t m = ev_meta[e];
-        uint32_t kept = 0;
-        if (!(m & CLOSE_BIT) && is_pushed(ev_meta, ev_xy, e)) {
-            const int32_t rf = run_first[m], rl = run_last[m];
-            const bool pop = int2_eq(ev_xy[rf], ev_xy[rl]);
-            kept = (pop && (int32_t)e == rl) ? 0u : 1u;
-        }
-        ev_kept[e] = kept;
-    }
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t bad = 0;
+    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
+        ev_kept[e] = kept_event(e, L, ev_meta, ev_xy, run_first, run_last, bad);
+    if (bad) raise_err(st, FT_EDEVICE);
 }
 
-// Local point index of kept event e in its path, or NO_IDX when the 65,535
-// cap drops it.  `repop` tells whether the cap's re-pop removed local 65534.
-__device__ inline uint32_t capped_local(uint32_t idx, uint32_t base, uint32_t repop) {
-    const uint32_t local = idx - base;
-    if (local >= FT_MAX_POINTS) return NO_IDX;
-    if (local == FT_MAX_POINTS - 1 && repop) return NO_IDX;
-    return local;
-}
-
-__device__ inline uint32_t path_event_begin(const PathDev &P, const unsigned long long *op_off) {
-    return (uint32_t)(op_off[P.op_begin] >> 32);
-}
-
-__device__ inline uint32_t path_event_end(const PathDev &P, const unsigned long long *op_off, uint32_t n_ops,
-                                          uint32_t n_events) {
-    const uint32_t k = P.op_begin + P.op_count;
-    return k < n_ops ? (uint32_t)(op_off[k] >> 32) : n_events;
-}
-
-__device__ inline unsigned long long point_key(int2 q) {
-    return ((unsigned long long)((uint32_t)q.y ^ 0x80000000u) << 32) | (unsigned long long)((uint32_t)q.x ^ 0x80000000u);
-}
-
-// Stage 4: compaction into the per-path point arrays (the Fig's `points`),
-// with the 65,535-point cap (vid.rs:17, fig.rs:429-431).  Once a path's
-// stored count reaches the cap every later point is ignored; the run holding
-// the cap point then closes against its truncated last point, which may pop
-// local point 65534.  Every later run only adds a point and pops it again.
-__global__ __launch_bounds__(BLOCK) void k_compact(const uint32_t *ev_meta, const int2 *ev_xy, const uint32_t *ev_kept,
-                                                   const uint32_t *ev_idx, const int32_t *run_first,
-                                                   const uint32_t *run_path, uint32_t *run_start,
-                                                   uint32_t *run_end, const PathDev *paths,
-                                                   const unsigned long long *op_off, uint32_t n_ops, int2 *pts,
-                                                   uint32_t *pt_run, PathState *ps, DevStatus *st) {
+__global__ __launch_bounds__(BLOCK) void k_cap(Limits L0, const PathDev *paths, const unsigned long long *op_off,
+                                               const uint32_t *ev_meta, const I2 *ev_xy, const uint32_t *ev_kept,
+                                               const uint32_t *ev_idx, const int32_t *run_first, PathState *ps,
+                                               DevStatus *st) {
     if (run_failed(st)) return;
-    const uint32_t n = st->n_events;
-    const int lane = threadIdx.x & 63;
-    for (uint32_t b = blockIdx.x * BLOCK; b < n; b += gridDim.x * BLOCK) {
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t bad = 0;
+    for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < L.n_paths; p += gridDim.x * BLOCK)
+        cap_path(p, L, paths, op_off, ev_meta, ev_xy, ev_kept, ev_idx, run_first, ps, bad);
+    if (bad) raise_err(st, FT_EDEVICE);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_compact(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
+                                                   const uint32_t *ev_kept, const uint32_t *ev_idx,
+                                                   const int32_t *run_first, const uint32_t *run_path,
+                                                   uint32_t *run_start, uint32_t *run_end, I2 *pts, uint32_t *pt_run,
+                                                   PathState *ps, DevStatus *st) {
+    if (run_failed(st)) return;
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t bad = 0;
+    for (uint32_t b = blockIdx.x * BLOCK; b < L.n_events; b += gridDim.x * BLOCK) {
         const uint32_t e = b + threadIdx.x;
-        bool has = false;
-        uint32_t p = NO_IDX;
-        unsigned long long key = NO_KEY;
-        if (e < n) {
-            const uint32_t m = ev_meta[e];
-            const uint32_t run = m & ~CLOSE_BIT;
-            p = run_path[run];
-            const PathDev &P = paths[p];
-            const uint32_t eb = path_event_begin(P, op_off);
-            const uint32_t base = ev_idx[eb];
-            if (e == eb) ps[p].pt_base = base;
-            if (ev_kept[e]) {
-                const uint32_t idx = ev_idx[e];
-                uint32_t local = idx - base;
-                bool keep = local < FT_MAX_POINTS;
-                if (local == FT_MAX_POINTS - 1) {
-                    // Find the next pushed point of the path: if it lies in the
-                    // same run, the cap hit this run and its close compares the
-                    // run's first point with this one (fig.rs:373-383).
-                    const uint32_t ee = path_event_end(P, op_off, n_ops, n);
-                    for (uint32_t e2 = e + 1; e2 < ee; ++e2) {
-                        if (ev_meta[e2] & CLOSE_BIT) continue;
-                        if (!is_pushed(ev_meta, ev_xy, e2)) continue;
-                        if (ev_meta[e2] == run && int2_eq(ev_xy[run_first[run]], ev_xy[e])) {
-                            keep = false;
-                            ps[p].repop = 1;
-                        }
-                        break;
-                    }
-                }
-                if (keep) {
-                    const int2 q = ev_xy[e];
-                    pts[idx] = q;
-                    pt_run[idx] = run;
-                    if ((int32_t)e == run_first[run]) run_start[run] = idx;
-                    atomicMax(&run_end[run], idx + 1);
-                    has = true;
-                    key = point_key(q);
-                }
-            }
-        }
-        // top-left point per path: min over (y, x), reduced per wavefront
-        const unsigned long long hm = __ballot(has);
-        if (hm) {
-            const int fl = __ffsll((long long)hm) - 1;
-            const uint32_t lead = __shfl(p, fl, 64);
-            unsigned long long k = (has && p == lead) ? key : NO_KEY;
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                unsigned long long t = __shfl_xor(k, o, 64);
-                k = t < k ? t : k;
-            }
-            if (lane == fl) atomicMin(&ps[lead].topkey, k);
-            if (has && p != lead) atomicMin(&ps[p].topkey, key);
-        }
+        Stored s;
+        s.path = NO_IDX;
+        s.stored = false;
+        s.key = NO_KEY;
+        if (e < L.n_events)
+            s = compact_event<DevAtom>(e, L, ev_meta, ev_xy, ev_kept, ev_idx, run_first, run_path, run_start, run_end,
+                                       pts, pt_run, ps, bad);
+        wave_min_key(ps, L.n_paths, s.stored, s.path, s.key);
     }
+    if (bad) raise_err(st, FT_EDEVICE);
 }
 
-// Scan rows of an edge mapped to raster rows (fig.rs:496-500,536-553): scan
-// row y_row = raster row + min(top_row, 0); rows run from max(top_row, 0) to
-// the raster bottom.  Returns false when the edge touches no raster row.
-__device__ inline bool edge_rows(const EdgeRec &E, int32_t top_row, uint32_t H, int32_t &rr0, int32_t &rr1) {
-    const int64_t first = top_row > 0 ? top_row : 0;
-    if (first >= (int64_t)H) return false;
-    const int64_t shift = top_row < 0 ? top_row : 0;
-    int64_t a = (int64_t)fx_int(E.y_upper) - shift;
-    int64_t b = (int64_t)fx_int(E.y_lower) - shift;
-    if (a < first) a = first;
-    if (b > (int64_t)H - 1) b = (int64_t)H - 1;
-    if (a > b) return false;
-    rr0 = (int32_t)a;
-    rr1 = (int32_t)b;
-    return true;
-}
-
-__device__ inline int32_t top_row_of(unsigned long long key) {
-    return ((int32_t)((uint32_t)(key >> 32) ^ 0x80000000u)) >> 16;
-}
-
-// Add `v` to a per-path counter for every lane with `has`, one atomic per
-// distinct path in the wavefront for the common single-path case.
-__device__ inline void wave_count(uint32_t *base, size_t stride_words, bool has, uint32_t p, uint32_t v) {
-    const int lane = threadIdx.x & 63;
-    const unsigned long long hm = __ballot(has);
-    if (!hm) return;
-    const int fl = __ffsll((long long)hm) - 1;
-    const uint32_t lead = __shfl(p, fl, 64);
-    const bool same = has && p == lead;
-    uint32_t t = same ? v : 0u;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-    if (lane == fl && t) atomicAdd(base + (size_t)lead * stride_words, t);
-    if (has && !same && v) atomicAdd(base + (size_t)p * stride_words, v);
-}
-
-constexpr size_t PS_WORDS = sizeof(PathState) / 4;
-
-// Stage 5: one edge per stored point: the segment to its successor within the
-// sub-figure (SubFig::next, fig.rs:143-163).  Edge::new (fig.rs:179-210) runs
-// once per edge; the edge's rows are binned into row bands.
-
-__global__ __launch_bounds__(BLOCK) void k_edges(const uint32_t *ev_meta, const uint32_t *ev_kept,
+// ---------------------------------------------------------------------------
+// Stage 5: edges and row-band binning.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, const uint32_t *ev_meta, const uint32_t *ev_kept,
                                                  const uint32_t *ev_idx, const uint32_t *run_path,
                                                  const uint32_t *run_start, const uint32_t *run_end,
-                                                 const PathDev *paths, const int2 *pts, EdgeRec *edges,
+                                                 const PathDev *paths, const I2 *pts, EdgeRec *edges,
                                                  uint32_t *band_cnt, PathState *ps, DevStatus *st) {
     if (run_failed(st)) return;
-    const uint32_t n = st->n_events;
-    for (uint32_t b = blockIdx.x * BLOCK; b < n; b += gridDim.x * BLOCK) {
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t bad = 0;
+    for (uint32_t b = blockIdx.x * BLOCK; b < L.n_events; b += gridDim.x * BLOCK) {
         const uint32_t e = b + threadIdx.x;
-        bool valid = false, has_edge = false;
-        uint32_t p = NO_IDX;
-        if (e < n && ev_kept[e]) {
-            const uint32_t run = ev_meta[e];
-            p = run_path[run];
-            const uint32_t i = ev_idx[e];
-            const uint32_t local = capped_local(i, ps[p].pt_base, ps[p].repop);
-            if (local != NO_IDX) {
-                valid = true;
-                const int2 q = pts[i];
-                if (point_key(q) == ps[p].topkey) atomicMin(&ps[p].topidx, local);
-                const uint32_t nx = (i + 1 < run_end[run]) ? i + 1 : run_start[run];
-                EdgeRec E;
-                E.flags = 0;
-                E.y_upper = E.y_lower = E.x_bot0 = E.inv = E.step = 0;
-                if (nx != i) {
-                    const int2 q1 = pts[nx];
-                    if (q1.y != q.y) {
-                        const bool fwd = q1.y > q.y;  // Forward: descends in vertex order
-                        const int2 up = fwd ? q : q1, lo = fwd ? q1 : q;
-                        const int32_t dx = wsub(lo.x, up.x), dy = wsub(lo.y, up.y);
-                        int32_t step = 0;
-                        if (dx != 0) {  // Edge::calculate_step
-                            step = fx_abs(fx_div(dy, dx));
-                            if (step > FX_ONE) step = FX_ONE;
-                        }
-                        E.inv = fx_div(dx, dy);
-                        E.step = step;
-                        E.y_upper = up.y;
-                        E.y_lower = lo.y;
-                        E.x_bot0 = wadd(up.x, fx_mul(E.inv, wsub(fx_floor(wadd(up.y, FX_ONE)), up.y)));
-                        E.flags = 1u | (fwd ? 2u : 0u);
-                        has_edge = true;
-                        const PathDev &P = paths[p];
-                        int32_t r0, r1;
-                        if (edge_rows(E, top_row_of(ps[p].topkey), P.height, r0, r1)) {
-                            const uint32_t b0 = (uint32_t)r0 / P.band_rows, b1 = (uint32_t)r1 / P.band_rows;
-                            for (uint32_t bb = b0; bb <= b1; ++bb) atomicAdd(&band_cnt[P.band_base + bb], 1u);
-                        }
-                    }
-                }
-                edges[i] = E;
-            }
-        }
-        wave_count(&ps[0].stored, PS_WORDS, valid, p, 1u);
-        wave_count(&ps[0].n_edges, PS_WORDS, has_edge, p, 1u);
+        uint32_t p = NO_IDX, r = 0;
+        if (e < L.n_events)
+            r = edge_event<DevAtom>(e, L, ev_meta, ev_kept, ev_idx, run_path, run_start, run_end, paths, pts, edges,
+                                    band_cnt, ps, &p, bad);
+        wave_count(&ps[0].stored, PS_WORDS, L.n_paths, (r & 1u) != 0, p, 1u);
+        wave_count(&ps[0].n_edges, PS_WORDS, L.n_paths, (r & 2u) != 0, p, 1u);
     }
+    if (bad) raise_err(st, FT_EDEVICE);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bin(const uint32_t *ev_meta, const uint32_t *ev_kept,
+__global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, const uint32_t *ev_meta, const uint32_t *ev_kept,
                                                const uint32_t *ev_idx, const uint32_t *run_path,
                                                const PathDev *paths, const EdgeRec *edges,
                                                const uint32_t *band_off, uint32_t *band_cur, uint32_t *band_list,
@@ -781,79 +378,17 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const uint32_t *ev_meta, const ui
         }
         return;
     }
-    const uint32_t n = st->n_events;
-    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < n; e += gridDim.x * BLOCK) {
-        if (!ev_kept[e]) continue;
-        const uint32_t run = ev_meta[e];
-        const uint32_t p = run_path[run];
-        const uint32_t i = ev_idx[e];
-        if (capped_local(i, ps[p].pt_base, ps[p].repop) == NO_IDX) continue;
-        const EdgeRec E = edges[i];
-        if (!(E.flags & 1u)) continue;
-        const PathDev &P = paths[p];
-        int32_t r0, r1;
-        if (!edge_rows(E, top_row_of(ps[p].topkey), P.height, r0, r1)) continue;
-        const uint32_t b0 = (uint32_t)r0 / P.band_rows, b1 = (uint32_t)r1 / P.band_rows;
-        for (uint32_t bb = b0; bb <= b1; ++bb) {
-            const uint32_t gb = P.band_base + bb;
-            band_list[band_off[gb] + atomicAdd(&band_cur[gb], 1u)] = i;
-        }
-    }
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t bad = 0;
+    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
+        bin_event<DevAtom>(e, L, ev_meta, ev_kept, ev_idx, run_path, paths, edges, band_off, band_cur, band_list, ps,
+                           bad);
+    if (bad) raise_err(st, FT_EDEVICE);
 }
 
 // ---------------------------------------------------------------------------
 // Stage 6: raster.  One block per row band.
 // ---------------------------------------------------------------------------
-
-// Signed-area cell deltas of one edge on one scan row: the closed form of
-// Edge::{starting_cov, continuing_cov, calculate_x_limits_*, scan_area,
-// first_cov, step_cov} (fig.rs:238-321).  Cells left of x = 0 land in cell 0;
-// cells at x >= W are dropped (fig.rs:292-295).
-__device__ inline void edge_row_cells(const EdgeRec &E, int32_t sr, int32_t ed, int32_t W, int32_t *cells) {
-    const int32_t r0 = fx_int(E.y_upper), r1 = fx_int(E.y_lower);
-    const bool starting = sr == r0, ending = sr == r1;
-    int32_t cov = ending ? pixel_cov(fx_fract(E.y_lower)) : 256;
-    if (starting) cov -= pixel_cov(fx_fract(E.y_upper));
-    if (cov <= 0) return;
-    const int32_t x_bot = (int32_t)((uint32_t)E.x_bot0 + (uint32_t)(sr - r0) * (uint32_t)E.inv);
-    const int32_t x0 = starting ? wsub(x_bot, fx_mul(E.inv, wsub(FX_ONE, fx_fract(E.y_upper)))) : wsub(x_bot, E.inv);
-    const int32_t x1 = ending ? wsub(x_bot, fx_mul(E.inv, wsub(fx_ceil(E.y_lower), E.y_lower))) : x_bot;
-    const int32_t min_x = x0 < x1 ? x0 : x1, max_x = x0 < x1 ? x1 : x0;
-    const int32_t min_pix = fx_int(min_x), max_pix = fx_int(max_x);
-    const int32_t full = cov << 8;  // Fixed::from(cov as f32 / 256.0), exact
-    const int32_t r = (min_pix == max_pix) ? fx_mul(wsub(FX_ONE, fx_fract(fx_avg(max_x, min_x))), full)
-                                           : fx_mul(wsub(FX_ONE, fx_fract(min_x)), FX_HALF);
-    const int64_t xc0 = E.step > 0 ? fx_mul(r, E.step) : r;
-    const int64_t st = E.step > 0 ? E.step : FX_ONE;
-    // x_cov after k steps: min(xc0 + k * st, ONE); x_pix_k = min(pixel_cov, cov)
-    auto xp_at = [&](int64_t k) -> int32_t {
-        int64_t xc = xc0 + k * st;
-        if (xc > FX_ONE) xc = FX_ONE;
-        int32_t xp = pixel_cov((int32_t)xc);
-        return xp < cov ? xp : cov;
-    };
-    int32_t prev = 0;
-    int64_t k = 0;
-    int32_t x = min_pix;
-    if (min_pix < 0) {
-        // every step at x <= 0 adds to cell 0: their sum is x_pix at x = 0
-        const int64_t k0 = -(int64_t)min_pix;
-        const int32_t xp = xp_at(k0 < 65537 ? k0 : 65537);
-        if (xp) atomicAdd(&cells[0], xp * ed);
-        if (xp >= cov) return;
-        prev = xp;
-        k = k0 + 1;
-        x = 1;
-    }
-    for (; x < W; ++x, ++k) {
-        const int32_t xp = xp_at(k);
-        const int32_t d = xp - prev;
-        if (d) atomicAdd(&cells[x], d * ed);
-        if (xp >= cov) break;
-        prev = xp;
-    }
-}
-
 __device__ inline uint32_t wave_incl_u32(uint32_t v) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -864,40 +399,22 @@ __device__ inline uint32_t wave_incl_u32(uint32_t v) {
     return v;
 }
 
-// SrcOver with coverage alpha a for one premultiplied pixel of N channels
-// (pix Pixel::composite_channels_alpha; alpha is the last channel).
-template <int N>
-__device__ inline void composite_px(uint8_t *d, const uint32_t *sw, uint32_t a) {
-    const uint32_t aw = ch8_widen(a);
-    const uint32_t sa = ch8_mul_w(sw[N - 1], aw);
-    const uint32_t invw = ch8_widen(255u - sa);
-#pragma unroll
-    for (int c = 0; c < N; ++c) d[c] = (uint8_t)sat_add_u8(ch8_mul_w(sw[c], aw), ch8_mul_w(ch8_widen(d[c]), invw));
-}
-
-template <int N>
-__device__ inline void composite_px_fresh(uint8_t *d, const uint32_t *sw, uint32_t a) {
-    const uint32_t aw = ch8_widen(a);
-#pragma unroll
-    for (int c = 0; c < N; ++c) d[c] = (uint8_t)ch8_mul_w(sw[c], aw);  // over zero: d*x = 0
-}
-
-// Prefix-sum one raster row of cells (one wavefront), map coverage with the
+// Prefix-sum cells [x_begin, W) of one raster row (one wavefront) starting
+// from `carry` (the sum of the cells left of x_begin), map coverage with the
 // fill rule and store / composite CPT pixels per lane.
 template <int FMT, int CPT>
-__device__ void row_pass(const int32_t *cells, uint32_t W, uint8_t *row, uint32_t rule, uint32_t clr,
-                         bool fresh) {
+__device__ void row_pass(const int32_t *cells, uint32_t x_begin, uint32_t W, uint8_t *row, uint32_t rule,
+                         uint32_t clr, bool fresh, uint32_t carry) {
     constexpr int BPP = FMT == FMT_RGBA8P ? 4 : (FMT == FMT_GRAYA8P ? 2 : 1);
     const int lane = threadIdx.x & 63;
-    uint32_t sw[4];
+    uint32_t sw[4] = {0, 0, 0, 0};
     if (FMT == FMT_RGBA8P) {
         for (int c = 0; c < 4; ++c) sw[c] = ch8_widen((clr >> (8 * c)) & 0xFFu);
     } else if (FMT == FMT_GRAYA8P) {
         sw[0] = ch8_widen(clr & 0xFFu);
         sw[1] = ch8_widen((clr >> 8) & 0xFFu);
     }
-    uint32_t carry = 0;
-    for (uint32_t c0 = 0; c0 < W; c0 += 64u * CPT) {
+    for (uint32_t c0 = x_begin; c0 < W; c0 += 64u * CPT) {
         const uint32_t x = c0 + (uint32_t)lane * CPT;
         uint32_t v[CPT];
         uint32_t run = 0;
@@ -913,10 +430,7 @@ __device__ void row_pass(const int32_t *cells, uint32_t W, uint8_t *row, uint32_
         if (x >= W) continue;
         uint8_t a[CPT];
 #pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-            const int32_t s = (int32_t)(base + v[j]);
-            a[j] = (uint8_t)(rule ? coverage_even_odd(s) : coverage_non_zero(s));
-        }
+        for (int j = 0; j < CPT; ++j) a[j] = (uint8_t)map_coverage((int32_t)(base + v[j]), rule);
         uint8_t *dst = row + (size_t)x * BPP;
         const bool full = x + CPT <= W && (((uintptr_t)dst) & (CPT * BPP >= 16 ? 15 : CPT * BPP - 1)) == 0;
         if (FMT == FMT_MATTE8) {
@@ -963,23 +477,29 @@ __device__ void row_pass(const int32_t *cells, uint32_t W, uint8_t *row, uint32_
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_raster(const PathDev *paths, const uint32_t *band_path,
+__global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *paths, const uint32_t *band_path,
                                                   const uint32_t *band_off, const uint32_t *band_cnt,
-                                                  const uint32_t *band_list, const EdgeRec *edges,
-                                                  const int2 *pts, const uint32_t *pt_run, const uint32_t *run_start,
+                                                  const uint32_t *band_list, const EdgeRec *edges, const I2 *pts,
+                                                  const uint32_t *pt_run, const uint32_t *run_start,
                                                   const uint32_t *run_end, const PathState *ps, DevStatus *st) {
     extern __shared__ int32_t s_cells[];
     __shared__ int s_fwd;
+    __shared__ uint32_t s_segsum[BLOCK / 64];
     if (run_failed(st)) return;
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     const uint32_t gb = blockIdx.x;
     const uint32_t p = band_path[gb];
+    if (p >= L.n_paths) {
+        if (threadIdx.x == 0) raise_err(st, FT_EDEVICE);
+        return;
+    }
     const PathDev &P = paths[p];
     const uint32_t R = P.band_rows, H = P.height, W = P.width;
     const uint32_t y0 = (gb - P.band_base) * R;
     const uint32_t y1 = y0 + R < H ? y0 + R : H;
-    const unsigned long long topkey = ps[p].topkey;
-    const bool has_pts = topkey != NO_KEY && ps[p].topidx != NO_IDX;
-    const int32_t top_row = has_pts ? top_row_of(topkey) : 0;
+    const PathState S = ps[p];
+    const bool has_pts = S.topkey != NO_KEY && S.topidx != NO_IDX;
+    const int32_t top_row = has_pts ? top_row_of(S.topkey) : 0;
     const int64_t first = !has_pts ? (int64_t)H : (top_row > 0 ? top_row : 0);
     const uint32_t bpp = P.fmt == FMT_RGBA8P ? 4 : (P.fmt == FMT_GRAYA8P ? 2 : 1);
     uint8_t *const base = reinterpret_cast<uint8_t *>(P.dst);
@@ -992,46 +512,75 @@ __global__ __launch_bounds__(BLOCK) void k_raster(const PathDev *paths, const ui
     }
     const uint32_t ry0 = first > (int64_t)y0 ? (uint32_t)first : y0;
     if ((int64_t)ry0 >= (int64_t)y1) return;
-    const uint32_t Wp = (W + CELL_ALIGN - 1) / CELL_ALIGN * CELL_ALIGN;
+    const uint32_t Wp = cell_stride(W);
     const uint32_t rows = y1 - ry0;
     if (threadIdx.x == 0) {
-        // get_dir at the top-left vertex (fig.rs:402-411,495)
-        const uint32_t i = ps[p].pt_base + ps[p].topidx;
-        const uint32_t rn = pt_run[i], rs = run_start[rn], re = run_end[rn];
-        const uint32_t nf = (i + 1 < re) ? i + 1 : rs;
-        const uint32_t nr = (i > rs) ? i - 1 : re - 1;
-        const int2 q = pts[i], qf = pts[nf], qr = pts[nr];
-        s_fwd = widdershins(wsub(qr.x, q.x), wsub(qr.y, q.y), wsub(qf.x, q.x), wsub(qf.y, q.y)) ? 1 : 0;
+        uint32_t bad = 0;
+        s_fwd = path_forward(S, L, pts, pt_run, run_start, run_end, bad) ? 1 : 0;
+        if (bad) raise_err(st, FT_EDEVICE);
     }
     for (uint32_t c = threadIdx.x; c < rows * Wp; c += BLOCK) s_cells[c] = 0;
     __syncthreads();
     const int fwd = s_fwd;
     const int32_t shift = top_row < 0 ? top_row : 0;
     const uint32_t off = band_off[gb], cnt = band_cnt[gb];
-    for (uint32_t k = threadIdx.x; k < cnt; k += BLOCK) {
-        const EdgeRec E = edges[band_list[off + k]];
+    const bool list_ok = (uint64_t)off + cnt <= L.n_pairs;
+    if (!list_ok && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
+    for (uint32_t k = threadIdx.x; list_ok && k < cnt; k += BLOCK) {
+        const uint32_t ei = band_list[off + k];
+        if (ei >= L.cap_points) {
+            raise_err(st, FT_EDEVICE);
+            continue;
+        }
+        const EdgeRec E = edges[ei];
         const int32_t ed = ((int)((E.flags >> 1) & 1u) == fwd) ? 1 : -1;
         int64_t a = (int64_t)fx_int(E.y_upper) - shift, b = (int64_t)fx_int(E.y_lower) - shift;
         if (a < ry0) a = ry0;
         if (b > (int64_t)y1 - 1) b = (int64_t)y1 - 1;
         for (int64_t r = a; r <= b; ++r)
-            edge_row_cells(E, (int32_t)(r + shift), ed, (int32_t)W, s_cells + (size_t)(r - ry0) * Wp);
+            edge_row_cells<DevAtom>(E, (int32_t)(r + shift), ed, (int32_t)W, s_cells + (size_t)(r - ry0) * Wp);
     }
     __syncthreads();
+    // Row pass.  Bands of at least 4 rows give each wavefront whole rows; a
+    // band of 1-3 rows splits every row into segments, one per wavefront,
+    // whose cell sums are exchanged through LDS to form the carry-in.
     const int wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    constexpr uint32_t NW = BLOCK / 64;
     const bool fresh = (P.flags & PATH_FRESH) != 0;
-    for (uint32_t r = wid; r < rows; r += BLOCK / 64) {
+    const uint32_t cpt = P.fmt == FMT_MATTE8 ? (W >= 1024 ? 16u : 4u) : (P.fmt == FMT_RGBA8P ? 4u : 8u);
+    const uint32_t seg = rows >= NW ? 1u : NW / rows;
+    const uint32_t chunk = 64u * cpt;
+    const uint32_t seg_len = seg == 1 ? W : ((W + seg - 1) / seg + chunk - 1) / chunk * chunk;
+    uint32_t r0 = wid, rstep = NW, carry = 0, xb = 0, xe = W;
+    if (seg > 1) {
+        r0 = wid / seg;
+        rstep = rows;  // one row per wavefront group
+        const uint32_t sgi = wid % seg;
+        xb = sgi * seg_len;
+        xe = xb + seg_len < W ? xb + seg_len : W;
+        uint32_t sum = 0;
+        if (r0 < rows)
+            for (uint32_t x = xb + lane; x < xe; x += 64) sum += (uint32_t)s_cells[(size_t)r0 * Wp + x];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+        if (lane == 0) s_segsum[wid] = sum;
+        __syncthreads();
+        for (uint32_t k = 0; k < sgi; ++k) carry += s_segsum[r0 * seg + k];
+    }
+    for (uint32_t r = r0; r < rows; r += rstep) {
+        if (xb >= xe) break;
         const int32_t *cells = s_cells + (size_t)r * Wp;
         uint8_t *row = base + (size_t)(ry0 + r) * P.pitch;
         switch (P.fmt) {
         case FMT_MATTE8:
-            if (W >= 1024)
-                row_pass<FMT_MATTE8, 16>(cells, W, row, P.rule, P.clr, fresh);
+            if (cpt == 16)
+                row_pass<FMT_MATTE8, 16>(cells, xb, xe, row, P.rule, P.clr, fresh, carry);
             else
-                row_pass<FMT_MATTE8, 4>(cells, W, row, P.rule, P.clr, fresh);
+                row_pass<FMT_MATTE8, 4>(cells, xb, xe, row, P.rule, P.clr, fresh, carry);
             break;
-        case FMT_RGBA8P: row_pass<FMT_RGBA8P, 4>(cells, W, row, P.rule, P.clr, fresh); break;
-        default: row_pass<FMT_GRAYA8P, 8>(cells, W, row, P.rule, P.clr, fresh); break;
+        case FMT_RGBA8P: row_pass<FMT_RGBA8P, 4>(cells, xb, xe, row, P.rule, P.clr, fresh, carry); break;
+        default: row_pass<FMT_GRAYA8P, 8>(cells, xb, xe, row, P.rule, P.clr, fresh, carry); break;
         }
     }
 }
@@ -1068,17 +617,6 @@ static ft_status dev_alloc(T *&ptr, size_t n) {
     }
     if (hipMalloc(reinterpret_cast<void **>(&ptr), (n ? n : 1) * sizeof(T)) != hipSuccess) return FT_ENOMEM;
     return FT_OK;
-}
-
-void band_geometry(uint32_t width, uint32_t height, uint32_t *rows, uint32_t *n_bands, uint32_t *cells) {
-    const uint32_t wp = (width + CELL_ALIGN - 1) / CELL_ALIGN * CELL_ALIGN;
-    uint32_t r = wp ? LDS_CELLS / wp : BAND_ROWS_MAX;
-    if (r > BAND_ROWS_MAX) r = BAND_ROWS_MAX;
-    if (r < 1) r = 1;
-    if (height && r > height) r = height;
-    *rows = r;
-    *n_bands = height ? (height + r - 1) / r : 0;
-    *cells = r * wp;
 }
 
 ft_status ws_init(Workspace &ws) {
@@ -1165,27 +703,11 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
             op_path[i] = (uint32_t)k;
         }
         PathDev &P = pd[k];
-        P.op_begin = (uint32_t)d.op_begin;
-        P.op_count = d.op_count;
-        P.rule = d.rule;
-        P.fmt = d.fmt;
-        P.dst = d.dst;
-        P.width = d.width;
-        P.height = d.height;
-        P.pitch = d.pitch;
-        P.clr = (uint32_t)d.clr[0] | ((uint32_t)d.clr[1] << 8) | ((uint32_t)d.clr[2] << 16) |
-                ((uint32_t)d.clr[3] << 24);
-        memcpy(P.tr, d.transform, sizeof P.tr);
-        float tol = d.tolerance > 0.01f ? d.tolerance : 0.01f;  // set_tolerance, plotter.rs:133-137
-        P.tol_sq = tol * tol;
-        P.s_width0 = d.pen_width;
+        P = path_from_desc(d, extra_flags);
         uint32_t rows, nb, cells;
         band_geometry(d.width, d.height, &rows, &nb, &cells);
-        if (!d.width) nb = 0;
-        P.band_rows = rows;
+        nb = P.n_bands;
         P.band_base = (uint32_t)band_path.size();
-        P.n_bands = nb;
-        P.flags = (d.fresh ? PATH_FRESH : 0u) | extra_flags;
         if (nb && cells > lds) lds = cells;
         band_path.insert(band_path.end(), nb, (uint32_t)k);
     }
@@ -1280,7 +802,9 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
                                                   st);
         k_scan<unsigned long long, false><<<scan_grid(n_ops), BLOCK, 0, s>>>(
             ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops, fl, ag, in, &st->tickets[0], &st->op_total, st, false);
-        Emitter em{};
+        EventSink em{};
+        em.cap_events = ws.cap_events;
+        em.cap_runs = (uint32_t)ws.cap_events;
         em.mode = mode;
         em.ev_xy = ws.d_ev_xy;
         em.ev_f = ws.d_ev_f;
@@ -1291,30 +815,38 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
         em.run_start = ws.d_run_start;
         em.run_end = ws.d_run_end;
         k_emit<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode, lastpw, ws.d_op_off,
-                                                 ws.cap_events, em, st);
+                                                 em, st);
     }
     if (mode == MODE_FILL) {
         const uint32_t ge = grid_for(ws.cap_events);
-        k_ingest<<<ge, BLOCK, 0, s>>>(ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_last, ws.d_run_path,
+        Limits L0{};
+        L0.n_ops = n_ops;
+        L0.n_paths = (uint32_t)ws.n_paths;
+        L0.n_bands = (uint32_t)ws.n_bands;
+        L0.cap_points = (uint32_t)ws.cap_events;
+        k_ingest<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_last, ws.d_run_path,
                                       ws.d_pstate, st);
-        k_kept<<<ge, BLOCK, 0, s>>>(ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_last, ws.d_ev_kept, st);
+        k_kept<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_last, ws.d_ev_kept, st);
         k_scan<uint32_t, false><<<scan_grid(ws.cap_events), BLOCK, 0, s>>>(
             ws.d_ev_kept, ws.d_ev_idx, &st->n_events, 0, (uint32_t)ws.cap_events, fl + tiles, ag + tiles,
             in + tiles, &st->tickets[1], &st->n_kept, st, true);
-        k_compact<<<ge, BLOCK, 0, s>>>(ws.d_ev_meta, ws.d_ev_xy, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first,
-                                       ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_paths, ws.d_op_off, n_ops,
-                                       ws.d_pts, ws.d_pt_run, ws.d_pstate, st);
-        k_edges<<<ge, BLOCK, 0, s>>>(ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_path, ws.d_run_start,
+        k_cap<<<grid_for(ws.n_paths), BLOCK, 0, s>>>(L0, ws.d_paths, ws.d_op_off, ws.d_ev_meta, ws.d_ev_xy,
+                                                     ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first, ws.d_pstate, st);
+        k_compact<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first,
+                                       ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_pts, ws.d_pt_run, ws.d_pstate,
+                                       st);
+        k_edges<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_path, ws.d_run_start,
                                      ws.d_run_end, ws.d_paths, ws.d_pts, ws.d_edges, ws.d_band_cnt, ws.d_pstate, st);
         if (ws.n_bands) {
             k_scan<uint32_t, false><<<scan_grid(ws.n_bands), BLOCK, 0, s>>>(
                 ws.d_band_cnt, ws.d_band_off, nullptr, (uint32_t)ws.n_bands, (uint32_t)ws.n_bands, fl + 2 * tiles,
                 ag + 2 * tiles, in + 2 * tiles, &st->tickets[2], &st->n_pairs, st, true);
-            k_bin<<<ge, BLOCK, 0, s>>>(ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_path, ws.d_paths, ws.d_edges,
-                                       ws.d_band_off, ws.d_band_cur, ws.d_band_list, ws.cap_pairs, ws.d_pstate, st);
+            k_bin<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_path, ws.d_paths,
+                                       ws.d_edges, ws.d_band_off, ws.d_band_cur, ws.d_band_list, ws.cap_pairs,
+                                       ws.d_pstate, st);
             if (ev) FT_HIP(hipEventRecord(ev[1], s));
             k_raster<<<(uint32_t)ws.n_bands, BLOCK, ws.lds_cells * sizeof(int32_t), s>>>(
-                ws.d_paths, ws.d_band_path, ws.d_band_off, ws.d_band_cnt, ws.d_band_list, ws.d_edges, ws.d_pts,
+                L0, ws.d_paths, ws.d_band_path, ws.d_band_off, ws.d_band_cnt, ws.d_band_list, ws.d_edges, ws.d_pts,
                 ws.d_pt_run, ws.d_run_start, ws.d_run_end, ws.d_pstate, st);
             if (ev) FT_HIP(hipEventRecord(ev[2], s));
         }

@@ -1,0 +1,771 @@
+// ft_stages.h -- per-item logic of every pipeline stage, host- and
+// device-compilable.
+//
+// The gfx950 kernels (ft_pipeline.hip) are thin loops around these functions;
+// tests/cpu_mirror runs the very same functions sequentially on the host
+// under AddressSanitizer and compares the result with the CPU oracle.
+// Atomic read-modify-writes go through a policy type `A` (device atomics on
+// the GPU, plain operations in the sequential host build).
+#pragma once
+
+#include <stdint.h>
+
+#include "ft_device.h"
+#include "ft_types.h"
+
+namespace ft {
+
+// ---------------------------------------------------------------------------
+// Flattening (src/plotter.rs:163-332, src/geom.rs:9-35)
+// ---------------------------------------------------------------------------
+struct WP {
+    float x, y, w;
+};
+
+// WidePt::midpoint: position midpoint (a+b)/2 per axis, width (w0+w1)/2.
+FT_HD inline WP wmid(WP a, WP b) {
+    WP r;
+    r.x = (a.x + b.x) / 2.0f;
+    r.y = (a.y + b.y) / 2.0f;
+    r.w = (a.w + b.w) / 2.0f;
+    return r;
+}
+
+// Plotter::is_within_tolerance2: distance_sq <= tol_sq (plotter.rs:273-276)
+FT_HD inline bool within_tol(WP a, WP b, float tol_sq) {
+    float dx = a.x - b.x;
+    float dy = a.y - b.y;
+    return dx * dx + dy * dy <= tol_sq;
+}
+
+// Plotter::transform_point (plotter.rs:169-172): width is not transformed.
+FT_HD inline WP xform(const PathDev &P, float x, float y, float w) {
+    WP r;
+    if (P.flags & PATH_RAW) {
+        r.x = x;
+        r.y = y;
+    } else {
+        r.x = P.tr[0] * x + P.tr[1] * y + P.tr[2];
+        r.y = P.tr[3] * x + P.tr[4] * y + P.tr[5];
+    }
+    r.w = w;
+    return r;
+}
+
+struct Quad {
+    WP a, b, c;
+};
+struct Cubic {
+    WP a, b, c, d;
+};
+
+// One subdivision step of quad_to_tran (plotter.rs:255-258).
+FT_HD inline bool split(const Quad &q, float tol_sq, Quad &L, Quad &R) {
+    WP ab = wmid(q.a, q.b);
+    WP bc = wmid(q.b, q.c);
+    WP ab_bc = wmid(ab, bc);
+    WP ac = wmid(q.a, q.c);
+    L.a = q.a; L.b = ab; L.c = ab_bc;
+    R.a = ab_bc; R.b = bc; R.c = q.c;
+    return within_tol(ab_bc, ac, tol_sq);
+}
+
+// One subdivision step of cubic_to_tran (plotter.rs:319-326).
+FT_HD inline bool split(const Cubic &q, float tol_sq, Cubic &L, Cubic &R) {
+    WP ab = wmid(q.a, q.b);
+    WP bc = wmid(q.b, q.c);
+    WP cd = wmid(q.c, q.d);
+    WP ab_bc = wmid(ab, bc);
+    WP bc_cd = wmid(bc, cd);
+    WP pe = wmid(ab_bc, bc_cd);
+    WP ad = wmid(q.a, q.d);
+    L.a = q.a; L.b = ab; L.c = ab_bc; L.d = pe;
+    R.a = pe; R.b = bc_cd; R.c = cd; R.d = q.d;
+    return within_tol(pe, ad, tol_sq);
+}
+
+FT_HD inline WP end_of(const Quad &q) { return q.c; }
+FT_HD inline WP end_of(const Cubic &q) { return q.d; }
+
+// Depth-first walk of the subdivision tree in the recursion's order (left
+// subtree, then right).  `bits` holds the left/right choices from the root;
+// moving to a right sibling replays those choices from the root, so every
+// node is computed by exactly the float operations the recursion performs.
+// Returns 0, or FT_ERANGE when a node at FT_FLATTEN_MAX_DEPTH is still out of
+// tolerance or the curve yields more than FT_FLATTEN_MAX_LEAVES points.
+template <class Curve, class Emit>
+FT_HD uint32_t flatten_walk(const Curve &root, float tol_sq, Emit &&emit) {
+    Curve cur = root;
+    uint32_t depth = 0, bits = 0, leaves = 0;
+    for (;;) {
+        Curve L, R;
+        if (split(cur, tol_sq, L, R)) {
+            if (++leaves > FT_FLATTEN_MAX_LEAVES) return FT_ERANGE;
+            emit(end_of(cur));
+            while (depth > 0 && (bits & 1u)) {
+                bits >>= 1;
+                --depth;
+            }
+            if (depth == 0) return 0;
+            bits |= 1u;
+            cur = root;
+            for (int lv = (int)depth - 1; lv >= 0; --lv) {
+                Curve l2, r2;
+                split(cur, tol_sq, l2, r2);
+                cur = ((bits >> lv) & 1u) ? r2 : l2;
+            }
+        } else {
+            if (depth >= FT_FLATTEN_MAX_DEPTH) return FT_ERANGE;
+            cur = L;
+            bits <<= 1;
+            ++depth;
+        }
+    }
+}
+
+// Index of the op that last set the pen before op i (plotter.rs:128-130,
+// 164-166): PenWidth ops do not move the pen.  Returns < begin at path start.
+FT_HD inline int64_t pen_event_before(const ft_path_op *ops, uint32_t i, uint32_t begin) {
+    int64_t j = (int64_t)i - 1;
+    while (j >= (int64_t)begin && ops[j].tag == TAG_PEN_WIDTH) --j;
+    return j;
+}
+
+// s_width in effect just before op k: the last PenWidth op of the path before
+// k, else the width the fill started with (plotter.rs:53,151-153).
+FT_HD inline float width_before(const ft_path_op *ops, const uint32_t *lastpw, int64_t k, const PathDev &P) {
+    if (!lastpw || k < (int64_t)P.op_begin) return P.s_width0;
+    uint32_t l = lastpw[k];
+    if (l == 0 || l - 1 < P.op_begin || (int64_t)(l - 1) >= k) return P.s_width0;
+    return ops[l - 1].v[0];
+}
+
+// Pen (untransformed WidePt) before op i.
+FT_HD inline WP pen_before(const ft_path_op *ops, const uint32_t *lastpw, uint32_t i, const PathDev &P) {
+    int64_t j = pen_event_before(ops, i, P.op_begin);
+    WP pen;
+    pen.x = 0.0f;
+    pen.y = 0.0f;
+    if (j < (int64_t)P.op_begin) {
+        pen.w = P.s_width0;
+        return pen;
+    }
+    pen.w = width_before(ops, lastpw, j, P);
+    const ft_path_op &o = ops[j];
+    switch (o.tag) {
+    case TAG_MOVE:
+    case TAG_LINE: pen.x = o.v[0]; pen.y = o.v[1]; break;
+    case TAG_QUAD: pen.x = o.v[2]; pen.y = o.v[3]; break;
+    case TAG_CUBIC: pen.x = o.v[4]; pen.y = o.v[5]; break;
+    default: break;  // Close resets the pen to the origin
+    }
+    return pen;
+}
+
+FT_HD inline Quad quad_of(const ft_path_op &o, WP pen, float sw, const PathDev &P) {
+    Quad q;  // plotter.rs:233-240
+    q.a = xform(P, pen.x, pen.y, pen.w);
+    q.b = xform(P, o.v[0], o.v[1], (pen.w + sw) / 2.0f);
+    q.c = xform(P, o.v[2], o.v[3], sw);
+    return q;
+}
+
+FT_HD inline Cubic cubic_of(const ft_path_op &o, WP pen, float sw, const PathDev &P) {
+    // float_lerp(a, b, t) = b + (a - b) * t   (geom.rs:14-16, plotter.rs:294-295)
+    const float t1 = 1.0f / 3.0f, t2 = 2.0f / 3.0f;
+    float w0 = sw + (pen.w - sw) * t1;
+    float w1 = sw + (pen.w - sw) * t2;
+    Cubic q;
+    q.a = xform(P, pen.x, pen.y, pen.w);
+    q.b = xform(P, o.v[0], o.v[1], w0);
+    q.c = xform(P, o.v[2], o.v[3], w1);
+    q.d = xform(P, o.v[4], o.v[5], sw);
+    return q;
+}
+
+// ---------------------------------------------------------------------------
+// Stage 1: events per op (plotter.rs:175-224 + Plotter::fill's final close,
+// plotter.rs:347).  Close: 1 close; Move: 1 close + 1 point; Line: 1 point;
+// Quad / Cubic: one point per subdivision leaf; PenWidth: nothing.
+// ---------------------------------------------------------------------------
+struct OpCount {
+    uint32_t ne, nc, err;
+};
+
+FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, const PathDev *paths, uint32_t i,
+                              uint32_t mode, const uint32_t *lastpw) {
+    OpCount c = {0, 0, 0};
+    const uint32_t p = op_path[i];
+    if (p == NO_IDX) return c;
+    const PathDev &P = paths[p];
+    const ft_path_op o = ops[i];
+    switch (o.tag) {
+    case TAG_CLOSE: c.ne = 1; c.nc = 1; break;
+    case TAG_MOVE: c.ne = 2; c.nc = 1; break;
+    case TAG_LINE: c.ne = 1; break;
+    case TAG_QUAD:
+    case TAG_CUBIC: {
+        const WP pen = pen_before(ops, lastpw, i, P);
+        const float sw = width_before(ops, lastpw, i, P);
+        uint32_t leaves = 0;
+        auto cnt = [&](WP) { ++leaves; };
+        const uint32_t rc = o.tag == TAG_QUAD ? flatten_walk(quad_of(o, pen, sw, P), P.tol_sq, cnt)
+                                              : flatten_walk(cubic_of(o, pen, sw, P), P.tol_sq, cnt);
+        if (rc)
+            c.err = rc;
+        else
+            c.ne = leaves;
+        break;
+    }
+    case TAG_PEN_WIDTH: break;
+    default: c.err = FT_EINVAL; break;
+    }
+    if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count) {
+        ++c.ne;
+        ++c.nc;
+    }
+    return c;
+}
+
+FT_HD inline unsigned long long pack_count(OpCount c) { return ((unsigned long long)c.ne << 32) | c.nc; }
+
+// ---------------------------------------------------------------------------
+// Stage 2: emit events.  Fill mode writes Fixed points (f32 -> Fixed as
+// Fig::add_point does, fig.rs:435-436) and the run (sub-figure) id of every
+// event; stroke mode writes (x, y, width, kind) for the host stroker.
+// ---------------------------------------------------------------------------
+struct EventSink {
+    uint32_t mode;
+    uint64_t e;
+    uint32_t run, p;
+    I2 *ev_xy;
+    F4 *ev_f;
+    uint32_t *ev_meta;
+    int32_t *run_first, *run_last;
+    uint32_t *run_path, *run_start, *run_end;
+    uint64_t cap_events;  // writes at or past these limits are dropped and
+    uint32_t cap_runs;    // flagged in `bad`
+    uint32_t bad;
+
+    FT_HD void point(WP q) {
+        const bool ok = e < cap_events;
+        if (ok & (mode == MODE_FILL)) {
+            I2 v;
+            v.x = fx_from_f32(q.x);
+            v.y = fx_from_f32(q.y);
+            ev_xy[e] = v;
+            ev_meta[e] = run;
+        }
+        if (ok & (mode != MODE_FILL)) {
+            F4 v;
+            v.x = q.x;
+            v.y = q.y;
+            v.z = q.w;
+            v.w = 0.0f;
+            ev_f[e] = v;
+        }
+        bad |= ok ? 0u : 1u;
+        ++e;
+    }
+    FT_HD void close(bool joined) {
+        const bool ok = (e < cap_events) & (run < cap_runs);
+        if (ok & (mode == MODE_FILL)) {
+            ev_meta[e] = run | CLOSE_BIT;
+            run_first[run] = -1;
+            run_last[run] = -1;
+            run_path[run] = p;
+            run_start[run] = NO_IDX;
+            run_end[run] = 0;
+        }
+        if (ok & (mode != MODE_FILL)) {
+            F4 v;
+            v.x = 0.0f;
+            v.y = 0.0f;
+            v.z = joined ? 1.0f : 0.0f;
+            v.w = joined ? 2.0f : 1.0f;
+            ev_f[e] = v;
+        }
+        bad |= ok ? 0u : 1u;
+        ++run;
+        ++e;
+    }
+};
+
+// Writes op i's events at offset `off` (events << 32 | closes before op i).
+FT_HD inline uint32_t emit_op(const ft_path_op *ops, const uint32_t *op_path, const PathDev *paths, uint32_t i,
+                              uint32_t mode, const uint32_t *lastpw, unsigned long long off, EventSink em) {
+    const uint32_t p = op_path[i];
+    if (p == NO_IDX) return 0;
+    const PathDev &P = paths[p];
+    const ft_path_op o = ops[i];
+    em.e = off >> 32;
+    em.run = (uint32_t)off;
+    em.p = p;
+    switch (o.tag) {
+    case TAG_CLOSE: em.close(true); break;  // plotter.rs:200-203
+    case TAG_MOVE:                           // plotter.rs:208-214
+        em.close(false);
+        em.point(xform(P, o.v[0], o.v[1], width_before(ops, lastpw, i, P)));
+        break;
+    case TAG_LINE:  // plotter.rs:219-224
+        em.point(xform(P, o.v[0], o.v[1], width_before(ops, lastpw, i, P)));
+        break;
+    case TAG_QUAD:
+    case TAG_CUBIC: {
+        const WP pen = pen_before(ops, lastpw, i, P);
+        const float sw = width_before(ops, lastpw, i, P);
+        auto put = [&](WP q) { em.point(q); };
+        if (o.tag == TAG_QUAD)
+            flatten_walk(quad_of(o, pen, sw, P), P.tol_sq, put);
+        else
+            flatten_walk(cubic_of(o, pen, sw, P), P.tol_sq, put);
+        break;
+    }
+    default: break;
+    }
+    if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count) em.close(true);
+    return em.bad;
+}
+
+// ---------------------------------------------------------------------------
+// Stages 3-5 are written branch-free: every value is computed for every lane
+// (with in-range stand-in indices) and only the stores and atomics are
+// predicated.  No value then takes different definitions along divergent
+// paths, a shape the gfx950 compiler has been seen to miscompile (a register
+// live for lanes inactive in a nested divergent block was reused inside that
+// block).  Every index read from memory is also checked against the run's
+// limits; a failed check skips the access and sets `bad`.
+// ---------------------------------------------------------------------------
+struct Limits {
+    uint32_t n_ops, n_paths, n_bands;
+    uint32_t n_events;    // events written by this run
+    uint32_t n_runs;      // runs (= closes) of this run
+    uint32_t cap_points;  // capacity of the point / edge arrays
+    uint32_t n_pairs;     // (edge, band) pairs of this run
+};
+
+FT_HD inline bool i2_eq(I2 a, I2 b) { return (a.x == b.x) & (a.y == b.y); }
+
+// Stage 3: Fig::add_point / Fig::close (fig.rs:373-383,428-451).  Within a
+// run (events between closes) a point is pushed if it is the run's first or
+// differs in Fixed from the previous point event; closing pops the last
+// pushed point when it equals the run's first point.
+struct Pushed {
+    uint32_t run;
+    bool is_pt, first, pushed, ok;
+};
+
+FT_HD inline Pushed pushed_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy) {
+    const uint32_t ep = e > 0 ? e - 1 : 0;
+    const uint32_t m = meta[e], mp = meta[ep];
+    const I2 a = xy[e], b = xy[ep];
+    Pushed r;
+    r.run = m & ~CLOSE_BIT;
+    r.ok = r.run < L.n_runs;
+    r.is_pt = r.ok & ((m & CLOSE_BIT) == 0);
+    r.first = (e == 0) | ((mp & CLOSE_BIT) != 0);
+    r.pushed = r.first | !i2_eq(a, b);
+    return r;
+}
+
+// Records the run's first point and last pushed point.  Returns the point
+// event's path, or NO_IDX for a close event.
+template <class A>
+FT_HD inline uint32_t ingest_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy,
+                                   int32_t *run_first, int32_t *run_last, const uint32_t *run_path, uint32_t &bad) {
+    const Pushed q = pushed_event(e, L, meta, xy);
+    const uint32_t p = run_path[q.ok ? q.run : 0];
+    if (q.is_pt & q.first) run_first[q.run] = (int32_t)e;
+    if (q.is_pt & q.pushed) A::max_i32(&run_last[q.run], (int32_t)e);
+    const bool ok = q.ok & (p < L.n_paths);
+    bad |= ok ? 0u : 1u;
+    return (q.is_pt & ok) ? p : NO_IDX;
+}
+
+// 1 if point event e survives the closing pop.
+FT_HD inline uint32_t kept_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy,
+                                 const int32_t *run_first, const int32_t *run_last, uint32_t &bad) {
+    const Pushed q = pushed_event(e, L, meta, xy);
+    const uint32_t run = q.ok ? q.run : 0;
+    const int32_t rf = run_first[run], rl = run_last[run];
+    const bool rok = (rf >= 0) & (rl >= 0) & ((uint32_t)rf < L.n_events) & ((uint32_t)rl < L.n_events);
+    const I2 f = xy[rok ? (uint32_t)rf : e], l = xy[rok ? (uint32_t)rl : e];
+    const bool pop = i2_eq(f, l) & ((int32_t)e == rl);
+    bad |= ((q.is_pt & q.pushed & !rok) ? 1u : 0u) | (q.ok ? 0u : 1u);
+    return (q.is_pt & q.pushed & rok & !pop) ? 1u : 0u;
+}
+
+// Local point index of a kept point with global index idx, or NO_IDX when the
+// 65,535 cap drops it; `repop` tells whether the cap's re-pop removed 65534.
+FT_HD inline uint32_t capped_local(uint32_t idx, uint32_t base, uint32_t repop) {
+    const uint32_t local = idx - base;
+    const bool keep = (local < FT_MAX_POINTS) & !((local == FT_MAX_POINTS - 1) & (repop != 0));
+    return keep ? local : NO_IDX;
+}
+
+FT_HD inline uint32_t path_event_begin(const PathDev &P, const unsigned long long *op_off) {
+    return (uint32_t)(op_off[P.op_begin] >> 32);
+}
+
+FT_HD inline uint32_t path_event_end(const PathDev &P, const unsigned long long *op_off, uint32_t n_ops,
+                                     uint32_t n_events) {
+    const uint32_t k = P.op_begin + P.op_count;
+    return k < n_ops ? (uint32_t)(op_off[k] >> 32) : n_events;
+}
+
+FT_HD inline unsigned long long point_key(I2 q) {
+    return ((unsigned long long)((uint32_t)q.y ^ 0x80000000u) << 32) |
+           (unsigned long long)((uint32_t)q.x ^ 0x80000000u);
+}
+
+FT_HD inline int32_t top_row_of(unsigned long long key) {
+    return ((int32_t)((uint32_t)(key >> 32) ^ 0x80000000u)) >> 16;
+}
+
+// Stage 4a, one thread per path: the path's first point index and the
+// 65,535-point cap (vid.rs:17, fig.rs:429-431).  Once the stored count
+// reaches the cap every later point is ignored; the run holding the cap point
+// then closes against its truncated last point (fig.rs:373-383), which pops
+// local point 65534 when it equals the run's first point.  Every later run
+// only adds a point and pops it again.
+FT_HD inline void cap_path(uint32_t p, const Limits &L, const PathDev *paths, const unsigned long long *op_off,
+                           const uint32_t *meta, const I2 *xy, const uint32_t *kept, const uint32_t *idx_of,
+                           const int32_t *run_first, PathState *ps, uint32_t &bad) {
+    const PathDev &P = paths[p];
+    uint32_t base = 0, repop = 0;
+    if (P.op_count && P.op_begin < L.n_ops) {
+        const uint32_t eb = path_event_begin(P, op_off);
+        const uint32_t ee = path_event_end(P, op_off, L.n_ops, L.n_events);
+        if (eb < ee && ee <= L.n_events) {
+            base = idx_of[eb];
+            const uint32_t n_kept = idx_of[ee - 1] + kept[ee - 1] - base;
+            if (n_kept >= FT_MAX_POINTS) {
+                // the kept event with local index 65534: first e with idx >= base + 65534
+                uint32_t lo = eb, hi = ee;
+                const uint32_t target = base + FT_MAX_POINTS - 1;
+                while (lo < hi) {
+                    const uint32_t mid = lo + (hi - lo) / 2;
+                    if (idx_of[mid] + kept[mid] <= target)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                const uint32_t ea = lo;  // kept, idx_of[ea] == target
+                const uint32_t run = meta[ea] & ~CLOSE_BIT;
+                // the next pushed point event of the path
+                uint32_t e2 = ea + 1;
+                while (e2 < ee) {
+                    const Pushed q = pushed_event(e2, L, meta, xy);
+                    if (q.is_pt & q.pushed) break;
+                    ++e2;
+                }
+                if (e2 < ee && meta[e2] == run && run < L.n_runs) {
+                    const int32_t rf = run_first[run];
+                    if (rf >= 0 && (uint32_t)rf < L.n_events && i2_eq(xy[rf], xy[ea])) repop = 1;
+                }
+            }
+        } else if (eb > ee || ee > L.n_events) {
+            bad = 1;
+        }
+    }
+    ps[p].pt_base = base;
+    ps[p].repop = repop;
+}
+
+struct Stored {
+    uint32_t path;           // path of the event (NO_IDX: none)
+    bool stored;             // the point went into the Fig
+    unsigned long long key;  // its (y, x) key
+};
+
+// Stage 4b: compaction into the per-path point arrays (the Fig's `points`).
+template <class A>
+FT_HD inline Stored compact_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy,
+                                  const uint32_t *kept, const uint32_t *idx_of, const int32_t *run_first,
+                                  const uint32_t *run_path, uint32_t *run_start, uint32_t *run_end, I2 *pts,
+                                  uint32_t *pt_run, const PathState *ps, uint32_t &bad) {
+    const uint32_t run = meta[e] & ~CLOSE_BIT;
+    const bool rok = run < L.n_runs;
+    const uint32_t p = run_path[rok ? run : 0];
+    const bool ok = rok & (p < L.n_paths);
+    const PathState s = ps[ok ? p : 0];
+    const uint32_t idx = idx_of[e];
+    const bool keep = ok & (kept[e] != 0) & (capped_local(idx, s.pt_base, s.repop) != NO_IDX);
+    const bool iok = idx < L.cap_points;
+    const I2 q = xy[e];
+    const int32_t rf = run_first[rok ? run : 0];
+    if (keep & iok) {
+        pts[idx] = q;
+        pt_run[idx] = run;
+        A::max_u32(&run_end[run], idx + 1);
+    }
+    if (keep & iok & ((int32_t)e == rf)) run_start[run] = idx;
+    bad |= (ok ? 0u : 1u) | ((keep & !iok) ? 1u : 0u);
+    Stored out;
+    out.path = ok ? p : NO_IDX;
+    out.stored = keep & iok;
+    out.key = point_key(q);
+    return out;
+}
+
+// Scan rows of an edge mapped to raster rows (fig.rs:496-500,536-553): scan
+// row y_row = raster row + min(top_row, 0); rows run from max(top_row, 0) to
+// the raster bottom.  Returns false when the edge touches no raster row.
+FT_HD inline bool edge_rows(const EdgeRec &E, int32_t top_row, uint32_t H, int32_t &rr0, int32_t &rr1) {
+    const int64_t first = top_row > 0 ? top_row : 0;
+    const int64_t shift = top_row < 0 ? top_row : 0;
+    int64_t a = (int64_t)fx_int(E.y_upper) - shift;
+    int64_t b = (int64_t)fx_int(E.y_lower) - shift;
+    a = a < first ? first : a;
+    b = b > (int64_t)H - 1 ? (int64_t)H - 1 : b;
+    rr0 = (int32_t)a;
+    rr1 = (int32_t)b;
+    return (first < (int64_t)H) & (a <= b);
+}
+
+// Stored point of event e, if any (branch-free).
+struct PointRef {
+    uint32_t path, idx, run;
+    bool valid;
+};
+
+FT_HD inline PointRef point_of(uint32_t e, const Limits &L, const uint32_t *meta, const uint32_t *kept,
+                               const uint32_t *idx_of, const uint32_t *run_path, const PathState *ps, uint32_t &bad) {
+    const uint32_t run = meta[e] & ~CLOSE_BIT;
+    const bool rok = run < L.n_runs;
+    const uint32_t p = run_path[rok ? run : 0];
+    const bool ok = rok & (p < L.n_paths);
+    const PathState s = ps[ok ? p : 0];
+    const uint32_t idx = idx_of[e];
+    const bool iok = idx < L.cap_points;
+    PointRef r;
+    r.path = ok ? p : NO_IDX;
+    r.idx = idx;
+    r.run = run;
+    r.valid = ok & iok & (kept[e] != 0) & (capped_local(idx, s.pt_base, s.repop) != NO_IDX);
+    bad |= (ok ? 0u : 1u) | (((kept[e] != 0) & !iok) ? 1u : 0u);
+    return r;
+}
+
+// Edge::new (fig.rs:179-210) for the segment q -> q1 (q1.y != q.y).
+FT_HD inline EdgeRec make_edge(I2 q, I2 q1) {
+    const bool fwd = q1.y > q.y;  // Forward: descends in vertex order
+    const I2 up = fwd ? q : q1, lo = fwd ? q1 : q;
+    const int32_t dx = wsub(lo.x, up.x), dy = wsub(lo.y, up.y);
+    const int32_t dyd = dy != 0 ? dy : 1, dxd = dx != 0 ? dx : 1;  // no division by zero
+    int32_t step = fx_abs(fx_div(dy, dxd));  // Edge::calculate_step
+    step = step > FX_ONE ? FX_ONE : step;
+    EdgeRec E;
+    E.inv = fx_div(dx, dyd);
+    E.step = dx != 0 ? step : 0;
+    E.y_upper = up.y;
+    E.y_lower = lo.y;
+    E.x_bot0 = wadd(up.x, fx_mul(E.inv, wsub(fx_floor(wadd(up.y, FX_ONE)), up.y)));
+    E.flags = 1u | (fwd ? 2u : 0u);
+    return E;
+}
+
+// Stage 5: one edge per stored point: the segment to its successor within the
+// sub-figure (SubFig::next, fig.rs:143-163).  Edge::new runs once per edge;
+// the edge's rows are counted into row bands.
+// Returns bit 0: a stored point, bit 1: it starts a non-horizontal edge.
+template <class A>
+FT_HD inline uint32_t edge_event(uint32_t e, const Limits &L, const uint32_t *meta, const uint32_t *kept,
+                                 const uint32_t *idx_of, const uint32_t *run_path, const uint32_t *run_start,
+                                 const uint32_t *run_end, const PathDev *paths, const I2 *pts, EdgeRec *edges,
+                                 uint32_t *band_cnt, PathState *ps, uint32_t *path_out, uint32_t &bad) {
+    const PointRef r = point_of(e, L, meta, kept, idx_of, run_path, ps, bad);
+    *path_out = r.path;
+    const uint32_t p = r.valid ? r.path : 0, i = r.valid ? r.idx : 0, run = r.valid ? r.run : 0;
+    const PathState s = ps[p];
+    const uint32_t rs = run_start[run], re = run_end[run];
+    const uint32_t nx = (i + 1 < re) ? i + 1 : rs;
+    const bool nok = nx < L.cap_points;
+    const I2 q = pts[i], q1 = pts[nok ? nx : i];
+    const bool has = r.valid & nok & (nx != i) & (q1.y != q.y);
+    bad |= (r.valid & !nok) ? 1u : 0u;
+    EdgeRec E = make_edge(q, q1);
+    if (!has) E.flags = 0;
+    if (r.valid & (point_key(q) == s.topkey)) A::min_u32(&ps[p].topidx, i - s.pt_base);
+    if (r.valid) edges[i] = E;
+    const PathDev &P = paths[p];
+    int32_t r0, r1;
+    const bool rows = edge_rows(E, top_row_of(s.topkey), P.height, r0, r1);
+    if (has & rows) {
+        const uint32_t b0 = (uint32_t)r0 / P.band_rows, b1 = (uint32_t)r1 / P.band_rows;
+        for (uint32_t bb = b0; bb <= b1; ++bb) {
+            const uint32_t gb = P.band_base + bb;
+            if (gb < L.n_bands) A::add_u32(&band_cnt[gb], 1u);
+        }
+    }
+    return (r.valid ? 1u : 0u) | (has ? 2u : 0u);
+}
+
+// Stage 5b: scatter the edge into the lists of the bands it touches.
+template <class A>
+FT_HD inline void bin_event(uint32_t e, const Limits &L, const uint32_t *meta, const uint32_t *kept,
+                            const uint32_t *idx_of, const uint32_t *run_path, const PathDev *paths,
+                            const EdgeRec *edges, const uint32_t *band_off, uint32_t *band_cur, uint32_t *band_list,
+                            const PathState *ps, uint32_t &bad) {
+    const PointRef r = point_of(e, L, meta, kept, idx_of, run_path, ps, bad);
+    const uint32_t p = r.valid ? r.path : 0, i = r.valid ? r.idx : 0;
+    const EdgeRec E = edges[i];
+    const PathDev &P = paths[p];
+    int32_t r0, r1;
+    const bool rows = edge_rows(E, top_row_of(ps[p].topkey), P.height, r0, r1);
+    if (r.valid & ((E.flags & 1u) != 0) & rows) {
+        const uint32_t b0 = (uint32_t)r0 / P.band_rows, b1 = (uint32_t)r1 / P.band_rows;
+        for (uint32_t bb = b0; bb <= b1; ++bb) {
+            const uint32_t gb = P.band_base + bb;
+            if (gb >= L.n_bands) {
+                bad = 1;
+                break;
+            }
+            const uint32_t slot = band_off[gb] + A::add_u32(&band_cur[gb], 1u);
+            if (slot < L.n_pairs)
+                band_list[slot] = i;
+            else
+                bad = 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 6: raster.
+// ---------------------------------------------------------------------------
+
+// Winding direction at the top-left vertex: Fig::get_dir (fig.rs:402-411).
+FT_HD inline bool path_forward(const PathState &s, const Limits &L, const I2 *pts, const uint32_t *pt_run,
+                               const uint32_t *run_start, const uint32_t *run_end, uint32_t &bad) {
+    const uint32_t i = s.pt_base + s.topidx;
+    const bool iok = i < L.cap_points;
+    const uint32_t ii = iok ? i : 0;
+    const uint32_t rn = pt_run[ii];
+    const bool rok = rn < L.n_runs;
+    const uint32_t rs = run_start[rok ? rn : 0], re = run_end[rok ? rn : 0];
+    const uint32_t nf = (ii + 1 < re) ? ii + 1 : rs;
+    const uint32_t nr = (ii > rs) ? ii - 1 : re - 1;
+    const bool ok = iok & rok & (nf < L.cap_points) & (nr < L.cap_points);
+    bad |= ok ? 0u : 1u;
+    const I2 q = pts[ii], qf = pts[ok ? nf : ii], qr = pts[ok ? nr : ii];
+    return widdershins(wsub(qr.x, q.x), wsub(qr.y, q.y), wsub(qf.x, q.x), wsub(qf.y, q.y));
+}
+
+// Signed-area cell deltas of one edge on one scan row: the closed form of
+// Edge::{starting_cov, continuing_cov, calculate_x_limits_*, scan_area,
+// first_cov, step_cov} (fig.rs:238-321).  Cells left of x = 0 land in cell 0;
+// cells at x >= W are dropped (fig.rs:292-295).
+template <class A>
+FT_HD inline void edge_row_cells(const EdgeRec &E, int32_t sr, int32_t ed, int32_t W, int32_t *cells) {
+    const int32_t r0 = fx_int(E.y_upper), r1 = fx_int(E.y_lower);
+    const bool starting = sr == r0, ending = sr == r1;
+    int32_t cov = ending ? pixel_cov(fx_fract(E.y_lower)) : 256;
+    if (starting) cov -= pixel_cov(fx_fract(E.y_upper));
+    if (cov <= 0) return;
+    const int32_t x_bot = (int32_t)((uint32_t)E.x_bot0 + (uint32_t)(sr - r0) * (uint32_t)E.inv);
+    const int32_t x0 = starting ? wsub(x_bot, fx_mul(E.inv, wsub(FX_ONE, fx_fract(E.y_upper)))) : wsub(x_bot, E.inv);
+    const int32_t x1 = ending ? wsub(x_bot, fx_mul(E.inv, wsub(fx_ceil(E.y_lower), E.y_lower))) : x_bot;
+    const int32_t min_x = x0 < x1 ? x0 : x1, max_x = x0 < x1 ? x1 : x0;
+    const int32_t min_pix = fx_int(min_x), max_pix = fx_int(max_x);
+    const int32_t full = cov << 8;  // Fixed::from(cov as f32 / 256.0), exact
+    const int32_t r = (min_pix == max_pix) ? fx_mul(wsub(FX_ONE, fx_fract(fx_avg(max_x, min_x))), full)
+                                           : fx_mul(wsub(FX_ONE, fx_fract(min_x)), FX_HALF);
+    const int64_t xc0 = E.step > 0 ? fx_mul(r, E.step) : r;
+    const int64_t st = E.step > 0 ? E.step : FX_ONE;
+    // x_cov after k steps: min(xc0 + k * st, ONE); x_pix_k = min(pixel_cov, cov)
+    auto xp_at = [&](int64_t k) -> int32_t {
+        int64_t xc = xc0 + k * st;
+        if (xc > FX_ONE) xc = FX_ONE;
+        int32_t xp = pixel_cov((int32_t)xc);
+        return xp < cov ? xp : cov;
+    };
+    int32_t prev = 0;
+    int64_t k = 0;
+    int32_t x = min_pix;
+    if (min_pix < 0) {
+        // every step at x <= 0 adds to cell 0: their sum is x_pix at x = 0
+        const int64_t k0 = -(int64_t)min_pix;
+        const int32_t xp = xp_at(k0 < 65537 ? k0 : 65537);
+        if (xp) A::add_cell(&cells[0], xp * ed);
+        if (xp >= cov) return;
+        prev = xp;
+        k = k0 + 1;
+        x = 1;
+    }
+    for (; x < W; ++x, ++k) {
+        const int32_t xp = xp_at(k);
+        const int32_t d = xp - prev;
+        if (d) A::add_cell(&cells[x], d * ed);
+        if (xp >= cov) break;
+        prev = xp;
+    }
+}
+
+FT_HD inline uint32_t map_coverage(int32_t sum, uint32_t rule) {
+    return rule ? coverage_even_odd(sum) : coverage_non_zero(sum);
+}
+
+// SrcOver with coverage alpha a for one premultiplied pixel of N channels
+// (pix Pixel::composite_channels_alpha; alpha is the last channel).
+// sw: source channels widened by ch8_widen.
+template <int N>
+FT_HD inline void composite_px(uint8_t *d, const uint32_t *sw, uint32_t a) {
+    const uint32_t aw = ch8_widen(a);
+    const uint32_t sa = ch8_mul_w(sw[N - 1], aw);
+    const uint32_t invw = ch8_widen(255u - sa);
+    for (int c = 0; c < N; ++c) d[c] = (uint8_t)sat_add_u8(ch8_mul_w(sw[c], aw), ch8_mul_w(ch8_widen(d[c]), invw));
+}
+
+// The same over a zero destination (a fresh raster): d * x = 0.
+template <int N>
+FT_HD inline void composite_px_fresh(uint8_t *d, const uint32_t *sw, uint32_t a) {
+    const uint32_t aw = ch8_widen(a);
+    for (int c = 0; c < N; ++c) d[c] = (uint8_t)ch8_mul_w(sw[c], aw);
+}
+
+// Band geometry: rows per band for a raster, and the LDS cells it needs.
+constexpr uint32_t LDS_CELLS = 4096;  // i32 signed-area cells per raster block (16 KiB)
+constexpr uint32_t BAND_ROWS_MAX = 64;
+constexpr uint32_t CELL_ALIGN = 16;   // row stride of cells in LDS
+
+FT_HD inline uint32_t cell_stride(uint32_t width) { return (width + CELL_ALIGN - 1) / CELL_ALIGN * CELL_ALIGN; }
+
+FT_HD inline void band_geometry(uint32_t width, uint32_t height, uint32_t *rows, uint32_t *n_bands, uint32_t *cells) {
+    const uint32_t wp = cell_stride(width);
+    uint32_t r = wp ? LDS_CELLS / wp : BAND_ROWS_MAX;
+    if (r > BAND_ROWS_MAX) r = BAND_ROWS_MAX;
+    if (r < 1) r = 1;
+    if (height && r > height) r = height;
+    *rows = r;
+    *n_bands = height ? (height + r - 1) / r : 0;
+    *cells = r * wp;
+}
+
+// Device form of one batch path (ft_fill_desc -> PathDev); band_base is set
+// by the caller.
+inline PathDev path_from_desc(const ft_fill_desc &d, uint32_t extra_flags) {
+    PathDev P;
+    P.op_begin = (uint32_t)d.op_begin;
+    P.op_count = d.op_count;
+    P.rule = d.rule;
+    P.fmt = d.fmt;
+    P.dst = d.dst;
+    P.width = d.width;
+    P.height = d.height;
+    P.pitch = d.pitch;
+    P.clr = (uint32_t)d.clr[0] | ((uint32_t)d.clr[1] << 8) | ((uint32_t)d.clr[2] << 16) | ((uint32_t)d.clr[3] << 24);
+    for (int k = 0; k < 6; ++k) P.tr[k] = d.transform[k];
+    const float tol = d.tolerance > 0.01f ? d.tolerance : 0.01f;  // set_tolerance, plotter.rs:133-137
+    P.tol_sq = tol * tol;
+    P.s_width0 = d.pen_width;
+    uint32_t rows, nb, cells;
+    band_geometry(d.width, d.height, &rows, &nb, &cells);
+    if (!d.width) nb = 0;
+    P.band_rows = rows;
+    P.band_base = 0;
+    P.n_bands = nb;
+    P.flags = (d.fresh ? PATH_FRESH : 0u) | extra_flags;
+    return P;
+}
+
+}  // namespace ft

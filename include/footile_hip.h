@@ -21,7 +21,7 @@ extern "C" {
 
 typedef enum {
     FT_OK = 0,
-    FT_EINVAL = 1,   /* null handle/pointer, zero size, bad tag, NaN tolerance */
+    FT_EINVAL = 1,   /* null handle/pointer, bad tag, bad enum value */
     FT_ENOMEM = 2,   /* host or device allocation failed */
     FT_EDEVICE = 3,  /* HIP runtime error */
     FT_ERANGE = 4,   /* curve flattening deeper than FT_FLATTEN_MAX_DEPTH or with

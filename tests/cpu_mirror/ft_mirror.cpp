@@ -1,0 +1,236 @@
+// ft_mirror.cpp -- sequential host run of the device pipeline's stage code.
+//
+// TEST INFRASTRUCTURE.  Runs the functions of footile_amd/csrc/ft_stages.h --
+// the same per-item code the gfx950 kernels execute -- one item at a time on
+// the CPU, with every intermediate buffer sized exactly to its logical length,
+// so that an AddressSanitizer build flags any index the device code would
+// take past the data it wrote.  The kernels' wavefront-level parts (scans,
+// LDS row scan) are replaced by plain sequential loops.
+#include <stdint.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../footile_amd/csrc/ft_stages.h"
+
+using namespace ft;
+
+namespace {
+
+struct HostAtom {
+    static void max_i32(int32_t *p, int32_t v) {
+        if (v > *p) *p = v;
+    }
+    static void max_u32(uint32_t *p, uint32_t v) {
+        if (v > *p) *p = v;
+    }
+    static void min_u32(uint32_t *p, uint32_t v) {
+        if (v < *p) *p = v;
+    }
+    static uint32_t add_u32(uint32_t *p, uint32_t v) {
+        uint32_t o = *p;
+        *p = o + v;
+        return o;
+    }
+    static void add_cell(int32_t *p, int32_t v) { *p = (int32_t)((uint32_t)*p + (uint32_t)v); }
+};
+
+// Every array holds at least one element: the stage code reads index 0 as an
+// in-range stand-in for lanes with nothing to do, as the device buffers allow.
+template <class T>
+std::vector<T> buf(size_t n) {
+    return std::vector<T>(n ? n : 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Run a batch (host raster pointers in desc.dst).  mode: MODE_FILL, or
+// MODE_STROKE_FLATTEN to only produce (x, y, w, kind) events into ev_out
+// (capacity ev_cap, count in *n_ev).  Per-path offered/stored counts are
+// written when the arrays are non-null.  Returns an ft_status.
+int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size_t n_paths, uint32_t mode,
+            uint32_t extra_flags, uint64_t *offered, uint64_t *stored, F4 *ev_out, size_t ev_cap, size_t *n_ev) {
+    std::vector<uint32_t> op_path(n_ops, NO_IDX);
+    std::vector<PathDev> paths(n_paths);
+    std::vector<uint32_t> band_path;
+    for (size_t k = 0; k < n_paths; ++k) {
+        const ft_fill_desc &d = descs[k];
+        if (d.op_begin > n_ops || d.op_count > n_ops - d.op_begin) return FT_EINVAL;
+        for (uint64_t i = d.op_begin; i < d.op_begin + d.op_count; ++i) {
+            if (op_path[i] != NO_IDX || ops[i].tag > FT_OP_PEN_WIDTH) return FT_EINVAL;
+            op_path[i] = (uint32_t)k;
+        }
+        paths[k] = path_from_desc(d, extra_flags);
+        paths[k].band_base = (uint32_t)band_path.size();
+        band_path.insert(band_path.end(), paths[k].n_bands, (uint32_t)k);
+    }
+    // stroke widths: last PenWidth index (+1) before each op
+    std::vector<uint32_t> lastpw;
+    if (mode != MODE_FILL) {
+        lastpw.assign(n_ops, 0);
+        uint32_t m = 0;
+        for (size_t i = 0; i < n_ops; ++i) {
+            lastpw[i] = m;
+            if (ops[i].tag == TAG_PEN_WIDTH) m = (uint32_t)i + 1;
+        }
+    }
+    // stage 1: counts + exclusive scan
+    std::vector<unsigned long long> op_off(n_ops);
+    unsigned long long tot = 0;
+    for (size_t i = 0; i < n_ops; ++i) {
+        OpCount c = count_op(ops, op_path.data(), paths.data(), (uint32_t)i, mode, mode == MODE_FILL ? nullptr : lastpw.data());
+        if (c.err) return (int)c.err;
+        op_off[i] = tot;
+        tot += pack_count(c);
+    }
+    const uint32_t n_events = (uint32_t)(tot >> 32), n_closes = (uint32_t)tot;
+    // stage 2: emit
+    auto ev_xy = buf<I2>(mode == MODE_FILL ? n_events : 0);
+    auto ev_f = buf<F4>(mode == MODE_FILL ? 0 : n_events);
+    auto ev_meta = buf<uint32_t>(mode == MODE_FILL ? n_events : 0);
+    auto run_first = buf<int32_t>(n_closes), run_last = buf<int32_t>(n_closes);
+    auto run_path = buf<uint32_t>(n_closes), run_start = buf<uint32_t>(n_closes), run_end = buf<uint32_t>(n_closes);
+    EventSink sink{};
+    sink.mode = mode;
+    sink.ev_xy = ev_xy.data();
+    sink.ev_f = ev_f.data();
+    sink.ev_meta = ev_meta.data();
+    sink.run_first = run_first.data();
+    sink.run_last = run_last.data();
+    sink.run_path = run_path.data();
+    sink.run_start = run_start.data();
+    sink.run_end = run_end.data();
+    sink.cap_events = n_events;
+    sink.cap_runs = n_closes;
+    uint32_t bad = 0;
+    for (size_t i = 0; i < n_ops; ++i)
+        bad |= emit_op(ops, op_path.data(), paths.data(), (uint32_t)i, mode, mode == MODE_FILL ? nullptr : lastpw.data(),
+                       op_off[i], sink);
+    if (bad) return FT_EDEVICE;
+    if (mode != MODE_FILL) {
+        if (n_ev) *n_ev = n_events;
+        if (ev_out) memcpy(ev_out, ev_f.data(), (n_events < ev_cap ? n_events : ev_cap) * sizeof(F4));
+        return FT_OK;
+    }
+    Limits L{};
+    L.n_ops = (uint32_t)n_ops;
+    L.n_paths = (uint32_t)n_paths;
+    L.n_bands = (uint32_t)band_path.size();
+    L.n_events = n_events;
+    L.n_runs = n_closes;
+    // stage 3: ingest + kept + exclusive scan
+    auto ps = buf<PathState>(n_paths);
+    for (PathState &s : ps) {
+        s.topkey = NO_KEY;
+        s.topidx = NO_IDX;
+        s.offered = s.repop = s.pt_base = s.n_edges = s.stored = 0;
+    }
+    for (uint32_t e = 0; e < n_events; ++e) {
+        uint32_t p = ingest_event<HostAtom>(e, L, ev_meta.data(), ev_xy.data(), run_first.data(), run_last.data(),
+                                            run_path.data(), bad);
+        if (p != NO_IDX) ps[p].offered++;
+    }
+    auto kept = buf<uint32_t>(n_events), idx = buf<uint32_t>(n_events);
+    uint32_t n_kept = 0;
+    for (uint32_t e = 0; e < n_events; ++e) {
+        kept[e] = kept_event(e, L, ev_meta.data(), ev_xy.data(), run_first.data(), run_last.data(), bad);
+        idx[e] = n_kept;
+        n_kept += kept[e];
+    }
+    L.cap_points = n_kept;
+    // stage 4: cap, compaction
+    for (uint32_t p = 0; p < n_paths; ++p)
+        cap_path(p, L, paths.data(), op_off.data(), ev_meta.data(), ev_xy.data(), kept.data(), idx.data(),
+                 run_first.data(), ps.data(), bad);
+    auto pts = buf<I2>(n_kept);
+    auto pt_run = buf<uint32_t>(n_kept);
+    for (uint32_t e = 0; e < n_events; ++e) {
+        Stored s = compact_event<HostAtom>(e, L, ev_meta.data(), ev_xy.data(), kept.data(), idx.data(),
+                                           run_first.data(), run_path.data(), run_start.data(), run_end.data(),
+                                           pts.data(), pt_run.data(), ps.data(), bad);
+        if (s.stored && s.key < ps[s.path].topkey) ps[s.path].topkey = s.key;
+    }
+    // stage 5: edges, band counts, scan, bin
+    const size_t n_bands = band_path.size();
+    auto edges = buf<EdgeRec>(n_kept);
+    auto band_cnt = buf<uint32_t>(n_bands), band_off = buf<uint32_t>(n_bands), band_cur = buf<uint32_t>(n_bands);
+    for (uint32_t e = 0; e < n_events; ++e) {
+        uint32_t p = NO_IDX;
+        uint32_t r = edge_event<HostAtom>(e, L, ev_meta.data(), kept.data(), idx.data(), run_path.data(),
+                                          run_start.data(), run_end.data(), paths.data(), pts.data(), edges.data(),
+                                          band_cnt.data(), ps.data(), &p, bad);
+        if (r & 1u) ps[p].stored++;
+        if (r & 2u) ps[p].n_edges++;
+    }
+    uint32_t n_pairs = 0;
+    for (size_t b = 0; b < n_bands; ++b) {
+        band_off[b] = n_pairs;
+        n_pairs += band_cnt[b];
+    }
+    L.n_pairs = n_pairs;
+    auto band_list = buf<uint32_t>(n_pairs);
+    for (uint32_t e = 0; e < n_events; ++e)
+        bin_event<HostAtom>(e, L, ev_meta.data(), kept.data(), idx.data(), run_path.data(), paths.data(), edges.data(),
+                            band_off.data(), band_cur.data(), band_list.data(), ps.data(), bad);
+    if (bad) return FT_EDEVICE;
+    // stage 6: raster, one band at a time (k_raster with a sequential row scan)
+    for (size_t gb = 0; gb < n_bands; ++gb) {
+        const uint32_t p = band_path[gb];
+        const PathDev &P = paths[p];
+        const uint32_t R = P.band_rows, H = P.height, W = P.width;
+        const uint32_t y0 = (uint32_t)(gb - P.band_base) * R;
+        const uint32_t y1 = y0 + R < H ? y0 + R : H;
+        const PathState &S = ps[p];
+        const bool has_pts = S.topkey != NO_KEY && S.topidx != NO_IDX;
+        const int32_t top_row = has_pts ? top_row_of(S.topkey) : 0;
+        const int64_t first = !has_pts ? (int64_t)H : (top_row > 0 ? top_row : 0);
+        const uint32_t bpp = P.fmt == FMT_RGBA8P ? 4 : (P.fmt == FMT_GRAYA8P ? 2 : 1);
+        uint8_t *const base = reinterpret_cast<uint8_t *>(P.dst);
+        if (P.flags & PATH_FRESH) {
+            const uint32_t ze = first < (int64_t)y1 ? (uint32_t)first : y1;
+            for (uint32_t r = y0; r < ze; ++r) memset(base + (size_t)r * P.pitch, 0, (size_t)W * bpp);
+        }
+        const uint32_t ry0 = first > (int64_t)y0 ? (uint32_t)first : y0;
+        if ((int64_t)ry0 >= (int64_t)y1) continue;
+        const uint32_t Wp = cell_stride(W);
+        const uint32_t rows = y1 - ry0;
+        std::vector<int32_t> cells((size_t)rows * Wp, 0);
+        const int fwd = path_forward(S, L, pts.data(), pt_run.data(), run_start.data(), run_end.data(), bad) ? 1 : 0;
+        const int32_t shift = top_row < 0 ? top_row : 0;
+        for (uint32_t k = 0; k < band_cnt[gb]; ++k) {
+            const EdgeRec E = edges[band_list[band_off[gb] + k]];
+            const int32_t ed = ((int)((E.flags >> 1) & 1u) == fwd) ? 1 : -1;
+            int64_t a = (int64_t)fx_int(E.y_upper) - shift, b = (int64_t)fx_int(E.y_lower) - shift;
+            if (a < ry0) a = ry0;
+            if (b > (int64_t)y1 - 1) b = (int64_t)y1 - 1;
+            for (int64_t r = a; r <= b; ++r)
+                edge_row_cells<HostAtom>(E, (int32_t)(r + shift), ed, (int32_t)W, cells.data() + (size_t)(r - ry0) * Wp);
+        }
+        uint32_t sw[4];
+        for (int c = 0; c < 4; ++c) sw[c] = ch8_widen((P.clr >> (8 * c)) & 0xFFu);
+        const bool fresh = (P.flags & PATH_FRESH) != 0;
+        for (uint32_t r = 0; r < rows; ++r) {
+            uint8_t *row = base + (size_t)(ry0 + r) * P.pitch;
+            uint32_t sum = 0;
+            for (uint32_t x = 0; x < W; ++x) {
+                sum += (uint32_t)cells[(size_t)r * Wp + x];
+                const uint32_t a = map_coverage((int32_t)sum, P.rule);
+                if (P.fmt == FMT_MATTE8)
+                    row[x] = (uint8_t)a;
+                else if (P.fmt == FMT_RGBA8P)
+                    fresh ? composite_px_fresh<4>(row + 4 * x, sw, a) : composite_px<4>(row + 4 * x, sw, a);
+                else
+                    fresh ? composite_px_fresh<2>(row + 2 * x, sw, a) : composite_px<2>(row + 2 * x, sw, a);
+            }
+        }
+    }
+    for (size_t k = 0; k < n_paths; ++k) {
+        if (offered) offered[k] = ps[k].offered;
+        if (stored) stored[k] = ps[k].stored;
+    }
+    return bad ? FT_EDEVICE : FT_OK;
+}
+
+}  // extern "C"
