@@ -218,7 +218,7 @@ def main():
 
     cpu_px = cpu_steps * 2 * W * (H - max(int(tops[0]), 0))
     line = {
-        "metric": "filled Mpixels/s (bit-exact vs CPU) at 1 GPU",
+        "metric": "filled Mpixels/s + paths/s (bit-exact vs CPU) at 1 GPU; fwd+bwd grad-steps/s",
         "value": px_per_step / sec_per_step / 1e6,
         "unit": "Mpixels/s",
         "n_gpus": 1,

@@ -335,6 +335,10 @@ ft_status ft_batch_run(ft_batch *b) {
 ft_status ft_batch_counts(ft_batch *b, uint64_t *offered, uint64_t *stored, uint64_t *edges,
                           int32_t *top_rows) {
     if (!b) return FT_EINVAL;
+    if (edges) {
+        const ft_status st = ft::ws_count_edges(b->ws);
+        if (st) return st;
+    }
     for (size_t i = 0; i < b->n_paths; ++i) {
         const ft::PathState &s = b->ws.h_pstate[i];
         if (offered) offered[i] = s.offered;

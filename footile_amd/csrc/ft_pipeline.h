@@ -3,8 +3,9 @@
 // One "run" renders a batch of independent fills (a single Plotter::fill is a
 // batch of one).  All stages run on the GPU from device-resident ops:
 //   count -> scan -> emit          (op interpretation + De Casteljau flatten)
-//   ingest -> kept -> scan -> compact   (Fig::add_point/close + 65,535 cap)
-//   edges -> scan -> bin           (Edge::new per segment, row-band binning)
+//   ingest -> max-scan -> kept -> scan -> cap -> compact -> top
+//                                  (Fig::add_point/close, 65,535 cap, top point)
+//   edges -> scan -> scan -> bin   (Edge::new per segment, row-band binning)
 //   raster                         (signed-area cells in LDS, row scan,
 //                                   NonZero/EvenOdd map, Matte/Rgba store)
 #pragma once
@@ -31,15 +32,21 @@ struct Workspace {
     PathDev *d_paths = nullptr;
     uint32_t *d_band_path = nullptr;
     uint32_t lds_cells = 0;  // dynamic LDS cells needed by the raster kernel
+    // raster blocks: (global band, row group) per block
+    size_t n_rblocks = 0, cap_rblocks = 0;
+    uint2 *d_rblock = nullptr;
     // per-op scratch
     unsigned long long *d_op_cnt = nullptr, *d_op_off = nullptr;
+    unsigned long long *d_sub_off = nullptr;  // per (op, lane): offset within the op
     uint32_t *d_op_pw = nullptr, *d_op_lastpw = nullptr;
     // events and everything sized by them
     size_t cap_events = 0;
     I2 *d_ev_xy = nullptr;
     F4 *d_ev_f = nullptr;  // stroke-mode events
     uint32_t *d_ev_meta = nullptr, *d_ev_kept = nullptr, *d_ev_idx = nullptr;
-    int32_t *d_run_first = nullptr, *d_run_last = nullptr;
+    uint32_t *d_ev_last = nullptr;  // pushed-point marks, then their running maximum
+    int32_t *d_run_first = nullptr;
+    uint32_t *d_run_close = nullptr;
     uint32_t *d_run_path = nullptr, *d_run_start = nullptr, *d_run_end = nullptr;
     I2 *d_pts = nullptr;
     uint32_t *d_pt_run = nullptr;
@@ -48,7 +55,7 @@ struct Workspace {
     uint32_t *d_band_cnt = nullptr, *d_band_off = nullptr, *d_band_cur = nullptr;
     size_t cap_pairs = 0;
     uint32_t *d_band_list = nullptr;
-    // scan look-back state (3 scans + stroke width scan)
+    // scan look-back state, SCAN_SLOTS instances
     size_t cap_tiles = 0;
     uint32_t *d_scan_flags = nullptr;
     unsigned long long *d_scan_agg = nullptr, *d_scan_incl = nullptr;
@@ -57,6 +64,7 @@ struct Workspace {
     DevStatus *d_status = nullptr;
     DevStatus *h_status = nullptr;  // pinned
     std::vector<PathState> h_pstate;
+    bool edges_counted = false;  // PathState::n_edges holds the last run's counts
     // optional event timing of whole runs and of the raster kernel alone:
     // launch k records into ring slot k (up to TIMING_RING launches between
     // two finishes); finish adds up every recorded slot.
@@ -84,5 +92,8 @@ ft_status ws_launch(Workspace &ws, uint32_t mode);
 ft_status ws_finish(Workspace &ws, bool read_pstate);
 // launch + finish, growing capacities and relaunching on overflow.
 ft_status ws_run(Workspace &ws, uint32_t mode, bool read_pstate);
+// Count the last fill run's non-horizontal edges per path into
+// PathState::n_edges (and h_pstate).
+ft_status ws_count_edges(Workspace &ws);
 
 }  // namespace ft

@@ -6,9 +6,10 @@
 // a per-row cumulative sum of i16 signed-area cells (src/imgbuf.rs,
 // src/fig.rs:619-682).  Here every stage is data-parallel:
 //
-//  * each curve is flattened by one lane with an iterative, stack-free
-//    subdivision walk that re-derives a node's control points from the root,
-//    so the emitted points (and their order) are those of the recursion;
+//  * each curve is flattened by OP_LANES lanes, one per subtree a few levels
+//    below the root, with an iterative, stack-free subdivision walk that
+//    re-derives a node's control points from the root, so the emitted points
+//    (and their order) are those of the recursion;
 //  * Fig::add_point / Fig::close sequencing (coincident-point drop, closing
 //    pop, the 65,535-point cap) is restated as flags + prefix sums;
 //  * the active-edge scan becomes independent (edge, row) items: an edge's
@@ -29,7 +30,10 @@ constexpr int BLOCK = 256;
 constexpr uint32_t GRID_MAX = 2048;     // grid-stride kernels: 8 blocks per CU
 constexpr uint32_t SCAN_ITEMS = 8;
 constexpr uint32_t SCAN_TILE = BLOCK * SCAN_ITEMS;
-constexpr uint32_t SCAN_SLOTS = 4;      // look-back state instances per run
+// look-back state instances per run: 0 op counts, 1 kept points, 2 band
+// offsets, 3 band differences (fill) or PenWidth indices (stroke), 4 pushed
+// points' running maximum
+constexpr uint32_t SCAN_SLOTS = 5;
 constexpr uint32_t SPIN_LIMIT = 1u << 24;
 
 // Device atomics for the stage code.
@@ -47,44 +51,80 @@ __device__ inline bool run_failed(const DevStatus *st) {
            __hip_atomic_load(&st->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
 }
 
-// Add `v` to a per-path counter (u32 field at `base + p * stride_words`) for
-// every lane with `has`: one atomic per wavefront for the lanes sharing the
-// first active lane's path, per-lane atomics for the rest.
-__device__ inline void wave_count(uint32_t *base, size_t stride_words, uint32_t n_paths, bool has, uint32_t p,
-                                  uint32_t v) {
+// Run-length aggregation of keyed atomic adds across a wavefront.  Lanes with
+// `has` add v to base[key]; lanes holding the same key as their left neighbour
+// form a run (neighbouring events mostly share a band), each run's sum is
+// formed with a segmented scan and its last lane issues one atomic.  Returns
+// the value base[key] had before this lane's add (its slot), for lanes with
+// `has`.  All 64 lanes must be active.
+__device__ inline uint32_t wave_add_runs(uint32_t *base, uint32_t n_keys, bool has, uint32_t key, uint32_t v) {
     const int lane = threadIdx.x & 63;
-    has = has & (p < n_paths);
-    const unsigned long long hm = __ballot(has);
-    if (!hm) return;
-    const int fl = __ffsll((long long)hm) - 1;
-    const uint32_t lead = __shfl(p, fl, 64);
-    const bool same = has && p == lead;
-    uint32_t t = same ? v : 0u;
+    const uint32_t k = (has & (key < n_keys)) ? key : NO_IDX;
+    const uint32_t x = k != NO_IDX ? v : 0u;
+    const uint32_t kp = __shfl_up(k, 1, 64);
+    const bool head = lane == 0 || kp != k;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes <= this one
+    const int rs = 63 - __clzll((long long)(heads & le));                       // this lane's run start
+    uint32_t sum = x;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-    if (lane == fl && t && lead < n_paths) atomicAdd(base + (size_t)lead * stride_words, t);
-    if (has && !same && v) atomicAdd(base + (size_t)p * stride_words, v);
-}
-
-// Per-path minimum of a u64 key, reduced per wavefront like wave_count.
-__device__ inline void wave_min_key(PathState *ps, uint32_t n_paths, bool has, uint32_t p, unsigned long long key) {
-    const int lane = threadIdx.x & 63;
-    has = has & (p < n_paths);
-    const unsigned long long hm = __ballot(has);
-    if (!hm) return;
-    const int fl = __ffsll((long long)hm) - 1;
-    const uint32_t lead = __shfl(p, fl, 64);
-    unsigned long long k = (has && p == lead) ? key : NO_KEY;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long t = __shfl_xor(k, o, 64);
-        k = t < k ? t : k;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(sum, o, 64);
+        if (lane - o >= rs) sum += u;
     }
-    if (lane == fl && lead < n_paths) atomicMin(&ps[lead].topkey, k);
-    if (has && p != lead) atomicMin(&ps[p].topkey, key);
+    const unsigned long long after = heads & ~le;  // heads right of this lane
+    const int tail = after ? __ffsll((long long)after) - 2 : 63;
+    uint32_t old = 0;
+    if (lane == tail && k != NO_IDX && sum) old = atomicAdd(base + k, sum);
+    const uint32_t run_old = __shfl(old, tail, 64);
+    return run_old + (sum - x);
 }
 
-constexpr size_t PS_WORDS = sizeof(PathState) / 4;
+// Block-wide keyed reduction with one atomic (flush) per distinct key per
+// block for the keys of the first two leaders of each wavefront; any other
+// lane flushes alone.  Keys are path indices of consecutive items, so a block
+// holds few.  Every thread of the block must call it.
+template <class T, class Comb, class Flush>
+__device__ inline void block_reduce_keyed(bool has, uint32_t key, T v, T identity, Comb comb, Flush flush) {
+    constexpr int NW = BLOCK / 64;
+    __shared__ uint32_t s_key[2 * NW];
+    __shared__ T s_val[2 * NW];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const unsigned long long hm = __ballot(has);
+        const int fl = hm ? __ffsll((long long)hm) - 1 : 0;
+        const uint32_t lead = __shfl(key, fl, 64);
+        const bool same = has && key == lead;
+        T m = same ? v : identity;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = comb(m, (T)__shfl_xor(m, o, 64));
+        if (lane == 0) {
+            s_key[2 * wid + it] = hm ? lead : NO_IDX;
+            s_val[2 * wid + it] = m;
+        }
+        has = has && !same;
+    }
+    if (has) flush(key, v);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ck = NO_IDX;
+        T cv = identity;
+        for (int k = 0; k < 2 * NW; ++k) {
+            const uint32_t kk = s_key[k];
+            if (kk == NO_IDX) continue;
+            if (kk == ck) {
+                cv = comb(cv, s_val[k]);
+            } else {
+                if (ck != NO_IDX) flush(ck, cv);
+                ck = kk;
+                cv = s_val[k];
+            }
+        }
+        if (ck != NO_IDX) flush(ck, cv);
+    }
+    __syncthreads();
+}
 
 // ---------------------------------------------------------------------------
 // Stage 0: clear per-run state (a kernel, not hipMemsetAsync: graph-safe).
@@ -143,7 +183,7 @@ template <typename T, bool IS_MAX>
 __global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint32_t *n_dev, uint32_t n_host,
                                                 uint32_t n_cap, uint32_t *flags, unsigned long long *agg,
                                                 unsigned long long *incl, uint32_t *ticket, T *total,
-                                                DevStatus *st, bool check_failed) {
+                                                DevStatus *st, bool check_failed, bool inclusive) {
     __shared__ uint32_t s_tile;
     __shared__ unsigned long long s_wave[BLOCK / 64];
     __shared__ unsigned long long s_prefix;
@@ -163,8 +203,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint3
 #pragma unroll
         for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
             unsigned long long x = (mine + j < n) ? (unsigned long long)in[mine + j] : 0ull;
-            v[j] = run;  // exclusive within the thread
+            const unsigned long long before = run;
             run = scan_op<IS_MAX>(run, x);
+            v[j] = inclusive ? run : before;  // within the thread
         }
         unsigned long long winc = wave_incl<IS_MAX>(run);
         unsigned long long wexc = __shfl_up(winc, 1, 64);
@@ -177,42 +218,62 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint3
             if (w < wid) wpre = scan_op<IS_MAX>(wpre, s_wave[w]);
             btot = scan_op<IS_MAX>(btot, s_wave[w]);
         }
-        if (threadIdx.x == 0) {
-            unsigned long long prefix = 0;
+        // Look-back by the first wavefront: lane l reads the state of tile
+        // (top - l), so 64 predecessors are examined per round trip.  A round
+        // ends at the nearest published inclusive value; every tile before it
+        // must have published at least its aggregate.
+        if (wid == 0) {
+            unsigned long long prefix = 0;  // identity of + and of max over unsigned values
             if (tile == 0) {
-                __hip_atomic_store(&incl[0], btot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&flags[0], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) {
+                    __hip_atomic_store(&incl[0], btot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&flags[0], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
             } else {
-                __hip_atomic_store(&agg[tile], btot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&flags[tile], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                int64_t j = (int64_t)tile - 1;
+                if (lane == 0) {
+                    __hip_atomic_store(&agg[tile], btot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&flags[tile], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                int64_t top = (int64_t)tile - 1;
                 uint32_t spins = 0;
-                while (j >= 0) {
-                    uint32_t f = __hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                    if (f == 0u) {
+                for (;;) {
+                    const int64_t j = top - lane;
+                    const uint32_t f =
+                        j >= 0 ? __hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 2u;
+                    const unsigned long long im = __ballot(f == 2u), zm = __ballot(f == 0u);
+                    const int fi = im ? __ffsll((long long)im) - 1 : 64;  // nearest inclusive
+                    const unsigned long long need = fi >= 63 ? ~0ull : ((2ull << fi) - 1ull);
+                    if (zm & need) {
                         if (++spins > SPIN_LIMIT) {  // never expected: report, do not hang
-                            atomicOr(&st->spin_fail, 1u);
-                            raise_err(st, FT_EDEVICE);
+                            if (lane == 0) {
+                                atomicOr(&st->spin_fail, 1u);
+                                raise_err(st, FT_EDEVICE);
+                            }
                             break;
                         }
                         __builtin_amdgcn_s_sleep(1);
                         continue;
                     }
-                    if (f == 2u) {
-                        prefix = scan_op<IS_MAX>(
-                            __hip_atomic_load(&incl[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), prefix);
-                        break;
-                    }
-                    prefix = scan_op<IS_MAX>(__hip_atomic_load(&agg[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                                             prefix);
-                    --j;
+                    unsigned long long v = 0;
+                    if (lane <= fi && j >= 0)
+                        v = f == 2u ? __hip_atomic_load(&incl[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                    : __hip_atomic_load(&agg[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) v = scan_op<IS_MAX>(v, __shfl_xor(v, o, 64));
+                    prefix = scan_op<IS_MAX>(prefix, v);
+                    if (fi < 64) break;
+                    top -= 64;
                 }
-                __hip_atomic_store(&incl[tile], scan_op<IS_MAX>(prefix, btot), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&flags[tile], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) {
+                    __hip_atomic_store(&incl[tile], scan_op<IS_MAX>(prefix, btot), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&flags[tile], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
-            s_prefix = prefix;
-            if (base + SCAN_TILE >= n && total) *total = (T)scan_op<IS_MAX>(prefix, btot);
+            if (lane == 0) {
+                s_prefix = prefix;
+                if (base + SCAN_TILE >= n && total) *total = (T)scan_op<IS_MAX>(prefix, btot);
+            }
         }
         __syncthreads();
         const unsigned long long pre = scan_op<IS_MAX>(s_prefix, scan_op<IS_MAX>(wpre, wexc));
@@ -231,21 +292,37 @@ __global__ __launch_bounds__(BLOCK) void k_pen_width_mark(const ft_path_op *ops,
         op_pw[i] = ops[i].tag == TAG_PEN_WIDTH ? i + 1 : 0u;
 }
 
+// One thread per (op, lane); an op's OP_LANES lanes are adjacent lanes of one
+// wavefront.  Writes the op's event/close count and each lane's offset within
+// the op (a prefix over the op's lanes).
 __global__ __launch_bounds__(BLOCK) void k_count(const ft_path_op *ops, const uint32_t *op_path,
                                                  const PathDev *paths, uint32_t n_ops, uint32_t mode,
                                                  const uint32_t *lastpw, unsigned long long *op_cnt,
-                                                 DevStatus *st) {
-    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n_ops; i += gridDim.x * BLOCK) {
-        const OpCount c = count_op(ops, op_path, paths, i, mode, lastpw);
+                                                 unsigned long long *sub_off, DevStatus *st) {
+    const uint64_t n_items = (uint64_t)n_ops * OP_LANES;
+    for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_items; t += (uint64_t)gridDim.x * BLOCK) {
+        const uint32_t i = (uint32_t)(t >> FLAT_SPLIT), s = (uint32_t)t & (OP_LANES - 1);
+        const OpCount c = count_op(ops, op_path, paths, i, s, mode, lastpw);
+        const unsigned long long v = pack_count(c);
+        unsigned long long inc = v;
+        uint32_t leaves = c.leaves;
+#pragma unroll
+        for (uint32_t o = 1; o < OP_LANES; o <<= 1) {
+            const unsigned long long u = __shfl_up(inc, o, OP_LANES);
+            if (s >= o) inc += u;
+            leaves += __shfl_xor(leaves, o, OP_LANES);
+        }
+        sub_off[t] = inc - v;
+        if (s == OP_LANES - 1) op_cnt[i] = inc;
         if (c.err) raise_err(st, c.err);
-        op_cnt[i] = pack_count(c);
+        if (s == 0 && leaves > FT_FLATTEN_MAX_LEAVES) raise_err(st, FT_ERANGE);
     }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uint32_t *op_path,
                                                 const PathDev *paths, uint32_t n_ops, uint32_t mode,
                                                 const uint32_t *lastpw, const unsigned long long *op_off,
-                                                EventSink sink, DevStatus *st) {
+                                                const unsigned long long *sub_off, EventSink sink, DevStatus *st) {
     if (run_failed(st)) return;
     const unsigned long long tot = st->op_total;
     const uint32_t n_events = (uint32_t)(tot >> 32);
@@ -259,8 +336,11 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uin
     }
     if (n_events > sink.cap_events) return;
     uint32_t bad = 0;
-    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n_ops; i += gridDim.x * BLOCK)
-        bad |= emit_op(ops, op_path, paths, i, mode, lastpw, op_off[i], sink);
+    const uint64_t n_items = (uint64_t)n_ops * OP_LANES;
+    for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_items; t += (uint64_t)gridDim.x * BLOCK) {
+        const uint32_t i = (uint32_t)(t >> FLAT_SPLIT), s = (uint32_t)t & (OP_LANES - 1);
+        bad |= emit_op(ops, op_path, paths, i, s, mode, lastpw, op_off[i] + sub_off[t], sink);
+    }
     if (bad) raise_err(st, FT_EDEVICE);
 }
 
@@ -279,31 +359,27 @@ __device__ inline Limits run_limits(const DevStatus *st, uint32_t n_ops, uint32_
 }
 
 // ---------------------------------------------------------------------------
-// Stages 3-4: Fig ingest (runs, closing pop, cap, compaction).
+// Stages 3-4: Fig ingest (runs, closing pop, cap, compaction, top point).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_ingest(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
-                                                  int32_t *run_first, int32_t *run_last, const uint32_t *run_path,
-                                                  PathState *ps, DevStatus *st) {
-    if (run_failed(st)) return;
-    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
-    uint32_t bad = 0;
-    for (uint32_t b = blockIdx.x * BLOCK; b < L.n_events; b += gridDim.x * BLOCK) {
-        const uint32_t e = b + threadIdx.x;
-        uint32_t p = NO_IDX;
-        if (e < L.n_events) p = ingest_event<DevAtom>(e, L, ev_meta, ev_xy, run_first, run_last, run_path, bad);
-        wave_count(&ps[0].offered, PS_WORDS, L.n_paths, p != NO_IDX, p, 1u);
-    }
-    if (bad) raise_err(st, FT_EDEVICE);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_kept(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
-                                                const int32_t *run_first, const int32_t *run_last, uint32_t *ev_kept,
-                                                DevStatus *st) {
+                                                  int32_t *run_first, const uint32_t *run_path, uint32_t *ev_last,
+                                                  DevStatus *st) {
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     uint32_t bad = 0;
     for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
-        ev_kept[e] = kept_event(e, L, ev_meta, ev_xy, run_first, run_last, bad);
+        ingest_event(e, L, ev_meta, ev_xy, run_first, run_path, ev_last, bad);
+    if (bad) raise_err(st, FT_EDEVICE);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_kept(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
+                                                const int32_t *run_first, const uint32_t *run_close,
+                                                const uint32_t *ev_last, uint32_t *ev_kept, DevStatus *st) {
+    if (run_failed(st)) return;
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t bad = 0;
+    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
+        ev_kept[e] = kept_event(e, L, ev_meta, ev_xy, run_first, run_close, ev_last, bad);
     if (bad) raise_err(st, FT_EDEVICE);
 }
 
@@ -323,20 +399,57 @@ __global__ __launch_bounds__(BLOCK) void k_compact(Limits L0, const uint32_t *ev
                                                    const uint32_t *ev_kept, const uint32_t *ev_idx,
                                                    const int32_t *run_first, const uint32_t *run_path,
                                                    uint32_t *run_start, uint32_t *run_end, I2 *pts, uint32_t *pt_run,
-                                                   PathState *ps, DevStatus *st) {
+                                                   const PathState *ps, DevStatus *st) {
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     uint32_t bad = 0;
-    for (uint32_t b = blockIdx.x * BLOCK; b < L.n_events; b += gridDim.x * BLOCK) {
-        const uint32_t e = b + threadIdx.x;
-        Stored s;
-        s.path = NO_IDX;
-        s.stored = false;
-        s.key = NO_KEY;
-        if (e < L.n_events)
-            s = compact_event<DevAtom>(e, L, ev_meta, ev_xy, ev_kept, ev_idx, run_first, run_path, run_start, run_end,
-                                       pts, pt_run, ps, bad);
-        wave_min_key(ps, L.n_paths, s.stored, s.path, s.key);
+    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
+        compact_event(e, L, ev_meta, ev_xy, ev_kept, ev_idx, run_first, run_path, run_start, run_end, pts, pt_run,
+                      ps, bad);
+    if (bad) raise_err(st, FT_EDEVICE);
+}
+
+// Per-path minimum (y, x) key over the stored points [0, n_kept): each thread
+// takes TOP_ITEMS points of a tile (loads first, then a running per-path
+// minimum), the block merges per path, and a path spanning many tiles gets
+// one atomic per tile.
+constexpr uint32_t TOP_ITEMS = 8;
+
+__global__ __launch_bounds__(BLOCK) void k_top(Limits L0, const I2 *pts, const uint32_t *pt_run,
+                                               const uint32_t *run_path, PathState *ps, DevStatus *st) {
+    if (run_failed(st)) return;
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t n = st->n_kept;
+    if (n > L.cap_points) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
+        return;
+    }
+    uint32_t bad = 0;
+    constexpr uint32_t TILE = TOP_ITEMS * BLOCK;
+    for (uint32_t base = blockIdx.x * TILE; base < n; base += gridDim.x * TILE) {
+        TopCand c[TOP_ITEMS];
+#pragma unroll
+        for (uint32_t j = 0; j < TOP_ITEMS; ++j) {
+            const uint32_t i = base + j * BLOCK + threadIdx.x;
+            c[j] = top_cand(i < n ? i : 0, L, pts, pt_run, run_path, bad);
+            if (i >= n) c[j].path = NO_IDX;
+        }
+        uint32_t ck = NO_IDX;
+        unsigned long long cv = NO_KEY;
+#pragma unroll
+        for (uint32_t j = 0; j < TOP_ITEMS; ++j) {
+            const bool v = c[j].path != NO_IDX;
+            const bool change = v & (c[j].path != ck) & (ck != NO_IDX);
+            if (change) atomicMin(&ps[ck].topkey, cv);
+            const bool fresh = v & (c[j].path != ck);
+            cv = fresh ? c[j].key : (v ? (c[j].key < cv ? c[j].key : cv) : cv);
+            ck = v ? c[j].path : ck;
+        }
+        block_reduce_keyed<unsigned long long>(
+            ck != NO_IDX, ck, cv, NO_KEY, [](unsigned long long a, unsigned long long b) { return a < b ? a : b; },
+            [ps, &L](uint32_t k, unsigned long long v) {
+                if (k < L.n_paths) atomicMin(&ps[k].topkey, v);
+            });
     }
     if (bad) raise_err(st, FT_EDEVICE);
 }
@@ -348,18 +461,20 @@ __global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, const uint32_t *ev_m
                                                  const uint32_t *ev_idx, const uint32_t *run_path,
                                                  const uint32_t *run_start, const uint32_t *run_end,
                                                  const PathDev *paths, const I2 *pts, EdgeRec *edges,
-                                                 uint32_t *band_cnt, PathState *ps, DevStatus *st) {
+                                                 uint32_t *band_diff, PathState *ps, DevStatus *st) {
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     uint32_t bad = 0;
     for (uint32_t b = blockIdx.x * BLOCK; b < L.n_events; b += gridDim.x * BLOCK) {
         const uint32_t e = b + threadIdx.x;
-        uint32_t p = NO_IDX, r = 0;
+        EdgeOut r;
+        r.g0 = r.g1 = 0;
+        r.banded = false;
         if (e < L.n_events)
             r = edge_event<DevAtom>(e, L, ev_meta, ev_kept, ev_idx, run_path, run_start, run_end, paths, pts, edges,
-                                    band_cnt, ps, &p, bad);
-        wave_count(&ps[0].stored, PS_WORDS, L.n_paths, (r & 1u) != 0, p, 1u);
-        wave_count(&ps[0].n_edges, PS_WORDS, L.n_paths, (r & 2u) != 0, p, 1u);
+                                    ps, bad);
+        wave_add_runs(band_diff, L.n_bands, r.banded, r.g0, 1u);
+        wave_add_runs(band_diff, L.n_bands, r.banded & (r.g1 + 1 < L.n_bands), r.g1 + 1, 0xFFFFFFFFu);
     }
     if (bad) raise_err(st, FT_EDEVICE);
 }
@@ -380,14 +495,50 @@ __global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, const uint32_t *ev_met
     }
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     uint32_t bad = 0;
-    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
-        bin_event<DevAtom>(e, L, ev_meta, ev_kept, ev_idx, run_path, paths, edges, band_off, band_cur, band_list, ps,
-                           bad);
+    for (uint32_t b = blockIdx.x * BLOCK; b < L.n_events; b += gridDim.x * BLOCK) {
+        const uint32_t e = b + threadIdx.x;
+        const BinItem it = bin_item(e < L.n_events ? e : 0, L, ev_meta, ev_kept, ev_idx, run_path, paths, edges,
+                                    ps, bad);
+        const bool has = it.banded & (e < L.n_events);
+        const uint32_t span = has ? it.g1 - it.g0 + 1 : 0u;
+        // the wavefront walks the bands of its edges together, so that lanes
+        // at the same band share one atomic
+        for (uint32_t j = 0; __ballot(j < span); ++j) {
+            const bool on = j < span;
+            const uint32_t g = it.g0 + j;
+            const uint32_t slot = wave_add_runs(band_cur, L.n_bands, on, g, 1u);
+            const uint32_t gi = on ? g : 0;
+            const uint32_t at = band_off[gi] + slot;
+            if (on & (at < L.n_pairs)) band_list[at] = it.idx;
+            bad |= (on & (at >= L.n_pairs)) ? 1u : 0u;
+        }
+    }
     if (bad) raise_err(st, FT_EDEVICE);
 }
 
+// Edges per path over the stored points (reported on request, not needed by
+// the raster).
+__global__ __launch_bounds__(BLOCK) void k_edge_count(Limits L0, const EdgeRec *edges, const uint32_t *pt_run,
+                                                      const uint32_t *run_path, PathState *ps, DevStatus *st) {
+    if (run_failed(st)) return;
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    const uint32_t n = st->n_kept < L.cap_points ? st->n_kept : L.cap_points;
+    for (uint32_t b = blockIdx.x * BLOCK; b < n; b += gridDim.x * BLOCK) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t ii = i < n ? i : 0;
+        const uint32_t run = pt_run[ii];
+        const uint32_t p = run_path[run < L.n_runs ? run : 0];
+        const bool has = (i < n) & (run < L.n_runs) & (p < L.n_paths) & ((edges[ii].flags & 1u) != 0);
+        block_reduce_keyed<uint32_t>(
+            has, p, 1u, 0u, [](uint32_t a, uint32_t c) { return a + c; },
+            [ps, &L](uint32_t k, uint32_t v) {
+                if (k < L.n_paths) atomicAdd(&ps[k].n_edges, v);
+            });
+    }
+}
+
 // ---------------------------------------------------------------------------
-// Stage 6: raster.  One block per row band.
+// Stage 6: raster.  One block per row band and LDS row group.
 // ---------------------------------------------------------------------------
 __device__ inline uint32_t wave_incl_u32(uint32_t v) {
     const int lane = threadIdx.x & 63;
@@ -477,26 +628,35 @@ __device__ void row_pass(const int32_t *cells, uint32_t x_begin, uint32_t W, uin
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *paths, const uint32_t *band_path,
-                                                  const uint32_t *band_off, const uint32_t *band_cnt,
-                                                  const uint32_t *band_list, const EdgeRec *edges, const I2 *pts,
-                                                  const uint32_t *pt_run, const uint32_t *run_start,
-                                                  const uint32_t *run_end, const PathState *ps, DevStatus *st) {
+// One block per (band, row group): `rblock[b]` = (global band, group).  The
+// block accumulates the band's listed edges for its G rows in LDS and scans
+// each row into the raster; the band's other groups are other blocks.
+__global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *paths, const uint2 *rblock,
+                                                  const uint32_t *band_path, const uint32_t *band_off,
+                                                  const uint32_t *band_cnt, const uint32_t *band_list,
+                                                  const EdgeRec *edges, const I2 *pts, const uint32_t *pt_run,
+                                                  const uint32_t *run_start, const uint32_t *run_end,
+                                                  const PathState *ps, DevStatus *st) {
     extern __shared__ int32_t s_cells[];
     __shared__ int s_fwd;
     __shared__ uint32_t s_segsum[BLOCK / 64];
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
-    const uint32_t gb = blockIdx.x;
-    const uint32_t p = band_path[gb];
+    const uint2 rb = rblock[blockIdx.x];
+    const uint32_t gb = rb.x;
+    const uint32_t p = gb < L.n_bands ? band_path[gb] : NO_IDX;
     if (p >= L.n_paths) {
         if (threadIdx.x == 0) raise_err(st, FT_EDEVICE);
         return;
     }
     const PathDev &P = paths[p];
     const uint32_t R = P.band_rows, H = P.height, W = P.width;
-    const uint32_t y0 = (gb - P.band_base) * R;
-    const uint32_t y1 = y0 + R < H ? y0 + R : H;
+    const uint32_t G = lds_rows(W, R);  // rows whose cells fit in LDS at once
+    const uint32_t b0 = (gb - P.band_base) * R;
+    const uint32_t b1 = b0 + R < H ? b0 + R : H;
+    const uint32_t y0 = b0 + rb.y * G;
+    const uint32_t y1 = y0 + G < b1 ? y0 + G : b1;
+    if (y0 >= y1) return;
     const PathState S = ps[p];
     const bool has_pts = S.topkey != NO_KEY && S.topidx != NO_IDX;
     const int32_t top_row = has_pts ? top_row_of(S.topkey) : 0;
@@ -541,8 +701,8 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
             edge_row_cells<DevAtom>(E, (int32_t)(r + shift), ed, (int32_t)W, s_cells + (size_t)(r - ry0) * Wp);
     }
     __syncthreads();
-    // Row pass.  Bands of at least 4 rows give each wavefront whole rows; a
-    // band of 1-3 rows splits every row into segments, one per wavefront,
+    // Row pass.  Groups of at least 4 rows give each wavefront whole rows; a
+    // group of 1-3 rows splits every row into segments, one per wavefront,
     // whose cell sums are exchanged through LDS to form the carry-in.
     const int wid = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -629,9 +789,10 @@ ft_status ws_init(Workspace &ws) {
 }
 
 void ws_free(Workspace &ws) {
-    void *bufs[] = {ws.d_ops, ws.d_op_path, ws.d_paths, ws.d_band_path, ws.d_op_cnt, ws.d_op_off, ws.d_op_pw,
-                    ws.d_op_lastpw, ws.d_ev_xy, ws.d_ev_f, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx,
-                    ws.d_run_first, ws.d_run_last, ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_pts,
+    void *bufs[] = {ws.d_ops, ws.d_op_path, ws.d_paths, ws.d_band_path, ws.d_op_cnt, ws.d_op_off, ws.d_sub_off,
+                    ws.d_op_pw,
+                    ws.d_op_lastpw, ws.d_ev_xy, ws.d_ev_f, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_ev_last,
+                    ws.d_run_first, ws.d_run_close, ws.d_run_path, ws.d_rblock, ws.d_run_start, ws.d_run_end, ws.d_pts,
                     ws.d_pt_run, ws.d_edges, ws.d_band_cnt, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
                     ws.d_scan_flags, ws.d_scan_agg, ws.d_scan_incl, ws.d_pstate, ws.d_status};
     for (void *b : bufs)
@@ -651,8 +812,8 @@ static ft_status ensure_event_capacity(Workspace &ws, size_t need) {
     size_t cap = need < 4096 ? 4096 : need;
     ft_status s;
     if ((s = dev_alloc(ws.d_ev_xy, cap)) || (s = dev_alloc(ws.d_ev_f, cap)) || (s = dev_alloc(ws.d_ev_meta, cap)) ||
-        (s = dev_alloc(ws.d_ev_kept, cap)) || (s = dev_alloc(ws.d_ev_idx, cap)) ||
-        (s = dev_alloc(ws.d_run_first, cap)) || (s = dev_alloc(ws.d_run_last, cap)) ||
+        (s = dev_alloc(ws.d_ev_kept, cap)) || (s = dev_alloc(ws.d_ev_idx, cap)) || (s = dev_alloc(ws.d_ev_last, cap)) ||
+        (s = dev_alloc(ws.d_run_first, cap)) || (s = dev_alloc(ws.d_run_close, cap)) ||
         (s = dev_alloc(ws.d_run_path, cap)) || (s = dev_alloc(ws.d_run_start, cap)) ||
         (s = dev_alloc(ws.d_run_end, cap)) || (s = dev_alloc(ws.d_pts, cap)) || (s = dev_alloc(ws.d_pt_run, cap)) ||
         (s = dev_alloc(ws.d_edges, cap))) {
@@ -689,6 +850,7 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     std::vector<uint32_t> op_path(n_ops, NO_IDX);
     std::vector<PathDev> pd(n_paths);
     std::vector<uint32_t> band_path;
+    std::vector<uint2> rblock;
     uint32_t lds = 0;
     for (size_t k = 0; k < n_paths; ++k) {
         const ft_fill_desc &d = paths[k];
@@ -710,13 +872,24 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
         P.band_base = (uint32_t)band_path.size();
         if (nb && cells > lds) lds = cells;
         band_path.insert(band_path.end(), nb, (uint32_t)k);
+        if (nb) {
+            const uint32_t G = lds_rows(d.width, P.band_rows);
+            const uint32_t groups = (P.band_rows + G - 1) / G;
+            for (uint32_t b = 0; b < nb; ++b) {
+                const uint32_t rows = std::min(P.band_rows, d.height - b * P.band_rows);
+                for (uint32_t g = 0; g < groups && g * G < rows; ++g)
+                    rblock.push_back(make_uint2(P.band_base + b, g));
+            }
+        }
     }
+    if (rblock.size() >= 0x7FFFFFFFu) return FT_ENOTSUP;
     ft_status s;
     const size_t old_ops = ws.cap_ops;
     if ((s = dev_grow(ws.d_ops, ws.cap_ops, n_ops))) return s;
     if (ws.cap_ops != old_ops || !ws.d_op_cnt) {  // per-op scratch follows the op capacity
         if ((s = dev_alloc(ws.d_op_path, ws.cap_ops)) || (s = dev_alloc(ws.d_op_cnt, ws.cap_ops)) ||
             (s = dev_alloc(ws.d_op_off, ws.cap_ops)) || (s = dev_alloc(ws.d_op_pw, ws.cap_ops)) ||
+            (s = dev_alloc(ws.d_sub_off, ws.cap_ops * OP_LANES)) ||
             (s = dev_alloc(ws.d_op_lastpw, ws.cap_ops)))
             return s;
     }
@@ -732,6 +905,8 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
             return s;
         ws.cap_bands = cap;
     }
+    if ((s = dev_grow(ws.d_rblock, ws.cap_rblocks, rblock.size()))) return s;
+    ws.n_rblocks = rblock.size();
     ws.n_ops = n_ops;
     ws.n_paths = n_paths;
     ws.n_bands = nb_total;
@@ -754,6 +929,9 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
         FT_HIP(hipMemcpyAsync(ws.d_paths, pd.data(), n_paths * sizeof(PathDev), hipMemcpyHostToDevice, ws.stream));
     if (nb_total)
         FT_HIP(hipMemcpyAsync(ws.d_band_path, band_path.data(), nb_total * 4, hipMemcpyHostToDevice, ws.stream));
+    if (!rblock.empty())
+        FT_HIP(hipMemcpyAsync(ws.d_rblock, rblock.data(), rblock.size() * sizeof(uint2), hipMemcpyHostToDevice,
+                              ws.stream));
     FT_HIP(hipStreamSynchronize(ws.stream));  // host vectors go out of scope
     return FT_OK;
 }
@@ -772,6 +950,7 @@ static inline uint32_t scan_grid(size_t n) {
 
 ft_status ws_launch(Workspace &ws, uint32_t mode) {
     hipStream_t s = ws.stream;
+    ws.edges_counted = false;
     const uint32_t n_ops = (uint32_t)ws.n_ops;
     const size_t tiles = ws.cap_tiles;
     uint32_t *fl = ws.d_scan_flags;
@@ -794,14 +973,15 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
         k_pen_width_mark<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, n_ops, ws.d_op_pw);
         k_scan<uint32_t, true><<<scan_grid(n_ops), BLOCK, 0, s>>>(ws.d_op_pw, ws.d_op_lastpw, nullptr, n_ops, n_ops,
                                                                   fl + 3 * tiles, ag + 3 * tiles, in + 3 * tiles,
-                                                                  &st->tickets[3], nullptr, st, false);
+                                                                  &st->tickets[3], nullptr, st, false, false);
         lastpw = ws.d_op_lastpw;
     }
     if (n_ops) {
-        k_count<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode, lastpw, ws.d_op_cnt,
-                                                  st);
+        k_count<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
+                                                                     lastpw, ws.d_op_cnt, ws.d_sub_off, st);
         k_scan<unsigned long long, false><<<scan_grid(n_ops), BLOCK, 0, s>>>(
-            ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops, fl, ag, in, &st->tickets[0], &st->op_total, st, false);
+            ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops, fl, ag, in, &st->tickets[0], &st->op_total, st, false,
+            false);
         EventSink em{};
         em.cap_events = ws.cap_events;
         em.cap_runs = (uint32_t)ws.cap_events;
@@ -810,12 +990,12 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
         em.ev_f = ws.d_ev_f;
         em.ev_meta = ws.d_ev_meta;
         em.run_first = ws.d_run_first;
-        em.run_last = ws.d_run_last;
+        em.run_close = ws.d_run_close;
         em.run_path = ws.d_run_path;
         em.run_start = ws.d_run_start;
         em.run_end = ws.d_run_end;
-        k_emit<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode, lastpw, ws.d_op_off,
-                                                 em, st);
+        k_emit<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
+                                                                    lastpw, ws.d_op_off, ws.d_sub_off, em, st);
     }
     if (mode == MODE_FILL) {
         const uint32_t ge = grid_for(ws.cap_events);
@@ -824,30 +1004,41 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
         L0.n_paths = (uint32_t)ws.n_paths;
         L0.n_bands = (uint32_t)ws.n_bands;
         L0.cap_points = (uint32_t)ws.cap_events;
-        k_ingest<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_last, ws.d_run_path,
-                                      ws.d_pstate, st);
-        k_kept<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_last, ws.d_ev_kept, st);
+        k_ingest<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, ws.d_ev_last, st);
+        k_scan<uint32_t, true><<<scan_grid(ws.cap_events), BLOCK, 0, s>>>(
+            ws.d_ev_last, ws.d_ev_last, &st->n_events, 0, (uint32_t)ws.cap_events, fl + 4 * tiles, ag + 4 * tiles,
+            in + 4 * tiles, &st->tickets[4], nullptr, st, true, true);
+        k_kept<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_last,
+                                    ws.d_ev_kept, st);
         k_scan<uint32_t, false><<<scan_grid(ws.cap_events), BLOCK, 0, s>>>(
             ws.d_ev_kept, ws.d_ev_idx, &st->n_events, 0, (uint32_t)ws.cap_events, fl + tiles, ag + tiles,
-            in + tiles, &st->tickets[1], &st->n_kept, st, true);
+            in + tiles, &st->tickets[1], &st->n_kept, st, true, false);
         k_cap<<<grid_for(ws.n_paths), BLOCK, 0, s>>>(L0, ws.d_paths, ws.d_op_off, ws.d_ev_meta, ws.d_ev_xy,
                                                      ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first, ws.d_pstate, st);
         k_compact<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first,
                                        ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_pts, ws.d_pt_run, ws.d_pstate,
                                        st);
+        const size_t top_tile = (size_t)TOP_ITEMS * BLOCK;
+        const size_t top_grid = std::max<size_t>(1, std::min<size_t>(GRID_MAX, (ws.cap_events + top_tile - 1) / top_tile));
+        k_top<<<(uint32_t)top_grid, BLOCK, 0, s>>>(L0, ws.d_pts, ws.d_pt_run, ws.d_run_path, ws.d_pstate, st);
         k_edges<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_path, ws.d_run_start,
                                      ws.d_run_end, ws.d_paths, ws.d_pts, ws.d_edges, ws.d_band_cnt, ws.d_pstate, st);
         if (ws.n_bands) {
+            // band differences -> edges per band (in place; slot 3 is free in fill mode) -> list offsets
+            k_scan<uint32_t, false><<<scan_grid(ws.n_bands), BLOCK, 0, s>>>(
+                ws.d_band_cnt, ws.d_band_cnt, nullptr, (uint32_t)ws.n_bands, (uint32_t)ws.n_bands, fl + 3 * tiles,
+                ag + 3 * tiles, in + 3 * tiles, &st->tickets[3], nullptr, st, true, true);
             k_scan<uint32_t, false><<<scan_grid(ws.n_bands), BLOCK, 0, s>>>(
                 ws.d_band_cnt, ws.d_band_off, nullptr, (uint32_t)ws.n_bands, (uint32_t)ws.n_bands, fl + 2 * tiles,
-                ag + 2 * tiles, in + 2 * tiles, &st->tickets[2], &st->n_pairs, st, true);
+                ag + 2 * tiles, in + 2 * tiles, &st->tickets[2], &st->n_pairs, st, true, false);
             k_bin<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_path, ws.d_paths,
                                        ws.d_edges, ws.d_band_off, ws.d_band_cur, ws.d_band_list, ws.cap_pairs,
                                        ws.d_pstate, st);
             if (ev) FT_HIP(hipEventRecord(ev[1], s));
-            k_raster<<<(uint32_t)ws.n_bands, BLOCK, ws.lds_cells * sizeof(int32_t), s>>>(
-                L0, ws.d_paths, ws.d_band_path, ws.d_band_off, ws.d_band_cnt, ws.d_band_list, ws.d_edges, ws.d_pts,
-                ws.d_pt_run, ws.d_run_start, ws.d_run_end, ws.d_pstate, st);
+            if (ws.n_rblocks)
+                k_raster<<<(uint32_t)ws.n_rblocks, BLOCK, ws.lds_cells * sizeof(int32_t), s>>>(
+                    L0, ws.d_paths, ws.d_rblock, ws.d_band_path, ws.d_band_off, ws.d_band_cnt, ws.d_band_list,
+                    ws.d_edges, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end, ws.d_pstate, st);
             if (ev) FT_HIP(hipEventRecord(ev[2], s));
         }
     }
@@ -875,6 +1066,24 @@ ft_status ws_finish(Workspace &ws, bool read_pstate) {
     const DevStatus &h = *ws.h_status;
     if (h.err) return (ft_status)h.err;
     if (h.overflow) return (ft_status)0x100;  // internal: capacity exceeded
+    return FT_OK;
+}
+
+ft_status ws_count_edges(Workspace &ws) {
+    if (!ws.edges_counted && ws.n_paths) {
+        Limits L0{};
+        L0.n_ops = (uint32_t)ws.n_ops;
+        L0.n_paths = (uint32_t)ws.n_paths;
+        L0.n_bands = (uint32_t)ws.n_bands;
+        L0.cap_points = (uint32_t)ws.cap_events;
+        k_edge_count<<<grid_for(ws.cap_events), BLOCK, 0, ws.stream>>>(L0, ws.d_edges, ws.d_pt_run, ws.d_run_path,
+                                                                        ws.d_pstate, ws.d_status);
+        FT_HIP(hipGetLastError());
+        FT_HIP(hipMemcpyAsync(ws.h_pstate.data(), ws.d_pstate, ws.n_paths * sizeof(PathState),
+                              hipMemcpyDeviceToHost, ws.stream));
+        FT_HIP(hipStreamSynchronize(ws.stream));
+        ws.edges_counted = true;
+    }
     return FT_OK;
 }
 

@@ -91,12 +91,12 @@ FT_HD inline WP end_of(const Cubic &q) { return q.d; }
 // subtree, then right).  `bits` holds the left/right choices from the root;
 // moving to a right sibling replays those choices from the root, so every
 // node is computed by exactly the float operations the recursion performs.
-// Returns 0, or FT_ERANGE when a node at FT_FLATTEN_MAX_DEPTH is still out of
-// tolerance or the curve yields more than FT_FLATTEN_MAX_LEAVES points.
+// Returns 0, or FT_ERANGE when a node `max_depth` levels below `root` is
+// still out of tolerance or `leaves` exceeds FT_FLATTEN_MAX_LEAVES.
 template <class Curve, class Emit>
-FT_HD uint32_t flatten_walk(const Curve &root, float tol_sq, Emit &&emit) {
+FT_HD uint32_t flatten_walk(const Curve &root, float tol_sq, uint32_t max_depth, uint32_t &leaves, Emit &&emit) {
     Curve cur = root;
-    uint32_t depth = 0, bits = 0, leaves = 0;
+    uint32_t depth = 0, bits = 0;
     for (;;) {
         Curve L, R;
         if (split(cur, tol_sq, L, R)) {
@@ -115,12 +115,38 @@ FT_HD uint32_t flatten_walk(const Curve &root, float tol_sq, Emit &&emit) {
                 cur = ((bits >> lv) & 1u) ? r2 : l2;
             }
         } else {
-            if (depth >= FT_FLATTEN_MAX_DEPTH) return FT_ERANGE;
+            if (depth >= max_depth) return FT_ERANGE;
             cur = L;
             bits <<= 1;
             ++depth;
         }
     }
+}
+
+// A curve is flattened by OP_LANES lanes.  Lane s takes the subtree
+// FLAT_SPLIT levels below the root whose left/right choices are the bits of
+// s, most significant first; a node that is already within tolerance above
+// that depth is a leaf of the lowest lane beneath it, and the other lanes
+// beneath it emit nothing.  Concatenated in lane order the lanes' points are
+// the recursion's points in order (plotter.rs:250-262,314-330).
+constexpr uint32_t FLAT_SPLIT = 3;
+constexpr uint32_t OP_LANES = 1u << FLAT_SPLIT;
+
+template <class Curve, class Emit>
+FT_HD uint32_t flatten_share(const Curve &root, float tol_sq, uint32_t s, uint32_t &leaves, Emit &&emit) {
+    Curve cur = root;
+    for (uint32_t lv = 0; lv < FLAT_SPLIT; ++lv) {
+        Curve L, R;
+        if (split(cur, tol_sq, L, R)) {
+            if ((s & ((1u << (FLAT_SPLIT - lv)) - 1u)) == 0) {
+                ++leaves;
+                emit(end_of(cur));
+            }
+            return 0;
+        }
+        cur = ((s >> (FLAT_SPLIT - 1 - lv)) & 1u) ? R : L;
+    }
+    return flatten_walk(cur, tol_sq, FT_FLATTEN_MAX_DEPTH - FLAT_SPLIT, leaves, emit);
 }
 
 // Index of the op that last set the pen before op i (plotter.rs:128-130,
@@ -186,41 +212,44 @@ FT_HD inline Cubic cubic_of(const ft_path_op &o, WP pen, float sw, const PathDev
 // ---------------------------------------------------------------------------
 // Stage 1: events per op (plotter.rs:175-224 + Plotter::fill's final close,
 // plotter.rs:347).  Close: 1 close; Move: 1 close + 1 point; Line: 1 point;
-// Quad / Cubic: one point per subdivision leaf; PenWidth: nothing.
+// Quad / Cubic: one point per subdivision leaf; PenWidth: nothing.  An op
+// is handled by OP_LANES lanes: lane 0 takes the non-curve events, each lane
+// its share of a curve, and the last lane the path's final close, so the
+// lanes' events in lane order are the op's events in order.
 // ---------------------------------------------------------------------------
 struct OpCount {
-    uint32_t ne, nc, err;
+    uint32_t ne, nc, err, leaves;
 };
 
 FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, const PathDev *paths, uint32_t i,
-                              uint32_t mode, const uint32_t *lastpw) {
-    OpCount c = {0, 0, 0};
+                              uint32_t s, uint32_t mode, const uint32_t *lastpw) {
+    OpCount c = {0, 0, 0, 0};
     const uint32_t p = op_path[i];
     if (p == NO_IDX) return c;
     const PathDev &P = paths[p];
     const ft_path_op o = ops[i];
+    const bool lead = s == 0;
     switch (o.tag) {
-    case TAG_CLOSE: c.ne = 1; c.nc = 1; break;
-    case TAG_MOVE: c.ne = 2; c.nc = 1; break;
-    case TAG_LINE: c.ne = 1; break;
+    case TAG_CLOSE: c.ne = lead ? 1 : 0; c.nc = lead ? 1 : 0; break;
+    case TAG_MOVE: c.ne = lead ? 2 : 0; c.nc = lead ? 1 : 0; break;
+    case TAG_LINE: c.ne = lead ? 1 : 0; break;
     case TAG_QUAD:
     case TAG_CUBIC: {
         const WP pen = pen_before(ops, lastpw, i, P);
         const float sw = width_before(ops, lastpw, i, P);
-        uint32_t leaves = 0;
-        auto cnt = [&](WP) { ++leaves; };
-        const uint32_t rc = o.tag == TAG_QUAD ? flatten_walk(quad_of(o, pen, sw, P), P.tol_sq, cnt)
-                                              : flatten_walk(cubic_of(o, pen, sw, P), P.tol_sq, cnt);
+        auto cnt = [&](WP) {};
+        const uint32_t rc = o.tag == TAG_QUAD ? flatten_share(quad_of(o, pen, sw, P), P.tol_sq, s, c.leaves, cnt)
+                                              : flatten_share(cubic_of(o, pen, sw, P), P.tol_sq, s, c.leaves, cnt);
         if (rc)
             c.err = rc;
         else
-            c.ne = leaves;
+            c.ne = c.leaves;
         break;
     }
     case TAG_PEN_WIDTH: break;
-    default: c.err = FT_EINVAL; break;
+    default: c.err = lead ? FT_EINVAL : 0u; break;
     }
-    if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count) {
+    if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count && s == OP_LANES - 1) {
         ++c.ne;
         ++c.nc;
     }
@@ -241,7 +270,8 @@ struct EventSink {
     I2 *ev_xy;
     F4 *ev_f;
     uint32_t *ev_meta;
-    int32_t *run_first, *run_last;
+    int32_t *run_first;
+    uint32_t *run_close;  // the run's close event
     uint32_t *run_path, *run_start, *run_end;
     uint64_t cap_events;  // writes at or past these limits are dropped and
     uint32_t cap_runs;    // flagged in `bad`
@@ -272,7 +302,7 @@ struct EventSink {
         if (ok & (mode == MODE_FILL)) {
             ev_meta[e] = run | CLOSE_BIT;
             run_first[run] = -1;
-            run_last[run] = -1;
+            run_close[run] = (uint32_t)e;
             run_path[run] = p;
             run_start[run] = NO_IDX;
             run_end[run] = 0;
@@ -291,39 +321,47 @@ struct EventSink {
     }
 };
 
-// Writes op i's events at offset `off` (events << 32 | closes before op i).
+// Writes lane s's events of op i at offset `off` (events << 32 | closes
+// before them).
 FT_HD inline uint32_t emit_op(const ft_path_op *ops, const uint32_t *op_path, const PathDev *paths, uint32_t i,
-                              uint32_t mode, const uint32_t *lastpw, unsigned long long off, EventSink em) {
+                              uint32_t s, uint32_t mode, const uint32_t *lastpw, unsigned long long off,
+                              EventSink em) {
     const uint32_t p = op_path[i];
     if (p == NO_IDX) return 0;
     const PathDev &P = paths[p];
     const ft_path_op o = ops[i];
+    const bool lead = s == 0;
     em.e = off >> 32;
     em.run = (uint32_t)off;
     em.p = p;
     switch (o.tag) {
-    case TAG_CLOSE: em.close(true); break;  // plotter.rs:200-203
-    case TAG_MOVE:                           // plotter.rs:208-214
-        em.close(false);
-        em.point(xform(P, o.v[0], o.v[1], width_before(ops, lastpw, i, P)));
+    case TAG_CLOSE:  // plotter.rs:200-203
+        if (lead) em.close(true);
+        break;
+    case TAG_MOVE:  // plotter.rs:208-214
+        if (lead) {
+            em.close(false);
+            em.point(xform(P, o.v[0], o.v[1], width_before(ops, lastpw, i, P)));
+        }
         break;
     case TAG_LINE:  // plotter.rs:219-224
-        em.point(xform(P, o.v[0], o.v[1], width_before(ops, lastpw, i, P)));
+        if (lead) em.point(xform(P, o.v[0], o.v[1], width_before(ops, lastpw, i, P)));
         break;
     case TAG_QUAD:
     case TAG_CUBIC: {
         const WP pen = pen_before(ops, lastpw, i, P);
         const float sw = width_before(ops, lastpw, i, P);
+        uint32_t leaves = 0;
         auto put = [&](WP q) { em.point(q); };
         if (o.tag == TAG_QUAD)
-            flatten_walk(quad_of(o, pen, sw, P), P.tol_sq, put);
+            flatten_share(quad_of(o, pen, sw, P), P.tol_sq, s, leaves, put);
         else
-            flatten_walk(cubic_of(o, pen, sw, P), P.tol_sq, put);
+            flatten_share(cubic_of(o, pen, sw, P), P.tol_sq, s, leaves, put);
         break;
     }
     default: break;
     }
-    if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count) em.close(true);
+    if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count && s == OP_LANES - 1) em.close(true);
     return em.bad;
 }
 
@@ -368,27 +406,31 @@ FT_HD inline Pushed pushed_event(uint32_t e, const Limits &L, const uint32_t *me
     return r;
 }
 
-// Records the run's first point and last pushed point.  Returns the point
-// event's path, or NO_IDX for a close event.
-template <class A>
-FT_HD inline uint32_t ingest_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy,
-                                   int32_t *run_first, int32_t *run_last, const uint32_t *run_path, uint32_t &bad) {
+// Records the run's first point and `last[e]` = e + 1 for a pushed point
+// event, else 0.  The inclusive max-scan of `last` at a run's close event is
+// then 1 + the run's last pushed point (pushed points rise with e, and the
+// run's first point is always pushed).
+FT_HD inline void ingest_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy, int32_t *run_first,
+                               const uint32_t *run_path, uint32_t *last, uint32_t &bad) {
     const Pushed q = pushed_event(e, L, meta, xy);
     const uint32_t p = run_path[q.ok ? q.run : 0];
     if (q.is_pt & q.first) run_first[q.run] = (int32_t)e;
-    if (q.is_pt & q.pushed) A::max_i32(&run_last[q.run], (int32_t)e);
-    const bool ok = q.ok & (p < L.n_paths);
-    bad |= ok ? 0u : 1u;
-    return (q.is_pt & ok) ? p : NO_IDX;
+    last[e] = (q.is_pt & q.pushed) ? e + 1 : 0u;
+    bad |= (q.ok & (p < L.n_paths)) ? 0u : 1u;
 }
 
-// 1 if point event e survives the closing pop.
+// 1 if point event e survives the closing pop.  `last_scan` is the inclusive
+// max-scan of ingest_event's `last`.
 FT_HD inline uint32_t kept_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy,
-                                 const int32_t *run_first, const int32_t *run_last, uint32_t &bad) {
+                                 const int32_t *run_first, const uint32_t *run_close, const uint32_t *last_scan,
+                                 uint32_t &bad) {
     const Pushed q = pushed_event(e, L, meta, xy);
     const uint32_t run = q.ok ? q.run : 0;
-    const int32_t rf = run_first[run], rl = run_last[run];
-    const bool rok = (rf >= 0) & (rl >= 0) & ((uint32_t)rf < L.n_events) & ((uint32_t)rl < L.n_events);
+    const int32_t rf = run_first[run];
+    const uint32_t c = run_close[run];
+    const bool cok = c < L.n_events;
+    const int32_t rl = (int32_t)last_scan[cok ? c : e] - 1;
+    const bool rok = cok & (rf >= 0) & (rl >= rf) & ((uint32_t)rf < L.n_events) & ((uint32_t)rl < L.n_events);
     const I2 f = xy[rok ? (uint32_t)rf : e], l = xy[rok ? (uint32_t)rl : e];
     const bool pop = i2_eq(f, l) & ((int32_t)e == rl);
     bad |= ((q.is_pt & q.pushed & !rok) ? 1u : 0u) | (q.ok ? 0u : 1u);
@@ -422,23 +464,28 @@ FT_HD inline int32_t top_row_of(unsigned long long key) {
     return ((int32_t)((uint32_t)(key >> 32) ^ 0x80000000u)) >> 16;
 }
 
-// Stage 4a, one thread per path: the path's first point index and the
-// 65,535-point cap (vid.rs:17, fig.rs:429-431).  Once the stored count
-// reaches the cap every later point is ignored; the run holding the cap point
-// then closes against its truncated last point (fig.rs:373-383), which pops
-// local point 65534 when it equals the run's first point.  Every later run
-// only adds a point and pops it again.
+// Stage 4a, one thread per path: the path's first point index, the 65,535-point
+// cap (vid.rs:17, fig.rs:429-431) and the offered / stored counts.  Once the
+// stored count reaches the cap every later point is ignored; the run holding
+// the cap point then closes against its truncated last point (fig.rs:373-383),
+// which pops local point 65534 when it equals the run's first point.  Every
+// later run only adds a point and pops it again.  The stored points are the
+// kept indices [pt_base, pt_base + stored).
 FT_HD inline void cap_path(uint32_t p, const Limits &L, const PathDev *paths, const unsigned long long *op_off,
                            const uint32_t *meta, const I2 *xy, const uint32_t *kept, const uint32_t *idx_of,
                            const int32_t *run_first, PathState *ps, uint32_t &bad) {
     const PathDev &P = paths[p];
-    uint32_t base = 0, repop = 0;
+    uint32_t base = 0, repop = 0, offered = 0, stored = 0;
     if (P.op_count && P.op_begin < L.n_ops) {
         const uint32_t eb = path_event_begin(P, op_off);
         const uint32_t ee = path_event_end(P, op_off, L.n_ops, L.n_events);
-        if (eb < ee && ee <= L.n_events) {
+        const uint32_t k = P.op_begin + P.op_count;
+        const uint32_t cb = (uint32_t)op_off[P.op_begin], ce = k < L.n_ops ? (uint32_t)op_off[k] : L.n_runs;
+        if (eb < ee && ee <= L.n_events && cb <= ce && ce - cb <= ee - eb) {
+            offered = (ee - eb) - (ce - cb);  // point events: events minus closes
             base = idx_of[eb];
             const uint32_t n_kept = idx_of[ee - 1] + kept[ee - 1] - base;
+            stored = n_kept;
             if (n_kept >= FT_MAX_POINTS) {
                 // the kept event with local index 65534: first e with idx >= base + 65534
                 uint32_t lo = eb, hi = ee;
@@ -463,49 +510,63 @@ FT_HD inline void cap_path(uint32_t p, const Limits &L, const PathDev *paths, co
                     const int32_t rf = run_first[run];
                     if (rf >= 0 && (uint32_t)rf < L.n_events && i2_eq(xy[rf], xy[ea])) repop = 1;
                 }
+                stored = FT_MAX_POINTS - repop;
             }
-        } else if (eb > ee || ee > L.n_events) {
+        } else if (eb > ee || ee > L.n_events || cb > ce) {
             bad = 1;
         }
     }
     ps[p].pt_base = base;
     ps[p].repop = repop;
+    ps[p].offered = offered;
+    ps[p].stored = stored;
 }
 
-struct Stored {
-    uint32_t path;           // path of the event (NO_IDX: none)
-    bool stored;             // the point went into the Fig
-    unsigned long long key;  // its (y, x) key
-};
-
 // Stage 4b: compaction into the per-path point arrays (the Fig's `points`).
-template <class A>
-FT_HD inline Stored compact_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy,
-                                  const uint32_t *kept, const uint32_t *idx_of, const int32_t *run_first,
-                                  const uint32_t *run_path, uint32_t *run_start, uint32_t *run_end, I2 *pts,
-                                  uint32_t *pt_run, const PathState *ps, uint32_t &bad) {
-    const uint32_t run = meta[e] & ~CLOSE_BIT;
+// A kept point beyond the cap gets pt_run = NO_IDX; a close event sets its
+// run's end (one past its last stored point).
+FT_HD inline void compact_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy,
+                                const uint32_t *kept, const uint32_t *idx_of, const int32_t *run_first,
+                                const uint32_t *run_path, uint32_t *run_start, uint32_t *run_end, I2 *pts,
+                                uint32_t *pt_run, const PathState *ps, uint32_t &bad) {
+    const uint32_t m = meta[e];
+    const uint32_t run = m & ~CLOSE_BIT;
     const bool rok = run < L.n_runs;
     const uint32_t p = run_path[rok ? run : 0];
     const bool ok = rok & (p < L.n_paths);
     const PathState s = ps[ok ? p : 0];
     const uint32_t idx = idx_of[e];
-    const bool keep = ok & (kept[e] != 0) & (capped_local(idx, s.pt_base, s.repop) != NO_IDX);
+    const bool kp = ok & (kept[e] != 0);
+    const bool keep = kp & (capped_local(idx, s.pt_base, s.repop) != NO_IDX);
     const bool iok = idx < L.cap_points;
     const I2 q = xy[e];
     const int32_t rf = run_first[rok ? run : 0];
-    if (keep & iok) {
-        pts[idx] = q;
-        pt_run[idx] = run;
-        A::max_u32(&run_end[run], idx + 1);
-    }
+    const uint32_t cap_end = s.pt_base + s.stored;
+    if (keep & iok) pts[idx] = q;
+    if (kp & iok) pt_run[idx] = keep ? run : NO_IDX;
     if (keep & iok & ((int32_t)e == rf)) run_start[run] = idx;
-    bad |= (ok ? 0u : 1u) | ((keep & !iok) ? 1u : 0u);
-    Stored out;
-    out.path = ok ? p : NO_IDX;
-    out.stored = keep & iok;
-    out.key = point_key(q);
-    return out;
+    if (ok & ((m & CLOSE_BIT) != 0)) run_end[run] = idx < cap_end ? idx : cap_end;
+    bad |= (ok ? 0u : 1u) | ((kp & !iok) ? 1u : 0u);
+}
+
+// Stage 4c: path and (y, x) key of stored point i, for the per-path top-left
+// point (Fig::sort_points' first vertex, fig.rs:467-478).
+struct TopCand {
+    uint32_t path;  // NO_IDX: not a stored point
+    unsigned long long key;
+};
+
+FT_HD inline TopCand top_cand(uint32_t i, const Limits &L, const I2 *pts, const uint32_t *pt_run,
+                              const uint32_t *run_path, uint32_t &bad) {
+    const uint32_t run = pt_run[i];
+    const bool rok = run < L.n_runs;
+    const uint32_t p = run_path[rok ? run : 0];
+    const bool ok = rok & (p < L.n_paths);
+    bad |= ((run != NO_IDX) & !ok) ? 1u : 0u;
+    TopCand c;
+    c.path = ok ? p : NO_IDX;
+    c.key = point_key(pts[i]);
+    return c;
 }
 
 // Scan rows of an edge mapped to raster rows (fig.rs:496-500,536-553): scan
@@ -565,17 +626,35 @@ FT_HD inline EdgeRec make_edge(I2 q, I2 q1) {
     return E;
 }
 
+// The global row bands [g0, g1] an edge touches.
+FT_HD inline bool edge_bands(const EdgeRec &E, const PathDev &P, unsigned long long topkey, const Limits &L,
+                             uint32_t &g0, uint32_t &g1) {
+    int32_t r0, r1;
+    const bool rows = edge_rows(E, top_row_of(topkey), P.height, r0, r1) & ((E.flags & 1u) != 0);
+    const uint32_t br = P.band_rows ? P.band_rows : 1;
+    g0 = P.band_base + (rows ? (uint32_t)r0 / br : 0u);
+    g1 = P.band_base + (rows ? (uint32_t)r1 / br : 0u);
+    return rows & (g1 < L.n_bands) & (g0 <= g1);
+}
+
 // Stage 5: one edge per stored point: the segment to its successor within the
-// sub-figure (SubFig::next, fig.rs:143-163).  Edge::new runs once per edge;
-// the edge's rows are counted into row bands.
-// Returns bit 0: a stored point, bit 1: it starts a non-horizontal edge.
+// sub-figure (SubFig::next, fig.rs:143-163).  Edge::new runs once per edge.
+// The caller records the edge's band range [g0, g1] as a difference (+1 at
+// g0, -1 at g1 + 1) whose inclusive prefix sum is the edges per band.
+struct EdgeOut {
+    uint32_t path;    // path of the stored point (NO_IDX: none)
+    uint32_t g0, g1;  // band range, when `banded`
+    bool valid;       // a stored point
+    bool has;         // it starts a non-horizontal edge
+    bool banded;      // that edge touches a raster row
+};
+
 template <class A>
-FT_HD inline uint32_t edge_event(uint32_t e, const Limits &L, const uint32_t *meta, const uint32_t *kept,
-                                 const uint32_t *idx_of, const uint32_t *run_path, const uint32_t *run_start,
-                                 const uint32_t *run_end, const PathDev *paths, const I2 *pts, EdgeRec *edges,
-                                 uint32_t *band_cnt, PathState *ps, uint32_t *path_out, uint32_t &bad) {
+FT_HD inline EdgeOut edge_event(uint32_t e, const Limits &L, const uint32_t *meta, const uint32_t *kept,
+                                const uint32_t *idx_of, const uint32_t *run_path, const uint32_t *run_start,
+                                const uint32_t *run_end, const PathDev *paths, const I2 *pts, EdgeRec *edges,
+                                PathState *ps, uint32_t &bad) {
     const PointRef r = point_of(e, L, meta, kept, idx_of, run_path, ps, bad);
-    *path_out = r.path;
     const uint32_t p = r.valid ? r.path : 0, i = r.valid ? r.idx : 0, run = r.valid ? r.run : 0;
     const PathState s = ps[p];
     const uint32_t rs = run_start[run], re = run_end[run];
@@ -588,46 +667,32 @@ FT_HD inline uint32_t edge_event(uint32_t e, const Limits &L, const uint32_t *me
     if (!has) E.flags = 0;
     if (r.valid & (point_key(q) == s.topkey)) A::min_u32(&ps[p].topidx, i - s.pt_base);
     if (r.valid) edges[i] = E;
-    const PathDev &P = paths[p];
-    int32_t r0, r1;
-    const bool rows = edge_rows(E, top_row_of(s.topkey), P.height, r0, r1);
-    if (has & rows) {
-        const uint32_t b0 = (uint32_t)r0 / P.band_rows, b1 = (uint32_t)r1 / P.band_rows;
-        for (uint32_t bb = b0; bb <= b1; ++bb) {
-            const uint32_t gb = P.band_base + bb;
-            if (gb < L.n_bands) A::add_u32(&band_cnt[gb], 1u);
-        }
-    }
-    return (r.valid ? 1u : 0u) | (has ? 2u : 0u);
+    EdgeOut out;
+    out.banded = edge_bands(E, paths[p], s.topkey, L, out.g0, out.g1) & has;
+    out.path = r.path;
+    out.valid = r.valid;
+    out.has = has;
+    return out;
 }
 
-// Stage 5b: scatter the edge into the lists of the bands it touches.
-template <class A>
-FT_HD inline void bin_event(uint32_t e, const Limits &L, const uint32_t *meta, const uint32_t *kept,
-                            const uint32_t *idx_of, const uint32_t *run_path, const PathDev *paths,
-                            const EdgeRec *edges, const uint32_t *band_off, uint32_t *band_cur, uint32_t *band_list,
-                            const PathState *ps, uint32_t &bad) {
+// Stage 5b: the edge of event e and its band range, for scattering into the
+// band lists (band_list[band_off[g] + slot] = edge index for g in [g0, g1]).
+struct BinItem {
+    uint32_t idx;     // edge index
+    uint32_t g0, g1;  // band range
+    bool banded;
+};
+
+FT_HD inline BinItem bin_item(uint32_t e, const Limits &L, const uint32_t *meta, const uint32_t *kept,
+                              const uint32_t *idx_of, const uint32_t *run_path, const PathDev *paths,
+                              const EdgeRec *edges, const PathState *ps, uint32_t &bad) {
     const PointRef r = point_of(e, L, meta, kept, idx_of, run_path, ps, bad);
     const uint32_t p = r.valid ? r.path : 0, i = r.valid ? r.idx : 0;
     const EdgeRec E = edges[i];
-    const PathDev &P = paths[p];
-    int32_t r0, r1;
-    const bool rows = edge_rows(E, top_row_of(ps[p].topkey), P.height, r0, r1);
-    if (r.valid & ((E.flags & 1u) != 0) & rows) {
-        const uint32_t b0 = (uint32_t)r0 / P.band_rows, b1 = (uint32_t)r1 / P.band_rows;
-        for (uint32_t bb = b0; bb <= b1; ++bb) {
-            const uint32_t gb = P.band_base + bb;
-            if (gb >= L.n_bands) {
-                bad = 1;
-                break;
-            }
-            const uint32_t slot = band_off[gb] + A::add_u32(&band_cur[gb], 1u);
-            if (slot < L.n_pairs)
-                band_list[slot] = i;
-            else
-                bad = 1;
-        }
-    }
+    BinItem b;
+    b.idx = i;
+    b.banded = edge_bands(E, paths[p], ps[p].topkey, L, b.g0, b.g1) & r.valid;
+    return b;
 }
 
 // ---------------------------------------------------------------------------
@@ -723,22 +788,36 @@ FT_HD inline void composite_px_fresh(uint8_t *d, const uint32_t *sw, uint32_t a)
     for (int c = 0; c < N; ++c) d[c] = (uint8_t)ch8_mul_w(sw[c], aw);
 }
 
-// Band geometry: rows per band for a raster, and the LDS cells it needs.
+// Band geometry.  A band is the unit of edge binning and one raster block;
+// its rows are rasterised in groups whose cells fit LDS_CELLS.  Bands are at
+// least BAND_ROWS_MIN rows tall so that wide rasters (one row per group) do
+// not list every edge once per row.
 constexpr uint32_t LDS_CELLS = 4096;  // i32 signed-area cells per raster block (16 KiB)
 constexpr uint32_t BAND_ROWS_MAX = 64;
+constexpr uint32_t BAND_ROWS_MIN = 16;
 constexpr uint32_t CELL_ALIGN = 16;   // row stride of cells in LDS
 
 FT_HD inline uint32_t cell_stride(uint32_t width) { return (width + CELL_ALIGN - 1) / CELL_ALIGN * CELL_ALIGN; }
 
+// Rows per LDS group for a raster of this width, within a band of `band_rows`.
+FT_HD inline uint32_t lds_rows(uint32_t width, uint32_t band_rows) {
+    const uint32_t wp = cell_stride(width);
+    uint32_t g = wp ? LDS_CELLS / wp : BAND_ROWS_MAX;
+    if (g < 1) g = 1;
+    return g < band_rows ? g : band_rows;
+}
+
 FT_HD inline void band_geometry(uint32_t width, uint32_t height, uint32_t *rows, uint32_t *n_bands, uint32_t *cells) {
     const uint32_t wp = cell_stride(width);
-    uint32_t r = wp ? LDS_CELLS / wp : BAND_ROWS_MAX;
-    if (r > BAND_ROWS_MAX) r = BAND_ROWS_MAX;
-    if (r < 1) r = 1;
+    uint32_t g = wp ? LDS_CELLS / wp : BAND_ROWS_MAX;
+    if (g > BAND_ROWS_MAX) g = BAND_ROWS_MAX;
+    if (g < 1) g = 1;
+    uint32_t r = g < BAND_ROWS_MIN ? BAND_ROWS_MIN : g;
     if (height && r > height) r = height;
+    if (g > r) g = r;
     *rows = r;
     *n_bands = height ? (height + r - 1) / r : 0;
-    *cells = r * wp;
+    *cells = g * wp;
 }
 
 // Device form of one batch path (ft_fill_desc -> PathDev); band_base is set

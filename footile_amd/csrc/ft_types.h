@@ -63,8 +63,8 @@ struct DevStatus {
     uint32_t need_events, need_pairs;
     unsigned long long op_total;  // packed (events << 32 | closes) of all ops
     uint32_t spin_fail;
-    uint32_t tickets[4];
-    uint32_t pad[3];
+    uint32_t tickets[5];
+    uint32_t pad[2];
 };
 
 struct EdgeRec {

@@ -77,20 +77,28 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
         }
     }
     // stage 1: counts + exclusive scan
-    std::vector<unsigned long long> op_off(n_ops);
+    std::vector<unsigned long long> op_off(n_ops), sub_off(n_ops * OP_LANES);
     unsigned long long tot = 0;
     for (size_t i = 0; i < n_ops; ++i) {
-        OpCount c = count_op(ops, op_path.data(), paths.data(), (uint32_t)i, mode, mode == MODE_FILL ? nullptr : lastpw.data());
-        if (c.err) return (int)c.err;
         op_off[i] = tot;
-        tot += pack_count(c);
+        uint32_t leaves = 0;
+        for (uint32_t s = 0; s < OP_LANES; ++s) {
+            OpCount c = count_op(ops, op_path.data(), paths.data(), (uint32_t)i, s, mode,
+                                 mode == MODE_FILL ? nullptr : lastpw.data());
+            if (c.err) return (int)c.err;
+            sub_off[i * OP_LANES + s] = tot - op_off[i];
+            tot += pack_count(c);
+            leaves += c.leaves;
+        }
+        if (leaves > FT_FLATTEN_MAX_LEAVES) return FT_ERANGE;
     }
     const uint32_t n_events = (uint32_t)(tot >> 32), n_closes = (uint32_t)tot;
     // stage 2: emit
     auto ev_xy = buf<I2>(mode == MODE_FILL ? n_events : 0);
     auto ev_f = buf<F4>(mode == MODE_FILL ? 0 : n_events);
     auto ev_meta = buf<uint32_t>(mode == MODE_FILL ? n_events : 0);
-    auto run_first = buf<int32_t>(n_closes), run_last = buf<int32_t>(n_closes);
+    auto run_first = buf<int32_t>(n_closes);
+    auto run_close = buf<uint32_t>(n_closes);
     auto run_path = buf<uint32_t>(n_closes), run_start = buf<uint32_t>(n_closes), run_end = buf<uint32_t>(n_closes);
     EventSink sink{};
     sink.mode = mode;
@@ -98,7 +106,7 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
     sink.ev_f = ev_f.data();
     sink.ev_meta = ev_meta.data();
     sink.run_first = run_first.data();
-    sink.run_last = run_last.data();
+    sink.run_close = run_close.data();
     sink.run_path = run_path.data();
     sink.run_start = run_start.data();
     sink.run_end = run_end.data();
@@ -106,8 +114,9 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
     sink.cap_runs = n_closes;
     uint32_t bad = 0;
     for (size_t i = 0; i < n_ops; ++i)
-        bad |= emit_op(ops, op_path.data(), paths.data(), (uint32_t)i, mode, mode == MODE_FILL ? nullptr : lastpw.data(),
-                       op_off[i], sink);
+        for (uint32_t s = 0; s < OP_LANES; ++s)
+            bad |= emit_op(ops, op_path.data(), paths.data(), (uint32_t)i, s, mode,
+                           mode == MODE_FILL ? nullptr : lastpw.data(), op_off[i] + sub_off[i * OP_LANES + s], sink);
     if (bad) return FT_EDEVICE;
     if (mode != MODE_FILL) {
         if (n_ev) *n_ev = n_events;
@@ -127,42 +136,50 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
         s.topidx = NO_IDX;
         s.offered = s.repop = s.pt_base = s.n_edges = s.stored = 0;
     }
-    for (uint32_t e = 0; e < n_events; ++e) {
-        uint32_t p = ingest_event<HostAtom>(e, L, ev_meta.data(), ev_xy.data(), run_first.data(), run_last.data(),
-                                            run_path.data(), bad);
-        if (p != NO_IDX) ps[p].offered++;
-    }
+    auto last = buf<uint32_t>(n_events);
+    for (uint32_t e = 0; e < n_events; ++e)
+        ingest_event(e, L, ev_meta.data(), ev_xy.data(), run_first.data(), run_path.data(), last.data(), bad);
+    for (uint32_t e = 1; e < n_events; ++e)  // inclusive running maximum
+        if (last[e - 1] > last[e]) last[e] = last[e - 1];
     auto kept = buf<uint32_t>(n_events), idx = buf<uint32_t>(n_events);
     uint32_t n_kept = 0;
     for (uint32_t e = 0; e < n_events; ++e) {
-        kept[e] = kept_event(e, L, ev_meta.data(), ev_xy.data(), run_first.data(), run_last.data(), bad);
+        kept[e] = kept_event(e, L, ev_meta.data(), ev_xy.data(), run_first.data(), run_close.data(), last.data(), bad);
         idx[e] = n_kept;
         n_kept += kept[e];
     }
     L.cap_points = n_kept;
-    // stage 4: cap, compaction
+    // stage 4: cap, compaction, top point
     for (uint32_t p = 0; p < n_paths; ++p)
         cap_path(p, L, paths.data(), op_off.data(), ev_meta.data(), ev_xy.data(), kept.data(), idx.data(),
                  run_first.data(), ps.data(), bad);
     auto pts = buf<I2>(n_kept);
     auto pt_run = buf<uint32_t>(n_kept);
-    for (uint32_t e = 0; e < n_events; ++e) {
-        Stored s = compact_event<HostAtom>(e, L, ev_meta.data(), ev_xy.data(), kept.data(), idx.data(),
-                                           run_first.data(), run_path.data(), run_start.data(), run_end.data(),
-                                           pts.data(), pt_run.data(), ps.data(), bad);
-        if (s.stored && s.key < ps[s.path].topkey) ps[s.path].topkey = s.key;
+    for (uint32_t e = 0; e < n_events; ++e)
+        compact_event(e, L, ev_meta.data(), ev_xy.data(), kept.data(), idx.data(), run_first.data(), run_path.data(),
+                      run_start.data(), run_end.data(), pts.data(), pt_run.data(), ps.data(), bad);
+    for (uint32_t i = 0; i < n_kept; ++i) {
+        const TopCand c = top_cand(i, L, pts.data(), pt_run.data(), run_path.data(), bad);
+        if (c.path != NO_IDX && c.key < ps[c.path].topkey) ps[c.path].topkey = c.key;
     }
     // stage 5: edges, band counts, scan, bin
     const size_t n_bands = band_path.size();
     auto edges = buf<EdgeRec>(n_kept);
     auto band_cnt = buf<uint32_t>(n_bands), band_off = buf<uint32_t>(n_bands), band_cur = buf<uint32_t>(n_bands);
     for (uint32_t e = 0; e < n_events; ++e) {
-        uint32_t p = NO_IDX;
-        uint32_t r = edge_event<HostAtom>(e, L, ev_meta.data(), kept.data(), idx.data(), run_path.data(),
-                                          run_start.data(), run_end.data(), paths.data(), pts.data(), edges.data(),
-                                          band_cnt.data(), ps.data(), &p, bad);
-        if (r & 1u) ps[p].stored++;
-        if (r & 2u) ps[p].n_edges++;
+        const EdgeOut r = edge_event<HostAtom>(e, L, ev_meta.data(), kept.data(), idx.data(), run_path.data(),
+                                               run_start.data(), run_end.data(), paths.data(), pts.data(),
+                                               edges.data(), ps.data(), bad);
+        if (r.has) ps[r.path].n_edges++;
+        if (r.banded) {
+            band_cnt[r.g0] += 1u;
+            if (r.g1 + 1 < n_bands) band_cnt[r.g1 + 1] -= 1u;
+        }
+    }
+    uint32_t acc = 0;
+    for (size_t b = 0; b < n_bands; ++b) {  // band differences -> edges per band
+        acc += band_cnt[b];
+        band_cnt[b] = acc;
     }
     uint32_t n_pairs = 0;
     for (size_t b = 0; b < n_bands; ++b) {
@@ -171,9 +188,18 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
     }
     L.n_pairs = n_pairs;
     auto band_list = buf<uint32_t>(n_pairs);
-    for (uint32_t e = 0; e < n_events; ++e)
-        bin_event<HostAtom>(e, L, ev_meta.data(), kept.data(), idx.data(), run_path.data(), paths.data(), edges.data(),
-                            band_off.data(), band_cur.data(), band_list.data(), ps.data(), bad);
+    for (uint32_t e = 0; e < n_events; ++e) {
+        const BinItem it = bin_item(e, L, ev_meta.data(), kept.data(), idx.data(), run_path.data(), paths.data(),
+                                    edges.data(), ps.data(), bad);
+        if (!it.banded) continue;
+        for (uint32_t g = it.g0; g <= it.g1; ++g) {
+            const uint32_t at = band_off[g] + band_cur[g]++;
+            if (at < n_pairs)
+                band_list[at] = it.idx;
+            else
+                bad = 1;
+        }
+    }
     if (bad) return FT_EDEVICE;
     // stage 6: raster, one band at a time (k_raster with a sequential row scan)
     for (size_t gb = 0; gb < n_bands; ++gb) {
