@@ -69,6 +69,13 @@ def _declare(L):
         "ft_memcpy_d2h": ([vp, vp, sz], i),
         "ft_memset_d": ([vp, i, sz], i),
         "ft_version": ([], ctypes.c_char_p),
+        "ft_diff_new": ([vp, pvp], i),
+        "ft_diff_free": ([vp], None),
+        "ft_diff_info": ([vp, pu64, pu64, pu64], i),
+        "ft_diff_forward": ([vp, vp, vp, vp], i),
+        "ft_diff_loss": ([vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)], i),
+        "ft_diff_backward": ([vp, vp, vp, vp, vp, vp], i),
+        "ft_diff_step": ([vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -166,6 +166,45 @@ ft_status ft_memcpy_h2d(void *dst, const void *src, size_t bytes);
 ft_status ft_memcpy_d2h(void *dst, const void *src, size_t bytes);
 ft_status ft_memset_d(void *dst, int value, size_t bytes);
 
+/* ------------------------------------------------------------------------
+ * Differentiable fill.  No footile counterpart: footile's fill is not
+ * differentiable.  A float32 analytic-coverage forward and a hand-written
+ * adjoint over a fixed scene layout, for callers that optimise path control
+ * points and colours (the forward/backward pair of an autograd function).
+ *
+ * Layout: n_images images of width x height premultiplied RGBA float32
+ * pixels; n_paths paths in composite order (path_image non-decreasing), path
+ * k having path_cubics[k] >= 1 cubics.  Parameters (device float32):
+ * points = per path (1 + 3 n) (x, y) pairs back to back (start point, then
+ * 3 control points per cubic); colours = [n_paths][4] premultiplied RGBA.
+ * Each cubic is flattened at t = j/16 (j = 1..16) and the outline closes back
+ * to the start point; a pixel's coverage is min(|A|, 1), A the integral of
+ * the winding number over the pixel square; paths composite SrcOver in order
+ * over a transparent canvas.  canvas / target / grad_canvas are
+ * [n_images][height][width][4].  Every call is synchronous.
+ * ------------------------------------------------------------------------ */
+typedef struct ft_diff ft_diff;
+typedef struct ft_diff_layout {
+    uint32_t n_images, width, height;
+    uint32_t n_paths;
+    const uint32_t *path_image;  /* host [n_paths] */
+    const uint32_t *path_cubics; /* host [n_paths] */
+} ft_diff_layout;
+ft_status ft_diff_new(const ft_diff_layout *layout, ft_diff **out);
+void ft_diff_free(ft_diff *d);
+ft_status ft_diff_info(const ft_diff *d, uint64_t *n_points, uint64_t *n_paths, uint64_t *canvas_floats);
+/* canvas = render(points, colours) */
+ft_status ft_diff_forward(ft_diff *d, const float *points, const float *colours, float *canvas);
+/* loss = mean((canvas - target)^2); grad_canvas (may be NULL) = d loss / d canvas */
+ft_status ft_diff_loss(ft_diff *d, const float *canvas, const float *target, float *grad_canvas, double *loss);
+/* Vector-Jacobian product: grad_canvas -> d/d points, d/d colours.  Coverage
+ * is recomputed, never stored per path. */
+ft_status ft_diff_backward(ft_diff *d, const float *points, const float *colours, const float *grad_canvas,
+                           float *grad_points, float *grad_colours);
+/* forward + loss + backward (one optimisation step's gradient) */
+ft_status ft_diff_step(ft_diff *d, const float *points, const float *colours, const float *target, float *canvas,
+                       float *grad_canvas, float *grad_points, float *grad_colours, double *loss);
+
 /* Library version string. */
 const char *ft_version(void);
 

@@ -41,16 +41,18 @@ def test_library_exports_every_declared_symbol(built_lib):
 
 def test_struct_layouts_match_c(tmp_path):
     from footile_amd._lib import FillDescC
+    from footile_amd.diff import _Layout
     from footile_amd.path import OP_DTYPE
     src = tmp_path / "layout.c"
     src.write_text(f'''#include <stdio.h>
 #include <stddef.h>
 #include "{HEADER}"
 int main(void) {{
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ft_path_op), sizeof(ft_fill_desc),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ft_path_op), sizeof(ft_fill_desc),
          offsetof(ft_fill_desc, dst), offsetof(ft_fill_desc, fmt), offsetof(ft_fill_desc, clr),
          offsetof(ft_fill_desc, transform), offsetof(ft_fill_desc, tolerance),
-         offsetof(ft_fill_desc, pen_width), offsetof(ft_fill_desc, fresh));
+         offsetof(ft_fill_desc, pen_width), offsetof(ft_fill_desc, fresh), sizeof(ft_diff_layout),
+         offsetof(ft_diff_layout, path_image), offsetof(ft_diff_layout, path_cubics));
   return 0;
 }}
 ''')
@@ -60,7 +62,8 @@ int main(void) {{
                                           check=True).stdout.split()]
     want = [OP_DTYPE.itemsize, ctypes.sizeof(FillDescC), FillDescC.dst.offset, FillDescC.fmt.offset,
             FillDescC.clr.offset, FillDescC.transform.offset, FillDescC.tolerance.offset,
-            FillDescC.pen_width.offset, FillDescC.fresh.offset]
+            FillDescC.pen_width.offset, FillDescC.fresh.offset, ctypes.sizeof(_Layout), _Layout.path_image.offset,
+            _Layout.path_cubics.offset]
     assert got == want
 
 

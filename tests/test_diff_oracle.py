@@ -1,0 +1,170 @@
+"""Pins the FP64 oracle of the differentiable fill (oracle/diff_oracle.c).
+
+No footile test covers this path (footile has none).  The oracle is checked
+against what the mathematics fixes:
+  * coverage: the winding-number integral over a pixel equals the signed area
+    of the polygon clipped to the pixel square (Sutherland-Hodgman against a
+    convex window, shoelace area) -- an independent FP64 computation;
+    axis-aligned rectangles give the product of overlap lengths;
+  * compositing: a covering opaque path leaves exactly its colour;
+  * gradients: central finite differences of the FP64 forward, for the loss
+    and for a general vector-Jacobian product.
+"""
+import numpy as np
+import pytest
+
+from footile_amd import diffscene as ds
+
+
+def _clip_area(poly, x0, y0, x1, y1):
+    """Signed area of polygon `poly` ((n, 2)) clipped to [x0, x1] x [y0, y1]."""
+    def clip(pts, inside, cut):
+        out = []
+        n = len(pts)
+        for i in range(n):
+            a, b = pts[i], pts[(i + 1) % n]
+            ia, ib = inside(a), inside(b)
+            if ia:
+                out.append(a)
+            if ia != ib:
+                out.append(cut(a, b))
+        return out
+
+    def cut_x(xc):
+        return lambda a, b: (xc, a[1] + (b[1] - a[1]) * (xc - a[0]) / (b[0] - a[0]))
+
+    def cut_y(yc):
+        return lambda a, b: (a[0] + (b[0] - a[0]) * (yc - a[1]) / (b[1] - a[1]), yc)
+
+    pts = [tuple(p) for p in poly]
+    for inside, cut in ((lambda p: p[0] >= x0, cut_x(x0)), (lambda p: p[0] <= x1, cut_x(x1)),
+                        (lambda p: p[1] >= y0, cut_y(y0)), (lambda p: p[1] <= y1, cut_y(y1))):
+        pts = clip(pts, inside, cut)
+        if not pts:
+            return 0.0
+    a = 0.0
+    for i in range(len(pts)):
+        (xa, ya), (xb, yb) = pts[i], pts[(i + 1) % len(pts)]
+        a += xa * yb - xb * ya
+    return 0.5 * a
+
+
+def _line_cubics(corners):
+    """A closed polygon as cubics with collinear control points (exactly linear)."""
+    pts = [corners[0]]
+    for i in range(len(corners)):
+        a = np.asarray(corners[i], float)
+        b = np.asarray(corners[(i + 1) % len(corners)], float)
+        pts += [tuple(a + (b - a) / 3.0), tuple(a + 2.0 * (b - a) / 3.0), tuple(b)]
+    return np.asarray(pts, float)
+
+
+def _scene(pd, n_images, w, h, image, cubics):
+    return pd.Scene(n_images, w, h, np.asarray(image, np.uint32), np.asarray(cubics, np.uint32))
+
+
+def test_rectangle_coverage_is_overlap_area(oracle):
+    from oracle import pydiff as pd
+    rect = (2.25, 1.5, 9.75, 6.125)
+    pts = _line_cubics([(rect[0], rect[1]), (rect[2], rect[1]), (rect[2], rect[3]), (rect[0], rect[3])])
+    s = _scene(pd, 1, 12, 8, [0], [4])
+    for y in range(8):
+        for x in range(12):
+            ox = max(0.0, min(x + 1, rect[2]) - max(x, rect[0]))
+            oy = max(0.0, min(y + 1, rect[3]) - max(y, rect[1]))
+            a = s.pixel_area(pts, 0, x, y)
+            assert abs(abs(a) - ox * oy) < 1e-12, (x, y, a, ox * oy)
+
+
+def test_coverage_equals_clipped_signed_area(oracle):
+    """Random (often self-intersecting) cubic paths: the winding integral."""
+    from oracle import pydiff as pd
+    rng = np.random.default_rng(5)
+    for t in range(12):
+        n = int(rng.integers(1, 4))
+        pts = rng.uniform(-2.0, 14.0, size=(1 + 3 * n, 2))
+        s = _scene(pd, 1, 12, 12, [0], [n])
+        poly = s.polygon(pts, 0)
+        for _ in range(25):
+            x, y = int(rng.integers(0, 12)), int(rng.integers(0, 12))
+            want = _clip_area(poly, x, y, x + 1, y + 1)
+            got = s.pixel_area(pts, 0, x, y)
+            # the polygon's orientation convention: A counts descending edges positive
+            assert abs(abs(got) - abs(want)) < 1e-9 and (abs(want) < 1e-12 or np.sign(got) == -np.sign(want)), \
+                (t, x, y, got, want)
+
+
+def test_opaque_cover_leaves_its_colour(oracle):
+    from oracle import pydiff as pd
+    pts = _line_cubics([(-1.0, -1.0), (9.0, -1.0), (9.0, 7.0), (-1.0, 7.0)])
+    s = _scene(pd, 1, 8, 6, [0], [4])
+    col = np.array([[0.25, 0.5, 0.125, 1.0]])
+    _, canvas = s.forward(pts, col)
+    assert np.allclose(canvas, col[0], atol=1e-15)
+
+
+def _small_scene(pd, rng, n_images=2, w=16, h=12, paths_per_image=3):
+    image, cubics = [], []
+    for i in range(n_images):
+        for _ in range(paths_per_image):
+            image.append(i)
+            cubics.append(int(rng.integers(1, 3)))
+    s = _scene(pd, n_images, w, h, image, cubics)
+    pts = rng.uniform(-1.5, [w + 1.5, h + 1.5], size=(s.n_points, 2))
+    a = rng.uniform(0.3, 0.9, size=len(image))
+    col = np.stack([a * rng.uniform(0, 1, len(image)) for _ in range(3)] + [a], axis=1)
+    return s, pts, col
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_loss_gradient_matches_central_differences(oracle, seed):
+    from oracle import pydiff as pd
+    rng = np.random.default_rng(seed)
+    s, pts, col = _small_scene(pd, rng)
+    target = rng.uniform(0, 1, size=(s.n_images, s.height, s.width, 4))
+    loss, gp, gc = s.backward(pts, col, target)
+    assert abs(loss - s.forward(pts, col, target)[0]) < 1e-14
+    h = 1e-6
+    for k in rng.choice(pts.size, size=24, replace=False):
+        p1, p2 = pts.copy().reshape(-1), pts.copy().reshape(-1)
+        p1[k] += h
+        p2[k] -= h
+        fd = (s.forward(p1, col, target)[0] - s.forward(p2, col, target)[0]) / (2 * h)
+        an = gp.reshape(-1)[k]
+        assert abs(fd - an) <= 1e-6 * max(1.0, np.abs(gp).max()) + 1e-4 * abs(an), (k, fd, an)
+    for k in range(col.size):
+        c1, c2 = col.copy().reshape(-1), col.copy().reshape(-1)
+        c1[k] += h
+        c2[k] -= h
+        fd = (s.forward(pts, c1, target)[0] - s.forward(pts, c2, target)[0]) / (2 * h)
+        an = gc.reshape(-1)[k]
+        assert abs(fd - an) <= 1e-8 + 1e-5 * abs(an), (k, fd, an)
+
+
+def test_vjp_matches_central_differences(oracle):
+    from oracle import pydiff as pd
+    rng = np.random.default_rng(21)
+    s, pts, col = _small_scene(pd, rng, n_images=1, w=14, h=10, paths_per_image=4)
+    G = rng.normal(size=(1, s.height, s.width, 4))
+    _, gp, gc = s.backward(pts, col, grad_canvas=G)
+
+    def f(p, c):
+        return float(np.sum(G * s.forward(p, c)[1]))
+
+    h = 1e-6
+    for k in rng.choice(pts.size, size=20, replace=False):
+        p1, p2 = pts.copy().reshape(-1), pts.copy().reshape(-1)
+        p1[k] += h
+        p2[k] -= h
+        fd = (f(p1, col) - f(p2, col)) / (2 * h)
+        an = gp.reshape(-1)[k]
+        assert abs(fd - an) <= 1e-6 * max(1.0, np.abs(gp).max()) + 1e-4 * abs(an), (k, fd, an)
+
+
+def test_scene_generators_are_deterministic():
+    a = ds.scene_c4(seed=3, width=64, height=48, n_paths=8)
+    b = ds.scene_c4(seed=3, width=64, height=48, n_paths=8)
+    assert np.array_equal(a.points, b.points) and np.array_equal(a.colours, b.colours)
+    c = ds.scene_c5(seed=4, n_images=3, size=32, min_paths=4, max_paths=20)
+    assert c.n_images == 3 and c.points.shape == (int(np.sum(1 + 3 * c.path_cubics)), 2)
+    assert np.all(np.diff(c.path_image.astype(np.int64)) >= 0)

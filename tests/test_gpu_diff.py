@@ -1,0 +1,141 @@
+"""Differentiable fill on the GPU against the FP64 oracle (oracle/diff_oracle.c,
+pinned by tests/test_diff_oracle.py).
+
+Tolerances: the forward canvas is float32, within 2e-5 of FP64 per channel;
+gradients are compared normwise, ||g - g64|| / ||g64|| <= 1e-4 (the bound the
+differentiable path is specified to), per parameter group.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _scene(seed, n_images=2, w=40, h=32, paths=6, cubics=(1, 4)):
+    from footile_amd.diffscene import DiffScene
+    rng = np.random.default_rng(seed)
+    image, cub = [], []
+    for i in range(n_images):
+        for _ in range(paths):
+            image.append(i)
+            cub.append(int(rng.integers(*cubics)))
+    image = np.asarray(image, np.uint32)
+    cub = np.asarray(cub, np.uint32)
+    n_pts = int(np.sum(1 + 3 * cub.astype(np.int64)))
+    pts = rng.uniform(-3.0, [w + 3.0, h + 3.0], size=(n_pts, 2)).astype(np.float32)
+    a = rng.uniform(0.3, 0.9, size=len(image))
+    col = np.stack([a * rng.uniform(0, 1, len(image)) for _ in range(3)] + [a], axis=1).astype(np.float32)
+    return DiffScene(n_images, w, h, image, cub, pts, col)
+
+
+def _oracle_scene(sc):
+    from oracle import pydiff as pd
+    return pd.Scene(sc.n_images, sc.width, sc.height, sc.path_image, sc.path_cubics)
+
+
+def _gpu(sc):
+    from footile_amd.diff import DiffRenderer, device_array
+    r = DiffRenderer.for_scene(sc)
+    return r, device_array(sc.points), device_array(sc.colours)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_forward_matches_fp64(oracle, seed):
+    sc = _scene(seed)
+    r, P, C = _gpu(sc)
+    canvas = r.new_canvas()
+    r.forward(P, C, canvas)
+    got = canvas.download(dtype=np.float32).reshape(r.canvas_shape)
+    _, want = _oracle_scene(sc).forward(sc.points, sc.colours)
+    assert np.abs(got - want).max() < 2e-5
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_loss_gradient_matches_fp64(oracle, seed):
+    from footile_amd.diff import device_array
+    from footile_amd.plotter import DeviceBuffer
+    sc = _scene(seed)
+    r, P, C = _gpu(sc)
+    target = np.random.default_rng(seed + 100).uniform(0, 1, size=r.canvas_shape).astype(np.float32)
+    T = device_array(target)
+    canvas, gcan = r.new_canvas(), r.new_canvas()
+    gp, gc = DeviceBuffer(8 * r.n_points), DeviceBuffer(16 * r.n_paths)
+    loss = r.step(P, C, T, canvas, gcan, gp, gc)
+    want_loss, want_gp, want_gc = _oracle_scene(sc).backward(sc.points, sc.colours, target)
+    assert abs(loss - want_loss) <= 1e-5 * abs(want_loss)
+    got_gp = gp.download(dtype=np.float32).reshape(-1, 2)
+    got_gc = gc.download(dtype=np.float32).reshape(-1, 4)
+    assert _rel(got_gp, want_gp) < 1e-4, _rel(got_gp, want_gp)
+    assert _rel(got_gc, want_gc) < 1e-4, _rel(got_gc, want_gc)
+
+
+def test_vjp_matches_fp64(oracle):
+    from footile_amd.diff import DiffFill, device_array
+    sc = _scene(9, n_images=1, w=48, h=40, paths=8)
+    r, P, C = _gpu(sc)
+    G = np.random.default_rng(3).normal(size=r.canvas_shape).astype(np.float32)
+    canvas, ctx = DiffFill.apply(r, P, C)
+    _, gp, gc = DiffFill.backward(ctx, device_array(G))
+    _, want_gp, want_gc = _oracle_scene(sc).backward(sc.points, sc.colours, grad_canvas=G)
+    assert _rel(gp.download(dtype=np.float32).reshape(-1, 2), want_gp) < 1e-4
+    assert _rel(gc.download(dtype=np.float32).reshape(-1, 4), want_gc) < 1e-4
+
+
+def test_c4_shape_reduced(oracle):
+    """The C4 scene law (8-cubic paths across the canvas) at 128x128, 32 paths."""
+    from footile_amd import diffscene as ds
+    from footile_amd.diff import device_array
+    from footile_amd.plotter import DeviceBuffer
+    sc = ds.scene_c4(seed=3, width=128, height=128, n_paths=32)
+    tp, tc = ds.perturbed(sc)
+    _, target = _oracle_scene(sc).forward(tp, tc)
+    target = target.astype(np.float32)
+    r, P, C = _gpu(sc)
+    T = device_array(target)
+    canvas, gcan = r.new_canvas(), r.new_canvas()
+    gp, gc = DeviceBuffer(8 * r.n_points), DeviceBuffer(16 * r.n_paths)
+    r.step(P, C, T, canvas, gcan, gp, gc)
+    _, want_gp, want_gc = _oracle_scene(sc).backward(sc.points, sc.colours, target)
+    assert _rel(gp.download(dtype=np.float32).reshape(-1, 2), want_gp) < 1e-4
+    assert _rel(gc.download(dtype=np.float32).reshape(-1, 4), want_gc) < 1e-4
+
+
+def test_step_equals_forward_loss_backward(oracle):
+    from footile_amd.diff import device_array
+    from footile_amd.plotter import DeviceBuffer
+    sc = _scene(12)
+    r, P, C = _gpu(sc)
+    T = device_array(np.full(r.canvas_shape, 0.25, np.float32))
+    c1, g1 = r.new_canvas(), r.new_canvas()
+    gp1, gc1 = DeviceBuffer(8 * r.n_points), DeviceBuffer(16 * r.n_paths)
+    l1 = r.step(P, C, T, c1, g1, gp1, gc1)
+    c2, g2 = r.new_canvas(), r.new_canvas()
+    r.forward(P, C, c2)
+    l2 = r.loss(c2, T, g2)
+    gp2, gc2 = DeviceBuffer(8 * r.n_points), DeviceBuffer(16 * r.n_paths)
+    r.backward(P, C, g2, gp2, gc2)
+    assert l1 == l2
+    assert np.array_equal(c1.download(), c2.download())
+    a, b = gp1.download(dtype=np.float32), gp2.download(dtype=np.float32)
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-7 * np.abs(b).max())
+
+
+def test_empty_image_and_bad_layout(oracle):
+    from footile_amd._lib import FootileError
+    from footile_amd.diff import DiffRenderer, device_array
+    # image 1 has no paths: its canvas stays transparent
+    r = DiffRenderer(2, 16, 8, np.array([0], np.uint32), np.array([2], np.uint32))
+    pts = np.array([[1, 1], [10, 1], [12, 6], [3, 7], [2, 3], [9, 2], [1, 1]], np.float32)
+    col = np.array([[0.2, 0.1, 0.3, 0.5]], np.float32)
+    canvas = r.new_canvas()
+    r.forward(device_array(pts), device_array(col), canvas)
+    got = canvas.download(dtype=np.float32).reshape(r.canvas_shape)
+    assert np.all(got[1] == 0.0) and got[0].max() > 0
+    with pytest.raises(FootileError):
+        DiffRenderer(1, 16, 8, np.array([1], np.uint32), np.array([2], np.uint32))  # image out of range
+    with pytest.raises(FootileError):
+        DiffRenderer(1, 16, 8, np.array([0], np.uint32), np.array([0], np.uint32))  # no cubics
