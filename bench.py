@@ -67,18 +67,29 @@ def bench_atlas(ft, steps, warmup):
     descs = [FillDesc(int(b), int(c), atlas.ptr + int(y) * side + int(x), int(s), int(s), side,
                       fresh=True) for b, c, x, y, s in tiles]
     batch = Batch(ops, descs)
+    from footile_amd.tune import autotune
+    tuned = autotune(batch, descs, len(ops), cache=None)
     batch.run()
-    for _ in range(warmup):
+    # raster-kernel time from eager launches (kernels are not timed inside a graph)
+    batch.set_timing(True)
+    batch.timing()
+    for _ in range(max(2, steps // 2)):
+        batch.launch()
+    batch.sync()
+    eager_total_ms, raster_ms, runs = batch.timing()
+    # the step: one replay of the captured launch graph
+    batch.set_graph(True)
+    for _ in range(warmup + 1):
         batch.launch()
         batch.sync()
-    batch.set_timing(True)
     batch.timing()
     t0 = time.perf_counter()
     for _ in range(steps):
         batch.launch()
     batch.sync()
     dt = (time.perf_counter() - t0) / steps
-    total_ms, raster_ms, runs = batch.timing()
+    total_ms, _, graph_runs = batch.timing()
+    batch.set_graph(False)
     tile_px = int((tiles[:, 4] ** 2).sum())
     bytes_per = tile_px + op_bytes(ops)
     offered, stored, edges = batch.counts()
@@ -97,11 +108,15 @@ def bench_atlas(ft, steps, warmup):
     raster_avg = raster_ms / max(runs, 1)
     return {
         "workload": "atlas: 16,384 paths, lognormal(3.5,1) segments, 60/30/10 cubic/quad/line, "
-                    "tiles 32..256 in a 16384^2 Matte8 atlas, one launch",
+                    "tiles 32..256 in a 16384^2 Matte8 atlas, one graph launch, autotuned raster geometry",
         "paths_per_s": len(tiles) / dt,
         "mpix_per_s": tile_px / dt / 1e6,
         "ms_per_step": dt * 1e3,
-        "device_ms_per_step": total_ms / max(runs, 1),
+        "device_ms_per_step": total_ms / max(graph_runs, 1),
+        "eager_device_ms_per_step": eager_total_ms / max(runs, 1),
+        "launch": "hipGraph replay of the whole launch sequence",
+        "tuning": {"band_rows_min": tuned["band_rows_min"], "lds_cells": tuned["lds_cells"],
+                   "candidates": tuned["tried"], "tune_s": round(tuned.get("tune_s", 0.0), 3)},
         "raster_kernel_ms": raster_avg,
         "raster_kernel_gbs": bytes_per / (raster_avg * 1e-3) / 1e9 if raster_avg > 0 else None,
         "edges_per_pixel": float(edges.sum()) / tile_px,
@@ -136,13 +151,118 @@ def bench_plain(ft, reps=50):
     return out
 
 
+def _grad_rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _diff_buffers(r, scene, target_params=None):
+    """Device parameters, a target rendered on the GPU from perturbed
+    parameters, and output buffers for one differentiable scene."""
+    from footile_amd.diff import device_array
+    from footile_amd.diffscene import perturbed
+    from footile_amd.plotter import DeviceBuffer
+    P, C = device_array(scene.points), device_array(scene.colours)
+    tp, tc = perturbed(scene) if target_params is None else target_params
+    T = r.new_canvas()
+    r.forward(device_array(tp), device_array(tc), T)
+    bufs = dict(P=P, C=C, T=T, canvas=r.new_canvas(), gcan=r.new_canvas(), gp=DeviceBuffer(8 * r.n_points),
+                gc=DeviceBuffer(16 * r.n_paths))
+    return bufs
+
+
+def _diff_time(r, b, steps, warmup):
+    for _ in range(warmup):
+        r.step(b["P"], b["C"], b["T"], b["canvas"], b["gcan"], b["gp"], b["gc"])
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = r.step(b["P"], b["C"], b["T"], b["canvas"], b["gcan"], b["gp"], b["gc"])
+    return (time.perf_counter() - t0) / steps, loss
+
+
+def bench_diff(steps, warmup, check=True):
+    """Differentiable configs: C4 (one 512^2 image, 256 paths x 8 cubics) and
+    C5 (64 images of 256^2, U[32,512] paths x U[4,16] cubics), fwd + loss +
+    bwd per step; gradients against the FP64 oracle."""
+    from footile_amd import diffscene as ds
+    from footile_amd.diff import DiffRenderer
+    out = {}
+    # C4
+    sc = ds.scene_c4()
+    r = DiffRenderer.for_scene(sc)
+    b = _diff_buffers(r, sc)
+    dt, loss = _diff_time(r, b, steps, warmup)
+    c4 = {"workload": "C4: 512x512 fp32 RGBA, 256 closed paths x 8 cubics (seed 3), L2 to a target rendered "
+                      "from params + N(0,2px) / colours + N(0,0.05)",
+          "grad_steps_per_s": 1.0 / dt, "ms_per_step": dt * 1e3, "loss": loss}
+    if check:
+        from oracle import pydiff as pd
+        target = b["T"].download(dtype=np.float32).reshape(r.canvas_shape)
+        t0 = time.perf_counter()
+        want_loss, want_gp, want_gc = pd.Scene(sc.n_images, sc.width, sc.height, sc.path_image,
+                                               sc.path_cubics).backward(sc.points, sc.colours, target)
+        c4["fp64_oracle_s"] = time.perf_counter() - t0
+        gp = b["gp"].download(dtype=np.float32).reshape(-1, 2)
+        gc = b["gc"].download(dtype=np.float32).reshape(-1, 4)
+        c4["grad_rel_err_points"] = _grad_rel(gp, want_gp)
+        c4["grad_rel_err_colours"] = _grad_rel(gc, want_gc)
+        c4["grad_max_abs_err_over_max_points"] = float(np.abs(gp - want_gp).max() / np.abs(want_gp).max())
+        c4["loss_rel_err"] = abs(loss - want_loss) / abs(want_loss)
+    out["c4"] = c4
+    # C5: the batched launch vs one launch per image
+    sc = ds.scene_c5()
+    r = DiffRenderer.for_scene(sc)
+    b = _diff_buffers(r, sc)
+    dt, loss = _diff_time(r, b, max(2, steps // 2), 1)
+    c5 = {"workload": "C5: 64 images 256x256 fp32 RGBA, U[32,512] paths x U[4,16] cubics (seed 4), one launch",
+          "grad_steps_per_s": 1.0 / dt, "ms_per_step": dt * 1e3, "paths": int(sc.n_paths)}
+    # per-image launches of the same scenes
+    per_image = 0.0
+    tp, tc = ds.perturbed(sc)
+    first = np.searchsorted(sc.path_image, np.arange(sc.n_images + 1))
+    pt_first = np.concatenate([[0], np.cumsum(1 + 3 * sc.path_cubics.astype(np.int64))])
+    subs = []
+    for i in range(sc.n_images):
+        k0, k1 = int(first[i]), int(first[i + 1])
+        sub = ds.DiffScene(1, sc.width, sc.height, np.zeros(k1 - k0, np.uint32), sc.path_cubics[k0:k1],
+                           sc.points[pt_first[k0]:pt_first[k1]], sc.colours[k0:k1])
+        rs = DiffRenderer.for_scene(sub)
+        bs = _diff_buffers(rs, sub, (tp[pt_first[k0]:pt_first[k1]], tc[k0:k1]))
+        subs.append((rs, bs, sub, k0, k1))
+    for rs, bs, *_ in subs:
+        rs.step(bs["P"], bs["C"], bs["T"], bs["canvas"], bs["gcan"], bs["gp"], bs["gc"])
+    reps = max(1, steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for rs, bs, *_ in subs:
+            rs.step(bs["P"], bs["C"], bs["T"], bs["canvas"], bs["gcan"], bs["gp"], bs["gc"])
+    per_image = (time.perf_counter() - t0) / reps
+    c5["per_image_launch_ms_per_step"] = per_image * 1e3
+    c5["speedup_vs_per_image_launch"] = per_image / dt
+    if check:
+        from oracle import pydiff as pd
+        gp_all = b["gp"].download(dtype=np.float32).reshape(-1, 2)
+        gc_all = b["gc"].download(dtype=np.float32).reshape(-1, 4)
+        target = b["T"].download(dtype=np.float32).reshape(r.canvas_shape)
+        errs_p, errs_c = [], []
+        for i, (rs, bs, sub, k0, k1) in enumerate(subs[:2]):  # two images against FP64 (own params each)
+            _, want_gp, want_gc = pd.Scene(1, sub.width, sub.height, sub.path_image, sub.path_cubics).backward(
+                sub.points, sub.colours, target[i:i + 1])
+            errs_p.append(_grad_rel(gp_all[pt_first[k0]:pt_first[k1]], want_gp))
+            errs_c.append(_grad_rel(gc_all[k0:k1], want_gc))
+        c5["grad_rel_err_points_images_0_1"] = max(errs_p)
+        c5["grad_rel_err_colours_images_0_1"] = max(errs_c)
+    out["c5"] = c5
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-extra", action="store_true", help="skip the atlas and plain measurements")
+    ap.add_argument("--no-extra", action="store_true", help="skip the atlas, plain and differentiable measurements")
+    ap.add_argument("--no-diff-check", action="store_true", help="skip the FP64 gradient checks")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -215,6 +335,8 @@ def main():
         extra["atlas"] = bench_atlas(ft, max(3, args.steps // 4), 1)
         print("[bench] plain 256", file=sys.stderr, flush=True)
         extra["plain_256"] = bench_plain(ft)
+        print("[bench] differentiable C4 / C5", file=sys.stderr, flush=True)
+        extra["diff"] = bench_diff(max(4, args.steps // 2), 2, check=not args.no_diff_check)
 
     cpu_px = cpu_steps * 2 * W * (H - max(int(tops[0]), 0))
     line = {

@@ -61,6 +61,9 @@ def _declare(L):
         "ft_batch_run": ([vp], i),
         "ft_batch_counts": ([vp, vp, vp, vp, vp], i),
         "ft_batch_stream": ([vp, pvp], i),
+        "ft_batch_set_tuning": ([vp, vp], i),
+        "ft_batch_get_tuning": ([vp, vp], i),
+        "ft_batch_set_graph": ([vp, i], i),
         "ft_batch_set_timing": ([vp, i], i),
         "ft_batch_timing": ([vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), pu64], i),
         "ft_device_alloc": ([sz, pvp], i),

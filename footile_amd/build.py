@@ -30,7 +30,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math
               f"-I{os.path.join(ROOT, 'include')}"]
 
 HIP_SOURCES = ["ft_pipeline.hip", "ft_diff.hip", "ft_api.cpp"]
-HOST_SOURCES = ["ft_stroke.cpp"]
+HOST_SOURCES = []
 
 
 def _run(cmd, verbose):

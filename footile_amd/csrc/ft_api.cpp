@@ -15,7 +15,6 @@
 
 #include "../../include/footile_hip.h"
 #include "ft_pipeline.h"
-#include "ft_stroke.h"
 
 using ft::Workspace;
 
@@ -37,6 +36,8 @@ struct ft_plotter {
 struct ft_batch {
     Workspace ws;
     size_t n_paths = 0;
+    std::vector<ft_path_op> ops;  // host copies, for a re-layout after a tuning change
+    std::vector<ft_fill_desc> descs;
 };
 
 namespace {
@@ -174,34 +175,24 @@ ft_status ft_plotter_fill(ft_plotter *p, ft_fill_rule rule, const ft_path_op *op
     return s;
 }
 
+// Plotter::stroke (plotter.rs:356-365): the ops are flattened with widths,
+// the stroker stages build the outline, and the outline is filled NonZero --
+// all in one device run.
 ft_status ft_plotter_stroke(ft_plotter *p, const ft_path_op *ops, size_t n, const uint8_t clr[4]) {
     if (!p || !valid_ops(ops, n)) return FT_EINVAL;
     if (n > 0x7FFFFFF0u) return FT_EINVAL;
-    // 1. device flattening with widths (plotter.rs:356-362 add_ops into a Stroke)
     ft_fill_desc d = plotter_desc(p, FT_NON_ZERO, n, clr);
-    ft_status s = ft::ws_set_batch(p->ws, ops, n, &d, 1);
+    ft_status s = ft::ws_set_batch(p->ws, ops, n, &d, 1, ft::PATH_WHOLE);
     if (s) return s;
-    s = ft::ws_run(p->ws, ft::MODE_STROKE_FLATTEN, false);
+    p->ws.join = (uint32_t)p->join;
+    p->ws.miter = p->miter;
+    p->ws.stroke_tol_sq = p->tolerance * p->tolerance;
+    s = ft::ws_run(p->ws, ft::MODE_STROKE, true);
     if (s) return s;
-    const uint32_t n_ev = p->ws.h_status->n_events;
-    std::vector<ft::F4> ev(n_ev);
-    if (n_ev && hipMemcpy(ev.data(), p->ws.d_ev_f, n_ev * sizeof(ft::F4), hipMemcpyDeviceToHost) != hipSuccess)
-        return FT_EDEVICE;
-    // 2. host stroker: Stroke::add_point / close / path_ops
-    const float tol = p->tolerance;
-    ft::Stroker st(p->join, p->miter, tol * tol);
-    for (const ft::F4 &e : ev) {
-        if (e.w == 0.0f)
-            st.add_point(ft::WidePoint{e.x, e.y, e.z});
-        else
-            st.close(e.w == 2.0f);
-    }
-    std::vector<ft_path_op> outline;
-    s = st.path_ops(outline);
-    if (s) return s;
+    p->last_offered = p->ws.h_pstate[0].offered;
+    p->last_stored = p->ws.h_pstate[0].stored;
     p->s_width = final_pen_width(ops, n, p->s_width);
-    // 3. device fill of the outline, NonZero (plotter.rs:364)
-    return plotter_fill_ops(p, FT_NON_ZERO, outline.data(), outline.size(), clr);
+    return FT_OK;
 }
 
 ft_status ft_plotter_read(const ft_plotter *p, void *dst, size_t bytes) {
@@ -306,7 +297,40 @@ ft_status ft_batch_new(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *
         return s;
     }
     b->n_paths = n_paths;
+    b->ops.assign(ops, ops + n_ops);
+    b->descs.assign(paths, paths + n_paths);
     *out = b;
+    return FT_OK;
+}
+
+ft_status ft_batch_set_tuning(ft_batch *b, const ft_tuning *t) {
+    if (!b || !t) return FT_EINVAL;
+    if (t->band_rows_min < 1 || t->band_rows_min > 256 || t->lds_cells < 1024 || t->lds_cells > 16384 ||
+        t->lds_cells % ft::CELL_ALIGN)
+        return FT_EINVAL;
+    ft::Tuning tu;
+    tu.band_rows_min = t->band_rows_min;
+    tu.lds_cells = t->lds_cells;
+    const ft::Tuning old = b->ws.tuning;
+    b->ws.tuning = tu;
+    const ft_status s = ft::ws_set_batch(b->ws, b->ops.data(), b->ops.size(), b->descs.data(), b->descs.size());
+    if (s) {
+        b->ws.tuning = old;
+        (void)ft::ws_set_batch(b->ws, b->ops.data(), b->ops.size(), b->descs.data(), b->descs.size());
+    }
+    return s;
+}
+
+ft_status ft_batch_get_tuning(const ft_batch *b, ft_tuning *t) {
+    if (!b || !t) return FT_EINVAL;
+    t->band_rows_min = b->ws.tuning.band_rows_min;
+    t->lds_cells = b->ws.tuning.lds_cells;
+    return FT_OK;
+}
+
+ft_status ft_batch_set_graph(ft_batch *b, int on) {
+    if (!b) return FT_EINVAL;
+    b->ws.graph = on != 0;
     return FT_OK;
 }
 

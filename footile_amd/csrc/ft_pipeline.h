@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/footile_hip.h"
+#include "ft_stages.h"
 #include "ft_types.h"
 
 namespace ft {
@@ -51,6 +52,15 @@ struct Workspace {
     I2 *d_pts = nullptr;
     uint32_t *d_pt_run = nullptr;
     EdgeRec *d_edges = nullptr;
+    // stroker stages (MODE_STROKE), sized by cap_events
+    size_t cap_stroke = 0;
+    unsigned long long *d_st_flags = nullptr, *d_st_off = nullptr;
+    uint32_t *d_st_last = nullptr;
+    F4 *d_spts = nullptr;
+    uint32_t *d_sub_start = nullptr, *d_sub_event = nullptr, *d_sub_n = nullptr, *d_sub_joined = nullptr;
+    unsigned long long *d_it_cnt = nullptr, *d_it_off = nullptr;  // 4 cap_events + 1 items
+    uint32_t join = 0;  // ft_join
+    float miter = 4.0f, stroke_tol_sq = 0.09f;
     // bands
     uint32_t *d_band_cnt = nullptr, *d_band_off = nullptr, *d_band_cur = nullptr;
     size_t cap_pairs = 0;
@@ -65,6 +75,11 @@ struct Workspace {
     DevStatus *h_status = nullptr;  // pinned
     std::vector<PathState> h_pstate;
     bool edges_counted = false;  // PathState::n_edges holds the last run's counts
+    // band / LDS geometry of the next ws_set_batch
+    Tuning tuning;
+    // graph mode: fill runs replay a captured graph of the launch sequence
+    bool graph = false;
+    hipGraphExec_t gexec = nullptr;
     // optional event timing of whole runs and of the raster kernel alone:
     // launch k records into ring slot k (up to TIMING_RING launches between
     // two finishes); finish adds up every recorded slot.

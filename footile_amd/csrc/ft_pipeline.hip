@@ -21,6 +21,7 @@
 #include "ft_pipeline.h"
 #include "ft_scan.h"
 #include "ft_stages.h"
+#include "ft_stroke_dev.h"
 
 #include <algorithm>
 #include <cstring>
@@ -385,6 +386,176 @@ __global__ __launch_bounds__(BLOCK) void k_edge_count(Limits L0, const EdgeRec *
 }
 
 // ---------------------------------------------------------------------------
+// Stroker stages (MODE_STROKE): flattened stroke events -> outline events.
+// ---------------------------------------------------------------------------
+// atan2 and sin evaluated in double and rounded once (they only choose joins)
+struct DevMath {
+    __device__ static float atan2(float y, float x) { return (float)::atan2((double)y, (double)x); }
+    __device__ static float sin(float t) { return (float)::sin((double)t); }
+};
+
+__global__ __launch_bounds__(BLOCK) void k_st_flags(const F4 *ev, unsigned long long *flags, uint32_t *last,
+                                                    DevStatus *st) {
+    if (run_failed(st)) return;
+    const uint32_t n = st->n_events;
+    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < n; e += gridDim.x * BLOCK) {
+        const StrokeFlags f = stroke_flags(e, ev);
+        flags[e] = ((unsigned long long)(f.pushed ? 1u : 0u) << 32) | (f.first ? 1u : 0u);
+        last[e] = f.is_pt ? 0u : e + 1;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_st_compact(const F4 *ev, const unsigned long long *off, F4 *spts,
+                                                      uint32_t *sub_start, uint32_t *sub_event, uint32_t cap,
+                                                      DevStatus *st) {
+    if (run_failed(st)) return;
+    const uint32_t n = st->n_events;
+    uint32_t bad = 0;
+    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < n; e += gridDim.x * BLOCK) {
+        const StrokeFlags f = stroke_flags(e, ev);
+        const unsigned long long o = off[e];
+        const uint32_t sidx = (uint32_t)(o >> 32), sub = (uint32_t)o;
+        if (f.pushed & (sidx < FT_MAX_POINTS) & (sidx < cap)) spts[sidx] = ev[e];
+        if (f.first & (sub < cap)) {
+            sub_start[sub] = sidx;
+            sub_event[sub] = e;
+        }
+        bad |= (f.first & (sub >= cap)) ? 1u : 0u;
+    }
+    if (bad) raise_err(st, FT_EDEVICE);
+}
+
+// One thread per sub-stroke: point count, joined flag, and the number of
+// sub-strokes below the cap; then the outline's item count.
+__global__ __launch_bounds__(BLOCK) void k_st_subs(const F4 *ev, const uint32_t *last_close,
+                                                   const uint32_t *sub_start, const uint32_t *sub_event,
+                                                   uint32_t *sub_n, uint32_t *sub_joined, uint32_t cap,
+                                                   DevStatus *st) {
+    if (run_failed(st)) return;
+    const unsigned long long tot = st->st_total;
+    const uint32_t n_ev = st->n_events, total_subs = (uint32_t)tot;
+    const uint32_t n_kept = (uint32_t)(tot >> 32) < FT_MAX_POINTS ? (uint32_t)(tot >> 32) : FT_MAX_POINTS;
+    if (total_subs > cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
+        return;
+    }
+    for (uint32_t s = blockIdx.x * BLOCK + threadIdx.x; s < total_subs; s += gridDim.x * BLOCK) {
+        const uint32_t a = sub_start[s];
+        const bool kept = a < FT_MAX_POINTS;
+        const bool more = s + 1 < total_subs;
+        const uint32_t a1 = more ? sub_start[s + 1] : n_kept;
+        const bool last_kept = kept & (!more | (a1 >= FT_MAX_POINTS));
+        const uint32_t end = a1 < n_kept ? a1 : n_kept;
+        // closes up to the next sub-stroke's first point (or the end) apply to this one
+        const uint32_t range_end = (last_kept | !more) ? n_ev : sub_event[s + 1];
+        const uint32_t lc = last_close[range_end ? range_end - 1 : 0];
+        const bool has_close = (range_end > 0) & (lc > 0) & (lc - 1 >= sub_event[s]) & (lc - 1 < n_ev);
+        const F4 ce = ev[has_close ? lc - 1 : 0];
+        const bool joined = has_close & (ce.w == 2.0f);
+        sub_n[s] = kept & (end > a) ? end - a : 0u;
+        sub_joined[s] = joined ? 1u : 0u;
+        if (last_kept) {
+            st->st_subs = s + 1;
+            st->st_items = 2u * (n_kept + s + 1) + 1u;
+        }
+    }
+    if (total_subs == 0 && blockIdx.x == 0 && threadIdx.x == 0) st->st_items = 1u;  // the final close only
+}
+
+__device__ inline uint32_t st_item_sub(uint32_t t, const uint32_t *sub_start, uint32_t n_subs) {
+    uint32_t lo = 0, hi = n_subs;  // last s with 2 (start_s + s) <= t
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (2u * (sub_start[mid] + mid) <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+struct StrokeParams {
+    uint32_t join;
+    float miter, tol_sq;
+};
+
+__global__ __launch_bounds__(BLOCK) void k_st_count(const F4 *spts, const uint32_t *sub_start, const uint32_t *sub_n,
+                                                    const uint32_t *sub_joined, StrokeParams sp,
+                                                    unsigned long long *cnt, DevStatus *st) {
+    if (run_failed(st)) return;
+    const uint32_t n_items = st->st_items, n_subs = st->st_subs;
+    for (uint32_t t = blockIdx.x * BLOCK + threadIdx.x; t < n_items; t += gridDim.x * BLOCK) {
+        uint32_t ne = 0, nc = 0, err = 0;
+        if (t + 1 == n_items) {
+            ne = nc = 1;  // Plotter::fill's final close
+        } else if (n_subs) {
+            const uint32_t si = st_item_sub(t, sub_start, n_subs);
+            StrokeSub sub;
+            sub.start = sub_start[si];
+            sub.n = sub_n[si];
+            sub.joined = sub_joined[si] != 0;
+            const uint32_t local = t - 2u * (sub.start + si);
+            const bool fwd = local <= sub.n;
+            const uint32_t k = fwd ? local : local - (sub.n + 1);
+            if (sub.n)
+                err = stroke_item<DevMath>(
+                    sub, fwd, k, spts, sp.join, sp.miter, sp.tol_sq, [&](SV2) { ++ne; },
+                    [&]() {
+                        ++ne;
+                        ++nc;
+                    });
+        }
+        if (err) raise_err(st, err);
+        cnt[t] = ((unsigned long long)ne << 32) | nc;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_st_emit(const F4 *spts, const uint32_t *sub_start, const uint32_t *sub_n,
+                                                   const uint32_t *sub_joined, StrokeParams sp,
+                                                   const unsigned long long *off, const PathDev *paths,
+                                                   EventSink sink, DevStatus *st) {
+    if (run_failed(st)) return;
+    const unsigned long long tot = st->stroke_total;
+    const uint32_t n_events = (uint32_t)(tot >> 32);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->n_events = n_events;
+        st->n_closes = (uint32_t)tot;
+        if (n_events > sink.cap_events) {
+            atomicOr(&st->overflow, 1u);
+            st->need_events = n_events;
+        }
+    }
+    if (n_events > sink.cap_events) return;
+    const uint32_t n_items = st->st_items, n_subs = st->st_subs;
+    const PathDev &P = paths[0];
+    uint32_t bad = 0;
+    for (uint32_t t = blockIdx.x * BLOCK + threadIdx.x; t < n_items; t += gridDim.x * BLOCK) {
+        EventSink em = sink;
+        em.e = off[t] >> 32;
+        em.run = (uint32_t)off[t];
+        em.p = 0;
+        if (t + 1 == n_items) {
+            em.close(true);
+        } else if (n_subs) {
+            const uint32_t si = st_item_sub(t, sub_start, n_subs);
+            StrokeSub sub;
+            sub.start = sub_start[si];
+            sub.n = sub_n[si];
+            sub.joined = sub_joined[si] != 0;
+            const uint32_t local = t - 2u * (sub.start + si);
+            const bool fwd = local <= sub.n;
+            const uint32_t k = fwd ? local : local - (sub.n + 1);
+            if (sub.n)
+                stroke_item<DevMath>(
+                    sub, fwd, k, spts, sp.join, sp.miter, sp.tol_sq,
+                    [&](SV2 q) { em.point(xform(P, q.x, q.y, 0.0f)); }, [&]() { em.close(true); });
+        }
+        bad |= em.bad;
+    }
+    if (bad) raise_err(st, FT_EDEVICE);
+}
+
+// ---------------------------------------------------------------------------
 // Stage 6: raster.  One block per row band and LDS row group.
 // ---------------------------------------------------------------------------
 __device__ inline uint32_t wave_incl_u32(uint32_t v) {
@@ -498,7 +669,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     }
     const PathDev &P = paths[p];
     const uint32_t R = P.band_rows, H = P.height, W = P.width;
-    const uint32_t G = lds_rows(W, R);  // rows whose cells fit in LDS at once
+    const uint32_t G = P.group_rows ? P.group_rows : 1u;  // rows whose cells fit in LDS at once
     const uint32_t b0 = (gb - P.band_base) * R;
     const uint32_t b1 = b0 + R < H ? b0 + R : H;
     const uint32_t y0 = b0 + rb.y * G;
@@ -600,6 +771,11 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
         if ((call) != hipSuccess) return FT_EDEVICE;   \
     } while (0)
 
+static void graph_drop(Workspace &ws) {
+    if (ws.gexec) (void)hipGraphExecDestroy(ws.gexec);
+    ws.gexec = nullptr;
+}
+
 template <typename T>
 static ft_status dev_grow(T *&ptr, size_t &cap, size_t need) {
     if (need <= cap && ptr) return FT_OK;
@@ -639,6 +815,8 @@ void ws_free(Workspace &ws) {
     void *bufs[] = {ws.d_ops, ws.d_op_path, ws.d_paths, ws.d_band_path, ws.d_op_cnt, ws.d_op_off, ws.d_sub_off,
                     ws.d_op_pw,
                     ws.d_op_lastpw, ws.d_ev_xy, ws.d_ev_f, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_ev_last,
+                    ws.d_st_flags, ws.d_st_off, ws.d_st_last, ws.d_spts, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
+                    ws.d_sub_joined, ws.d_it_cnt, ws.d_it_off,
                     ws.d_run_first, ws.d_run_close, ws.d_run_path, ws.d_rblock, ws.d_run_start, ws.d_run_end, ws.d_pts,
                     ws.d_pt_run, ws.d_edges, ws.d_band_cnt, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
                     ws.d_scan_flags, ws.d_scan_agg, ws.d_scan_incl, ws.d_pstate, ws.d_status};
@@ -650,6 +828,7 @@ void ws_free(Workspace &ws) {
             if (ws.ev[i]) (void)hipEventDestroy(ws.ev[i]);
         delete[] ws.ev;
     }
+    graph_drop(ws);
     if (ws.stream) (void)hipStreamDestroy(ws.stream);
     ws = Workspace();
 }
@@ -671,8 +850,24 @@ static ft_status ensure_event_capacity(Workspace &ws, size_t need) {
     return FT_OK;
 }
 
+static ft_status ensure_stroke_capacity(Workspace &ws) {
+    if (ws.cap_stroke == ws.cap_events && ws.d_st_flags) return FT_OK;
+    const size_t cap = ws.cap_events, items = 4 * cap + 1;
+    ft_status s;
+    if ((s = dev_alloc(ws.d_st_flags, cap)) || (s = dev_alloc(ws.d_st_off, cap)) ||
+        (s = dev_alloc(ws.d_st_last, cap)) || (s = dev_alloc(ws.d_spts, cap)) || (s = dev_alloc(ws.d_sub_start, cap)) ||
+        (s = dev_alloc(ws.d_sub_event, cap)) || (s = dev_alloc(ws.d_sub_n, cap)) ||
+        (s = dev_alloc(ws.d_sub_joined, cap)) || (s = dev_alloc(ws.d_it_cnt, items)) ||
+        (s = dev_alloc(ws.d_it_off, items))) {
+        ws.cap_stroke = 0;
+        return s;
+    }
+    ws.cap_stroke = cap;
+    return FT_OK;
+}
+
 static ft_status ensure_scan_capacity(Workspace &ws) {
-    size_t n = ws.cap_events;
+    size_t n = 4 * ws.cap_events + 1;  // stroke outline items bound every other scan length
     if (ws.n_ops > n) n = ws.n_ops;
     if (ws.n_bands > n) n = ws.n_bands;
     size_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE + 1;
@@ -694,6 +889,7 @@ ft_status ws_upload_ops(Workspace &ws, const ft_path_op *ops, size_t n_ops) {
 ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const ft_fill_desc *paths,
                        size_t n_paths, uint32_t extra_flags) {
     if (n_ops >= 0x7FFFFFFFu || n_paths >= 0x7FFFFFFFu) return FT_EINVAL;
+    graph_drop(ws);
     std::vector<uint32_t> op_path(n_ops, NO_IDX);
     std::vector<PathDev> pd(n_paths);
     std::vector<uint32_t> band_path;
@@ -712,15 +908,15 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
             op_path[i] = (uint32_t)k;
         }
         PathDev &P = pd[k];
-        P = path_from_desc(d, extra_flags);
-        uint32_t rows, nb, cells;
-        band_geometry(d.width, d.height, &rows, &nb, &cells);
+        P = path_from_desc(d, extra_flags, ws.tuning);
+        uint32_t rows, group, nb, cells;
+        band_geometry(d.width, d.height, ws.tuning, &rows, &group, &nb, &cells);
         nb = P.n_bands;
         P.band_base = (uint32_t)band_path.size();
         if (nb && cells > lds) lds = cells;
         band_path.insert(band_path.end(), nb, (uint32_t)k);
         if (nb) {
-            const uint32_t G = lds_rows(d.width, P.band_rows);
+            const uint32_t G = P.group_rows;
             const uint32_t groups = (P.band_rows + G - 1) / G;
             for (uint32_t b = 0; b < nb; ++b) {
                 const uint32_t rows = std::min(P.band_rows, d.height - b * P.band_rows);
@@ -795,15 +991,13 @@ static inline uint32_t scan_grid(size_t n) {
     return (uint32_t)(g > 1024 ? 1024 : g);
 }
 
+static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev);
+
+// Enqueue one run.  In graph mode a fill run is captured once into a HIP
+// graph (the launch sequence has no memset nodes and no host waits) and then
+// replayed; any change of buffers or layout drops the graph.
 ft_status ws_launch(Workspace &ws, uint32_t mode) {
-    hipStream_t s = ws.stream;
     ws.edges_counted = false;
-    const uint32_t n_ops = (uint32_t)ws.n_ops;
-    const size_t tiles = ws.cap_tiles;
-    uint32_t *fl = ws.d_scan_flags;
-    unsigned long long *ag = ws.d_scan_agg, *in = ws.d_scan_incl;
-    DevStatus *st = ws.d_status;
-    const uint32_t n_flags = (uint32_t)(tiles * SCAN_SLOTS);
     hipEvent_t *ev = nullptr;
     if (ws.timing && mode == MODE_FILL && ws.n_bands && ws.n_pending < Workspace::TIMING_RING) {
         if (!ws.ev) {
@@ -812,6 +1006,46 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
         }
         ev = ws.ev + 4 * ws.n_pending++;
     }
+    if (!(ws.graph && mode == MODE_FILL)) return ws_enqueue(ws, mode, ev);
+    if (!ws.gexec) {
+        hipGraph_t g = nullptr;
+        FT_HIP(hipStreamBeginCapture(ws.stream, hipStreamCaptureModeThreadLocal));
+        const ft_status st = ws_enqueue(ws, mode, nullptr);
+        const hipError_t e = hipStreamEndCapture(ws.stream, &g);
+        if (st) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess || !g) return FT_EDEVICE;
+        const hipError_t ie = hipGraphInstantiate(&ws.gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            ws.gexec = nullptr;
+            return FT_EDEVICE;
+        }
+    }
+    if (ev) {
+        FT_HIP(hipEventRecord(ev[0], ws.stream));
+        FT_HIP(hipEventRecord(ev[1], ws.stream));  // the raster kernel is not timed inside a graph
+        FT_HIP(hipEventRecord(ev[2], ws.stream));
+    }
+    FT_HIP(hipGraphLaunch(ws.gexec, ws.stream));
+    if (ev) FT_HIP(hipEventRecord(ev[3], ws.stream));
+    return FT_OK;
+}
+
+static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
+    if (mode == MODE_STROKE) {
+        const ft_status e = ensure_stroke_capacity(ws);
+        if (e) return e;
+    }
+    hipStream_t s = ws.stream;
+    const uint32_t n_ops = (uint32_t)ws.n_ops;
+    const size_t tiles = ws.cap_tiles;
+    uint32_t *fl = ws.d_scan_flags;
+    unsigned long long *ag = ws.d_scan_agg, *in = ws.d_scan_incl;
+    DevStatus *st = ws.d_status;
+    const uint32_t n_flags = (uint32_t)(tiles * SCAN_SLOTS);
     if (ev) FT_HIP(hipEventRecord(ev[0], s));
     k_clear<<<grid_for(std::max(std::max(ws.n_paths, ws.n_bands), (size_t)n_flags)), BLOCK, 0, s>>>(
         ws.d_pstate, (uint32_t)ws.n_paths, ws.d_band_cnt, ws.d_band_cur, (uint32_t)ws.n_bands, fl, n_flags, st);
@@ -819,8 +1053,8 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
     if (mode != MODE_FILL && n_ops) {
         k_pen_width_mark<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, n_ops, ws.d_op_pw);
         k_scan<uint32_t, true><<<scan_grid(n_ops), BLOCK, 0, s>>>(ws.d_op_pw, ws.d_op_lastpw, nullptr, n_ops, n_ops,
-                                                                  fl + 3 * tiles, ag + 3 * tiles, in + 3 * tiles,
-                                                                  &st->tickets[3], nullptr, st, false, false);
+                                                                  fl + 5 * tiles, ag + 5 * tiles, in + 5 * tiles,
+                                                                  &st->tickets[5], nullptr, st, false, false);
         lastpw = ws.d_op_lastpw;
     }
     if (n_ops) {
@@ -844,7 +1078,45 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
         k_emit<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
                                                                     lastpw, ws.d_op_off, ws.d_sub_off, em, st);
     }
-    if (mode == MODE_FILL) {
+    if (mode == MODE_STROKE) {
+        const uint32_t ge = grid_for(ws.cap_events);
+        const uint32_t cap = (uint32_t)ws.cap_events;
+        const size_t cap_items = 4 * ws.cap_events + 1;
+        StrokeParams sp;
+        sp.join = ws.join;
+        sp.miter = ws.miter;
+        sp.tol_sq = ws.n_paths ? ws.stroke_tol_sq : 0.0f;
+        k_st_flags<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_flags, ws.d_st_last, st);
+        k_scan<unsigned long long, false><<<scan_grid(ws.cap_events), BLOCK, 0, s>>>(
+            ws.d_st_flags, ws.d_st_off, &st->n_events, 0, cap, fl + 6 * tiles, ag + 6 * tiles, in + 6 * tiles,
+            &st->tickets[6], &st->st_total, st, true, false);
+        k_scan<uint32_t, true><<<scan_grid(ws.cap_events), BLOCK, 0, s>>>(
+            ws.d_st_last, ws.d_st_last, &st->n_events, 0, cap, fl + 7 * tiles, ag + 7 * tiles, in + 7 * tiles,
+            &st->tickets[7], nullptr, st, true, true);
+        k_st_compact<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_off, ws.d_spts, ws.d_sub_start, ws.d_sub_event, cap, st);
+        k_st_subs<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_last, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
+                                       ws.d_sub_joined, cap, st);
+        k_st_count<<<grid_for(cap_items), BLOCK, 0, s>>>(ws.d_spts, ws.d_sub_start, ws.d_sub_n, ws.d_sub_joined, sp,
+                                                         ws.d_it_cnt, st);
+        k_scan<unsigned long long, false><<<scan_grid(cap_items), BLOCK, 0, s>>>(
+            ws.d_it_cnt, ws.d_it_off, &st->st_items, 0, (uint32_t)cap_items, fl + 8 * tiles, ag + 8 * tiles,
+            in + 8 * tiles, &st->tickets[8], &st->stroke_total, st, true, false);
+        EventSink em{};
+        em.cap_events = ws.cap_events;
+        em.cap_runs = cap;
+        em.mode = MODE_FILL;
+        em.ev_xy = ws.d_ev_xy;
+        em.ev_f = ws.d_ev_f;
+        em.ev_meta = ws.d_ev_meta;
+        em.run_first = ws.d_run_first;
+        em.run_close = ws.d_run_close;
+        em.run_path = ws.d_run_path;
+        em.run_start = ws.d_run_start;
+        em.run_end = ws.d_run_end;
+        k_st_emit<<<grid_for(cap_items), BLOCK, 0, s>>>(ws.d_spts, ws.d_sub_start, ws.d_sub_n, ws.d_sub_joined, sp,
+                                                        ws.d_it_off, ws.d_paths, em, st);
+    }
+    if (mode == MODE_FILL || mode == MODE_STROKE) {
         const uint32_t ge = grid_for(ws.cap_events);
         Limits L0{};
         L0.n_ops = n_ops;
@@ -941,6 +1213,7 @@ ft_status ws_run(Workspace &ws, uint32_t mode, bool read_pstate) {
         s = ws_finish(ws, read_pstate);
         if (s != (ft_status)0x100) return s;
         const DevStatus &h = *ws.h_status;
+        graph_drop(ws);
         if (h.overflow & 1u) {
             if ((s = ensure_event_capacity(ws, (size_t)h.need_events + h.need_events / 4 + 1024))) return s;
             if ((s = ensure_scan_capacity(ws))) return s;

@@ -16,9 +16,11 @@ constexpr uint32_t GRID_MAX = 2048;     // grid-stride kernels: 8 blocks per CU
 constexpr uint32_t SCAN_ITEMS = 8;
 constexpr uint32_t SCAN_TILE = BLOCK * SCAN_ITEMS;
 // look-back state instances per run: 0 op counts, 1 kept points, 2 band
-// offsets, 3 band differences (fill) or PenWidth indices (stroke), 4 pushed
-// points' running maximum
-constexpr uint32_t SCAN_SLOTS = 5;
+// offsets, 3 band differences, 4 pushed points' running maximum, 5 PenWidth
+// indices, 6 stroker point/sub-stroke flags, 7 stroker last closes, 8 stroke
+// outline items
+constexpr uint32_t SCAN_SLOTS = N_SCAN_SLOTS;
+static_assert(SCAN_SLOTS == 9, "scan slot assignment");
 constexpr uint32_t SPIN_LIMIT = 1u << 24;
 
 // Device atomics for the stage code.

@@ -476,11 +476,13 @@ FT_HD inline void cap_path(uint32_t p, const Limits &L, const PathDev *paths, co
                            const int32_t *run_first, PathState *ps, uint32_t &bad) {
     const PathDev &P = paths[p];
     uint32_t base = 0, repop = 0, offered = 0, stored = 0;
-    if (P.op_count && P.op_begin < L.n_ops) {
-        const uint32_t eb = path_event_begin(P, op_off);
-        const uint32_t ee = path_event_end(P, op_off, L.n_ops, L.n_events);
+    const bool whole = (P.flags & PATH_WHOLE) != 0;
+    if (whole || (P.op_count && P.op_begin < L.n_ops)) {
         const uint32_t k = P.op_begin + P.op_count;
-        const uint32_t cb = (uint32_t)op_off[P.op_begin], ce = k < L.n_ops ? (uint32_t)op_off[k] : L.n_runs;
+        const uint32_t eb = whole ? 0u : path_event_begin(P, op_off);
+        const uint32_t ee = whole ? L.n_events : path_event_end(P, op_off, L.n_ops, L.n_events);
+        const uint32_t cb = whole ? 0u : (uint32_t)op_off[P.op_begin];
+        const uint32_t ce = whole ? L.n_runs : (k < L.n_ops ? (uint32_t)op_off[k] : L.n_runs);
         if (eb < ee && ee <= L.n_events && cb <= ce && ce - cb <= ee - eb) {
             offered = (ee - eb) - (ce - cb);  // point events: events minus closes
             base = idx_of[eb];
@@ -788,41 +790,43 @@ FT_HD inline void composite_px_fresh(uint8_t *d, const uint32_t *sw, uint32_t a)
     for (int c = 0; c < N; ++c) d[c] = (uint8_t)ch8_mul_w(sw[c], aw);
 }
 
-// Band geometry.  A band is the unit of edge binning and one raster block;
-// its rows are rasterised in groups whose cells fit LDS_CELLS.  Bands are at
-// least BAND_ROWS_MIN rows tall so that wide rasters (one row per group) do
-// not list every edge once per row.
-constexpr uint32_t LDS_CELLS = 4096;  // i32 signed-area cells per raster block (16 KiB)
+// Band geometry.  A band is the unit of edge binning; its rows are rasterised
+// in groups whose i32 cells fit the LDS budget, one raster block per group.
+// Bands are at least `band_rows_min` rows tall so that wide rasters (one row
+// per group) do not list every edge once per row.  Both figures are tunable
+// per batch (Tuning); the defaults suit the measured workloads.
+constexpr uint32_t LDS_CELLS = 4096;  // default i32 cells per raster block (16 KiB)
 constexpr uint32_t BAND_ROWS_MAX = 64;
 constexpr uint32_t BAND_ROWS_MIN = 16;
 constexpr uint32_t CELL_ALIGN = 16;   // row stride of cells in LDS
 
+struct Tuning {
+    uint32_t band_rows_min = BAND_ROWS_MIN;  // 1 .. 256
+    uint32_t lds_cells = LDS_CELLS;          // 1024 .. 16384 (64 KiB), a multiple of CELL_ALIGN
+};
+
 FT_HD inline uint32_t cell_stride(uint32_t width) { return (width + CELL_ALIGN - 1) / CELL_ALIGN * CELL_ALIGN; }
 
-// Rows per LDS group for a raster of this width, within a band of `band_rows`.
-FT_HD inline uint32_t lds_rows(uint32_t width, uint32_t band_rows) {
+// rows: band height; group: rows per LDS group; cells: LDS cells per block.
+FT_HD inline void band_geometry(uint32_t width, uint32_t height, const Tuning &t, uint32_t *rows, uint32_t *group,
+                                uint32_t *n_bands, uint32_t *cells) {
     const uint32_t wp = cell_stride(width);
-    uint32_t g = wp ? LDS_CELLS / wp : BAND_ROWS_MAX;
-    if (g < 1) g = 1;
-    return g < band_rows ? g : band_rows;
-}
-
-FT_HD inline void band_geometry(uint32_t width, uint32_t height, uint32_t *rows, uint32_t *n_bands, uint32_t *cells) {
-    const uint32_t wp = cell_stride(width);
-    uint32_t g = wp ? LDS_CELLS / wp : BAND_ROWS_MAX;
+    uint32_t g = wp ? t.lds_cells / wp : BAND_ROWS_MAX;
     if (g > BAND_ROWS_MAX) g = BAND_ROWS_MAX;
     if (g < 1) g = 1;
-    uint32_t r = g < BAND_ROWS_MIN ? BAND_ROWS_MIN : g;
+    uint32_t r = g < t.band_rows_min ? t.band_rows_min : g;
     if (height && r > height) r = height;
+    if (r < 1) r = 1;
     if (g > r) g = r;
     *rows = r;
+    *group = g;
     *n_bands = height ? (height + r - 1) / r : 0;
     *cells = g * wp;
 }
 
 // Device form of one batch path (ft_fill_desc -> PathDev); band_base is set
 // by the caller.
-inline PathDev path_from_desc(const ft_fill_desc &d, uint32_t extra_flags) {
+inline PathDev path_from_desc(const ft_fill_desc &d, uint32_t extra_flags, const Tuning &t = Tuning()) {
     PathDev P;
     P.op_begin = (uint32_t)d.op_begin;
     P.op_count = d.op_count;
@@ -837,10 +841,11 @@ inline PathDev path_from_desc(const ft_fill_desc &d, uint32_t extra_flags) {
     const float tol = d.tolerance > 0.01f ? d.tolerance : 0.01f;  // set_tolerance, plotter.rs:133-137
     P.tol_sq = tol * tol;
     P.s_width0 = d.pen_width;
-    uint32_t rows, nb, cells;
-    band_geometry(d.width, d.height, &rows, &nb, &cells);
+    uint32_t rows, group, nb, cells;
+    band_geometry(d.width, d.height, t, &rows, &group, &nb, &cells);
     if (!d.width) nb = 0;
     P.band_rows = rows;
+    P.group_rows = group;
     P.band_base = 0;
     P.n_bands = nb;
     P.flags = (d.fresh ? PATH_FRESH : 0u) | extra_flags;

@@ -17,7 +17,10 @@ struct F4 {  // stroke-mode event (x, y, width, kind), 16 bytes
     float x, y, z, w;
 };
 
-enum : uint32_t { MODE_FILL = 0, MODE_STROKE_FLATTEN = 1 };
+// MODE_FILL: Plotter::fill.  MODE_STROKE_FLATTEN: flatten only, writing
+// (x, y, width, kind) events.  MODE_STROKE: Plotter::stroke (flatten, the
+// stroker stages, then the fill of the outline).
+enum : uint32_t { MODE_FILL = 0, MODE_STROKE_FLATTEN = 1, MODE_STROKE = 2 };
 enum : uint32_t { TAG_CLOSE = 0, TAG_MOVE, TAG_LINE, TAG_QUAD, TAG_CUBIC, TAG_PEN_WIDTH };
 enum : uint32_t { FMT_MATTE8 = 0, FMT_RGBA8P = 1, FMT_GRAYA8P = 2 };
 
@@ -36,12 +39,14 @@ struct PathDev {
     float tol_sq;
     float s_width0;
     uint32_t band_base, band_rows, n_bands;
-    uint32_t flags;  // PATH_FRESH | PATH_RAW
+    uint32_t group_rows;  // rows per raster block (LDS group) within a band
+    uint32_t flags;       // PATH_FRESH | PATH_RAW
 };
 
 enum : uint32_t {
     PATH_FRESH = 1u,  // region starts cleared: rows above the figure are zeroed
-    PATH_RAW = 2u     // points go to Fig::add_point untransformed (Fig-level entry)
+    PATH_RAW = 2u,    // points go to Fig::add_point untransformed (Fig-level entry)
+    PATH_WHOLE = 4u   // the path owns every event of the run (a stroke outline)
 };
 
 // Per-path results of a run (cleared at the start of each run).
@@ -55,15 +60,20 @@ struct PathState {
     uint32_t stored;            // points stored in the Fig after the cap
 };
 
+constexpr uint32_t N_SCAN_SLOTS = 9;
+
 struct DevStatus {
     uint32_t err;        // first ft_status error raised by a kernel
     uint32_t overflow;   // bit 0: events, bit 1: band pairs
     uint32_t n_events, n_closes;
     uint32_t n_kept, n_pairs;
     uint32_t need_events, need_pairs;
-    unsigned long long op_total;  // packed (events << 32 | closes) of all ops
+    unsigned long long op_total;      // packed (events << 32 | closes) of all ops
+    unsigned long long st_total;      // stroker: points stored << 32 | sub-strokes, before the cap
+    unsigned long long stroke_total;  // stroker outline: events << 32 | closes
+    uint32_t st_subs, st_items;       // sub-strokes kept; outline items
     uint32_t spin_fail;
-    uint32_t tickets[5];
+    uint32_t tickets[N_SCAN_SLOTS];
     uint32_t pad[2];
 };
 

@@ -218,6 +218,10 @@ class FillDesc:
         self.c.fresh = 1 if fresh else 0
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [("band_rows_min", ctypes.c_uint32), ("lds_cells", ctypes.c_uint32)]
+
+
 class Batch:
     """Many independent fills from device-resident ops (ft_batch_*)."""
 
@@ -253,6 +257,20 @@ class Batch:
         check(lib().ft_batch_timing(self._h, ctypes.byref(t), ctypes.byref(r), ctypes.byref(n)),
               "ft_batch_timing")
         return t.value, r.value, n.value
+
+    def set_tuning(self, band_rows_min, lds_cells):
+        """Raster geometry (ft_batch_set_tuning); re-lays the batch out."""
+        t = _Tuning(int(band_rows_min), int(lds_cells))
+        check(lib().ft_batch_set_tuning(self._h, ctypes.byref(t)), "ft_batch_set_tuning")
+
+    def tuning(self):
+        t = _Tuning()
+        check(lib().ft_batch_get_tuning(self._h, ctypes.byref(t)), "ft_batch_get_tuning")
+        return t.band_rows_min, t.lds_cells
+
+    def set_graph(self, on=True):
+        """Replay launches from a captured HIP graph (ft_batch_set_graph)."""
+        check(lib().ft_batch_set_graph(self._h, 1 if on else 0), "ft_batch_set_graph")
 
     def counts(self):
         """(offered, stored, edges) per path of the last run."""

@@ -152,6 +152,21 @@ ft_status ft_batch_run(ft_batch *b);
  * the figure's top row (row_of of the smallest y; INT32_MAX if empty). */
 ft_status ft_batch_counts(ft_batch *b, uint64_t *offered, uint64_t *stored, uint64_t *edges,
                           int32_t *top_rows);
+/* Raster geometry of a batch: row bands of at least band_rows_min rows
+ * (1..256) are the unit of edge binning, and each band is rasterised in
+ * groups of rows whose i32 cells fit lds_cells (1024..16384, a multiple of
+ * 16); one block per group.  Setting it re-lays the batch out.  No reference
+ * counterpart: the autotuner (footile_amd.tune) picks it per workload. */
+typedef struct ft_tuning {
+    uint32_t band_rows_min;
+    uint32_t lds_cells;
+} ft_tuning;
+ft_status ft_batch_set_tuning(ft_batch *b, const ft_tuning *t);
+ft_status ft_batch_get_tuning(const ft_batch *b, ft_tuning *t);
+/* Graph mode: ft_batch_launch captures the launch sequence into a HIP graph
+ * once and replays it (re-captured after any re-layout or capacity growth);
+ * the raster kernel is then not timed separately. */
+ft_status ft_batch_set_graph(ft_batch *b, int on);
 /* The batch's HIP stream (hipStream_t) for event timing by the caller. */
 ft_status ft_batch_stream(ft_batch *b, void **stream);
 /* Event timing on the batch stream: when on, every launch records HIP events

@@ -52,6 +52,36 @@ def check(tag, w, h, fmt, rule, ops, clr=(255, 255, 255, 255), init=None, transf
         FAILS.append(f"{tag}: counts {counts} vs oracle {o.last_counts()}")
 
 
+def check_stroke(tag, w, h, fmt, ops, clr=(255, 255, 255, 255), transform=None, tol=None, join=None):
+    o = po.Plotter(w, h, fmt)
+    if transform is not None:
+        o.set_transform(transform)
+    if tol is not None:
+        o.set_tolerance(tol)
+    if join is not None:
+        o.set_join(*join)
+    rc_o = o.stroke(ops, clr)
+    kw = {}
+    if transform is not None:
+        kw["transform"] = transform
+    if tol is not None:
+        kw["tolerance"] = tol
+    if join is not None:
+        kw["join"] = join
+    rc, px, counts = mirror.stroke(w, h, fmt, ops, clr, **kw)
+    if rc != rc_o:
+        FAILS.append(f"{tag}: status {rc} vs oracle {rc_o}")
+        return
+    if rc:
+        return
+    want = o.pixels()
+    if not np.array_equal(px, want):
+        bad = np.flatnonzero(px != want)
+        FAILS.append(f"{tag}: {bad.size} bytes differ, first {bad[:5]}")
+    elif counts != o.last_counts():
+        FAILS.append(f"{tag}: counts {counts} vs oracle {o.last_counts()}")
+
+
 def rect(x0, y0, x1, y1, ccw=False):
     pts = [(x0, y0), (x1, y0), (x1, y1), (x0, y1)]
     if ccw:
@@ -102,6 +132,17 @@ def main(quick=False):
                       (LINE, (96.0, 64.0)), (CLOSE, ())])
     check("fish scale 2", 256, 256, MATTE8, 0, fish, transform=(2, 0, 0, 0, 2, 0))
     check("fish tol", 128, 128, RGBA8P, 1, fish, (127, 96, 96, 255), tol=0.05)
+    # strokes: the stroker stages against the oracle's stroker
+    check_stroke("plain_256 stroke", 256, 256, MATTE8, workloads.plain_256_stroke(0))
+    check_stroke("fish stroke scale 2", 256, 256, MATTE8, fish, transform=(2, 0, 0, 0, 2, 0))
+    check_stroke("fish stroke rgba", 128, 128, RGBA8P, fish, (255, 208, 208, 255), tol=0.05)
+    rng = np.random.default_rng(5)
+    for t in range(12 if quick else 36):
+        ops = random_path(rng, 96, 96, n_sub=2, kinds=(LINE, QUAD, CUBIC), margin=-8.0, close_prob=0.5)
+        ops = np.concatenate([ops_array([(PEN_WIDTH, (float(rng.uniform(1, 9)),))]), ops])
+        join = [(0, 4.0), (1, 0.0), (2, 0.0), (0, 1.5)][t % 4]
+        check_stroke(f"stroke{t} join{join}", 96, 96, MATTE8, ops, join=join)
+    check_stroke("empty stroke", 32, 32, MATTE8, ops_array([(MOVE, (3.0, 3.0)), (CLOSE, ())]))
     # the 65,535-point cap and its close-pop interplay
     rng = np.random.default_rng(8)
     n = 66000

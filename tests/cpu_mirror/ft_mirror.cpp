@@ -9,9 +9,11 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <math.h>
 #include <vector>
 
 #include "../../footile_amd/csrc/ft_stages.h"
+#include "../../footile_amd/csrc/ft_stroke_dev.h"
 
 using namespace ft;
 
@@ -42,6 +44,101 @@ std::vector<T> buf(size_t n) {
     return std::vector<T>(n ? n : 1);
 }
 
+struct HostMath {  // the C library's float functions, as the oracle calls them
+    static float atan2(float y, float x) { return atan2f(y, x); }
+    static float sin(float t) { return sinf(t); }
+};
+
+// The stroker stages in order, on the host: flattened stroke events -> the
+// outline's fill events (and run tables).
+int stroke_outline(const F4 *ev, uint32_t n_ev, const PathDev &P, uint32_t join, float miter, uint32_t &n_out,
+                   uint32_t &n_closes_out, std::vector<I2> &ev_xy, std::vector<uint32_t> &ev_meta,
+                   std::vector<int32_t> &run_first, std::vector<uint32_t> &run_close,
+                   std::vector<uint32_t> &run_path, std::vector<uint32_t> &run_start, std::vector<uint32_t> &run_end) {
+    // S1: flags, their scans, compaction, sub-strokes
+    std::vector<uint32_t> sidx(n_ev ? n_ev : 1), subi(n_ev ? n_ev : 1), lastc(n_ev ? n_ev : 1);
+    uint32_t pushed = 0, subs = 0, lc = 0;
+    for (uint32_t e = 0; e < n_ev; ++e) {
+        const StrokeFlags f = stroke_flags(e, ev);
+        sidx[e] = pushed;
+        subi[e] = subs;
+        pushed += f.pushed ? 1 : 0;
+        subs += f.first ? 1 : 0;
+        if (!f.is_pt) lc = e + 1;
+        lastc[e] = lc;
+    }
+    const uint32_t n_kept = pushed < FT_MAX_POINTS ? pushed : FT_MAX_POINTS;
+    auto spts = buf<F4>(n_kept);
+    auto sub_start = buf<uint32_t>(subs), sub_event = buf<uint32_t>(subs);
+    for (uint32_t e = 0; e < n_ev; ++e) {
+        const StrokeFlags f = stroke_flags(e, ev);
+        if (f.pushed && sidx[e] < FT_MAX_POINTS) spts[sidx[e]] = ev[e];
+        if (f.first) {
+            sub_start[subi[e]] = sidx[e];
+            sub_event[subi[e]] = e;
+        }
+    }
+    std::vector<StrokeSub> S;
+    for (uint32_t s = 0; s < subs; ++s) {
+        if (sub_start[s] >= FT_MAX_POINTS) break;
+        const bool more = s + 1 < subs;
+        const uint32_t a1 = more ? sub_start[s + 1] : n_kept;
+        const bool last_kept = !more || a1 >= FT_MAX_POINTS;
+        const uint32_t end = a1 < n_kept ? a1 : n_kept;
+        const uint32_t range_end = last_kept ? n_ev : sub_event[s + 1];
+        const uint32_t c = range_end ? lastc[range_end - 1] : 0;
+        StrokeSub sub;
+        sub.start = sub_start[s];
+        sub.n = end - sub.start;
+        sub.joined = c > 0 && c - 1 >= sub_event[s] && ev[c - 1].w == 2.0f;
+        S.push_back(sub);
+    }
+    // S2: the outline, item by item (sub-stroke, side, segment), in output order
+    const float tol_sq = P.tol_sq;
+    uint32_t count_e = 0, count_c = 0, err = 0;
+    for (const StrokeSub &sub : S)
+        for (int side = 0; side < 2; ++side)
+            for (uint32_t k = 0; k <= sub.n; ++k)
+                err |= stroke_item<HostMath>(
+                    sub, side == 0, k, spts.data(), join, miter, tol_sq, [&](SV2) { ++count_e; },
+                    [&]() {
+                        ++count_e;
+                        ++count_c;
+                    });
+    if (err) return (int)err;
+    n_out = count_e + 1;  // + Plotter::fill's final close
+    n_closes_out = count_c + 1;
+    ev_xy = buf<I2>(n_out);
+    ev_meta = buf<uint32_t>(n_out);
+    run_first = buf<int32_t>(n_closes_out);
+    run_close = buf<uint32_t>(n_closes_out);
+    run_path = buf<uint32_t>(n_closes_out);
+    run_start = buf<uint32_t>(n_closes_out);
+    run_end = buf<uint32_t>(n_closes_out);
+    EventSink em{};
+    em.mode = MODE_FILL;
+    em.ev_xy = ev_xy.data();
+    em.ev_meta = ev_meta.data();
+    em.run_first = run_first.data();
+    em.run_close = run_close.data();
+    em.run_path = run_path.data();
+    em.run_start = run_start.data();
+    em.run_end = run_end.data();
+    em.cap_events = n_out;
+    em.cap_runs = n_closes_out;
+    em.e = 0;
+    em.run = 0;
+    em.p = 0;
+    for (const StrokeSub &sub : S)
+        for (int side = 0; side < 2; ++side)
+            for (uint32_t k = 0; k <= sub.n; ++k)
+                stroke_item<HostMath>(
+                    sub, side == 0, k, spts.data(), join, miter, tol_sq,
+                    [&](SV2 q) { em.point(xform(P, q.x, q.y, 0.0f)); }, [&]() { em.close(true); });
+    em.close(true);
+    return em.bad ? FT_EDEVICE : 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -51,7 +148,8 @@ extern "C" {
 // (capacity ev_cap, count in *n_ev).  Per-path offered/stored counts are
 // written when the arrays are non-null.  Returns an ft_status.
 int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size_t n_paths, uint32_t mode,
-            uint32_t extra_flags, uint64_t *offered, uint64_t *stored, F4 *ev_out, size_t ev_cap, size_t *n_ev) {
+            uint32_t extra_flags, uint64_t *offered, uint64_t *stored, F4 *ev_out, size_t ev_cap, size_t *n_ev,
+            uint32_t join, float miter) {
     std::vector<uint32_t> op_path(n_ops, NO_IDX);
     std::vector<PathDev> paths(n_paths);
     std::vector<uint32_t> band_path;
@@ -92,7 +190,7 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
         }
         if (leaves > FT_FLATTEN_MAX_LEAVES) return FT_ERANGE;
     }
-    const uint32_t n_events = (uint32_t)(tot >> 32), n_closes = (uint32_t)tot;
+    uint32_t n_events = (uint32_t)(tot >> 32), n_closes = (uint32_t)tot;  // the outline's, for a stroke
     // stage 2: emit
     auto ev_xy = buf<I2>(mode == MODE_FILL ? n_events : 0);
     auto ev_f = buf<F4>(mode == MODE_FILL ? 0 : n_events);
@@ -118,10 +216,18 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
             bad |= emit_op(ops, op_path.data(), paths.data(), (uint32_t)i, s, mode,
                            mode == MODE_FILL ? nullptr : lastpw.data(), op_off[i] + sub_off[i * OP_LANES + s], sink);
     if (bad) return FT_EDEVICE;
-    if (mode != MODE_FILL) {
+    if (mode == MODE_STROKE_FLATTEN) {
         if (n_ev) *n_ev = n_events;
         if (ev_out) memcpy(ev_out, ev_f.data(), (n_events < ev_cap ? n_events : ev_cap) * sizeof(F4));
         return FT_OK;
+    }
+    if (mode == MODE_STROKE) {
+        uint32_t n_out = 0, n_cl = 0;
+        const int rc = stroke_outline(ev_f.data(), n_events, paths[0], join, miter, n_out, n_cl, ev_xy, ev_meta,
+                                      run_first, run_close, run_path, run_start, run_end);
+        if (rc) return rc;
+        n_events = n_out;
+        n_closes = n_cl;
     }
     Limits L{};
     L.n_ops = (uint32_t)n_ops;

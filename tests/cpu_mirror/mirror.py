@@ -28,7 +28,7 @@ def lib():
         L = ctypes.CDLL(os.path.join(HERE, name))
         vp, sz = ctypes.c_void_p, ctypes.c_size_t
         L.ftm_run.argtypes = [vp, sz, vp, sz, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, sz,
-                              ctypes.POINTER(sz)]
+                              ctypes.POINTER(sz), ctypes.c_uint32, ctypes.c_float]
         L.ftm_run.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -38,7 +38,11 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def run_batch(ops, descs, raw=False):
+MODE_FILL, MODE_STROKE = 0, 2
+PATH_RAW, PATH_WHOLE = 2, 4
+
+
+def run_batch(ops, descs, raw=False, mode=MODE_FILL, join=(0, 4.0)):
     """descs: list of FillDescC whose dst point at host memory.  Returns
     (status, offered, stored)."""
     from footile_amd._lib import FillDescC
@@ -47,13 +51,20 @@ def run_batch(ops, descs, raw=False):
     arr = (FillDescC * max(len(descs), 1))(*descs)
     off = np.zeros(max(len(descs), 1), np.uint64)
     st = np.zeros(max(len(descs), 1), np.uint64)
-    rc = lib().ftm_run(_ptr(a), len(a), arr, len(descs), 0, 2 if raw else 0, _ptr(off), _ptr(st),
-                       None, 0, None)
+    flags = (PATH_RAW if raw else 0) | (PATH_WHOLE if mode == MODE_STROKE else 0)
+    rc = lib().ftm_run(_ptr(a), len(a), arr, len(descs), mode, flags, _ptr(off), _ptr(st), None, 0, None,
+                       int(join[0]), float(join[1]))
     return rc, off[: len(descs)], st[: len(descs)]
 
 
+def stroke(width, height, fmt, ops, clr=(255, 255, 255, 255), init=None, transform=(1, 0, 0, 0, 1, 0),
+           tolerance=0.3, pen_width=1.0, join=(0, 4.0)):
+    """One Plotter::stroke on a host raster through the mirrored stages."""
+    return fill(width, height, fmt, 0, ops, clr, init, transform, tolerance, pen_width, mode=MODE_STROKE, join=join)
+
+
 def fill(width, height, fmt, rule, ops, clr=(255, 255, 255, 255), init=None, transform=(1, 0, 0, 0, 1, 0),
-         tolerance=0.3, pen_width=1.0, raw=False):
+         tolerance=0.3, pen_width=1.0, raw=False, mode=MODE_FILL, join=(0, 4.0)):
     """One Plotter::fill on a host raster through the mirrored stages."""
     from footile_amd._lib import FillDescC
     from footile_amd.path import ops_array
@@ -70,5 +81,5 @@ def fill(width, height, fmt, rule, ops, clr=(255, 255, 255, 255), init=None, tra
     for i, v in enumerate(transform):
         d.transform[i] = float(v)
     d.tolerance, d.pen_width, d.fresh = float(tolerance), float(pen_width), 0
-    rc, off, st = run_batch(a, [d], raw=raw)
+    rc, off, st = run_batch(a, [d], raw=raw, mode=mode, join=join)
     return rc, px, (int(off[0]), int(st[0]))

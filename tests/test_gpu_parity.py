@@ -244,3 +244,60 @@ def test_batch_matches_individual_plotters(ft, oracle):
         want = o.pixels().reshape(s, s)
         assert np.array_equal(img[y:y + s, x:x + s], want), f"tile {t}"
         assert (offered[t], stored[t]) == o.last_counts()
+
+
+def _atlas_batch(n_paths=96, side=1024):
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    ops, tiles = workloads.atlas(2, n_paths=n_paths, atlas_side=side)
+    atlas = DeviceBuffer(side * side)
+    atlas.upload(np.full(side * side, 77, dtype=np.uint8))
+    descs = [FillDesc(int(b), int(c), atlas.ptr + int(y) * side + int(x), int(s), int(s), side,
+                      fresh=True) for b, c, x, y, s in tiles]
+    return Batch(ops, descs), atlas, descs, ops, tiles
+
+
+def test_batch_tuning_keeps_rasters(ft, oracle):
+    """Every raster geometry gives the same bytes (and the oracle's)."""
+    batch, atlas, descs, ops, tiles = _atlas_batch()
+    batch.run()
+    want = atlas.download()
+    for t, (b, c, x, y, s) in enumerate(tiles[:24]):
+        o = oracle.Plotter(int(s), int(s))
+        assert o.fill(0, ops[b:b + c]) == 0
+        assert np.array_equal(want.reshape(1024, 1024)[y:y + s, x:x + s].reshape(-1), o.pixels()), f"tile {t}"
+    for band_rows, cells in [(1, 1024), (4, 2048), (64, 16384), (256, 8192), (16, 4096)]:
+        batch.set_tuning(band_rows, cells)
+        assert batch.tuning() == (band_rows, cells)
+        atlas.upload(np.full(1024 * 1024, 77, dtype=np.uint8))
+        batch.run()
+        assert np.array_equal(atlas.download(), want), (band_rows, cells)
+    with pytest.raises(ft.FootileError):
+        batch.set_tuning(0, 4096)
+    with pytest.raises(ft.FootileError):
+        batch.set_tuning(16, 1000)
+
+
+def test_batch_graph_replays_match_eager(ft, oracle):
+    """Graph mode: several replays with eager work and synchronisation in
+    between; every replay equals the eager result."""
+    batch, atlas, descs, ops, tiles = _atlas_batch()
+    batch.run()
+    want = atlas.download()
+    offered, stored, _, tops = batch.results()
+    batch.set_graph(True)
+    other = ft.Plotter(ft.Raster.with_clear(128, 96))
+    for rep in range(4):
+        atlas.upload(np.full(1024 * 1024, 77 + rep, dtype=np.uint8))
+        other.fill(0, workloads.plain_256(rep))  # eager work on another stream between replays
+        batch.launch()
+        batch.sync()
+        assert np.array_equal(atlas.download(), want), f"replay {rep}"
+        o2, s2, _, t2 = batch.results()
+        assert np.array_equal(o2, offered) and np.array_equal(s2, stored) and np.array_equal(t2, tops)
+    # a re-layout drops the graph; the next launch captures again
+    batch.set_tuning(8, 8192)
+    atlas.upload(np.full(1024 * 1024, 5, dtype=np.uint8))
+    batch.launch()
+    batch.sync()
+    assert np.array_equal(atlas.download(), want)
+    batch.set_graph(False)
