@@ -286,7 +286,12 @@ def main():
     descs = [FillDesc(0, len(ops), A.ptr, W, H, W * 4, RGBA8P, rule=0, clr=clr),
              FillDesc(len(ops), len(ops), B.ptr, W, H, W * 4, RGBA8P, rule=1, clr=clr)]
     batch = Batch(both, descs)
-    batch.run()  # first step on fresh rasters: the parity sample
+    from footile_amd.tune import autotune
+    tuned = autotune(batch, descs, len(both), cache=None)
+    print(f"[bench] tuned: {tuned}", file=sys.stderr, flush=True)
+    A.zero()
+    B.zero()
+    batch.run()  # first step on zeroed rasters with the tuned geometry: the parity sample
     img_a, img_b = A.download(), B.download()
     offered, stored, edges, tops = batch.results()
     px_per_step = int(sum(W * (H - max(int(t), 0)) for t in tops if t < H))
@@ -329,6 +334,8 @@ def main():
         "points_stored_per_fill": int(stored[0]),
         "edges_per_fill": int(edges[0]),
         "top_rows": [int(t) for t in tops],
+        "tuning": {"band_rows_min": tuned["band_rows_min"], "lds_cells": tuned["lds_cells"],
+                   "dst_prefetch": tuned["dst_prefetch"], "candidates": tuned["tried"]},
     }
     if not args.no_extra:
         print("[bench] atlas", file=sys.stderr, flush=True)

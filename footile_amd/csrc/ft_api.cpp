@@ -311,6 +311,7 @@ ft_status ft_batch_set_tuning(ft_batch *b, const ft_tuning *t) {
     ft::Tuning tu;
     tu.band_rows_min = t->band_rows_min;
     tu.lds_cells = t->lds_cells;
+    tu.dst_prefetch = t->dst_prefetch ? 1u : 0u;
     const ft::Tuning old = b->ws.tuning;
     b->ws.tuning = tu;
     const ft_status s = ft::ws_set_batch(b->ws, b->ops.data(), b->ops.size(), b->descs.data(), b->descs.size());
@@ -325,6 +326,7 @@ ft_status ft_batch_get_tuning(const ft_batch *b, ft_tuning *t) {
     if (!b || !t) return FT_EINVAL;
     t->band_rows_min = b->ws.tuning.band_rows_min;
     t->lds_cells = b->ws.tuning.lds_cells;
+    t->dst_prefetch = b->ws.tuning.dst_prefetch;
     return FT_OK;
 }
 

@@ -571,9 +571,31 @@ __device__ inline uint32_t wave_incl_u32(uint32_t v) {
 // Prefix-sum cells [x_begin, W) of one raster row (one wavefront) starting
 // from `carry` (the sum of the cells left of x_begin), map coverage with the
 // fill rule and store / composite CPT pixels per lane.
+// Destination chunks (16 bytes per lane) a wavefront loads before its cell
+// pass, in the order the row pass consumes them, so that the HBM latency of
+// the read-modify-write formats overlaps the LDS work.
+constexpr uint32_t PRE = 4;
+
+struct Prefetch {
+    uint4 q0, q1, q2, q3;  // chunks 0..3 in consumption order
+    uint32_t n;            // chunks loaded
+};
+
+// Chunk i by masks (a select chain would be turned into an indexed stack load).
+__device__ inline uint4 pre_pick(const Prefetch &p, uint32_t i) {
+    const uint32_t m0 = 0u - (uint32_t)(i == 0), m1 = 0u - (uint32_t)(i == 1);
+    const uint32_t m2 = 0u - (uint32_t)(i == 2), m3 = 0u - (uint32_t)(i == 3);
+    uint4 r;
+    r.x = (p.q0.x & m0) | (p.q1.x & m1) | (p.q2.x & m2) | (p.q3.x & m3);
+    r.y = (p.q0.y & m0) | (p.q1.y & m1) | (p.q2.y & m2) | (p.q3.y & m3);
+    r.z = (p.q0.z & m0) | (p.q1.z & m1) | (p.q2.z & m2) | (p.q3.z & m3);
+    r.w = (p.q0.w & m0) | (p.q1.w & m1) | (p.q2.w & m2) | (p.q3.w & m3);
+    return r;
+}
+
 template <int FMT, int CPT>
-__device__ void row_pass(const int32_t *cells, uint32_t x_begin, uint32_t W, uint8_t *row, uint32_t rule,
-                         uint32_t clr, bool fresh, uint32_t carry) {
+__device__ __forceinline__ void row_pass(const int32_t *cells, uint32_t x_begin, uint32_t W, uint8_t *row, uint32_t rule,
+                         uint32_t clr, bool fresh, uint32_t carry, const Prefetch pf, uint32_t &next) {
     constexpr int BPP = FMT == FMT_RGBA8P ? 4 : (FMT == FMT_GRAYA8P ? 2 : 1);
     const int lane = threadIdx.x & 63;
     uint32_t sw[4] = {0, 0, 0, 0};
@@ -584,6 +606,7 @@ __device__ void row_pass(const int32_t *cells, uint32_t x_begin, uint32_t W, uin
         sw[1] = ch8_widen((clr >> 8) & 0xFFu);
     }
     for (uint32_t c0 = x_begin; c0 < W; c0 += 64u * CPT) {
+        const uint32_t ci = next++;
         const uint32_t x = c0 + (uint32_t)lane * CPT;
         uint32_t v[CPT];
         uint32_t run = 0;
@@ -623,7 +646,8 @@ __device__ void row_pass(const int32_t *cells, uint32_t x_begin, uint32_t W, uin
                 if (x + j < W) dst[j] = a[j];
         } else {
             if (full && CPT * BPP == 16) {
-                uint4 px = fresh ? make_uint4(0, 0, 0, 0) : *reinterpret_cast<const uint4 *>(dst);
+                uint4 px = fresh ? make_uint4(0, 0, 0, 0)
+                                 : (ci < pf.n ? pre_pick(pf, ci) : *reinterpret_cast<const uint4 *>(dst));
                 uint8_t *pb = reinterpret_cast<uint8_t *>(&px);
 #pragma unroll
                 for (int j = 0; j < CPT; ++j) {
@@ -649,6 +673,7 @@ __device__ void row_pass(const int32_t *cells, uint32_t x_begin, uint32_t W, uin
 // One block per (band, row group): `rblock[b]` = (global band, group).  The
 // block accumulates the band's listed edges for its G rows in LDS and scans
 // each row into the raster; the band's other groups are other blocks.
+template <bool PF>
 __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *paths, const uint2 *rblock,
                                                   const uint32_t *band_path, const uint32_t *band_off,
                                                   const uint32_t *band_cnt, const uint32_t *band_list,
@@ -697,31 +722,9 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
         s_fwd = path_forward(S, L, pts, pt_run, run_start, run_end, bad) ? 1 : 0;
         if (bad) raise_err(st, FT_EDEVICE);
     }
-    for (uint32_t c = threadIdx.x; c < rows * Wp; c += BLOCK) s_cells[c] = 0;
-    __syncthreads();
-    const int fwd = s_fwd;
-    const int32_t shift = top_row < 0 ? top_row : 0;
-    const uint32_t off = band_off[gb], cnt = band_cnt[gb];
-    const bool list_ok = (uint64_t)off + cnt <= L.n_pairs;
-    if (!list_ok && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
-    for (uint32_t k = threadIdx.x; list_ok && k < cnt; k += BLOCK) {
-        const uint32_t ei = band_list[off + k];
-        if (ei >= L.cap_points) {
-            raise_err(st, FT_EDEVICE);
-            continue;
-        }
-        const EdgeRec E = edges[ei];
-        const int32_t ed = ((int)((E.flags >> 1) & 1u) == fwd) ? 1 : -1;
-        int64_t a = (int64_t)fx_int(E.y_upper) - shift, b = (int64_t)fx_int(E.y_lower) - shift;
-        if (a < ry0) a = ry0;
-        if (b > (int64_t)y1 - 1) b = (int64_t)y1 - 1;
-        for (int64_t r = a; r <= b; ++r)
-            edge_row_cells<DevAtom>(E, (int32_t)(r + shift), ed, (int32_t)W, s_cells + (size_t)(r - ry0) * Wp);
-    }
-    __syncthreads();
-    // Row pass.  Groups of at least 4 rows give each wavefront whole rows; a
-    // group of 1-3 rows splits every row into segments, one per wavefront,
-    // whose cell sums are exchanged through LDS to form the carry-in.
+    // Row pass assignment.  Groups of at least 4 rows give each wavefront
+    // whole rows; a group of 1-3 rows splits every row into segments, one per
+    // wavefront, whose cell sums are exchanged through LDS as carry-in.
     const int wid = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     constexpr uint32_t NW = BLOCK / 64;
@@ -730,13 +733,81 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     const uint32_t seg = rows >= NW ? 1u : NW / rows;
     const uint32_t chunk = 64u * cpt;
     const uint32_t seg_len = seg == 1 ? W : ((W + seg - 1) / seg + chunk - 1) / chunk * chunk;
-    uint32_t r0 = wid, rstep = NW, carry = 0, xb = 0, xe = W;
+    uint32_t r0 = wid, rstep = NW, xb = 0, xe = W;
     if (seg > 1) {
         r0 = wid / seg;
         rstep = rows;  // one row per wavefront group
-        const uint32_t sgi = wid % seg;
-        xb = sgi * seg_len;
+        xb = (wid % seg) * seg_len;
         xe = xb + seg_len < W ? xb + seg_len : W;
+    }
+    // load the first destination chunks of a read-modify-write format now
+    Prefetch pf;
+    pf.q0 = pf.q1 = pf.q2 = pf.q3 = make_uint4(0, 0, 0, 0);
+    pf.n = 0;
+    uint32_t next_chunk = 0;
+    if (PF && !fresh && P.fmt != FMT_MATTE8 && xb < xe && r0 < rows) {
+        const uint32_t cpr = (xe - xb + chunk - 1) / chunk;
+        const uint32_t nrows = (rows - r0 + rstep - 1) / rstep;
+        const uint32_t total = cpr * nrows;
+        pf.n = total < PRE ? total : PRE;
+        const uint32_t bpp_l = P.fmt == FMT_RGBA8P ? 4u : 2u;
+        auto load = [&](uint32_t k) {
+            const uint32_t rr = r0 + (k / cpr) * rstep;
+            const uint32_t x = xb + (k % cpr) * chunk + (uint32_t)lane * cpt;
+            const uint8_t *dst = base + (size_t)(ry0 + (rr < rows ? rr : 0)) * P.pitch + (size_t)x * bpp_l;
+            const bool full = (k < pf.n) & (x + cpt <= xe) & ((((uintptr_t)dst) & 15) == 0);
+            return full ? *reinterpret_cast<const uint4 *>(dst) : make_uint4(0, 0, 0, 0);
+        };
+        pf.q0 = load(0);
+        pf.q1 = load(1);
+        pf.q2 = load(2);
+        pf.q3 = load(3);
+    }
+    for (uint32_t c = threadIdx.x; c < rows * Wp; c += BLOCK) s_cells[c] = 0;
+    __syncthreads();
+    const int fwd = s_fwd;
+    const int32_t shift = top_row < 0 ? top_row : 0;
+    const uint32_t off = band_off[gb], cnt = band_cnt[gb];
+    const bool list_ok = (uint64_t)off + cnt <= L.n_pairs;
+    if (!list_ok && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
+    // the band's edges, four per thread per round: list entries, then edge
+    // records, then their cells
+    constexpr int EB = 4;
+    for (uint32_t k0 = 0; list_ok && k0 < cnt; k0 += (uint32_t)BLOCK * EB) {
+        uint32_t ei[EB];
+        bool ok[EB];
+#pragma unroll
+        for (int j = 0; j < EB; ++j) {
+            const uint32_t k = k0 + (uint32_t)j * BLOCK + threadIdx.x;
+            ok[j] = k < cnt;
+            ei[j] = band_list[off + (ok[j] ? k : 0u)];
+        }
+        EdgeRec E[EB];
+#pragma unroll
+        for (int j = 0; j < EB; ++j) {
+            if (ok[j] & (ei[j] >= L.cap_points)) raise_err(st, FT_EDEVICE);
+            ok[j] = ok[j] & (ei[j] < L.cap_points);
+            E[j] = edges[ok[j] ? ei[j] : 0u];
+        }
+        auto cells_of = [&](const EdgeRec &e, bool on) {
+            if (!on) return;
+            const int32_t ed = ((int)((e.flags >> 1) & 1u) == fwd) ? 1 : -1;
+            int64_t a = (int64_t)fx_int(e.y_upper) - shift, b = (int64_t)fx_int(e.y_lower) - shift;
+            if (a < ry0) a = ry0;
+            if (b > (int64_t)y1 - 1) b = (int64_t)y1 - 1;
+            for (int64_t r = a; r <= b; ++r)
+                edge_row_cells<DevAtom>(e, (int32_t)(r + shift), ed, (int32_t)W, s_cells + (size_t)(r - ry0) * Wp);
+        };
+        static_assert(EB == 4, "four explicit calls");
+        cells_of(E[0], ok[0]);
+        cells_of(E[1], ok[1]);
+        cells_of(E[2], ok[2]);
+        cells_of(E[3], ok[3]);
+    }
+    __syncthreads();
+    uint32_t carry = 0;
+    if (seg > 1) {
+        const uint32_t sgi = wid % seg;
         uint32_t sum = 0;
         if (r0 < rows)
             for (uint32_t x = xb + lane; x < xe; x += 64) sum += (uint32_t)s_cells[(size_t)r0 * Wp + x];
@@ -753,12 +824,12 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
         switch (P.fmt) {
         case FMT_MATTE8:
             if (cpt == 16)
-                row_pass<FMT_MATTE8, 16>(cells, xb, xe, row, P.rule, P.clr, fresh, carry);
+                row_pass<FMT_MATTE8, 16>(cells, xb, xe, row, P.rule, P.clr, fresh, carry, pf, next_chunk);
             else
-                row_pass<FMT_MATTE8, 4>(cells, xb, xe, row, P.rule, P.clr, fresh, carry);
+                row_pass<FMT_MATTE8, 4>(cells, xb, xe, row, P.rule, P.clr, fresh, carry, pf, next_chunk);
             break;
-        case FMT_RGBA8P: row_pass<FMT_RGBA8P, 4>(cells, xb, xe, row, P.rule, P.clr, fresh, carry); break;
-        default: row_pass<FMT_GRAYA8P, 8>(cells, xb, xe, row, P.rule, P.clr, fresh, carry); break;
+        case FMT_RGBA8P: row_pass<FMT_RGBA8P, 4>(cells, xb, xe, row, P.rule, P.clr, fresh, carry, pf, next_chunk); break;
+        default: row_pass<FMT_GRAYA8P, 8>(cells, xb, xe, row, P.rule, P.clr, fresh, carry, pf, next_chunk); break;
         }
     }
 }
@@ -1154,10 +1225,12 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
                                        ws.d_edges, ws.d_band_off, ws.d_band_cur, ws.d_band_list, ws.cap_pairs,
                                        ws.d_pstate, st);
             if (ev) FT_HIP(hipEventRecord(ev[1], s));
-            if (ws.n_rblocks)
-                k_raster<<<(uint32_t)ws.n_rblocks, BLOCK, ws.lds_cells * sizeof(int32_t), s>>>(
+            if (ws.n_rblocks) {
+                auto kr = ws.tuning.dst_prefetch ? k_raster<true> : k_raster<false>;
+                kr<<<(uint32_t)ws.n_rblocks, BLOCK, ws.lds_cells * sizeof(int32_t), s>>>(
                     L0, ws.d_paths, ws.d_rblock, ws.d_band_path, ws.d_band_off, ws.d_band_cnt, ws.d_band_list,
                     ws.d_edges, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end, ws.d_pstate, st);
+            }
             if (ev) FT_HIP(hipEventRecord(ev[2], s));
         }
     }

@@ -803,6 +803,7 @@ constexpr uint32_t CELL_ALIGN = 16;   // row stride of cells in LDS
 struct Tuning {
     uint32_t band_rows_min = BAND_ROWS_MIN;  // 1 .. 256
     uint32_t lds_cells = LDS_CELLS;          // 1024 .. 16384 (64 KiB), a multiple of CELL_ALIGN
+    uint32_t dst_prefetch = 1;               // raster blocks load their destination chunks first
 };
 
 FT_HD inline uint32_t cell_stride(uint32_t width) { return (width + CELL_ALIGN - 1) / CELL_ALIGN * CELL_ALIGN; }

@@ -219,7 +219,8 @@ class FillDesc:
 
 
 class _Tuning(ctypes.Structure):
-    _fields_ = [("band_rows_min", ctypes.c_uint32), ("lds_cells", ctypes.c_uint32)]
+    _fields_ = [("band_rows_min", ctypes.c_uint32), ("lds_cells", ctypes.c_uint32),
+                ("dst_prefetch", ctypes.c_uint32)]
 
 
 class Batch:
@@ -258,15 +259,15 @@ class Batch:
               "ft_batch_timing")
         return t.value, r.value, n.value
 
-    def set_tuning(self, band_rows_min, lds_cells):
+    def set_tuning(self, band_rows_min, lds_cells, dst_prefetch=True):
         """Raster geometry (ft_batch_set_tuning); re-lays the batch out."""
-        t = _Tuning(int(band_rows_min), int(lds_cells))
+        t = _Tuning(int(band_rows_min), int(lds_cells), 1 if dst_prefetch else 0)
         check(lib().ft_batch_set_tuning(self._h, ctypes.byref(t)), "ft_batch_set_tuning")
 
     def tuning(self):
         t = _Tuning()
         check(lib().ft_batch_get_tuning(self._h, ctypes.byref(t)), "ft_batch_get_tuning")
-        return t.band_rows_min, t.lds_cells
+        return t.band_rows_min, t.lds_cells, bool(t.dst_prefetch)
 
     def set_graph(self, on=True):
         """Replay launches from a captured HIP graph (ft_batch_set_graph)."""

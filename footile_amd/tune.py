@@ -1,7 +1,9 @@
 """Autotuner of a batch's raster geometry (ft_batch_set_tuning).
 
-Candidates are (band_rows_min, lds_cells) pairs: band height for edge
-binning, and the LDS cell budget that fixes the rows per raster block.  Each
+Candidates are (band_rows_min, lds_cells, dst_prefetch): band height for
+edge binning, the LDS cell budget that fixes the rows per raster block, and
+(for read-modify-write rasters) whether raster blocks load their destination
+before the cell pass.  Each
 candidate is timed on the batch itself with device events (whole launch
 sequence, median of a few launches after one warm-up run); the fastest wins.
 Results are cached in a JSON file keyed by a signature of the batch layout
@@ -56,21 +58,24 @@ def autotune(batch, descs, n_ops, cache=DEFAULT_CACHE, reps=3, candidates=None, 
             table = {}
     if key in table:
         e = table[key]
-        batch.set_tuning(e["band_rows_min"], e["lds_cells"])
+        batch.set_tuning(e["band_rows_min"], e["lds_cells"], e.get("dst_prefetch", True))
         return dict(e, cached=True, tried=0)
-    cands = candidates or [(b, c) for b in BAND_ROWS for c in LDS_CELLS]
+    # the destination prefetch only matters for read-modify-write rasters
+    rmw = any(int(d.c.fmt) != 0 and not int(d.c.fresh) for d in descs)
+    prefetch = (True, False) if rmw else (True,)
+    cands = candidates or [(b, c, p) for b in BAND_ROWS for c in LDS_CELLS for p in prefetch]
     best = None
     t0 = time.perf_counter()
-    for b, c in cands:
-        batch.set_tuning(b, c)
+    for b, c, p in cands:
+        batch.set_tuning(b, c, p)
         batch.run()
         ms = _time(batch, reps)
         if verbose:
-            print(f"[tune] band_rows_min={b} lds_cells={c}: {ms:.3f} ms")
-        if best is None or ms < best[2]:
-            best = (b, c, ms)
-    batch.set_tuning(best[0], best[1])
-    entry = {"band_rows_min": best[0], "lds_cells": best[1], "ms": best[2],
+            print(f"[tune] band_rows_min={b} lds_cells={c} dst_prefetch={p}: {ms:.3f} ms")
+        if best is None or ms < best[3]:
+            best = (b, c, p, ms)
+    batch.set_tuning(best[0], best[1], best[2])
+    entry = {"band_rows_min": best[0], "lds_cells": best[1], "dst_prefetch": best[2], "ms": best[3],
              "tune_s": time.perf_counter() - t0}
     if cache:
         try:

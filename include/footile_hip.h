@@ -160,6 +160,8 @@ ft_status ft_batch_counts(ft_batch *b, uint64_t *offered, uint64_t *stored, uint
 typedef struct ft_tuning {
     uint32_t band_rows_min;
     uint32_t lds_cells;
+    uint32_t dst_prefetch; /* 1: raster blocks of Rgba8p/Graya8p read their
+                              destination before the cell pass */
 } ft_tuning;
 ft_status ft_batch_set_tuning(ft_batch *b, const ft_tuning *t);
 ft_status ft_batch_get_tuning(const ft_batch *b, ft_tuning *t);

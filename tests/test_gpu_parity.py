@@ -265,9 +265,10 @@ def test_batch_tuning_keeps_rasters(ft, oracle):
         o = oracle.Plotter(int(s), int(s))
         assert o.fill(0, ops[b:b + c]) == 0
         assert np.array_equal(want.reshape(1024, 1024)[y:y + s, x:x + s].reshape(-1), o.pixels()), f"tile {t}"
-    for band_rows, cells in [(1, 1024), (4, 2048), (64, 16384), (256, 8192), (16, 4096)]:
-        batch.set_tuning(band_rows, cells)
-        assert batch.tuning() == (band_rows, cells)
+    for band_rows, cells, pf in [(1, 1024, True), (4, 2048, False), (64, 16384, True), (256, 8192, False),
+                                 (16, 4096, True)]:
+        batch.set_tuning(band_rows, cells, pf)
+        assert batch.tuning() == (band_rows, cells, pf)
         atlas.upload(np.full(1024 * 1024, 77, dtype=np.uint8))
         batch.run()
         assert np.array_equal(atlas.download(), want), (band_rows, cells)
@@ -301,3 +302,42 @@ def test_batch_graph_replays_match_eager(ft, oracle):
     batch.sync()
     assert np.array_equal(atlas.download(), want)
     batch.set_graph(False)
+
+
+def test_batch_rmw_formats_with_and_without_prefetch(ft, oracle):
+    """Rgba8p / Graya8p batches over existing content: the destination
+    prefetch on and off give the oracle's bytes."""
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    rng = np.random.default_rng(31)
+    for fmt in (RGBA8P, GRAYA8P):
+        bpp = BPP[fmt]
+        side = 512
+        init = rng.integers(0, 256, size=side * side * bpp).astype(np.uint8)
+        dev = DeviceBuffer(side * side * bpp)
+        ops_all, descs, tiles = [], [], []
+        at = 0
+        for t in range(12):  # disjoint tiles: fills of one batch run concurrently
+            w, h = int(rng.integers(40, 168)), int(rng.integers(8, 126))
+            x, y = (t % 3) * 170 + int(rng.integers(0, 170 - w)), (t // 3) * 128 + int(rng.integers(0, 128 - h))
+            ops = random_path(rng, w, h, n_sub=2, kinds=(LINE, QUAD, CUBIC))
+            clr = tuple(int(v) for v in rng.integers(0, 256, size=4))
+            if fmt == RGBA8P:
+                clr = (min(clr[0], clr[3]), min(clr[1], clr[3]), min(clr[2], clr[3]), clr[3])
+            descs.append(FillDesc(at, len(ops), dev.ptr + (y * side + x) * bpp, w, h, side * bpp, fmt,
+                                  rule=t % 2, clr=clr))
+            tiles.append((x, y, w, h, ops, clr, t % 2))
+            ops_all.append(ops)
+            at += len(ops)
+        batch = Batch(np.concatenate(ops_all), descs)
+        for pf in (True, False):
+            dev.upload(init)
+            batch.set_tuning(16, 4096, pf)
+            batch.run()
+            got = dev.download().reshape(side, side * bpp)
+            want = init.reshape(side, side * bpp).copy()
+            for x, y, w, h, ops, clr, rule in tiles:
+                o = oracle.Plotter(w, h, fmt)
+                o.set_pixels(want[y:y + h, x * bpp:(x + w) * bpp].reshape(-1))
+                assert o.fill(rule, ops, clr) == 0
+                want[y:y + h, x * bpp:(x + w) * bpp] = o.pixels().reshape(h, w * bpp)
+            assert np.array_equal(got, want), (fmt, pf)
