@@ -244,9 +244,11 @@ def bench_diff(steps, warmup, check=True):
         gc_all = b["gc"].download(dtype=np.float32).reshape(-1, 4)
         target = b["T"].download(dtype=np.float32).reshape(r.canvas_shape)
         errs_p, errs_c = [], []
+        scale = 1.0 / sc.n_images  # the batched loss is the mean over every image's channels
         for i, (rs, bs, sub, k0, k1) in enumerate(subs[:2]):  # two images against FP64 (own params each)
             _, want_gp, want_gc = pd.Scene(1, sub.width, sub.height, sub.path_image, sub.path_cubics).backward(
                 sub.points, sub.colours, target[i:i + 1])
+            want_gp, want_gc = want_gp * scale, want_gc * scale
             errs_p.append(_grad_rel(gp_all[pt_first[k0]:pt_first[k1]], want_gp))
             errs_c.append(_grad_rel(gc_all[k0:k1], want_gc))
         c5["grad_rel_err_points_images_0_1"] = max(errs_p)

