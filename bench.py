@@ -265,6 +265,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the atlas, plain and differentiable measurements")
     ap.add_argument("--no-diff-check", action="store_true", help="skip the FP64 gradient checks")
+    ap.add_argument("--tuning", default=None,
+                    help="band_rows_min,lds_cells,dst_prefetch for the random-walk batch instead of autotuning "
+                         "(profiling runs)")
+    ap.add_argument("--only", choices=["atlas", "diff"], default=None,
+                    help="run only the atlas or the differentiable measurement and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -277,6 +282,13 @@ def main():
     from footile_amd.pixel import RGBA8P
     from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
 
+    if args.only == "atlas":
+        print(json.dumps(bench_atlas(ft, args.steps, args.warmup)), flush=True)
+        return
+    if args.only == "diff":
+        print(json.dumps(bench_diff(args.steps, args.warmup, check=not args.no_diff_check)), flush=True)
+        return
+
     W = H = 4096
     clr = (200, 120, 40, 255)
     ops = workloads.random_walk(1)
@@ -288,8 +300,13 @@ def main():
     descs = [FillDesc(0, len(ops), A.ptr, W, H, W * 4, RGBA8P, rule=0, clr=clr),
              FillDesc(len(ops), len(ops), B.ptr, W, H, W * 4, RGBA8P, rule=1, clr=clr)]
     batch = Batch(both, descs)
-    from footile_amd.tune import autotune
-    tuned = autotune(batch, descs, len(both), cache=None)
+    if args.tuning:
+        b_rows, cells, pf = (int(v) for v in args.tuning.split(","))
+        batch.set_tuning(b_rows, cells, bool(pf))
+        tuned = {"band_rows_min": b_rows, "lds_cells": cells, "dst_prefetch": bool(pf), "tried": 0}
+    else:
+        from footile_amd.tune import autotune
+        tuned = autotune(batch, descs, len(both), cache=None)
     print(f"[bench] tuned: {tuned}", file=sys.stderr, flush=True)
     A.zero()
     B.zero()

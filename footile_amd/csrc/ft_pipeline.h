@@ -52,6 +52,7 @@ struct Workspace {
     I2 *d_pts = nullptr;
     uint32_t *d_pt_run = nullptr;
     EdgeRec *d_edges = nullptr;
+    U2 *d_pt_band = nullptr;  // per stored point: its edge's band range
     // stroker stages (MODE_STROKE), sized by cap_events
     size_t cap_stroke = 0;
     unsigned long long *d_st_flags = nullptr, *d_st_off = nullptr;

@@ -305,33 +305,34 @@ __global__ __launch_bounds__(BLOCK) void k_top(Limits L0, const I2 *pts, const u
 // ---------------------------------------------------------------------------
 // Stage 5: edges and row-band binning.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, const uint32_t *ev_meta, const uint32_t *ev_kept,
-                                                 const uint32_t *ev_idx, const uint32_t *run_path,
+// One thread per stored point [0, n_kept): Edge::new, the band range as a
+// difference, and the point's band record for k_bin.
+__global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, const uint32_t *pt_run, const uint32_t *run_path,
                                                  const uint32_t *run_start, const uint32_t *run_end,
-                                                 const PathDev *paths, const I2 *pts, EdgeRec *edges,
+                                                 const PathDev *paths, const I2 *pts, EdgeRec *edges, U2 *pt_band,
                                                  uint32_t *band_diff, PathState *ps, DevStatus *st) {
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    const uint32_t n = st->n_kept < L.cap_points ? st->n_kept : L.cap_points;
     uint32_t bad = 0;
-    for (uint32_t b = blockIdx.x * BLOCK; b < L.n_events; b += gridDim.x * BLOCK) {
-        const uint32_t e = b + threadIdx.x;
+    for (uint32_t b = blockIdx.x * BLOCK; b < n; b += gridDim.x * BLOCK) {
+        const uint32_t i = b + threadIdx.x;
         EdgeOut r;
         r.g0 = r.g1 = 0;
         r.banded = false;
-        if (e < L.n_events)
-            r = edge_event<DevAtom>(e, L, ev_meta, ev_kept, ev_idx, run_path, run_start, run_end, paths, pts, edges,
-                                    ps, bad);
+        if (i < n) {
+            r = edge_point<DevAtom>(i, L, pt_run, run_path, run_start, run_end, paths, pts, edges, ps, bad);
+            pt_band[i] = band_record(r);
+        }
         wave_add_runs(band_diff, L.n_bands, r.banded, r.g0, 1u);
         wave_add_runs(band_diff, L.n_bands, r.banded & (r.g1 + 1 < L.n_bands), r.g1 + 1, 0xFFFFFFFFu);
     }
     if (bad) raise_err(st, FT_EDEVICE);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, const uint32_t *ev_meta, const uint32_t *ev_kept,
-                                               const uint32_t *ev_idx, const uint32_t *run_path,
-                                               const PathDev *paths, const EdgeRec *edges,
-                                               const uint32_t *band_off, uint32_t *band_cur, uint32_t *band_list,
-                                               uint64_t cap_pairs, const PathState *ps, DevStatus *st) {
+__global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, const U2 *pt_band, const uint32_t *band_off,
+                                               uint32_t *band_cur, uint32_t *band_list, uint64_t cap_pairs,
+                                               DevStatus *st) {
     if (run_failed(st)) return;
     const uint32_t n_pairs = st->n_pairs;
     if (n_pairs > cap_pairs) {
@@ -342,12 +343,12 @@ __global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, const uint32_t *ev_met
         return;
     }
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    const uint32_t n = st->n_kept < L.cap_points ? st->n_kept : L.cap_points;
     uint32_t bad = 0;
-    for (uint32_t b = blockIdx.x * BLOCK; b < L.n_events; b += gridDim.x * BLOCK) {
-        const uint32_t e = b + threadIdx.x;
-        const BinItem it = bin_item(e < L.n_events ? e : 0, L, ev_meta, ev_kept, ev_idx, run_path, paths, edges,
-                                    ps, bad);
-        const bool has = it.banded & (e < L.n_events);
+    for (uint32_t b = blockIdx.x * BLOCK; b < n; b += gridDim.x * BLOCK) {
+        const uint32_t i = b + threadIdx.x;
+        const BinItem it = bin_item(i < n ? i : 0u, pt_band);
+        const bool has = it.banded & (i < n) & (it.g1 < L.n_bands);
         const uint32_t span = has ? it.g1 - it.g0 + 1 : 0u;
         // the wavefront walks the bands of its edges together, so that lanes
         // at the same band share one atomic
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, const uint32_t *ev_met
             const uint32_t slot = wave_add_runs(band_cur, L.n_bands, on, g, 1u);
             const uint32_t gi = on ? g : 0;
             const uint32_t at = band_off[gi] + slot;
-            if (on & (at < L.n_pairs)) band_list[at] = it.idx;
+            if (on & (at < L.n_pairs)) band_list[at] = i;
             bad |= (on & (at >= L.n_pairs)) ? 1u : 0u;
         }
     }
@@ -889,7 +890,7 @@ void ws_free(Workspace &ws) {
                     ws.d_st_flags, ws.d_st_off, ws.d_st_last, ws.d_spts, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
                     ws.d_sub_joined, ws.d_it_cnt, ws.d_it_off,
                     ws.d_run_first, ws.d_run_close, ws.d_run_path, ws.d_rblock, ws.d_run_start, ws.d_run_end, ws.d_pts,
-                    ws.d_pt_run, ws.d_edges, ws.d_band_cnt, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
+                    ws.d_pt_run, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
                     ws.d_scan_flags, ws.d_scan_agg, ws.d_scan_incl, ws.d_pstate, ws.d_status};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -913,7 +914,7 @@ static ft_status ensure_event_capacity(Workspace &ws, size_t need) {
         (s = dev_alloc(ws.d_run_first, cap)) || (s = dev_alloc(ws.d_run_close, cap)) ||
         (s = dev_alloc(ws.d_run_path, cap)) || (s = dev_alloc(ws.d_run_start, cap)) ||
         (s = dev_alloc(ws.d_run_end, cap)) || (s = dev_alloc(ws.d_pts, cap)) || (s = dev_alloc(ws.d_pt_run, cap)) ||
-        (s = dev_alloc(ws.d_edges, cap))) {
+        (s = dev_alloc(ws.d_edges, cap)) || (s = dev_alloc(ws.d_pt_band, cap))) {
         ws.cap_events = 0;
         return s;
     }
@@ -1211,8 +1212,8 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         const size_t top_tile = (size_t)TOP_ITEMS * BLOCK;
         const size_t top_grid = std::max<size_t>(1, std::min<size_t>(GRID_MAX, (ws.cap_events + top_tile - 1) / top_tile));
         k_top<<<(uint32_t)top_grid, BLOCK, 0, s>>>(L0, ws.d_pts, ws.d_pt_run, ws.d_run_path, ws.d_pstate, st);
-        k_edges<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_path, ws.d_run_start,
-                                     ws.d_run_end, ws.d_paths, ws.d_pts, ws.d_edges, ws.d_band_cnt, ws.d_pstate, st);
+        k_edges<<<ge, BLOCK, 0, s>>>(L0, ws.d_pt_run, ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_paths,
+                                     ws.d_pts, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_pstate, st);
         if (ws.n_bands) {
             // band differences -> edges per band (in place; slot 3 is free in fill mode) -> list offsets
             k_scan<uint32_t, false><<<scan_grid(ws.n_bands), BLOCK, 0, s>>>(
@@ -1221,9 +1222,8 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
             k_scan<uint32_t, false><<<scan_grid(ws.n_bands), BLOCK, 0, s>>>(
                 ws.d_band_cnt, ws.d_band_off, nullptr, (uint32_t)ws.n_bands, (uint32_t)ws.n_bands, fl + 2 * tiles,
                 ag + 2 * tiles, in + 2 * tiles, &st->tickets[2], &st->n_pairs, st, true, false);
-            k_bin<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_path, ws.d_paths,
-                                       ws.d_edges, ws.d_band_off, ws.d_band_cur, ws.d_band_list, ws.cap_pairs,
-                                       ws.d_pstate, st);
+            k_bin<<<ge, BLOCK, 0, s>>>(L0, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list, ws.cap_pairs,
+                                       st);
             if (ev) FT_HIP(hipEventRecord(ev[1], s));
             if (ws.n_rblocks) {
                 auto kr = ws.tuning.dst_prefetch ? k_raster<true> : k_raster<false>;

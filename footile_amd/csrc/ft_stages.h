@@ -652,49 +652,57 @@ struct EdgeOut {
 };
 
 template <class A>
-FT_HD inline EdgeOut edge_event(uint32_t e, const Limits &L, const uint32_t *meta, const uint32_t *kept,
-                                const uint32_t *idx_of, const uint32_t *run_path, const uint32_t *run_start,
-                                const uint32_t *run_end, const PathDev *paths, const I2 *pts, EdgeRec *edges,
-                                PathState *ps, uint32_t &bad) {
-    const PointRef r = point_of(e, L, meta, kept, idx_of, run_path, ps, bad);
-    const uint32_t p = r.valid ? r.path : 0, i = r.valid ? r.idx : 0, run = r.valid ? r.run : 0;
+FT_HD inline EdgeOut edge_point(uint32_t i, const Limits &L, const uint32_t *pt_run, const uint32_t *run_path,
+                                const uint32_t *run_start, const uint32_t *run_end, const PathDev *paths,
+                                const I2 *pts, EdgeRec *edges, PathState *ps, uint32_t &bad) {
+    // point i is stored iff pt_run[i] names a run (compact_event writes NO_IDX for capped points)
+    const uint32_t run0 = pt_run[i];
+    const bool rok = run0 < L.n_runs;
+    const uint32_t p0 = run_path[rok ? run0 : 0];
+    const bool valid = rok & (p0 < L.n_paths);
+    bad |= ((run0 != NO_IDX) & !valid) ? 1u : 0u;
+    const uint32_t p = valid ? p0 : 0, run = valid ? run0 : 0;
     const PathState s = ps[p];
     const uint32_t rs = run_start[run], re = run_end[run];
     const uint32_t nx = (i + 1 < re) ? i + 1 : rs;
     const bool nok = nx < L.cap_points;
     const I2 q = pts[i], q1 = pts[nok ? nx : i];
-    const bool has = r.valid & nok & (nx != i) & (q1.y != q.y);
-    bad |= (r.valid & !nok) ? 1u : 0u;
+    const bool has = valid & nok & (nx != i) & (q1.y != q.y);
+    bad |= (valid & !nok) ? 1u : 0u;
     EdgeRec E = make_edge(q, q1);
     if (!has) E.flags = 0;
-    if (r.valid & (point_key(q) == s.topkey)) A::min_u32(&ps[p].topidx, i - s.pt_base);
-    if (r.valid) edges[i] = E;
+    if (valid & (point_key(q) == s.topkey)) A::min_u32(&ps[p].topidx, i - s.pt_base);
+    if (valid) edges[i] = E;
     EdgeOut out;
     out.banded = edge_bands(E, paths[p], s.topkey, L, out.g0, out.g1) & has;
-    out.path = r.path;
-    out.valid = r.valid;
+    out.path = valid ? p : NO_IDX;
+    out.valid = valid;
     out.has = has;
     return out;
 }
 
-// Stage 5b: the edge of event e and its band range, for scattering into the
-// band lists (band_list[band_off[g] + slot] = edge index for g in [g0, g1]).
+// Stage 5b: the band range k_edges recorded for point i, for scattering the
+// edge into the band lists (band_list[band_off[g] + slot] = i, g0 <= g <= g1).
 struct BinItem {
-    uint32_t idx;     // edge index
     uint32_t g0, g1;  // band range
     bool banded;
 };
 
-FT_HD inline BinItem bin_item(uint32_t e, const Limits &L, const uint32_t *meta, const uint32_t *kept,
-                              const uint32_t *idx_of, const uint32_t *run_path, const PathDev *paths,
-                              const EdgeRec *edges, const PathState *ps, uint32_t &bad) {
-    const PointRef r = point_of(e, L, meta, kept, idx_of, run_path, ps, bad);
-    const uint32_t p = r.valid ? r.path : 0, i = r.valid ? r.idx : 0;
-    const EdgeRec E = edges[i];
+FT_HD inline BinItem bin_item(uint32_t i, const U2 *pt_band) {
+    const U2 r = pt_band[i];
     BinItem b;
-    b.idx = i;
-    b.banded = edge_bands(E, paths[p], ps[p].topkey, L, b.g0, b.g1) & r.valid;
+    b.g0 = r.x;
+    b.g1 = r.y;
+    b.banded = r.y >= r.x;
     return b;
+}
+
+// The record bin_item reads: [g0, g1], or an empty range.
+FT_HD inline U2 band_record(const EdgeOut &o) {
+    U2 r;
+    r.x = o.banded ? o.g0 : 1u;
+    r.y = o.banded ? o.g1 : 0u;
+    return r;
 }
 
 // ---------------------------------------------------------------------------

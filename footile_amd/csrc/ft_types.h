@@ -13,6 +13,10 @@ namespace ft {
 struct I2 {  // Fixed point (x, y), 8 bytes (same layout as HIP int2)
     int32_t x, y;
 };
+
+struct U2 {
+    uint32_t x, y;
+};
 struct F4 {  // stroke-mode event (x, y, width, kind), 16 bytes
     float x, y, z, w;
 };

@@ -272,10 +272,12 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
     const size_t n_bands = band_path.size();
     auto edges = buf<EdgeRec>(n_kept);
     auto band_cnt = buf<uint32_t>(n_bands), band_off = buf<uint32_t>(n_bands), band_cur = buf<uint32_t>(n_bands);
-    for (uint32_t e = 0; e < n_events; ++e) {
-        const EdgeOut r = edge_event<HostAtom>(e, L, ev_meta.data(), kept.data(), idx.data(), run_path.data(),
-                                               run_start.data(), run_end.data(), paths.data(), pts.data(),
-                                               edges.data(), ps.data(), bad);
+    auto pt_band = buf<U2>(n_kept);
+    for (uint32_t i = 0; i < n_kept; ++i) {
+        const EdgeOut r = edge_point<HostAtom>(i, L, pt_run.data(), run_path.data(), run_start.data(),
+                                               run_end.data(), paths.data(), pts.data(), edges.data(), ps.data(),
+                                               bad);
+        pt_band[i] = band_record(r);
         if (r.has) ps[r.path].n_edges++;
         if (r.banded) {
             band_cnt[r.g0] += 1u;
@@ -294,14 +296,13 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
     }
     L.n_pairs = n_pairs;
     auto band_list = buf<uint32_t>(n_pairs);
-    for (uint32_t e = 0; e < n_events; ++e) {
-        const BinItem it = bin_item(e, L, ev_meta.data(), kept.data(), idx.data(), run_path.data(), paths.data(),
-                                    edges.data(), ps.data(), bad);
+    for (uint32_t i = 0; i < n_kept; ++i) {
+        const BinItem it = bin_item(i, pt_band.data());
         if (!it.banded) continue;
         for (uint32_t g = it.g0; g <= it.g1; ++g) {
             const uint32_t at = band_off[g] + band_cur[g]++;
             if (at < n_pairs)
-                band_list[at] = it.idx;
+                band_list[at] = i;
             else
                 bad = 1;
         }
