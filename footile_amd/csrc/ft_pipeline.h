@@ -63,7 +63,11 @@ struct Workspace {
     uint32_t join = 0;  // ft_join
     float miter = 4.0f, stroke_tol_sq = 0.09f;
     // bands
+    // pairs, list offsets and fill cursors per (band, replica) slot: band g,
+    // replica r at (g << band_shift) + r
     uint32_t *d_band_cnt = nullptr, *d_band_off = nullptr, *d_band_cur = nullptr;
+    size_t cap_band_slots = 0;
+    uint32_t band_shift = 0;
     size_t cap_pairs = 0;
     uint32_t *d_band_list = nullptr;
     // scan look-back state, SCAN_SLOTS instances

@@ -28,35 +28,6 @@
 
 namespace ft {
 
-// Run-length aggregation of keyed atomic adds across a wavefront.  Lanes with
-// `has` add v to base[key]; lanes holding the same key as their left neighbour
-// form a run (neighbouring events mostly share a band), each run's sum is
-// formed with a segmented scan and its last lane issues one atomic.  Returns
-// the value base[key] had before this lane's add (its slot), for lanes with
-// `has`.  All 64 lanes must be active.
-__device__ inline uint32_t wave_add_runs(uint32_t *base, uint32_t n_keys, bool has, uint32_t key, uint32_t v) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t k = (has & (key < n_keys)) ? key : NO_IDX;
-    const uint32_t x = k != NO_IDX ? v : 0u;
-    const uint32_t kp = __shfl_up(k, 1, 64);
-    const bool head = lane == 0 || kp != k;
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes <= this one
-    const int rs = 63 - __clzll((long long)(heads & le));                       // this lane's run start
-    uint32_t sum = x;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(sum, o, 64);
-        if (lane - o >= rs) sum += u;
-    }
-    const unsigned long long after = heads & ~le;  // heads right of this lane
-    const int tail = after ? __ffsll((long long)after) - 2 : 63;
-    uint32_t old = 0;
-    if (lane == tail && k != NO_IDX && sum) old = atomicAdd(base + k, sum);
-    const uint32_t run_old = __shfl(old, tail, 64);
-    return run_old + (sum - x);
-}
-
 // Block-wide keyed reduction with one atomic (flush) per distinct key per
 // block for the keys of the first two leaders of each wavefront; any other
 // lane flushes alone.  Keys are path indices of consecutive items, so a block
@@ -107,9 +78,9 @@ __device__ inline void block_reduce_keyed(bool has, uint32_t key, T v, T identit
 // Stage 0: clear per-run state (a kernel, not hipMemsetAsync: graph-safe).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, uint32_t n_paths, uint32_t *band_cnt,
-                                                 uint32_t *band_cur, uint32_t n_bands, uint32_t *flags,
+                                                 uint32_t *band_cur, uint32_t n_slots, uint32_t *flags,
                                                  uint32_t n_flags, DevStatus *st) {
-    uint32_t n = n_paths > n_bands ? n_paths : n_bands;
+    uint32_t n = n_paths > n_slots ? n_paths : n_slots;
     n = n > n_flags ? n : n_flags;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
         if (i < n_paths) {
@@ -123,7 +94,7 @@ __global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, uint32_t n_paths
             s.stored = 0;
             ps[i] = s;
         }
-        if (i < n_bands) {
+        if (i < n_slots) {
             band_cnt[i] = 0;
             band_cur[i] = 0;
         }
@@ -305,34 +276,124 @@ __global__ __launch_bounds__(BLOCK) void k_top(Limits L0, const I2 *pts, const u
 // ---------------------------------------------------------------------------
 // Stage 5: edges and row-band binning.
 // ---------------------------------------------------------------------------
-// One thread per stored point [0, n_kept): Edge::new, the band range as a
-// difference, and the point's band record for k_bin.
-__global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, const uint32_t *pt_run, const uint32_t *run_path,
-                                                 const uint32_t *run_start, const uint32_t *run_end,
-                                                 const PathDev *paths, const I2 *pts, EdgeRec *edges, U2 *pt_band,
-                                                 uint32_t *band_diff, PathState *ps, DevStatus *st) {
+// Band binning.  The (edge, band) pairs of a chunk of BLOCK consecutive
+// stored points are counted in an LDS histogram over the window of
+// BIN_WINDOW bands starting at the chunk's lowest band, and each nonzero bin
+// goes to global memory with one atomic.  The global counters are replicated:
+// band g, replica r sits at slot (g << band_shift) + r, and chunk c uses
+// replica c mod 2^band_shift, so few bands do not put every chunk's atomics
+// on the same cache lines.  An exclusive scan over the slots gives each
+// (band, replica) its list segment; a band's list is its replicas' segments
+// back to back, starting at slot g << band_shift.
+constexpr uint32_t BIN_WINDOW = 1024;
+constexpr size_t BAND_SLOTS_MIN = 16384;
+constexpr uint32_t BAND_SHIFT_MAX = 5;
+static_assert(BIN_WINDOW == 4 * BLOCK, "the window scan gives each thread four bins");
+
+__device__ inline uint32_t block_min_u32(uint32_t v, uint32_t *s_w) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t u = __shfl_xor(v, o, 64);
+        v = u < v ? u : v;
+    }
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t m = s_w[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) m = s_w[w] < m ? s_w[w] : m;
+    __syncthreads();
+    return m;
+}
+
+// Pairs per band of the chunk's window [gmin, gmin + BIN_WINDOW) into s_h,
+// from a difference per edge and an inclusive scan over the window.  Returns
+// gmin (NO_IDX: no banded edge in the chunk).  Every thread of the block calls
+// it; the caller synchronises before s_h is written again.
+__device__ inline uint32_t bin_window_counts(bool on, uint32_t g0, uint32_t g1, uint32_t *s_h, uint32_t *s_w) {
+    const uint32_t gmin = block_min_u32(on ? g0 : NO_IDX, s_w);
+    if (gmin == NO_IDX) return NO_IDX;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    reinterpret_cast<uint4 *>(s_h)[t] = make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    if (on & (g0 - gmin < BIN_WINDOW)) {
+        const uint32_t hi = g1 - gmin < BIN_WINDOW - 1 ? g1 - gmin : BIN_WINDOW - 1;
+        atomicAdd(&s_h[g0 - gmin], 1u);
+        if (hi + 1 < BIN_WINDOW) atomicAdd(&s_h[hi + 1], 0xFFFFFFFFu);
+    }
+    __syncthreads();
+    const uint4 q = reinterpret_cast<const uint4 *>(s_h)[t];
+    const uint32_t a0 = q.x, a1 = a0 + q.y, a2 = a1 + q.z, a3 = a2 + q.w;
+    uint32_t x = a3;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(x, o, 64);
+        if (lane >= o) x += u;
+    }
+    if (lane == 63) s_w[wid] = x;
+    __syncthreads();
+    uint32_t pre = x - a3;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) pre += w < wid ? s_w[w] : 0u;
+    reinterpret_cast<uint4 *>(s_h)[t] = make_uint4(pre + a0, pre + a1, pre + a2, pre + a3);
+    __syncthreads();
+    return gmin;
+}
+
+// One thread per stored point [0, n_kept): Edge::new, the point's band
+// record for k_bin, and the pairs per (band, replica).
+__global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, uint32_t band_shift, const uint32_t *pt_run,
+                                                 const uint32_t *run_path, const uint32_t *run_start,
+                                                 const uint32_t *run_end, const PathDev *paths, const I2 *pts,
+                                                 EdgeRec *edges, U2 *pt_band, uint32_t *band_cnt, PathState *ps,
+                                                 DevStatus *st) {
+    __shared__ __align__(16) uint32_t s_h[BIN_WINDOW];
+    __shared__ uint32_t s_w[BLOCK / 64];
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     const uint32_t n = st->n_kept < L.cap_points ? st->n_kept : L.cap_points;
+    const uint32_t rep_mask = (1u << band_shift) - 1u;
     uint32_t bad = 0;
     for (uint32_t b = blockIdx.x * BLOCK; b < n; b += gridDim.x * BLOCK) {
         const uint32_t i = b + threadIdx.x;
-        EdgeOut r;
-        r.g0 = r.g1 = 0;
-        r.banded = false;
+        U2 rec;
+        rec.x = 1;
+        rec.y = 0;
         if (i < n) {
-            r = edge_point<DevAtom>(i, L, pt_run, run_path, run_start, run_end, paths, pts, edges, ps, bad);
-            pt_band[i] = band_record(r);
+            const EdgeOut r = edge_point<DevAtom>(i, L, pt_run, run_path, run_start, run_end, paths, pts, edges, ps,
+                                                  bad);
+            rec = band_record(r);
+            pt_band[i] = rec;
         }
-        wave_add_runs(band_diff, L.n_bands, r.banded, r.g0, 1u);
-        wave_add_runs(band_diff, L.n_bands, r.banded & (r.g1 + 1 < L.n_bands), r.g1 + 1, 0xFFFFFFFFu);
+        const bool on = (rec.y >= rec.x) & (rec.y < L.n_bands);
+        const uint32_t rep = (b / BLOCK) & rep_mask;
+        const uint32_t gmin = bin_window_counts(on, rec.x, rec.y, s_h, s_w);
+        if (gmin == NO_IDX) continue;
+        const uint4 q = reinterpret_cast<const uint4 *>(s_h)[threadIdx.x];
+        const uint32_t c[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t g = gmin + 4 * threadIdx.x + j;
+            if (c[j] && g < L.n_bands) atomicAdd(&band_cnt[(g << band_shift) + rep], c[j]);
+        }
+        // bands past the window, one atomic each (edges far below the chunk's top band)
+        if (on)
+            for (uint32_t g = gmin + BIN_WINDOW > rec.x ? gmin + BIN_WINDOW : rec.x; g <= rec.y; ++g)
+                atomicAdd(&band_cnt[(g << band_shift) + rep], 1u);
+        __syncthreads();
     }
     if (bad) raise_err(st, FT_EDEVICE);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, const U2 *pt_band, const uint32_t *band_off,
-                                               uint32_t *band_cur, uint32_t *band_list, uint64_t cap_pairs,
-                                               DevStatus *st) {
+// Scatters each point's pairs into the band lists: one atomic per nonzero
+// (chunk, band) reserves the chunk's range in the replica's segment, and LDS
+// cursors hand out slots within it.  Order within a list is not fixed; the
+// raster's integer sums do not depend on it.
+__global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, uint32_t band_shift, const U2 *pt_band,
+                                               const uint32_t *band_off, uint32_t *band_cur, uint32_t *band_list,
+                                               uint64_t cap_pairs, DevStatus *st) {
+    __shared__ __align__(16) uint32_t s_h[BIN_WINDOW];
+    __shared__ __align__(16) uint32_t s_base[BIN_WINDOW];
+    __shared__ uint32_t s_w[BLOCK / 64];
     if (run_failed(st)) return;
     const uint32_t n_pairs = st->n_pairs;
     if (n_pairs > cap_pairs) {
@@ -344,23 +405,56 @@ __global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, const U2 *pt_band, con
     }
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     const uint32_t n = st->n_kept < L.cap_points ? st->n_kept : L.cap_points;
+    const uint32_t rep_mask = (1u << band_shift) - 1u;
+    const uint32_t n_slots = L.n_bands << band_shift;
     uint32_t bad = 0;
+    // list segment [begin, end) of a slot
+    auto seg_end = [&](uint32_t slot) { return slot + 1 < n_slots ? band_off[slot + 1] : n_pairs; };
     for (uint32_t b = blockIdx.x * BLOCK; b < n; b += gridDim.x * BLOCK) {
         const uint32_t i = b + threadIdx.x;
         const BinItem it = bin_item(i < n ? i : 0u, pt_band);
-        const bool has = it.banded & (i < n) & (it.g1 < L.n_bands);
-        const uint32_t span = has ? it.g1 - it.g0 + 1 : 0u;
-        // the wavefront walks the bands of its edges together, so that lanes
-        // at the same band share one atomic
-        for (uint32_t j = 0; __ballot(j < span); ++j) {
-            const bool on = j < span;
-            const uint32_t g = it.g0 + j;
-            const uint32_t slot = wave_add_runs(band_cur, L.n_bands, on, g, 1u);
-            const uint32_t gi = on ? g : 0;
-            const uint32_t at = band_off[gi] + slot;
-            if (on & (at < L.n_pairs)) band_list[at] = i;
-            bad |= (on & (at >= L.n_pairs)) ? 1u : 0u;
+        const bool on = it.banded & (i < n) & (it.g1 < L.n_bands);
+        const uint32_t rep = (b / BLOCK) & rep_mask;
+        const uint32_t gmin = bin_window_counts(on, it.g0, it.g1, s_h, s_w);
+        if (gmin == NO_IDX) continue;
+        {
+            const uint4 q = reinterpret_cast<const uint4 *>(s_h)[threadIdx.x];
+            const uint32_t c[4] = {q.x, q.y, q.z, q.w};
+            uint32_t base[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t g = gmin + 4 * threadIdx.x + j;
+                base[j] = NO_IDX;
+                if (c[j] && g < L.n_bands) {
+                    const uint32_t slot = (g << band_shift) + rep;
+                    const uint32_t at = band_off[slot] + atomicAdd(&band_cur[slot], c[j]);
+                    const bool fits = (uint64_t)at + c[j] <= seg_end(slot);
+                    bad |= fits ? 0u : 1u;
+                    base[j] = fits ? at : NO_IDX;
+                }
+            }
+            reinterpret_cast<uint4 *>(s_base)[threadIdx.x] = make_uint4(base[0], base[1], base[2], base[3]);
+            reinterpret_cast<uint4 *>(s_h)[threadIdx.x] = make_uint4(0, 0, 0, 0);  // cursors
         }
+        __syncthreads();
+        if (on) {
+            const uint32_t hi = it.g1 - gmin < BIN_WINDOW - 1 ? it.g1 : gmin + BIN_WINDOW - 1;
+            for (uint32_t g = it.g0; g <= hi; ++g) {
+                const uint32_t k = g - gmin;
+                const uint32_t slot = atomicAdd(&s_h[k], 1u);
+                const uint32_t at = s_base[k];
+                if (at != NO_IDX) band_list[at + slot] = i;
+            }
+            for (uint32_t g = gmin + BIN_WINDOW > it.g0 ? gmin + BIN_WINDOW : it.g0; g <= it.g1; ++g) {
+                const uint32_t slot = (g << band_shift) + rep;
+                const uint32_t at = band_off[slot] + atomicAdd(&band_cur[slot], 1u);
+                if (at < seg_end(slot))
+                    band_list[at] = i;
+                else
+                    bad = 1;
+            }
+        }
+        __syncthreads();
     }
     if (bad) raise_err(st, FT_EDEVICE);
 }
@@ -676,8 +770,8 @@ __device__ __forceinline__ void row_pass(const int32_t *cells, uint32_t x_begin,
 // each row into the raster; the band's other groups are other blocks.
 template <bool PF>
 __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *paths, const uint2 *rblock,
-                                                  const uint32_t *band_path, const uint32_t *band_off,
-                                                  const uint32_t *band_cnt, const uint32_t *band_list,
+                                                  const uint32_t *band_path, uint32_t band_shift,
+                                                  const uint32_t *band_off, const uint32_t *band_list,
                                                   const EdgeRec *edges, const I2 *pts, const uint32_t *pt_run,
                                                   const uint32_t *run_start, const uint32_t *run_end,
                                                   const PathState *ps, DevStatus *st) {
@@ -768,8 +862,11 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     __syncthreads();
     const int fwd = s_fwd;
     const int32_t shift = top_row < 0 ? top_row : 0;
-    const uint32_t off = band_off[gb], cnt = band_cnt[gb];
-    const bool list_ok = (uint64_t)off + cnt <= L.n_pairs;
+    // the band's list: its replica segments, back to back
+    const uint32_t off = band_off[gb << band_shift];
+    const uint32_t end = gb + 1 < L.n_bands ? band_off[(gb + 1) << band_shift] : L.n_pairs;
+    const bool list_ok = (off <= end) & (end <= L.n_pairs);
+    const uint32_t cnt = list_ok ? end - off : 0u;
     if (!list_ok && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
     // the band's edges, four per thread per round: list entries, then edge
     // records, then their cells
@@ -941,7 +1038,7 @@ static ft_status ensure_stroke_capacity(Workspace &ws) {
 static ft_status ensure_scan_capacity(Workspace &ws) {
     size_t n = 4 * ws.cap_events + 1;  // stroke outline items bound every other scan length
     if (ws.n_ops > n) n = ws.n_ops;
-    if (ws.n_bands > n) n = ws.n_bands;
+    if ((ws.n_bands << ws.band_shift) > n) n = ws.n_bands << ws.band_shift;
     size_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE + 1;
     if (tiles <= ws.cap_tiles && ws.d_scan_flags) return FT_OK;
     ft_status s;
@@ -1013,13 +1110,25 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     if (ws.cap_paths != old_paths || !ws.d_pstate)
         if ((s = dev_alloc(ws.d_pstate, ws.cap_paths))) return s;
     size_t nb_total = band_path.size();
+    // replicas per band counter: enough that the slots spread over about
+    // BAND_SLOTS_MIN counters, at most 2^BAND_SHIFT_MAX
+    uint32_t shift = 0;
+    while (shift < BAND_SHIFT_MAX && (nb_total << shift) < BAND_SLOTS_MIN) ++shift;
+    if ((nb_total << shift) >= 0x7FFFFFFFu) return FT_ENOTSUP;
     if (nb_total > ws.cap_bands || !ws.d_band_path) {
         size_t cap = nb_total ? nb_total : 1;
-        if ((s = dev_alloc(ws.d_band_path, cap)) || (s = dev_alloc(ws.d_band_cnt, cap)) ||
-            (s = dev_alloc(ws.d_band_off, cap)) || (s = dev_alloc(ws.d_band_cur, cap)))
-            return s;
+        if ((s = dev_alloc(ws.d_band_path, cap))) return s;
         ws.cap_bands = cap;
     }
+    const size_t n_slots = nb_total << shift;
+    if (n_slots > ws.cap_band_slots || !ws.d_band_cnt) {
+        size_t cap = n_slots ? n_slots : 1;
+        if ((s = dev_alloc(ws.d_band_cnt, cap)) || (s = dev_alloc(ws.d_band_off, cap)) ||
+            (s = dev_alloc(ws.d_band_cur, cap)))
+            return s;
+        ws.cap_band_slots = cap;
+    }
+    ws.band_shift = shift;
     if ((s = dev_grow(ws.d_rblock, ws.cap_rblocks, rblock.size()))) return s;
     ws.n_rblocks = rblock.size();
     ws.n_ops = n_ops;
@@ -1119,8 +1228,9 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
     DevStatus *st = ws.d_status;
     const uint32_t n_flags = (uint32_t)(tiles * SCAN_SLOTS);
     if (ev) FT_HIP(hipEventRecord(ev[0], s));
-    k_clear<<<grid_for(std::max(std::max(ws.n_paths, ws.n_bands), (size_t)n_flags)), BLOCK, 0, s>>>(
-        ws.d_pstate, (uint32_t)ws.n_paths, ws.d_band_cnt, ws.d_band_cur, (uint32_t)ws.n_bands, fl, n_flags, st);
+    const size_t n_slots = ws.n_bands << ws.band_shift;
+    k_clear<<<grid_for(std::max(std::max(ws.n_paths, n_slots), (size_t)n_flags)), BLOCK, 0, s>>>(
+        ws.d_pstate, (uint32_t)ws.n_paths, ws.d_band_cnt, ws.d_band_cur, (uint32_t)n_slots, fl, n_flags, st);
     const uint32_t *lastpw = nullptr;
     if (mode != MODE_FILL && n_ops) {
         k_pen_width_mark<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, n_ops, ws.d_op_pw);
@@ -1212,23 +1322,21 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         const size_t top_tile = (size_t)TOP_ITEMS * BLOCK;
         const size_t top_grid = std::max<size_t>(1, std::min<size_t>(GRID_MAX, (ws.cap_events + top_tile - 1) / top_tile));
         k_top<<<(uint32_t)top_grid, BLOCK, 0, s>>>(L0, ws.d_pts, ws.d_pt_run, ws.d_run_path, ws.d_pstate, st);
-        k_edges<<<ge, BLOCK, 0, s>>>(L0, ws.d_pt_run, ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_paths,
-                                     ws.d_pts, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_pstate, st);
+        k_edges<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_run, ws.d_run_path, ws.d_run_start, ws.d_run_end,
+                                     ws.d_paths, ws.d_pts, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_pstate, st);
         if (ws.n_bands) {
-            // band differences -> edges per band (in place; slot 3 is free in fill mode) -> list offsets
-            k_scan<uint32_t, false><<<scan_grid(ws.n_bands), BLOCK, 0, s>>>(
-                ws.d_band_cnt, ws.d_band_cnt, nullptr, (uint32_t)ws.n_bands, (uint32_t)ws.n_bands, fl + 3 * tiles,
-                ag + 3 * tiles, in + 3 * tiles, &st->tickets[3], nullptr, st, true, true);
-            k_scan<uint32_t, false><<<scan_grid(ws.n_bands), BLOCK, 0, s>>>(
-                ws.d_band_cnt, ws.d_band_off, nullptr, (uint32_t)ws.n_bands, (uint32_t)ws.n_bands, fl + 2 * tiles,
-                ag + 2 * tiles, in + 2 * tiles, &st->tickets[2], &st->n_pairs, st, true, false);
-            k_bin<<<ge, BLOCK, 0, s>>>(L0, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list, ws.cap_pairs,
-                                       st);
+            // pairs per (band, replica) -> list segment offsets and the pair total
+            const uint32_t n_slots = (uint32_t)(ws.n_bands << ws.band_shift);
+            k_scan<uint32_t, false><<<scan_grid(n_slots), BLOCK, 0, s>>>(
+                ws.d_band_cnt, ws.d_band_off, nullptr, n_slots, n_slots, fl + 2 * tiles, ag + 2 * tiles,
+                in + 2 * tiles, &st->tickets[2], &st->n_pairs, st, true, false);
+            k_bin<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
+                                       ws.cap_pairs, st);
             if (ev) FT_HIP(hipEventRecord(ev[1], s));
             if (ws.n_rblocks) {
                 auto kr = ws.tuning.dst_prefetch ? k_raster<true> : k_raster<false>;
                 kr<<<(uint32_t)ws.n_rblocks, BLOCK, ws.lds_cells * sizeof(int32_t), s>>>(
-                    L0, ws.d_paths, ws.d_rblock, ws.d_band_path, ws.d_band_off, ws.d_band_cnt, ws.d_band_list,
+                    L0, ws.d_paths, ws.d_rblock, ws.d_band_path, ws.band_shift, ws.d_band_off, ws.d_band_list,
                     ws.d_edges, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end, ws.d_pstate, st);
             }
             if (ev) FT_HIP(hipEventRecord(ev[2], s));

@@ -641,8 +641,7 @@ FT_HD inline bool edge_bands(const EdgeRec &E, const PathDev &P, unsigned long l
 
 // Stage 5: one edge per stored point: the segment to its successor within the
 // sub-figure (SubFig::next, fig.rs:143-163).  Edge::new runs once per edge.
-// The caller records the edge's band range [g0, g1] as a difference (+1 at
-// g0, -1 at g1 + 1) whose inclusive prefix sum is the edges per band.
+// The caller records the edge's band range [g0, g1] for binning.
 struct EdgeOut {
     uint32_t path;    // path of the stored point (NO_IDX: none)
     uint32_t g0, g1;  // band range, when `banded`
