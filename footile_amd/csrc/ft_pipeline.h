@@ -25,6 +25,7 @@ namespace ft {
 struct Workspace {
     hipStream_t stream = nullptr;
     int device = 0;
+    size_t lds_max = 65536;  // LDS bytes per block on the device
     // batch layout
     size_t n_ops = 0, n_paths = 0, n_bands = 0;
     size_t cap_ops = 0, cap_paths = 0, cap_bands = 0;
@@ -35,7 +36,7 @@ struct Workspace {
     uint32_t lds_cells = 0;  // dynamic LDS cells needed by the raster kernel
     // raster blocks: (global band, row group) per block
     size_t n_rblocks = 0, cap_rblocks = 0;
-    uint2 *d_rblock = nullptr;
+    uint4 *d_rblock = nullptr;  // raster blocks: (global band, row group, path, 0)
     // per-op scratch
     unsigned long long *d_op_cnt = nullptr, *d_op_off = nullptr;
     unsigned long long *d_sub_off = nullptr;  // per (op, lane): offset within the op

@@ -92,6 +92,7 @@ __global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, uint32_t n_paths
             s.pt_base = 0;
             s.n_edges = 0;
             s.stored = 0;
+            s.fwd = 0;
             ps[i] = s;
         }
         if (i < n_slots) {
@@ -390,7 +391,9 @@ __global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, uint32_t band_shift,
 // raster's integer sums do not depend on it.
 __global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, uint32_t band_shift, const U2 *pt_band,
                                                const uint32_t *band_off, uint32_t *band_cur, uint32_t *band_list,
-                                               uint64_t cap_pairs, DevStatus *st) {
+                                               uint64_t cap_pairs, const I2 *pts, const uint32_t *pt_run,
+                                               const uint32_t *run_start, const uint32_t *run_end, PathState *ps,
+                                               DevStatus *st) {
     __shared__ __align__(16) uint32_t s_h[BIN_WINDOW];
     __shared__ __align__(16) uint32_t s_base[BIN_WINDOW];
     __shared__ uint32_t s_w[BLOCK / 64];
@@ -408,6 +411,12 @@ __global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, uint32_t band_shift, c
     const uint32_t rep_mask = (1u << band_shift) - 1u;
     const uint32_t n_slots = L.n_bands << band_shift;
     uint32_t bad = 0;
+    // winding direction per path (its top point is final after k_edges)
+    for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < L.n_paths; p += gridDim.x * BLOCK) {
+        const PathState S = ps[p];
+        if ((S.topkey != NO_KEY) & (S.topidx != NO_IDX))
+            ps[p].fwd = path_forward(S, L, pts, pt_run, run_start, run_end, bad) ? 1u : 0u;
+    }
     // list segment [begin, end) of a slot
     auto seg_end = [&](uint32_t slot) { return slot + 1 < n_slots ? band_off[slot + 1] : n_pairs; };
     for (uint32_t b = blockIdx.x * BLOCK; b < n; b += gridDim.x * BLOCK) {
@@ -653,6 +662,11 @@ __global__ __launch_bounds__(BLOCK) void k_st_emit(const F4 *spts, const uint32_
 // ---------------------------------------------------------------------------
 // Stage 6: raster.  One block per row band and LDS row group.
 // ---------------------------------------------------------------------------
+typedef __attribute__((address_space(1))) uint8_t gbyte;  // raster memory
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) u32x4 gu32x4;
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
 __device__ inline uint32_t wave_incl_u32(uint32_t v) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -663,130 +677,261 @@ __device__ inline uint32_t wave_incl_u32(uint32_t v) {
     return v;
 }
 
-// Prefix-sum cells [x_begin, W) of one raster row (one wavefront) starting
-// from `carry` (the sum of the cells left of x_begin), map coverage with the
-// fill rule and store / composite CPT pixels per lane.
-// Destination chunks (16 bytes per lane) a wavefront loads before its cell
-// pass, in the order the row pass consumes them, so that the HBM latency of
-// the read-modify-write formats overlaps the LDS work.
-constexpr uint32_t PRE = 4;
+// Coverage of a running cell sum under a fixed fill rule (0: NonZero,
+// 1: EvenOdd), as map_coverage.
+template <uint32_t RULE>
+__device__ __forceinline__ uint32_t cov_of(uint32_t sum) {
+    if (RULE) {  // coverage_even_odd: |(s & 0xFF) - (s & 0x100)| capped at 255, s = the sum as i16
+        const uint32_t c = __sad(sum & 0xFFu, sum & 0x100u, 0u);
+        return c < 255u ? c : 255u;
+    }
+    return coverage_non_zero((int32_t)sum);
+}
 
-struct Prefetch {
-    uint4 q0, q1, q2, q3;  // chunks 0..3 in consumption order
-    uint32_t n;            // chunks loaded
+// SrcOver of the path's constant premultiplied colour (composite_px) depends
+// on the coverage a only through S_c(a) = ch8_mul_w(widen(src_c), widen(a))
+// and invw(a) = widen(255 - S_alpha(a)):
+//     out_c = min(255, S_c(a) + ch8_mul_w(widen(d_c), invw(a))).
+// A raster block tabulates them for a = 0..255 in LDS.  Entry, Rgba8p:
+// x = S0 | S2 << 16, y = S1 | S3 << 16, z = invw, w = the S bytes as a pixel;
+// Graya8p: x = S0 | S1 << 16, y = 0, z = invw, w = S0 | S1 << 8.
+constexpr uint32_t LUT_ENTRIES = 256;
+constexpr uint32_t LUT_BYTES = LUT_ENTRIES * sizeof(uint4);
+
+static inline size_t raster_lds_bytes(size_t cells) { return LUT_BYTES + cells * sizeof(int32_t); }
+
+__device__ inline uint4 lut_entry(uint32_t fmt, uint32_t clr, uint32_t a) {
+    const uint32_t aw = ch8_widen(a);
+    if (fmt == FMT_RGBA8P) {
+        uint32_t s[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s[c] = ch8_mul_w(ch8_widen((clr >> (8 * c)) & 0xFFu), aw);
+        return make_uint4(s[0] | (s[2] << 16), s[1] | (s[3] << 16), ch8_widen(255u - s[3]),
+                          s[0] | (s[1] << 8) | (s[2] << 16) | (s[3] << 24));
+    }
+    const uint32_t s0 = ch8_mul_w(ch8_widen(clr & 0xFFu), aw), s1 = ch8_mul_w(ch8_widen((clr >> 8) & 0xFFu), aw);
+    return make_uint4(s0 | (s1 << 16), 0u, ch8_widen(255u - s1), s0 | (s1 << 8));
+}
+
+// Two channels at once: d2 and s2 hold a channel in the low byte of each
+// 16-bit half; returns min(255, s + (widen(d) * invw >> 16)) per half.
+// widen(d) and invw are 12-bit, so each product fits 24 bits and its >> 16 is
+// the product's byte 2.
+__device__ __forceinline__ uint32_t blend2(uint32_t d2, uint32_t s2, uint32_t invw) {
+    const uint32_t w2 = (d2 << 4) | __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, d2) >> (u16x2)(4));
+    const uint32_t p0 = __umul24(w2 & 0xFFFFu, invw), p1 = __umul24(w2 >> 16, invw);
+    const uint32_t t2 = __builtin_amdgcn_perm(p1, p0, 0x0C060C02u);
+    const u16x2 r = __builtin_elementwise_min(__builtin_bit_cast(u16x2, s2 + t2), (u16x2)(255));
+    return __builtin_bit_cast(uint32_t, r);
+}
+
+__device__ __forceinline__ uint32_t blend_rgba(uint32_t d, uint4 e) {
+    const uint32_t r02 = blend2(__builtin_amdgcn_perm(0u, d, 0x0C020C00u), e.x, e.z);
+    const uint32_t r13 = blend2(__builtin_amdgcn_perm(0u, d, 0x0C030C01u), e.y, e.z);
+    return __builtin_amdgcn_perm(r13, r02, 0x06020400u);
+}
+
+// Two Graya8p pixels (one 32-bit word) with entries e0 (pixel 0), e1.
+__device__ __forceinline__ uint32_t blend_graya2(uint32_t d, uint4 e0, uint4 e1) {
+    const uint32_t r0 = blend2(__builtin_amdgcn_perm(0u, d, 0x0C010C00u), e0.x, e0.z);
+    const uint32_t r1 = blend2(__builtin_amdgcn_perm(0u, d, 0x0C030C02u), e1.x, e1.z);
+    return __builtin_amdgcn_perm(r1, r0, 0x06040200u);
+}
+
+// Read-modify-write formats keep the destination loads of the next
+// DST_DEPTH chunks of the row segment in flight.
+constexpr uint32_t DST_DEPTH = 4;
+struct DstPipe {
+    uint4 q0, q1, q2, q3;
 };
 
-// Chunk i by masks (a select chain would be turned into an indexed stack load).
-__device__ inline uint4 pre_pick(const Prefetch &p, uint32_t i) {
-    const uint32_t m0 = 0u - (uint32_t)(i == 0), m1 = 0u - (uint32_t)(i == 1);
-    const uint32_t m2 = 0u - (uint32_t)(i == 2), m3 = 0u - (uint32_t)(i == 3);
-    uint4 r;
-    r.x = (p.q0.x & m0) | (p.q1.x & m1) | (p.q2.x & m2) | (p.q3.x & m3);
-    r.y = (p.q0.y & m0) | (p.q1.y & m1) | (p.q2.y & m2) | (p.q3.y & m3);
-    r.z = (p.q0.z & m0) | (p.q1.z & m1) | (p.q2.z & m2) | (p.q3.z & m3);
-    r.w = (p.q0.w & m0) | (p.q1.w & m1) | (p.q2.w & m2) | (p.q3.w & m3);
-    return r;
+template <int FMT>
+constexpr int bpp_of() {
+    return FMT == FMT_RGBA8P ? 4 : (FMT == FMT_GRAYA8P ? 2 : 1);
 }
 
+// Destination chunk k (16 bytes per lane) of the segment [xb, xe) of a row,
+// or zero where the chunk is partial, misaligned or past the segment (the row
+// pass then takes its per-pixel path).
 template <int FMT, int CPT>
-__device__ __forceinline__ void row_pass(const int32_t *cells, uint32_t x_begin, uint32_t W, uint8_t *row, uint32_t rule,
-                         uint32_t clr, bool fresh, uint32_t carry, const Prefetch pf, uint32_t &next) {
-    constexpr int BPP = FMT == FMT_RGBA8P ? 4 : (FMT == FMT_GRAYA8P ? 2 : 1);
-    const int lane = threadIdx.x & 63;
-    uint32_t sw[4] = {0, 0, 0, 0};
-    if (FMT == FMT_RGBA8P) {
-        for (int c = 0; c < 4; ++c) sw[c] = ch8_widen((clr >> (8 * c)) & 0xFFu);
-    } else if (FMT == FMT_GRAYA8P) {
-        sw[0] = ch8_widen(clr & 0xFFu);
-        sw[1] = ch8_widen((clr >> 8) & 0xFFu);
+__device__ __forceinline__ uint4 dst_load(const gbyte *row, uint32_t xb, uint32_t xe, uint32_t k, uint32_t nck) {
+    constexpr int BPP = bpp_of<FMT>();
+    static_assert(CPT * BPP == 16, "one 16-byte chunk per lane");
+    const uint32_t x = xb + k * 64u * CPT + (uint32_t)(threadIdx.x & 63) * CPT;
+    const gbyte *p = row + (size_t)x * BPP;
+    const bool full = (k < nck) & (x + CPT <= xe) & ((((uintptr_t)p) & 15) == 0);
+    u32x4 v = (u32x4)(0u);
+    if (full) v = *reinterpret_cast<const gu32x4 *>(p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+// One chunk of a row pass: CPT cells per lane at x from LDS, their prefix
+// sum from `carry` (the sum of the cells left of the chunk), coverage, and
+// the pixels stored or composited (q: the chunk's destination, when loaded).
+template <int FMT, int CPT, uint32_t RULE, bool FRESH>
+__device__ __forceinline__ void row_chunk(const int32_t *cells, uint32_t x, uint32_t xe, gbyte *row,
+                                          const uint4 *lut, uint32_t clr, uint32_t &carry, uint4 q) {
+    constexpr int BPP = bpp_of<FMT>();
+    const bool in = x < xe;
+    const uint32_t xr = in ? x : 0u;  // a readable address for lanes past the segment
+    uint32_t c[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; j += 4) {
+        const int4 c4 = *reinterpret_cast<const int4 *>(cells + xr + j);
+        c[j] = (uint32_t)c4.x;
+        c[j + 1] = (uint32_t)c4.y;
+        c[j + 2] = (uint32_t)c4.z;
+        c[j + 3] = (uint32_t)c4.w;
     }
-    for (uint32_t c0 = x_begin; c0 < W; c0 += 64u * CPT) {
-        const uint32_t ci = next++;
-        const uint32_t x = c0 + (uint32_t)lane * CPT;
-        uint32_t v[CPT];
-        uint32_t run = 0;
+    uint32_t v[CPT];
+    uint32_t run = 0;
 #pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-            run += (x + j < W) ? (uint32_t)cells[x + j] : 0u;
-            v[j] = run;
-        }
-        const uint32_t incl = wave_incl_u32(run);
-        const uint32_t total = __shfl(incl, 63, 64);
-        const uint32_t base = carry + incl - run;
-        carry += total;
-        if (x >= W) continue;
-        uint8_t a[CPT];
+    for (int j = 0; j < CPT; ++j) {
+        run += (x + j < xe) ? c[j] : 0u;
+        v[j] = run;
+    }
+    const uint32_t incl = wave_incl_u32(run);
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t base = carry + incl - run;
+    carry += total;
+    if (!in) return;
+    gbyte *dst = row + (size_t)x * BPP;
+    constexpr uint32_t AL = CPT * BPP >= 16 ? 15 : CPT * BPP - 1;
+    const bool full = (x + CPT <= xe) & ((((uintptr_t)dst) & AL) == 0);
+    if (FMT == FMT_MATTE8) {
+        uint32_t a[CPT];
 #pragma unroll
-        for (int j = 0; j < CPT; ++j) a[j] = (uint8_t)map_coverage((int32_t)(base + v[j]), rule);
-        uint8_t *dst = row + (size_t)x * BPP;
-        const bool full = x + CPT <= W && (((uintptr_t)dst) & (CPT * BPP >= 16 ? 15 : CPT * BPP - 1)) == 0;
-        if (FMT == FMT_MATTE8) {
+        for (int j = 0; j < CPT; ++j) a[j] = cov_of<RULE>(base + v[j]);
+        if (full) {
             if constexpr (CPT == 16) {
-                if (full) {
-                    uint4 pk;
-                    pk.x = a[0] | (a[1] << 8) | (a[2] << 16) | ((uint32_t)a[3] << 24);
-                    pk.y = a[4] | (a[5] << 8) | (a[6] << 16) | ((uint32_t)a[7] << 24);
-                    pk.z = a[8] | (a[9] << 8) | (a[10] << 16) | ((uint32_t)a[11] << 24);
-                    pk.w = a[12] | (a[13] << 8) | (a[14] << 16) | ((uint32_t)a[15] << 24);
-                    *reinterpret_cast<uint4 *>(dst) = pk;
-                    continue;
-                }
+                uint4 pk;
+                pk.x = a[0] | (a[1] << 8) | (a[2] << 16) | (a[3] << 24);
+                pk.y = a[4] | (a[5] << 8) | (a[6] << 16) | (a[7] << 24);
+                pk.z = a[8] | (a[9] << 8) | (a[10] << 16) | (a[11] << 24);
+                pk.w = a[12] | (a[13] << 8) | (a[14] << 16) | (a[15] << 24);
+                *reinterpret_cast<gu32x4 *>(dst) = (u32x4){pk.x, pk.y, pk.z, pk.w};
+                return;
             } else if constexpr (CPT == 4) {
-                if (full) {
-                    *reinterpret_cast<uint32_t *>(dst) = a[0] | (a[1] << 8) | (a[2] << 16) | ((uint32_t)a[3] << 24);
-                    continue;
-                }
+                *reinterpret_cast<__attribute__((address_space(1))) uint32_t *>(dst) =
+                    a[0] | (a[1] << 8) | (a[2] << 16) | (a[3] << 24);
+                return;
             }
-            for (int j = 0; j < CPT; ++j)
-                if (x + j < W) dst[j] = a[j];
-        } else {
-            if (full && CPT * BPP == 16) {
-                uint4 px = fresh ? make_uint4(0, 0, 0, 0)
-                                 : (ci < pf.n ? pre_pick(pf, ci) : *reinterpret_cast<const uint4 *>(dst));
-                uint8_t *pb = reinterpret_cast<uint8_t *>(&px);
+        }
 #pragma unroll
-                for (int j = 0; j < CPT; ++j) {
-                    if (fresh)
-                        composite_px_fresh<BPP>(pb + j * BPP, sw, a[j]);
-                    else
-                        composite_px<BPP>(pb + j * BPP, sw, a[j]);
-                }
-                *reinterpret_cast<uint4 *>(dst) = px;
-            } else {
-                for (int j = 0; j < CPT; ++j) {
-                    if (x + j >= W) break;
-                    if (fresh)
-                        composite_px_fresh<BPP>(dst + j * BPP, sw, a[j]);
-                    else
-                        composite_px<BPP>(dst + j * BPP, sw, a[j]);
-                }
+        for (int j = 0; j < CPT; ++j)
+            if (x + j < xe) dst[j] = (uint8_t)a[j];
+        return;
+    }
+    if (full) {
+        uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        if constexpr (FMT == FMT_RGBA8P) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint4 e = lut[cov_of<RULE>(base + v[j])];
+                w[j] = FRESH ? e.w : blend_rgba(w[j], e);
             }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint4 e0 = lut[cov_of<RULE>(base + v[2 * j])];
+                const uint4 e1 = lut[cov_of<RULE>(base + v[2 * j + 1])];
+                w[j] = FRESH ? (e0.w | (e1.w << 16)) : blend_graya2(w[j], e0, e1);
+            }
+        }
+        *reinterpret_cast<gu32x4 *>(dst) = (u32x4){w[0], w[1], w[2], w[3]};
+        return;
+    }
+    // partial or misaligned chunk: per pixel, composite_px on a copy
+    uint32_t sw[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int ch = 0; ch < BPP; ++ch) sw[ch] = ch8_widen((clr >> (8 * ch)) & 0xFFu);
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        if (x + j >= xe) break;
+        uint8_t px[BPP];
+#pragma unroll
+        for (int ch = 0; ch < BPP; ++ch) px[ch] = FRESH ? 0 : dst[j * BPP + ch];
+        const uint32_t a = cov_of<RULE>(base + v[j]);
+        if (FRESH)
+            composite_px_fresh<BPP>(px, sw, a);
+        else
+            composite_px<BPP>(px, sw, a);
+#pragma unroll
+        for (int ch = 0; ch < BPP; ++ch) dst[j * BPP + ch] = px[ch];
+    }
+}
+
+// Prefix-sum the cells [xb, xe) of one raster row (one wavefront) from
+// `carry`, map coverage and store / composite CPT pixels per lane per chunk.
+// `loaded`: the pipe already holds the segment's first DST_DEPTH chunks.
+template <int FMT, int CPT, uint32_t RULE, bool FRESH>
+__device__ __forceinline__ void row_pass(const int32_t *cells, uint32_t xb, uint32_t xe, gbyte *row,
+                                         const uint4 *lut, uint32_t clr, uint32_t carry, DstPipe &pp, bool loaded) {
+    constexpr bool RMW = FMT != FMT_MATTE8 && !FRESH;
+    constexpr uint32_t chunk = 64u * CPT;
+    const uint32_t nck = (xe - xb + chunk - 1) / chunk;
+    if constexpr (RMW) {
+        if (!loaded) {
+            pp.q0 = dst_load<FMT, CPT>(row, xb, xe, 0, nck);
+            pp.q1 = dst_load<FMT, CPT>(row, xb, xe, 1, nck);
+            pp.q2 = dst_load<FMT, CPT>(row, xb, xe, 2, nck);
+            pp.q3 = dst_load<FMT, CPT>(row, xb, xe, 3, nck);
+        }
+    }
+    const uint32_t lx = (uint32_t)(threadIdx.x & 63) * CPT;
+    for (uint32_t k = 0; k < nck; k += DST_DEPTH) {
+        const uint32_t x0 = xb + k * chunk + lx;
+        row_chunk<FMT, CPT, RULE, FRESH>(cells, x0, xe, row, lut, clr, carry, pp.q0);
+        if constexpr (RMW) pp.q0 = dst_load<FMT, CPT>(row, xb, xe, k + 4, nck);
+        if (k + 1 < nck) {
+            row_chunk<FMT, CPT, RULE, FRESH>(cells, x0 + chunk, xe, row, lut, clr, carry, pp.q1);
+            if constexpr (RMW) pp.q1 = dst_load<FMT, CPT>(row, xb, xe, k + 5, nck);
+        }
+        if (k + 2 < nck) {
+            row_chunk<FMT, CPT, RULE, FRESH>(cells, x0 + 2 * chunk, xe, row, lut, clr, carry, pp.q2);
+            if constexpr (RMW) pp.q2 = dst_load<FMT, CPT>(row, xb, xe, k + 6, nck);
+        }
+        if (k + 3 < nck) {
+            row_chunk<FMT, CPT, RULE, FRESH>(cells, x0 + 3 * chunk, xe, row, lut, clr, carry, pp.q3);
+            if constexpr (RMW) pp.q3 = dst_load<FMT, CPT>(row, xb, xe, k + 7, nck);
         }
     }
 }
 
-// One block per (band, row group): `rblock[b]` = (global band, group).  The
-// block accumulates the band's listed edges for its G rows in LDS and scans
-// each row into the raster; the band's other groups are other blocks.
+template <int FMT, int CPT, bool FRESH>
+__device__ __forceinline__ void row_pass_rule(uint32_t rule, const int32_t *cells, uint32_t xb, uint32_t xe,
+                                              gbyte *row, const uint4 *lut, uint32_t clr, uint32_t carry,
+                                              DstPipe &pp, bool loaded) {
+    if (rule)
+        row_pass<FMT, CPT, 1, FRESH>(cells, xb, xe, row, lut, clr, carry, pp, loaded);
+    else
+        row_pass<FMT, CPT, 0, FRESH>(cells, xb, xe, row, lut, clr, carry, pp, loaded);
+}
+
+// One block per (band, row group): `rblock[b]` = (global band, group, path).
+// The block accumulates the band's listed edges for its G rows in LDS and
+// scans each row into the raster; the band's other groups are other blocks.
+// Dynamic LDS: the composite table (LUT_BYTES), then the rows' i32 cells.
 template <bool PF>
-__global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *paths, const uint2 *rblock,
-                                                  const uint32_t *band_path, uint32_t band_shift,
-                                                  const uint32_t *band_off, const uint32_t *band_list,
-                                                  const EdgeRec *edges, const I2 *pts, const uint32_t *pt_run,
-                                                  const uint32_t *run_start, const uint32_t *run_end,
+__global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *paths, const uint4 *rblock,
+                                                  uint32_t band_shift, const uint32_t *band_off,
+                                                  const uint32_t *band_list, const EdgeRec *edges,
                                                   const PathState *ps, DevStatus *st) {
-    extern __shared__ int32_t s_cells[];
-    __shared__ int s_fwd;
+    extern __shared__ uint4 s_dyn[];
     __shared__ uint32_t s_segsum[BLOCK / 64];
+    uint4 *const lut = s_dyn;
+    int32_t *const s_cells = reinterpret_cast<int32_t *>(s_dyn + LUT_ENTRIES);
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
-    const uint2 rb = rblock[blockIdx.x];
-    const uint32_t gb = rb.x;
-    const uint32_t p = gb < L.n_bands ? band_path[gb] : NO_IDX;
-    if (p >= L.n_paths) {
+    const uint4 rb = rblock[blockIdx.x];
+    const uint32_t gb = rb.x, p = rb.z;
+    if ((gb >= L.n_bands) | (p >= L.n_paths)) {
         if (threadIdx.x == 0) raise_err(st, FT_EDEVICE);
         return;
     }
+    // the band's list: its replica segments, back to back
+    const uint32_t off = band_off[gb << band_shift];
+    const uint32_t end = gb + 1 < L.n_bands ? band_off[(gb + 1) << band_shift] : L.n_pairs;
     const PathDev &P = paths[p];
     const uint32_t R = P.band_rows, H = P.height, W = P.width;
     const uint32_t G = P.group_rows ? P.group_rows : 1u;  // rows whose cells fit in LDS at once
@@ -799,12 +944,14 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     const bool has_pts = S.topkey != NO_KEY && S.topidx != NO_IDX;
     const int32_t top_row = has_pts ? top_row_of(S.topkey) : 0;
     const int64_t first = !has_pts ? (int64_t)H : (top_row > 0 ? top_row : 0);
-    const uint32_t bpp = P.fmt == FMT_RGBA8P ? 4 : (P.fmt == FMT_GRAYA8P ? 2 : 1);
-    uint8_t *const base = reinterpret_cast<uint8_t *>(P.dst);
-    if (P.flags & PATH_FRESH) {  // rows above the figure stay as a fresh raster: zero
+    const uint32_t fmt = P.fmt;
+    const uint32_t bpp = fmt == FMT_RGBA8P ? 4 : (fmt == FMT_GRAYA8P ? 2 : 1);
+    gbyte *const base = (gbyte *)(P.dst);
+    const bool fresh = (P.flags & PATH_FRESH) != 0;
+    if (fresh) {  // rows above the figure stay as a fresh raster: zero
         const uint32_t ze = first < (int64_t)y1 ? (uint32_t)first : y1;
         for (uint32_t r = y0; r < ze; ++r) {
-            uint8_t *row = base + (size_t)r * P.pitch;
+            gbyte *row = base + (size_t)r * P.pitch;
             for (uint32_t x = threadIdx.x; x < W * bpp; x += BLOCK) row[x] = 0;
         }
     }
@@ -812,19 +959,13 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     if ((int64_t)ry0 >= (int64_t)y1) return;
     const uint32_t Wp = cell_stride(W);
     const uint32_t rows = y1 - ry0;
-    if (threadIdx.x == 0) {
-        uint32_t bad = 0;
-        s_fwd = path_forward(S, L, pts, pt_run, run_start, run_end, bad) ? 1 : 0;
-        if (bad) raise_err(st, FT_EDEVICE);
-    }
     // Row pass assignment.  Groups of at least 4 rows give each wavefront
     // whole rows; a group of 1-3 rows splits every row into segments, one per
     // wavefront, whose cell sums are exchanged through LDS as carry-in.
     const int wid = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     constexpr uint32_t NW = BLOCK / 64;
-    const bool fresh = (P.flags & PATH_FRESH) != 0;
-    const uint32_t cpt = P.fmt == FMT_MATTE8 ? (W >= 1024 ? 16u : 4u) : (P.fmt == FMT_RGBA8P ? 4u : 8u);
+    const uint32_t cpt = fmt == FMT_MATTE8 ? (W >= 1024 ? 16u : 4u) : (fmt == FMT_RGBA8P ? 4u : 8u);
     const uint32_t seg = rows >= NW ? 1u : NW / rows;
     const uint32_t chunk = 64u * cpt;
     const uint32_t seg_len = seg == 1 ? W : ((W + seg - 1) / seg + chunk - 1) / chunk * chunk;
@@ -835,36 +976,32 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
         xb = (wid % seg) * seg_len;
         xe = xb + seg_len < W ? xb + seg_len : W;
     }
-    // load the first destination chunks of a read-modify-write format now
-    Prefetch pf;
-    pf.q0 = pf.q1 = pf.q2 = pf.q3 = make_uint4(0, 0, 0, 0);
-    pf.n = 0;
-    uint32_t next_chunk = 0;
-    if (PF && !fresh && P.fmt != FMT_MATTE8 && xb < xe && r0 < rows) {
-        const uint32_t cpr = (xe - xb + chunk - 1) / chunk;
-        const uint32_t nrows = (rows - r0 + rstep - 1) / rstep;
-        const uint32_t total = cpr * nrows;
-        pf.n = total < PRE ? total : PRE;
-        const uint32_t bpp_l = P.fmt == FMT_RGBA8P ? 4u : 2u;
-        auto load = [&](uint32_t k) {
-            const uint32_t rr = r0 + (k / cpr) * rstep;
-            const uint32_t x = xb + (k % cpr) * chunk + (uint32_t)lane * cpt;
-            const uint8_t *dst = base + (size_t)(ry0 + (rr < rows ? rr : 0)) * P.pitch + (size_t)x * bpp_l;
-            const bool full = (k < pf.n) & (x + cpt <= xe) & ((((uintptr_t)dst) & 15) == 0);
-            return full ? *reinterpret_cast<const uint4 *>(dst) : make_uint4(0, 0, 0, 0);
-        };
-        pf.q0 = load(0);
-        pf.q1 = load(1);
-        pf.q2 = load(2);
-        pf.q3 = load(3);
+    // the first destination chunks of a read-modify-write format, now
+    DstPipe pp;
+    pp.q0 = pp.q1 = pp.q2 = pp.q3 = make_uint4(0, 0, 0, 0);
+    const bool rmw = !fresh & (fmt != FMT_MATTE8);
+    const bool pre = PF & rmw & (xb < xe) & (r0 < rows);
+    if (pre) {
+        const gbyte *row = base + (size_t)(ry0 + r0) * P.pitch;
+        const uint32_t nck = (xe - xb + chunk - 1) / chunk;
+        if (fmt == FMT_RGBA8P) {
+            pp.q0 = dst_load<FMT_RGBA8P, 4>(row, xb, xe, 0, nck);
+            pp.q1 = dst_load<FMT_RGBA8P, 4>(row, xb, xe, 1, nck);
+            pp.q2 = dst_load<FMT_RGBA8P, 4>(row, xb, xe, 2, nck);
+            pp.q3 = dst_load<FMT_RGBA8P, 4>(row, xb, xe, 3, nck);
+        } else {
+            pp.q0 = dst_load<FMT_GRAYA8P, 8>(row, xb, xe, 0, nck);
+            pp.q1 = dst_load<FMT_GRAYA8P, 8>(row, xb, xe, 1, nck);
+            pp.q2 = dst_load<FMT_GRAYA8P, 8>(row, xb, xe, 2, nck);
+            pp.q3 = dst_load<FMT_GRAYA8P, 8>(row, xb, xe, 3, nck);
+        }
     }
-    for (uint32_t c = threadIdx.x; c < rows * Wp; c += BLOCK) s_cells[c] = 0;
+    if (fmt != FMT_MATTE8)
+        for (uint32_t a = threadIdx.x; a < LUT_ENTRIES; a += BLOCK) lut[a] = lut_entry(fmt, P.clr, a);
+    for (uint32_t c = threadIdx.x; c < rows * Wp / 4; c += BLOCK) reinterpret_cast<int4 *>(s_cells)[c] = make_int4(0, 0, 0, 0);
     __syncthreads();
-    const int fwd = s_fwd;
+    const int fwd = (int)S.fwd;
     const int32_t shift = top_row < 0 ? top_row : 0;
-    // the band's list: its replica segments, back to back
-    const uint32_t off = band_off[gb << band_shift];
-    const uint32_t end = gb + 1 < L.n_bands ? band_off[(gb + 1) << band_shift] : L.n_pairs;
     const bool list_ok = (off <= end) & (end <= L.n_pairs);
     const uint32_t cnt = list_ok ? end - off : 0u;
     if (!list_ok && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
@@ -908,27 +1045,43 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
         const uint32_t sgi = wid % seg;
         uint32_t sum = 0;
         if (r0 < rows)
-            for (uint32_t x = xb + lane; x < xe; x += 64) sum += (uint32_t)s_cells[(size_t)r0 * Wp + x];
+            for (uint32_t x = xb + 4u * (uint32_t)lane; x < xe; x += 256) {
+                const int4 c4 = *reinterpret_cast<const int4 *>(s_cells + (size_t)r0 * Wp + x);
+                sum += (uint32_t)c4.x + (x + 1 < xe ? (uint32_t)c4.y : 0u) + (x + 2 < xe ? (uint32_t)c4.z : 0u) +
+                       (x + 3 < xe ? (uint32_t)c4.w : 0u);
+            }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
         if (lane == 0) s_segsum[wid] = sum;
         __syncthreads();
         for (uint32_t k = 0; k < sgi; ++k) carry += s_segsum[r0 * seg + k];
     }
+    bool loaded = pre;
     for (uint32_t r = r0; r < rows; r += rstep) {
         if (xb >= xe) break;
         const int32_t *cells = s_cells + (size_t)r * Wp;
-        uint8_t *row = base + (size_t)(ry0 + r) * P.pitch;
-        switch (P.fmt) {
+        gbyte *row = base + (size_t)(ry0 + r) * P.pitch;
+        switch (fmt) {
         case FMT_MATTE8:
             if (cpt == 16)
-                row_pass<FMT_MATTE8, 16>(cells, xb, xe, row, P.rule, P.clr, fresh, carry, pf, next_chunk);
+                row_pass_rule<FMT_MATTE8, 16, false>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, false);
             else
-                row_pass<FMT_MATTE8, 4>(cells, xb, xe, row, P.rule, P.clr, fresh, carry, pf, next_chunk);
+                row_pass_rule<FMT_MATTE8, 4, false>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, false);
             break;
-        case FMT_RGBA8P: row_pass<FMT_RGBA8P, 4>(cells, xb, xe, row, P.rule, P.clr, fresh, carry, pf, next_chunk); break;
-        default: row_pass<FMT_GRAYA8P, 8>(cells, xb, xe, row, P.rule, P.clr, fresh, carry, pf, next_chunk); break;
+        case FMT_RGBA8P:
+            if (fresh)
+                row_pass_rule<FMT_RGBA8P, 4, true>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, false);
+            else
+                row_pass_rule<FMT_RGBA8P, 4, false>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, loaded);
+            break;
+        default:
+            if (fresh)
+                row_pass_rule<FMT_GRAYA8P, 8, true>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, false);
+            else
+                row_pass_rule<FMT_GRAYA8P, 8, false>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, loaded);
+            break;
         }
+        loaded = false;
     }
 }
 
@@ -973,6 +1126,9 @@ static ft_status dev_alloc(T *&ptr, size_t n) {
 
 ft_status ws_init(Workspace &ws) {
     FT_HIP(hipGetDevice(&ws.device));
+    int lds_max = 0;
+    FT_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ws.device));
+    ws.lds_max = (size_t)lds_max;
     FT_HIP(hipStreamCreateWithFlags(&ws.stream, hipStreamNonBlocking));
     FT_HIP(hipMalloc(reinterpret_cast<void **>(&ws.d_status), sizeof(DevStatus)));
     FT_HIP(hipHostMalloc(reinterpret_cast<void **>(&ws.h_status), sizeof(DevStatus), hipHostMallocDefault));
@@ -1062,7 +1218,7 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     std::vector<uint32_t> op_path(n_ops, NO_IDX);
     std::vector<PathDev> pd(n_paths);
     std::vector<uint32_t> band_path;
-    std::vector<uint2> rblock;
+    std::vector<uint4> rblock;
     uint32_t lds = 0;
     for (size_t k = 0; k < n_paths; ++k) {
         const ft_fill_desc &d = paths[k];
@@ -1090,11 +1246,12 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
             for (uint32_t b = 0; b < nb; ++b) {
                 const uint32_t rows = std::min(P.band_rows, d.height - b * P.band_rows);
                 for (uint32_t g = 0; g < groups && g * G < rows; ++g)
-                    rblock.push_back(make_uint2(P.band_base + b, g));
+                    rblock.push_back(make_uint4(P.band_base + b, g, (uint32_t)k, 0u));
             }
         }
     }
     if (rblock.size() >= 0x7FFFFFFFu) return FT_ENOTSUP;
+    if (raster_lds_bytes(lds) > ws.lds_max) return FT_ENOTSUP;  // the tuning asks for more LDS than a block has
     ft_status s;
     const size_t old_ops = ws.cap_ops;
     if ((s = dev_grow(ws.d_ops, ws.cap_ops, n_ops))) return s;
@@ -1154,7 +1311,7 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     if (nb_total)
         FT_HIP(hipMemcpyAsync(ws.d_band_path, band_path.data(), nb_total * 4, hipMemcpyHostToDevice, ws.stream));
     if (!rblock.empty())
-        FT_HIP(hipMemcpyAsync(ws.d_rblock, rblock.data(), rblock.size() * sizeof(uint2), hipMemcpyHostToDevice,
+        FT_HIP(hipMemcpyAsync(ws.d_rblock, rblock.data(), rblock.size() * sizeof(uint4), hipMemcpyHostToDevice,
                               ws.stream));
     FT_HIP(hipStreamSynchronize(ws.stream));  // host vectors go out of scope
     return FT_OK;
@@ -1331,13 +1488,14 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
                 ws.d_band_cnt, ws.d_band_off, nullptr, n_slots, n_slots, fl + 2 * tiles, ag + 2 * tiles,
                 in + 2 * tiles, &st->tickets[2], &st->n_pairs, st, true, false);
             k_bin<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
-                                       ws.cap_pairs, st);
+                                       ws.cap_pairs, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end,
+                                       ws.d_pstate, st);
             if (ev) FT_HIP(hipEventRecord(ev[1], s));
             if (ws.n_rblocks) {
                 auto kr = ws.tuning.dst_prefetch ? k_raster<true> : k_raster<false>;
-                kr<<<(uint32_t)ws.n_rblocks, BLOCK, ws.lds_cells * sizeof(int32_t), s>>>(
-                    L0, ws.d_paths, ws.d_rblock, ws.d_band_path, ws.band_shift, ws.d_band_off, ws.d_band_list,
-                    ws.d_edges, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end, ws.d_pstate, st);
+                kr<<<(uint32_t)ws.n_rblocks, BLOCK, raster_lds_bytes(ws.lds_cells), s>>>(
+                    L0, ws.d_paths, ws.d_rblock, ws.band_shift, ws.d_band_off, ws.d_band_list, ws.d_edges,
+                    ws.d_pstate, st);
             }
             if (ev) FT_HIP(hipEventRecord(ev[2], s));
         }

@@ -62,6 +62,7 @@ struct PathState {
     uint32_t pt_base;           // global index of the path's first point
     uint32_t n_edges;           // non-horizontal segments
     uint32_t stored;            // points stored in the Fig after the cap
+    uint32_t fwd;               // Fig::get_dir at the top point (set before the raster)
 };
 
 constexpr uint32_t N_SCAN_SLOTS = 9;

@@ -17,6 +17,8 @@ import time
 
 import numpy as np
 
+from ._lib import FootileError
+
 BAND_ROWS = (4, 8, 16, 32, 64)
 LDS_CELLS = (2048, 4096, 8192, 16384)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "footile_amd", "tuning.json")
@@ -67,7 +69,12 @@ def autotune(batch, descs, n_ops, cache=DEFAULT_CACHE, reps=3, candidates=None, 
     best = None
     t0 = time.perf_counter()
     for b, c, p in cands:
-        batch.set_tuning(b, c, p)
+        try:
+            batch.set_tuning(b, c, p)
+        except FootileError as e:
+            if e.code != 5:  # FT_ENOTSUP
+                raise
+            continue  # more LDS per block than the device has
         batch.run()
         ms = _time(batch, reps)
         if verbose:
