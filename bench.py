@@ -25,6 +25,7 @@ import time
 
 import numpy as np
 
+TIMING_EVERY = 4
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -327,13 +328,17 @@ def main():
     for _ in range(args.warmup):
         batch.launch()
         batch.sync()
-    batch.set_timing(True)
+    # Event timing of the raster kernel on every TIMING_EVERY-th step of the
+    # timed loop: timed launches carry events in their first and raster
+    # dispatches, which costs a few microseconds of queue time per launch.
     batch.timing()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.steps):
+        batch.set_timing(k % TIMING_EVERY == 0)
         batch.launch()
     batch.sync()
     t1 = time.perf_counter()
+    batch.set_timing(False)
     total_ms, raster_ms, runs = batch.timing()
     sec_per_step = (t1 - t0) / args.steps
     raster_avg_ms = raster_ms / max(runs, 1)

@@ -561,8 +561,8 @@ uint32_t grid_of(uint64_t n) {
 }
 
 uint32_t scan_blocks(uint64_t n) {
-    uint64_t g = (n + SCAN_TILE - 1) / SCAN_TILE;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 1024));
+    uint64_t g = (n + scan_tile<uint32_t>() - 1) / scan_tile<uint32_t>();
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, SCAN_GRID_MAX));
 }
 
 // Geometry for one parameter set: vertices, per-(path, row) edge lists.
@@ -577,12 +577,12 @@ ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, flo
     k_vertices<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, reinterpret_cast<const float2 *>(points), d->d_status);
     k_rows<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, d->d_status);
     const uint32_t nc = (uint32_t)d->n_cells;
-    k_scan<uint32_t, false><<<scan_blocks(nc), BLOCK, 0, s>>>(d->d_cell_cnt, d->d_cell_cnt, nullptr, nc, nc, d->d_flags,
+    k_scan<uint32_t, false><<<scan_blocks(nc), BLOCK, 0, s>>>(array_src(d->d_cell_cnt), d->d_cell_cnt, nullptr, nc, nc, d->d_flags,
                                                               d->d_agg, d->d_incl, &d->d_status->tickets[0], nullptr,
-                                                              d->d_status, false, true);
+                                                              d->d_status, false, true, scan_resident_blocks());
     k_scan<uint32_t, false><<<scan_blocks(nc), BLOCK, 0, s>>>(
-        d->d_cell_cnt, d->d_cell_off, nullptr, nc, nc, d->d_flags + d->cap_tiles, d->d_agg + d->cap_tiles,
-        d->d_incl + d->cap_tiles, &d->d_status->tickets[1], &d->d_status->n_pairs, d->d_status, false, false);
+        array_src(d->d_cell_cnt), d->d_cell_off, nullptr, nc, nc, d->d_flags + d->cap_tiles, d->d_agg + d->cap_tiles,
+        d->d_incl + d->cap_tiles, &d->d_status->tickets[1], &d->d_status->n_pairs, d->d_status, false, false, scan_resident_blocks());
     k_fill_cells<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, d->d_status);
     k_sort_cells<<<grid_of(d->n_cells), BLOCK, 0, s>>>(S, d->d_status);
     FD_HIP(hipGetLastError());
@@ -703,7 +703,7 @@ ft_status ft_diff_new(const ft_diff_layout *L, ft_diff **out) {
     const dim3 g = raster_grid(d);
     d->ck_stride = (uint64_t)g.x * g.y * g.z * DBLOCK;
     const uint64_t n_ck = std::max<uint64_t>(1, (max_paths + KSEG - 1) / KSEG);
-    d->cap_tiles = (cells + SCAN_TILE - 1) / SCAN_TILE + 1;
+    d->cap_tiles = (cells + SCAN_TILE_MIN - 1) / SCAN_TILE_MIN + 1;
     d->cap_pairs = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, 8 * nv + 4096);
     ft_status st;
     if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) st = FT_EDEVICE;

@@ -26,6 +26,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     int device = 0;
     size_t lds_max = 65536;  // LDS bytes per block on the device
+    uint32_t scan_resident = 0;  // scan blocks resident at once (0: unknown)
     // batch layout
     size_t n_ops = 0, n_paths = 0, n_bands = 0;
     size_t cap_ops = 0, cap_paths = 0, cap_bands = 0;
@@ -92,7 +93,7 @@ struct Workspace {
     static constexpr int TIMING_RING = 256;
     bool timing = false;
     int n_pending = 0;
-    hipEvent_t *ev = nullptr;  // TIMING_RING x 4 events, created on first use
+    hipEvent_t *ev = nullptr;  // TIMING_RING x 3 events (start, raster start, raster end), created on first use
     double t_total_ms = 0.0, t_raster_ms = 0.0;
     uint64_t n_timed = 0;
 };

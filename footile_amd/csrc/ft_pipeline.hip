@@ -23,6 +23,8 @@
 #include "ft_stages.h"
 #include "ft_stroke_dev.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cstring>
 
@@ -203,15 +205,45 @@ __global__ __launch_bounds__(BLOCK) void k_kept(Limits L0, const uint32_t *ev_me
     if (bad) raise_err(st, FT_EDEVICE);
 }
 
+// One wavefront per path; the cap point is found by a 64-way search (each
+// round probes 64 events at once), a few round trips for any path length.
 __global__ __launch_bounds__(BLOCK) void k_cap(Limits L0, const PathDev *paths, const unsigned long long *op_off,
                                                const uint32_t *ev_meta, const I2 *ev_xy, const uint32_t *ev_kept,
                                                const uint32_t *ev_idx, const int32_t *run_first, PathState *ps,
                                                DevStatus *st) {
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (BLOCK / 64);
     uint32_t bad = 0;
-    for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < L.n_paths; p += gridDim.x * BLOCK)
-        cap_path(p, L, paths, op_off, ev_meta, ev_xy, ev_kept, ev_idx, run_first, ps, bad);
+    for (uint32_t p = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); p < L.n_paths; p += waves) {
+        const CapSpan c = cap_span(p, L, paths, op_off, ev_kept, ev_idx, bad);
+        uint32_t repop = 0;
+        if (c.over) {
+            // cap_after is false below lo and true from hi on (hi = ee: none)
+            uint32_t lo = c.eb, hi = c.ee;
+            while (hi - lo > 64) {
+                const uint32_t step = (hi - lo + 63) / 64;
+                uint32_t m = lo + (lane + 1) * step - 1;
+                m = m < hi ? m : hi - 1;
+                const unsigned long long t = __ballot(cap_after(m, c, ev_kept, ev_idx));
+                if (t) {
+                    const int f = __ffsll((long long)t) - 1;
+                    const uint32_t mf = __shfl(m, f, 64), mp = __shfl(m, f > 0 ? f - 1 : 0, 64);
+                    hi = mf;
+                    lo = f > 0 ? mp + 1 : lo;
+                } else {
+                    lo = hi;
+                }
+            }
+            const uint32_t m = lo + lane;
+            const unsigned long long t = __ballot((m < hi) && cap_after(m < hi ? m : lo, c, ev_kept, ev_idx));
+            const uint32_t ea = t ? lo + (uint32_t)(__ffsll((long long)t) - 1) : hi;
+            bad |= ea < c.ee ? 0u : 1u;
+            if (lane == 0 && ea < c.ee) repop = cap_repop(ea, c, L, ev_meta, ev_xy, run_first);
+        }
+        if (lane == 0) cap_store(p, c, repop, ps);
+    }
     if (bad) raise_err(st, FT_EDEVICE);
 }
 
@@ -1126,6 +1158,7 @@ static ft_status dev_alloc(T *&ptr, size_t n) {
 
 ft_status ws_init(Workspace &ws) {
     FT_HIP(hipGetDevice(&ws.device));
+    ws.scan_resident = scan_resident_blocks();
     int lds_max = 0;
     FT_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ws.device));
     ws.lds_max = (size_t)lds_max;
@@ -1149,7 +1182,7 @@ void ws_free(Workspace &ws) {
         if (b) (void)hipFree(b);
     if (ws.h_status) (void)hipHostFree(ws.h_status);
     if (ws.ev) {
-        for (int i = 0; i < 4 * Workspace::TIMING_RING; ++i)
+        for (int i = 0; i < 3 * Workspace::TIMING_RING; ++i)
             if (ws.ev[i]) (void)hipEventDestroy(ws.ev[i]);
         delete[] ws.ev;
     }
@@ -1195,7 +1228,7 @@ static ft_status ensure_scan_capacity(Workspace &ws) {
     size_t n = 4 * ws.cap_events + 1;  // stroke outline items bound every other scan length
     if (ws.n_ops > n) n = ws.n_ops;
     if ((ws.n_bands << ws.band_shift) > n) n = ws.n_bands << ws.band_shift;
-    size_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE + 1;
+    size_t tiles = (n + SCAN_TILE_MIN - 1) / SCAN_TILE_MIN + 1;
     if (tiles <= ws.cap_tiles && ws.d_scan_flags) return FT_OK;
     ft_status s;
     if ((s = dev_alloc(ws.d_scan_flags, tiles * SCAN_SLOTS)) || (s = dev_alloc(ws.d_scan_agg, tiles * SCAN_SLOTS)) ||
@@ -1323,10 +1356,11 @@ static inline uint32_t grid_for(size_t n) {
     return (uint32_t)(g > GRID_MAX ? GRID_MAX : g);
 }
 
+template <typename T>
 static inline uint32_t scan_grid(size_t n) {
-    size_t g = (n + SCAN_TILE - 1) / SCAN_TILE;
+    size_t g = (n + scan_tile<T>() - 1) / scan_tile<T>();
     if (g < 1) g = 1;
-    return (uint32_t)(g > 1024 ? 1024 : g);
+    return (uint32_t)(g > SCAN_GRID_MAX ? SCAN_GRID_MAX : g);
 }
 
 static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev);
@@ -1339,10 +1373,10 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
     hipEvent_t *ev = nullptr;
     if (ws.timing && mode == MODE_FILL && ws.n_bands && ws.n_pending < Workspace::TIMING_RING) {
         if (!ws.ev) {
-            ws.ev = new hipEvent_t[4 * Workspace::TIMING_RING]();
-            for (int i = 0; i < 4 * Workspace::TIMING_RING; ++i) FT_HIP(hipEventCreate(&ws.ev[i]));
+            ws.ev = new hipEvent_t[3 * Workspace::TIMING_RING]();
+            for (int i = 0; i < 3 * Workspace::TIMING_RING; ++i) FT_HIP(hipEventCreate(&ws.ev[i]));
         }
-        ev = ws.ev + 4 * ws.n_pending++;
+        ev = ws.ev + 3 * ws.n_pending++;
     }
     if (!(ws.graph && mode == MODE_FILL)) return ws_enqueue(ws, mode, ev);
     if (!ws.gexec) {
@@ -1362,13 +1396,12 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
             return FT_EDEVICE;
         }
     }
-    if (ev) {
-        FT_HIP(hipEventRecord(ev[0], ws.stream));
-        FT_HIP(hipEventRecord(ev[1], ws.stream));  // the raster kernel is not timed inside a graph
+    if (ev) FT_HIP(hipEventRecord(ev[0], ws.stream));
+    FT_HIP(hipGraphLaunch(ws.gexec, ws.stream));
+    if (ev) {  // the raster kernel is not timed inside a graph: ev[1] = ev[2]
+        FT_HIP(hipEventRecord(ev[1], ws.stream));
         FT_HIP(hipEventRecord(ev[2], ws.stream));
     }
-    FT_HIP(hipGraphLaunch(ws.gexec, ws.stream));
-    if (ev) FT_HIP(hipEventRecord(ev[3], ws.stream));
     return FT_OK;
 }
 
@@ -1384,24 +1417,27 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
     unsigned long long *ag = ws.d_scan_agg, *in = ws.d_scan_incl;
     DevStatus *st = ws.d_status;
     const uint32_t n_flags = (uint32_t)(tiles * SCAN_SLOTS);
-    if (ev) FT_HIP(hipEventRecord(ev[0], s));
+    // Timing: the first and the raster kernels carry the events in their
+    // dispatch (ev[0] at k_clear's start, ev[1] / ev[2] around the raster), so
+    // no marker packet stalls the stream between kernels.
     const size_t n_slots = ws.n_bands << ws.band_shift;
-    k_clear<<<grid_for(std::max(std::max(ws.n_paths, n_slots), (size_t)n_flags)), BLOCK, 0, s>>>(
-        ws.d_pstate, (uint32_t)ws.n_paths, ws.d_band_cnt, ws.d_band_cur, (uint32_t)n_slots, fl, n_flags, st);
+    hipExtLaunchKernelGGL(k_clear, dim3(grid_for(std::max(std::max(ws.n_paths, n_slots), (size_t)n_flags))),
+                          dim3(BLOCK), 0, s, ev ? ev[0] : nullptr, nullptr, 0, ws.d_pstate, (uint32_t)ws.n_paths,
+                          ws.d_band_cnt, ws.d_band_cur, (uint32_t)n_slots, fl, n_flags, st);
     const uint32_t *lastpw = nullptr;
     if (mode != MODE_FILL && n_ops) {
         k_pen_width_mark<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, n_ops, ws.d_op_pw);
-        k_scan<uint32_t, true><<<scan_grid(n_ops), BLOCK, 0, s>>>(ws.d_op_pw, ws.d_op_lastpw, nullptr, n_ops, n_ops,
+        k_scan<uint32_t, true><<<scan_grid<uint32_t>(n_ops), BLOCK, 0, s>>>(array_src(ws.d_op_pw), ws.d_op_lastpw, nullptr, n_ops, n_ops,
                                                                   fl + 5 * tiles, ag + 5 * tiles, in + 5 * tiles,
-                                                                  &st->tickets[5], nullptr, st, false, false);
+                                                                  &st->tickets[5], nullptr, st, false, false, ws.scan_resident);
         lastpw = ws.d_op_lastpw;
     }
     if (n_ops) {
         k_count<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
                                                                      lastpw, ws.d_op_cnt, ws.d_sub_off, st);
-        k_scan<unsigned long long, false><<<scan_grid(n_ops), BLOCK, 0, s>>>(
-            ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops, fl, ag, in, &st->tickets[0], &st->op_total, st, false,
-            false);
+        k_scan<unsigned long long, false><<<scan_grid<unsigned long long>(n_ops), BLOCK, 0, s>>>(
+            array_src(ws.d_op_cnt), ws.d_op_off, nullptr, n_ops, n_ops, fl, ag, in, &st->tickets[0], &st->op_total, st, false,
+            false, ws.scan_resident);
         EventSink em{};
         em.cap_events = ws.cap_events;
         em.cap_runs = (uint32_t)ws.cap_events;
@@ -1426,20 +1462,20 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         sp.miter = ws.miter;
         sp.tol_sq = ws.n_paths ? ws.stroke_tol_sq : 0.0f;
         k_st_flags<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_flags, ws.d_st_last, st);
-        k_scan<unsigned long long, false><<<scan_grid(ws.cap_events), BLOCK, 0, s>>>(
-            ws.d_st_flags, ws.d_st_off, &st->n_events, 0, cap, fl + 6 * tiles, ag + 6 * tiles, in + 6 * tiles,
-            &st->tickets[6], &st->st_total, st, true, false);
-        k_scan<uint32_t, true><<<scan_grid(ws.cap_events), BLOCK, 0, s>>>(
-            ws.d_st_last, ws.d_st_last, &st->n_events, 0, cap, fl + 7 * tiles, ag + 7 * tiles, in + 7 * tiles,
-            &st->tickets[7], nullptr, st, true, true);
+        k_scan<unsigned long long, false><<<scan_grid<unsigned long long>(ws.cap_events), BLOCK, 0, s>>>(
+            array_src(ws.d_st_flags), ws.d_st_off, &st->n_events, 0, cap, fl + 6 * tiles, ag + 6 * tiles, in + 6 * tiles,
+            &st->tickets[6], &st->st_total, st, true, false, ws.scan_resident);
+        k_scan<uint32_t, true><<<scan_grid<uint32_t>(ws.cap_events), BLOCK, 0, s>>>(
+            array_src(ws.d_st_last), ws.d_st_last, &st->n_events, 0, cap, fl + 7 * tiles, ag + 7 * tiles, in + 7 * tiles,
+            &st->tickets[7], nullptr, st, true, true, ws.scan_resident);
         k_st_compact<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_off, ws.d_spts, ws.d_sub_start, ws.d_sub_event, cap, st);
         k_st_subs<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_last, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
                                        ws.d_sub_joined, cap, st);
         k_st_count<<<grid_for(cap_items), BLOCK, 0, s>>>(ws.d_spts, ws.d_sub_start, ws.d_sub_n, ws.d_sub_joined, sp,
                                                          ws.d_it_cnt, st);
-        k_scan<unsigned long long, false><<<scan_grid(cap_items), BLOCK, 0, s>>>(
-            ws.d_it_cnt, ws.d_it_off, &st->st_items, 0, (uint32_t)cap_items, fl + 8 * tiles, ag + 8 * tiles,
-            in + 8 * tiles, &st->tickets[8], &st->stroke_total, st, true, false);
+        k_scan<unsigned long long, false><<<scan_grid<unsigned long long>(cap_items), BLOCK, 0, s>>>(
+            array_src(ws.d_it_cnt), ws.d_it_off, &st->st_items, 0, (uint32_t)cap_items, fl + 8 * tiles, ag + 8 * tiles,
+            in + 8 * tiles, &st->tickets[8], &st->stroke_total, st, true, false, ws.scan_resident);
         EventSink em{};
         em.cap_events = ws.cap_events;
         em.cap_runs = cap;
@@ -1462,16 +1498,18 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         L0.n_paths = (uint32_t)ws.n_paths;
         L0.n_bands = (uint32_t)ws.n_bands;
         L0.cap_points = (uint32_t)ws.cap_events;
+        // pushed-point marks -> their running maximum (the last pushed point so far)
         k_ingest<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, ws.d_ev_last, st);
-        k_scan<uint32_t, true><<<scan_grid(ws.cap_events), BLOCK, 0, s>>>(
-            ws.d_ev_last, ws.d_ev_last, &st->n_events, 0, (uint32_t)ws.cap_events, fl + 4 * tiles, ag + 4 * tiles,
-            in + 4 * tiles, &st->tickets[4], nullptr, st, true, true);
+        k_scan<uint32_t, true><<<scan_grid<uint32_t>(ws.cap_events), BLOCK, 0, s>>>(
+            array_src(ws.d_ev_last), ws.d_ev_last, &st->n_events, 0, (uint32_t)ws.cap_events, fl + 4 * tiles,
+            ag + 4 * tiles, in + 4 * tiles, &st->tickets[4], nullptr, st, true, true, ws.scan_resident);
+        // kept flags -> point indices
         k_kept<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_last,
                                     ws.d_ev_kept, st);
-        k_scan<uint32_t, false><<<scan_grid(ws.cap_events), BLOCK, 0, s>>>(
-            ws.d_ev_kept, ws.d_ev_idx, &st->n_events, 0, (uint32_t)ws.cap_events, fl + tiles, ag + tiles,
-            in + tiles, &st->tickets[1], &st->n_kept, st, true, false);
-        k_cap<<<grid_for(ws.n_paths), BLOCK, 0, s>>>(L0, ws.d_paths, ws.d_op_off, ws.d_ev_meta, ws.d_ev_xy,
+        k_scan<uint32_t, false><<<scan_grid<uint32_t>(ws.cap_events), BLOCK, 0, s>>>(
+            array_src(ws.d_ev_kept), ws.d_ev_idx, &st->n_events, 0, (uint32_t)ws.cap_events, fl + tiles, ag + tiles,
+            in + tiles, &st->tickets[1], &st->n_kept, st, true, false, ws.scan_resident);
+        k_cap<<<grid_for(ws.n_paths * 64), BLOCK, 0, s>>>(L0, ws.d_paths, ws.d_op_off, ws.d_ev_meta, ws.d_ev_xy,
                                                      ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first, ws.d_pstate, st);
         k_compact<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first,
                                        ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_pts, ws.d_pt_run, ws.d_pstate,
@@ -1484,23 +1522,24 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         if (ws.n_bands) {
             // pairs per (band, replica) -> list segment offsets and the pair total
             const uint32_t n_slots = (uint32_t)(ws.n_bands << ws.band_shift);
-            k_scan<uint32_t, false><<<scan_grid(n_slots), BLOCK, 0, s>>>(
-                ws.d_band_cnt, ws.d_band_off, nullptr, n_slots, n_slots, fl + 2 * tiles, ag + 2 * tiles,
-                in + 2 * tiles, &st->tickets[2], &st->n_pairs, st, true, false);
+            k_scan<uint32_t, false><<<scan_grid<uint32_t>(n_slots), BLOCK, 0, s>>>(
+                array_src(ws.d_band_cnt), ws.d_band_off, nullptr, n_slots, n_slots, fl + 2 * tiles, ag + 2 * tiles,
+                in + 2 * tiles, &st->tickets[2], &st->n_pairs, st, true, false, ws.scan_resident);
             k_bin<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
                                        ws.cap_pairs, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end,
                                        ws.d_pstate, st);
-            if (ev) FT_HIP(hipEventRecord(ev[1], s));
             if (ws.n_rblocks) {
                 auto kr = ws.tuning.dst_prefetch ? k_raster<true> : k_raster<false>;
-                kr<<<(uint32_t)ws.n_rblocks, BLOCK, raster_lds_bytes(ws.lds_cells), s>>>(
-                    L0, ws.d_paths, ws.d_rblock, ws.band_shift, ws.d_band_off, ws.d_band_list, ws.d_edges,
-                    ws.d_pstate, st);
+                hipExtLaunchKernelGGL(kr, dim3((uint32_t)ws.n_rblocks), dim3(BLOCK),
+                                      (uint32_t)raster_lds_bytes(ws.lds_cells), s, ev ? ev[1] : nullptr,
+                                      ev ? ev[2] : nullptr, 0, L0, ws.d_paths, ws.d_rblock, ws.band_shift,
+                                      ws.d_band_off, ws.d_band_list, ws.d_edges, ws.d_pstate, st);
+            } else if (ev) {
+                FT_HIP(hipEventRecord(ev[1], s));
+                FT_HIP(hipEventRecord(ev[2], s));
             }
-            if (ev) FT_HIP(hipEventRecord(ev[2], s));
         }
     }
-    if (ev) FT_HIP(hipEventRecord(ev[3], s));
     FT_HIP(hipGetLastError());
     FT_HIP(hipMemcpyAsync(ws.h_status, ws.d_status, sizeof(DevStatus), hipMemcpyDeviceToHost, s));
     return FT_OK;
@@ -1513,8 +1552,8 @@ ft_status ws_finish(Workspace &ws, bool read_pstate) {
     FT_HIP(hipStreamSynchronize(ws.stream));
     for (int k = 0; k < ws.n_pending; ++k) {
         float total = 0.0f, raster = 0.0f;
-        hipEvent_t *ev = ws.ev + 4 * k;
-        FT_HIP(hipEventElapsedTime(&total, ev[0], ev[3]));
+        hipEvent_t *ev = ws.ev + 3 * k;
+        FT_HIP(hipEventElapsedTime(&total, ev[0], ev[2]));
         FT_HIP(hipEventElapsedTime(&raster, ev[1], ev[2]));
         ws.t_total_ms += total;
         ws.t_raster_ms += raster;

@@ -410,13 +410,22 @@ FT_HD inline Pushed pushed_event(uint32_t e, const Limits &L, const uint32_t *me
 // event, else 0.  The inclusive max-scan of `last` at a run's close event is
 // then 1 + the run's last pushed point (pushed points rise with e, and the
 // run's first point is always pushed).
-FT_HD inline void ingest_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy, int32_t *run_first,
-                               const uint32_t *run_path, uint32_t *last, uint32_t &bad) {
+// ingest_mark returns last[e] and, in first_run, the run whose first point is
+// e (NO_IDX: none); ingest_event stores both.
+FT_HD inline uint32_t ingest_mark(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy,
+                                  const uint32_t *run_path, uint32_t &bad, uint32_t &first_run) {
     const Pushed q = pushed_event(e, L, meta, xy);
     const uint32_t p = run_path[q.ok ? q.run : 0];
-    if (q.is_pt & q.first) run_first[q.run] = (int32_t)e;
-    last[e] = (q.is_pt & q.pushed) ? e + 1 : 0u;
+    first_run = (q.is_pt & q.first) ? q.run : NO_IDX;
     bad |= (q.ok & (p < L.n_paths)) ? 0u : 1u;
+    return (q.is_pt & q.pushed) ? e + 1 : 0u;
+}
+
+FT_HD inline void ingest_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy, int32_t *run_first,
+                               const uint32_t *run_path, uint32_t *last, uint32_t &bad) {
+    uint32_t first_run;
+    last[e] = ingest_mark(e, L, meta, xy, run_path, bad, first_run);
+    if (first_run != NO_IDX) run_first[first_run] = (int32_t)e;
 }
 
 // 1 if point event e survives the closing pop.  `last_scan` is the inclusive
@@ -464,18 +473,26 @@ FT_HD inline int32_t top_row_of(unsigned long long key) {
     return ((int32_t)((uint32_t)(key >> 32) ^ 0x80000000u)) >> 16;
 }
 
-// Stage 4a, one thread per path: the path's first point index, the 65,535-point
-// cap (vid.rs:17, fig.rs:429-431) and the offered / stored counts.  Once the
+// Stage 4a, per path: the path's first point index, the 65,535-point cap
+// (vid.rs:17, fig.rs:429-431) and the offered / stored counts.  Once the
 // stored count reaches the cap every later point is ignored; the run holding
 // the cap point then closes against its truncated last point (fig.rs:373-383),
 // which pops local point 65534 when it equals the run's first point.  Every
 // later run only adds a point and pops it again.  The stored points are the
 // kept indices [pt_base, pt_base + stored).
-FT_HD inline void cap_path(uint32_t p, const Limits &L, const PathDev *paths, const unsigned long long *op_off,
-                           const uint32_t *meta, const I2 *xy, const uint32_t *kept, const uint32_t *idx_of,
-                           const int32_t *run_first, PathState *ps, uint32_t &bad) {
+struct CapSpan {
+    uint32_t eb, ee;     // the path's events
+    uint32_t base;       // kept index of its first point
+    uint32_t offered, stored;
+    bool over;           // stored reached the cap: cap_point / cap_repop decide
+};
+
+FT_HD inline CapSpan cap_span(uint32_t p, const Limits &L, const PathDev *paths, const unsigned long long *op_off,
+                              const uint32_t *kept, const uint32_t *idx_of, uint32_t &bad) {
     const PathDev &P = paths[p];
-    uint32_t base = 0, repop = 0, offered = 0, stored = 0;
+    CapSpan c;
+    c.eb = c.ee = c.base = c.offered = c.stored = 0;
+    c.over = false;
     const bool whole = (P.flags & PATH_WHOLE) != 0;
     if (whole || (P.op_count && P.op_begin < L.n_ops)) {
         const uint32_t k = P.op_begin + P.op_count;
@@ -484,44 +501,69 @@ FT_HD inline void cap_path(uint32_t p, const Limits &L, const PathDev *paths, co
         const uint32_t cb = whole ? 0u : (uint32_t)op_off[P.op_begin];
         const uint32_t ce = whole ? L.n_runs : (k < L.n_ops ? (uint32_t)op_off[k] : L.n_runs);
         if (eb < ee && ee <= L.n_events && cb <= ce && ce - cb <= ee - eb) {
-            offered = (ee - eb) - (ce - cb);  // point events: events minus closes
-            base = idx_of[eb];
-            const uint32_t n_kept = idx_of[ee - 1] + kept[ee - 1] - base;
-            stored = n_kept;
-            if (n_kept >= FT_MAX_POINTS) {
-                // the kept event with local index 65534: first e with idx >= base + 65534
-                uint32_t lo = eb, hi = ee;
-                const uint32_t target = base + FT_MAX_POINTS - 1;
-                while (lo < hi) {
-                    const uint32_t mid = lo + (hi - lo) / 2;
-                    if (idx_of[mid] + kept[mid] <= target)
-                        lo = mid + 1;
-                    else
-                        hi = mid;
-                }
-                const uint32_t ea = lo;  // kept, idx_of[ea] == target
-                const uint32_t run = meta[ea] & ~CLOSE_BIT;
-                // the next pushed point event of the path
-                uint32_t e2 = ea + 1;
-                while (e2 < ee) {
-                    const Pushed q = pushed_event(e2, L, meta, xy);
-                    if (q.is_pt & q.pushed) break;
-                    ++e2;
-                }
-                if (e2 < ee && meta[e2] == run && run < L.n_runs) {
-                    const int32_t rf = run_first[run];
-                    if (rf >= 0 && (uint32_t)rf < L.n_events && i2_eq(xy[rf], xy[ea])) repop = 1;
-                }
-                stored = FT_MAX_POINTS - repop;
-            }
+            c.eb = eb;
+            c.ee = ee;
+            c.offered = (ee - eb) - (ce - cb);  // point events: events minus closes
+            c.base = idx_of[eb];
+            c.stored = idx_of[ee - 1] + kept[ee - 1] - c.base;
+            c.over = c.stored >= FT_MAX_POINTS;
         } else if (eb > ee || ee > L.n_events || cb > ce) {
             bad = 1;
         }
     }
-    ps[p].pt_base = base;
+    return c;
+}
+
+// The cap point is the kept event with local index 65534: the first event e of
+// the path with cap_after(e) (a monotone predicate over the path's events).
+FT_HD inline bool cap_after(uint32_t e, const CapSpan &c, const uint32_t *kept, const uint32_t *idx_of) {
+    return idx_of[e] + kept[e] > c.base + FT_MAX_POINTS - 1;
+}
+
+// 1 if the close of the cap point's run pops it: the run's next pushed point
+// is offered and dropped, and the close compares the run's first point with
+// the cap point.
+FT_HD inline uint32_t cap_repop(uint32_t ea, const CapSpan &c, const Limits &L, const uint32_t *meta, const I2 *xy,
+                                const int32_t *run_first) {
+    const uint32_t run = meta[ea] & ~CLOSE_BIT;
+    uint32_t e2 = ea + 1;  // the next pushed point event of the path
+    while (e2 < c.ee) {
+        const Pushed q = pushed_event(e2, L, meta, xy);
+        if (q.is_pt & q.pushed) break;
+        ++e2;
+    }
+    if (e2 < c.ee && meta[e2] == run && run < L.n_runs) {
+        const int32_t rf = run_first[run];
+        if (rf >= 0 && (uint32_t)rf < L.n_events && i2_eq(xy[rf], xy[ea])) return 1;
+    }
+    return 0;
+}
+
+FT_HD inline void cap_store(uint32_t p, const CapSpan &c, uint32_t repop, PathState *ps) {
+    ps[p].pt_base = c.base;
     ps[p].repop = repop;
-    ps[p].offered = offered;
-    ps[p].stored = stored;
+    ps[p].offered = c.offered;
+    ps[p].stored = c.over ? FT_MAX_POINTS - repop : c.stored;
+}
+
+// The whole stage for one path, sequentially (binary search for the cap point).
+FT_HD inline void cap_path(uint32_t p, const Limits &L, const PathDev *paths, const unsigned long long *op_off,
+                           const uint32_t *meta, const I2 *xy, const uint32_t *kept, const uint32_t *idx_of,
+                           const int32_t *run_first, PathState *ps, uint32_t &bad) {
+    const CapSpan c = cap_span(p, L, paths, op_off, kept, idx_of, bad);
+    uint32_t repop = 0;
+    if (c.over) {
+        uint32_t lo = c.eb, hi = c.ee;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (cap_after(mid, c, kept, idx_of))
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        repop = cap_repop(lo, c, L, meta, xy, run_first);
+    }
+    cap_store(p, c, repop, ps);
 }
 
 // Stage 4b: compaction into the per-path point arrays (the Fig's `points`).
