@@ -560,11 +560,6 @@ uint32_t grid_of(uint64_t n) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, GRID_MAX));
 }
 
-uint32_t scan_blocks(uint64_t n) {
-    uint64_t g = (n + scan_tile<uint32_t>() - 1) / scan_tile<uint32_t>();
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, SCAN_GRID_MAX));
-}
-
 // Geometry for one parameter set: vertices, per-(path, row) edge lists.
 ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, float *gcol) {
     hipStream_t s = d->stream;
@@ -577,12 +572,8 @@ ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, flo
     k_vertices<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, reinterpret_cast<const float2 *>(points), d->d_status);
     k_rows<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, d->d_status);
     const uint32_t nc = (uint32_t)d->n_cells;
-    k_scan<uint32_t, false><<<scan_blocks(nc), BLOCK, 0, s>>>(array_src(d->d_cell_cnt), d->d_cell_cnt, nullptr, nc, nc, d->d_flags,
-                                                              d->d_agg, d->d_incl, &d->d_status->tickets[0], nullptr,
-                                                              d->d_status, false, true, scan_resident_blocks());
-    k_scan<uint32_t, false><<<scan_blocks(nc), BLOCK, 0, s>>>(
-        array_src(d->d_cell_cnt), d->d_cell_off, nullptr, nc, nc, d->d_flags + d->cap_tiles, d->d_agg + d->cap_tiles,
-        d->d_incl + d->cap_tiles, &d->d_status->tickets[1], &d->d_status->n_pairs, d->d_status, false, false, scan_resident_blocks());
+    scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_cnt, nullptr, nc, nc, d->d_flags, d->d_agg, d->d_incl, nullptr, d->d_status, false, true);
+    scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_off, nullptr, nc, nc, d->d_flags + d->cap_tiles, d->d_agg + d->cap_tiles, d->d_incl + d->cap_tiles, &d->d_status->n_pairs, d->d_status, false, false);
     k_fill_cells<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, d->d_status);
     k_sort_cells<<<grid_of(d->n_cells), BLOCK, 0, s>>>(S, d->d_status);
     FD_HIP(hipGetLastError());

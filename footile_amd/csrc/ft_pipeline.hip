@@ -1356,13 +1356,6 @@ static inline uint32_t grid_for(size_t n) {
     return (uint32_t)(g > GRID_MAX ? GRID_MAX : g);
 }
 
-template <typename T>
-static inline uint32_t scan_grid(size_t n) {
-    size_t g = (n + scan_tile<T>() - 1) / scan_tile<T>();
-    if (g < 1) g = 1;
-    return (uint32_t)(g > SCAN_GRID_MAX ? SCAN_GRID_MAX : g);
-}
-
 static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev);
 
 // Enqueue one run.  In graph mode a fill run is captured once into a HIP
@@ -1427,17 +1420,13 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
     const uint32_t *lastpw = nullptr;
     if (mode != MODE_FILL && n_ops) {
         k_pen_width_mark<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, n_ops, ws.d_op_pw);
-        k_scan<uint32_t, true><<<scan_grid<uint32_t>(n_ops), BLOCK, 0, s>>>(array_src(ws.d_op_pw), ws.d_op_lastpw, nullptr, n_ops, n_ops,
-                                                                  fl + 5 * tiles, ag + 5 * tiles, in + 5 * tiles,
-                                                                  &st->tickets[5], nullptr, st, false, false, ws.scan_resident);
+        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_op_pw, ws.d_op_lastpw, nullptr, n_ops, n_ops, fl + 5 * tiles, ag + 5 * tiles, in + 5 * tiles, nullptr, st, false, false);
         lastpw = ws.d_op_lastpw;
     }
     if (n_ops) {
         k_count<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
                                                                      lastpw, ws.d_op_cnt, ws.d_sub_off, st);
-        k_scan<unsigned long long, false><<<scan_grid<unsigned long long>(n_ops), BLOCK, 0, s>>>(
-            array_src(ws.d_op_cnt), ws.d_op_off, nullptr, n_ops, n_ops, fl, ag, in, &st->tickets[0], &st->op_total, st, false,
-            false, ws.scan_resident);
+        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops, fl, ag, in, &st->op_total, st, false, false);
         EventSink em{};
         em.cap_events = ws.cap_events;
         em.cap_runs = (uint32_t)ws.cap_events;
@@ -1462,20 +1451,14 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         sp.miter = ws.miter;
         sp.tol_sq = ws.n_paths ? ws.stroke_tol_sq : 0.0f;
         k_st_flags<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_flags, ws.d_st_last, st);
-        k_scan<unsigned long long, false><<<scan_grid<unsigned long long>(ws.cap_events), BLOCK, 0, s>>>(
-            array_src(ws.d_st_flags), ws.d_st_off, &st->n_events, 0, cap, fl + 6 * tiles, ag + 6 * tiles, in + 6 * tiles,
-            &st->tickets[6], &st->st_total, st, true, false, ws.scan_resident);
-        k_scan<uint32_t, true><<<scan_grid<uint32_t>(ws.cap_events), BLOCK, 0, s>>>(
-            array_src(ws.d_st_last), ws.d_st_last, &st->n_events, 0, cap, fl + 7 * tiles, ag + 7 * tiles, in + 7 * tiles,
-            &st->tickets[7], nullptr, st, true, true, ws.scan_resident);
+        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_st_flags, ws.d_st_off, &st->n_events, 0, cap, fl + 6 * tiles, ag + 6 * tiles, in + 6 * tiles, &st->st_total, st, true, false);
+        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_st_last, ws.d_st_last, &st->n_events, 0, cap, fl + 7 * tiles, ag + 7 * tiles, in + 7 * tiles, nullptr, st, true, true);
         k_st_compact<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_off, ws.d_spts, ws.d_sub_start, ws.d_sub_event, cap, st);
         k_st_subs<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_last, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
                                        ws.d_sub_joined, cap, st);
         k_st_count<<<grid_for(cap_items), BLOCK, 0, s>>>(ws.d_spts, ws.d_sub_start, ws.d_sub_n, ws.d_sub_joined, sp,
                                                          ws.d_it_cnt, st);
-        k_scan<unsigned long long, false><<<scan_grid<unsigned long long>(cap_items), BLOCK, 0, s>>>(
-            array_src(ws.d_it_cnt), ws.d_it_off, &st->st_items, 0, (uint32_t)cap_items, fl + 8 * tiles, ag + 8 * tiles,
-            in + 8 * tiles, &st->tickets[8], &st->stroke_total, st, true, false, ws.scan_resident);
+        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_it_cnt, ws.d_it_off, &st->st_items, 0, (uint32_t)cap_items, fl + 8 * tiles, ag + 8 * tiles, in + 8 * tiles, &st->stroke_total, st, true, false);
         EventSink em{};
         em.cap_events = ws.cap_events;
         em.cap_runs = cap;
@@ -1500,15 +1483,11 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         L0.cap_points = (uint32_t)ws.cap_events;
         // pushed-point marks -> their running maximum (the last pushed point so far)
         k_ingest<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, ws.d_ev_last, st);
-        k_scan<uint32_t, true><<<scan_grid<uint32_t>(ws.cap_events), BLOCK, 0, s>>>(
-            array_src(ws.d_ev_last), ws.d_ev_last, &st->n_events, 0, (uint32_t)ws.cap_events, fl + 4 * tiles,
-            ag + 4 * tiles, in + 4 * tiles, &st->tickets[4], nullptr, st, true, true, ws.scan_resident);
+        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_ev_last, ws.d_ev_last, &st->n_events, 0, (uint32_t)ws.cap_events, fl + 4 * tiles, ag + 4 * tiles, in + 4 * tiles, nullptr, st, true, true);
         // kept flags -> point indices
         k_kept<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_last,
                                     ws.d_ev_kept, st);
-        k_scan<uint32_t, false><<<scan_grid<uint32_t>(ws.cap_events), BLOCK, 0, s>>>(
-            array_src(ws.d_ev_kept), ws.d_ev_idx, &st->n_events, 0, (uint32_t)ws.cap_events, fl + tiles, ag + tiles,
-            in + tiles, &st->tickets[1], &st->n_kept, st, true, false, ws.scan_resident);
+        scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_ev_kept, ws.d_ev_idx, &st->n_events, 0, (uint32_t)ws.cap_events, fl + tiles, ag + tiles, in + tiles, &st->n_kept, st, true, false);
         k_cap<<<grid_for(ws.n_paths * 64), BLOCK, 0, s>>>(L0, ws.d_paths, ws.d_op_off, ws.d_ev_meta, ws.d_ev_xy,
                                                      ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first, ws.d_pstate, st);
         k_compact<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first,
@@ -1522,9 +1501,7 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         if (ws.n_bands) {
             // pairs per (band, replica) -> list segment offsets and the pair total
             const uint32_t n_slots = (uint32_t)(ws.n_bands << ws.band_shift);
-            k_scan<uint32_t, false><<<scan_grid<uint32_t>(n_slots), BLOCK, 0, s>>>(
-                array_src(ws.d_band_cnt), ws.d_band_off, nullptr, n_slots, n_slots, fl + 2 * tiles, ag + 2 * tiles,
-                in + 2 * tiles, &st->tickets[2], &st->n_pairs, st, true, false, ws.scan_resident);
+            scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_band_cnt, ws.d_band_off, nullptr, n_slots, n_slots, fl + 2 * tiles, ag + 2 * tiles, in + 2 * tiles, &st->n_pairs, st, true, false);
             k_bin<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
                                        ws.cap_pairs, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end,
                                        ws.d_pstate, st);

@@ -66,25 +66,6 @@ __device__ inline bool run_failed(const DevStatus *st) {
 // ticket order, so a tile only waits on tiles already owned by running blocks,
 // each of which publishes its aggregate before it waits on anything.
 // ---------------------------------------------------------------------------
-// Scan inputs.  load() reads item i (and may return a 32-bit side value);
-// commit() then performs the item's side effects.  A block loads all of its
-// items before it commits any, so the loads of a tile go out together.  A
-// pure source has no side effects, so a block may load its likely tile before
-// it holds the tile's ticket.  bind() runs once per block before any item.
-template <typename T>
-struct ArraySrc {
-    static constexpr bool kPure = true;
-    const T *in;
-    __device__ void bind(const DevStatus *) {}
-    __device__ T load(uint32_t i, uint32_t &, uint32_t &) const { return in[i]; }
-    __device__ void commit(uint32_t, T, uint32_t) const {}
-};
-
-template <typename T>
-inline ArraySrc<T> array_src(const T *in) {
-    return ArraySrc<T>{in};
-}
-
 template <typename T, bool IS_MAX>
 __device__ inline T scan_op_t(T a, T b) {
     if constexpr (IS_MAX)
@@ -104,154 +85,189 @@ __device__ inline T wave_incl_t(T v) {
     return v;
 }
 
-template <typename T, bool IS_MAX, class Src>
-__global__ __launch_bounds__(BLOCK) void k_scan(Src src, T *out, const uint32_t *n_dev, uint32_t n_host,
-                                                uint32_t n_cap, uint32_t *flags, unsigned long long *agg,
-                                                unsigned long long *incl, uint32_t *ticket, T *total,
-                                                DevStatus *st, bool check_failed, bool inclusive,
-                                                uint32_t resident) {
-    constexpr uint32_t SCAN_ITEMS = scan_items<T>(), SCAN_TILE = scan_tile<T>();
-    __shared__ uint32_t s_tile, s_fail, s_n;
+// Host side: blocks of a scan over n_cap items and the tiles each block owns.
+// The grid never exceeds the blocks resident at once, so every block's
+// predecessors are running when it waits on them.
+struct ScanGrid {
+    uint32_t blocks, tiles_per_block;
+};
+
+template <typename T>
+inline ScanGrid scan_grid_for(size_t n_cap, uint32_t resident) {
+    const uint64_t tiles = n_cap ? (n_cap + scan_tile<T>() - 1) / scan_tile<T>() : 1;
+    uint64_t g = resident ? resident : 64;  // unknown device: a grid any GPU holds
+    if (g > SCAN_GRID_MAX) g = SCAN_GRID_MAX;
+    if (g > tiles) g = tiles;
+    ScanGrid r;
+    r.blocks = (uint32_t)g;
+    r.tiles_per_block = (uint32_t)((tiles + g - 1) / g);
+    return r;
+}
+
+// Prefix scan (sum or running maximum) of in[0, n) into out, n = *n_dev if
+// given, else n_host, at most n_cap.  Block b owns the tiles_per_block tiles
+// from b * tiles_per_block: it reduces them (coalesced loads), takes its
+// prefix by decoupled look-back over the blocks before it (lane l of the
+// first wavefront reads block top - l, 64 blocks per round trip), and then
+// scans the tiles again, transposing each through LDS so that both the loads
+// and the stores stay coalesced.  A single-tile block keeps its items in
+// registers between the passes.  `total` receives the whole reduction.
+template <typename T, bool IS_MAX>
+__global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint32_t *n_dev, uint32_t n_host,
+                                                uint32_t n_cap, uint32_t tiles_per_block, uint32_t *flags,
+                                                unsigned long long *agg, unsigned long long *incl, T *total,
+                                                DevStatus *st, bool check_failed, bool inclusive) {
+    constexpr uint32_t ITEMS = scan_items<T>(), TILE = scan_tile<T>(), ROW = ITEMS + 1;
+    __shared__ T s_t[BLOCK * ROW];  // a tile, one padded row of ITEMS per thread
+    __shared__ uint32_t s_fail, s_n;
     __shared__ T s_wave[BLOCK / 64];
     __shared__ T s_prefix;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    // One pass: the grid covers every tile and fits on the device at once
-    // (`resident` blocks, from the host), so block b owns tile b and every
-    // tile's predecessors are running.  Otherwise blocks take tiles in ticket
-    // order, so a tile only waits on tiles owned by running blocks.
-    const bool one_pass = ((uint64_t)gridDim.x * SCAN_TILE >= n_cap) & (gridDim.x <= resident);
-    if (threadIdx.x == 0) {  // tile, status and length in one round trip
-        const uint32_t t = one_pass ? blockIdx.x : atomicAdd(ticket, 1u);
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {  // status and length in one round trip
         const uint32_t e = st->err, o = st->overflow;
         const uint32_t nn = n_dev ? *n_dev : n_host;
-        s_tile = t;
         s_fail = (check_failed & ((e | o) != 0u)) ? 1u : 0u;
         s_n = nn;
     }
-    // the loads of a pure source's likely tile (tickets mostly follow block
-    // order) go out with the ticket; items past n are masked once n is known
-    T x[SCAN_ITEMS];
-    uint32_t aux[SCAN_ITEMS];
-    uint32_t loaded = NO_IDX, bad = 0;
-    if constexpr (Src::kPure) {
-        loaded = blockIdx.x;
-        const uint64_t g0 = (uint64_t)loaded * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
-#pragma unroll
-        for (uint32_t j = 0; j < SCAN_ITEMS; ++j)
-            x[j] = (g0 + j < n_cap) ? src.load((uint32_t)(g0 + j), bad, aux[j]) : (T)0;
-    }
     __syncthreads();
     if (s_fail) return;
-    const uint32_t n = s_n < n_cap ? s_n : n_cap;
-    src.bind(st);
-    uint32_t tile = s_tile;
-    for (;;) {
-        const uint64_t base = (uint64_t)tile * SCAN_TILE;
-        if (base >= n) break;
-        const uint64_t mine = base + (uint64_t)threadIdx.x * SCAN_ITEMS;
-        if (tile != loaded) {
+    const uint64_t n = s_n < n_cap ? s_n : n_cap;
+    const uint64_t n_tiles = (n + TILE - 1) / TILE;
+    const uint64_t t0 = (uint64_t)blockIdx.x * tiles_per_block;
+    if (t0 >= n_tiles) return;  // no items: no later block has any either
+    const uint64_t t1 = t0 + tiles_per_block < n_tiles ? t0 + tiles_per_block : n_tiles;
+    const bool single = t1 - t0 == 1;
+    // pass 1: the chunk's reduction
+    T xr[ITEMS];
+    T acc = 0;
+    for (uint64_t t = t0; t < t1; ++t) {
+        const uint64_t base = t * TILE;
 #pragma unroll
-            for (uint32_t j = 0; j < SCAN_ITEMS; ++j)
-                x[j] = (mine + j < n) ? src.load((uint32_t)(mine + j), bad, aux[j]) : (T)0;
-            if constexpr (!Src::kPure) {
+        for (uint32_t j = 0; j < ITEMS; ++j) {
+            const uint64_t i = base + (uint64_t)j * BLOCK + tid;
+            xr[j] = i < n ? in[i] : (T)0;
+        }
 #pragma unroll
-                for (uint32_t j = 0; j < SCAN_ITEMS; ++j)
-                    if (mine + j < n) src.commit((uint32_t)(mine + j), x[j], aux[j]);
+        for (uint32_t j = 0; j < ITEMS; ++j) acc = scan_op_t<T, IS_MAX>(acc, xr[j]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc = scan_op_t<T, IS_MAX>(acc, __shfl_xor(acc, o, 64));
+    if (lane == 0) s_wave[wid] = acc;
+    __syncthreads();
+    T btot = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) btot = scan_op_t<T, IS_MAX>(btot, s_wave[w]);
+    // the chunk's prefix
+    const uint32_t b = blockIdx.x;
+    if (wid == 0) {
+        T prefix = 0;  // identity of + and of max over unsigned values
+        if (b == 0) {
+            if (lane == 0) {
+                __hip_atomic_store(&incl[0], (unsigned long long)btot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&flags[0], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else {
+            if (lane == 0) {
+                __hip_atomic_store(&agg[b], (unsigned long long)btot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&flags[b], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            int64_t top = (int64_t)b - 1;
+            uint32_t spins = 0;
+            for (;;) {
+                const int64_t j = top - lane;
+                const uint32_t f =
+                    j >= 0 ? __hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 2u;
+                const unsigned long long im = __ballot(f == 2u), zm = __ballot(f == 0u);
+                const int fi = im ? __ffsll((long long)im) - 1 : 64;  // nearest inclusive
+                const unsigned long long need = fi >= 63 ? ~0ull : ((2ull << fi) - 1ull);
+                if (zm & need) {
+                    if (++spins > SPIN_LIMIT) {  // never expected: report, do not hang
+                        if (lane == 0) {
+                            atomicOr(&st->spin_fail, 1u);
+                            raise_err(st, FT_EDEVICE);
+                        }
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                T vv = 0;
+                if (lane <= fi && j >= 0)
+                    vv = (T)(f == 2u ? __hip_atomic_load(&incl[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                     : __hip_atomic_load(&agg[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) vv = scan_op_t<T, IS_MAX>(vv, __shfl_xor(vv, o, 64));
+                prefix = scan_op_t<T, IS_MAX>(prefix, vv);
+                if (fi < 64) break;
+                top -= 64;
+            }
+            if (lane == 0) {
+                __hip_atomic_store(&incl[b], (unsigned long long)scan_op_t<T, IS_MAX>(prefix, btot),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&flags[b], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        T v[SCAN_ITEMS];
+        if (lane == 0) {
+            s_prefix = prefix;
+            if (t1 == n_tiles && total) *total = scan_op_t<T, IS_MAX>(prefix, btot);
+        }
+    }
+    __syncthreads();
+    T running = s_prefix;
+    // pass 2: scan each tile from the running prefix
+    for (uint64_t t = t0; t < t1; ++t) {
+        const uint64_t base = t * TILE;
+#pragma unroll
+        for (uint32_t j = 0; j < ITEMS; ++j) {
+            const uint32_t k = j * BLOCK + tid;
+            const uint64_t i = base + k;
+            const T x = single ? xr[j] : (i < n ? in[i] : (T)0);
+            s_t[(k / ITEMS) * ROW + k % ITEMS] = x;
+        }
+        __syncthreads();
+        T v[ITEMS];
         T run = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
-            const T xi = (mine + j < n) ? x[j] : (T)0;
+        for (uint32_t j = 0; j < ITEMS; ++j) {
             const T before = run;
-            run = scan_op_t<T, IS_MAX>(run, xi);
-            v[j] = inclusive ? run : before;  // within the thread
+            run = scan_op_t<T, IS_MAX>(run, s_t[tid * ROW + j]);
+            v[j] = inclusive ? run : before;
         }
         const T winc = wave_incl_t<T, IS_MAX>(run);
         T wexc = __shfl_up(winc, 1, 64);
         if (lane == 0) wexc = 0;
         if (lane == 63) s_wave[wid] = winc;
         __syncthreads();
-        T wpre = 0, btot = 0;
+        T wpre = 0, ttot = 0;
 #pragma unroll
         for (int w = 0; w < BLOCK / 64; ++w) {
             if (w < wid) wpre = scan_op_t<T, IS_MAX>(wpre, s_wave[w]);
-            btot = scan_op_t<T, IS_MAX>(btot, s_wave[w]);
+            ttot = scan_op_t<T, IS_MAX>(ttot, s_wave[w]);
         }
-        // Look-back by the first wavefront: lane l reads the state of tile
-        // (top - l), so 64 predecessors are examined per round trip.  A round
-        // ends at the nearest published inclusive value; every tile before it
-        // must have published at least its aggregate.
-        if (wid == 0) {
-            T prefix = 0;  // identity of + and of max over unsigned values
-            if (tile == 0) {
-                if (lane == 0) {
-                    __hip_atomic_store(&incl[0], (unsigned long long)btot, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(&flags[0], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            } else {
-                if (lane == 0) {
-                    __hip_atomic_store(&agg[tile], (unsigned long long)btot, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(&flags[tile], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                int64_t top = (int64_t)tile - 1;
-                uint32_t spins = 0;
-                for (;;) {
-                    const int64_t j = top - lane;
-                    const uint32_t f =
-                        j >= 0 ? __hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 2u;
-                    const unsigned long long im = __ballot(f == 2u), zm = __ballot(f == 0u);
-                    const int fi = im ? __ffsll((long long)im) - 1 : 64;  // nearest inclusive
-                    const unsigned long long need = fi >= 63 ? ~0ull : ((2ull << fi) - 1ull);
-                    if (zm & need) {
-                        if (++spins > SPIN_LIMIT) {  // never expected: report, do not hang
-                            if (lane == 0) {
-                                atomicOr(&st->spin_fail, 1u);
-                                raise_err(st, FT_EDEVICE);
-                            }
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    T vv = 0;
-                    if (lane <= fi && j >= 0)
-                        vv = (T)(f == 2u
-                                     ? __hip_atomic_load(&incl[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                     : __hip_atomic_load(&agg[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const T pre = scan_op_t<T, IS_MAX>(running, scan_op_t<T, IS_MAX>(wpre, wexc));
 #pragma unroll
-                    for (int o = 32; o > 0; o >>= 1) vv = scan_op_t<T, IS_MAX>(vv, __shfl_xor(vv, o, 64));
-                    prefix = scan_op_t<T, IS_MAX>(prefix, vv);
-                    if (fi < 64) break;
-                    top -= 64;
-                }
-                if (lane == 0) {
-                    __hip_atomic_store(&incl[tile], (unsigned long long)scan_op_t<T, IS_MAX>(prefix, btot),
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(&flags[tile], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
-            if (lane == 0) {
-                s_prefix = prefix;
-                if (base + SCAN_TILE >= n && total) *total = scan_op_t<T, IS_MAX>(prefix, btot);
-            }
+        for (uint32_t j = 0; j < ITEMS; ++j) s_t[tid * ROW + j] = scan_op_t<T, IS_MAX>(pre, v[j]);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < ITEMS; ++j) {
+            const uint32_t k = j * BLOCK + tid;
+            const uint64_t i = base + k;
+            if (i < n) out[i] = s_t[(k / ITEMS) * ROW + k % ITEMS];
         }
+        running = scan_op_t<T, IS_MAX>(running, ttot);
         __syncthreads();
-        const T pre = scan_op_t<T, IS_MAX>(s_prefix, scan_op_t<T, IS_MAX>(wpre, wexc));
-#pragma unroll
-        for (uint32_t j = 0; j < SCAN_ITEMS; ++j)
-            if (mine + j < n) out[mine + j] = scan_op_t<T, IS_MAX>(pre, v[j]);
-        if (one_pass) break;
-        loaded = NO_IDX;
-        __syncthreads();
-        if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-        __syncthreads();
-        tile = s_tile;
     }
-    if (bad) raise_err(st, FT_EDEVICE);
+}
+
+// Launches k_scan over a resident grid (see scan_grid_for).
+template <typename T, bool IS_MAX>
+inline void scan_launch(hipStream_t s, uint32_t resident, const T *in, T *out, const uint32_t *n_dev,
+                        uint32_t n_host, uint32_t n_cap, uint32_t *flags, unsigned long long *agg,
+                        unsigned long long *incl, T *total, DevStatus *st, bool check_failed, bool inclusive) {
+    const ScanGrid g = scan_grid_for<T>(n_cap, resident);
+    k_scan<T, IS_MAX><<<g.blocks, BLOCK, 0, s>>>(in, out, n_dev, n_host, n_cap, g.tiles_per_block, flags, agg, incl,
+                                                 total, st, check_failed, inclusive);
 }
 
 }  // namespace ft

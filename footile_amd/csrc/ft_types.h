@@ -78,7 +78,6 @@ struct DevStatus {
     unsigned long long stroke_total;  // stroker outline: events << 32 | closes
     uint32_t st_subs, st_items;       // sub-strokes kept; outline items
     uint32_t spin_fail;
-    uint32_t tickets[N_SCAN_SLOTS];
     uint32_t pad[2];
 };
 
