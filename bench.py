@@ -58,7 +58,7 @@ def cpu_random_walk(ops, budget_s):
     return outs, steps, dt
 
 
-def bench_atlas(ft, steps, warmup):
+def bench_atlas(ft, steps, warmup, tuning=None):
     from footile_amd import workloads
     from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
     side = 16384
@@ -68,8 +68,13 @@ def bench_atlas(ft, steps, warmup):
     descs = [FillDesc(int(b), int(c), atlas.ptr + int(y) * side + int(x), int(s), int(s), side,
                       fresh=True) for b, c, x, y, s in tiles]
     batch = Batch(ops, descs)
-    from footile_amd.tune import autotune
-    tuned = autotune(batch, descs, len(ops), cache=None)
+    if tuning:
+        b_rows, cells = (int(v) for v in tuning.split(",")[:2])
+        batch.set_tuning(b_rows, cells, True)
+        tuned = {"band_rows_min": b_rows, "lds_cells": cells, "tried": 0}
+    else:
+        from footile_amd.tune import autotune
+        tuned = autotune(batch, descs, len(ops), cache=None)
     batch.run()
     # raster-kernel time from eager launches (kernels are not timed inside a graph)
     batch.set_timing(True)
@@ -269,6 +274,8 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="band_rows_min,lds_cells,dst_prefetch for the random-walk batch instead of autotuning "
                          "(profiling runs)")
+    ap.add_argument("--atlas-tuning", default=None,
+                    help="band_rows_min,lds_cells for the atlas instead of autotuning (profiling runs)")
     ap.add_argument("--only", choices=["atlas", "diff"], default=None,
                     help="run only the atlas or the differentiable measurement and print its JSON (profiling runs)")
     args = ap.parse_args()
@@ -284,7 +291,7 @@ def main():
     from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
 
     if args.only == "atlas":
-        print(json.dumps(bench_atlas(ft, args.steps, args.warmup)), flush=True)
+        print(json.dumps(bench_atlas(ft, args.steps, args.warmup, args.atlas_tuning)), flush=True)
         return
     if args.only == "diff":
         print(json.dumps(bench_diff(args.steps, args.warmup, check=not args.no_diff_check)), flush=True)
@@ -363,7 +370,7 @@ def main():
     }
     if not args.no_extra:
         print("[bench] atlas", file=sys.stderr, flush=True)
-        extra["atlas"] = bench_atlas(ft, max(3, args.steps // 4), 1)
+        extra["atlas"] = bench_atlas(ft, max(3, args.steps // 4), 1, args.atlas_tuning)
         print("[bench] plain 256", file=sys.stderr, flush=True)
         extra["plain_256"] = bench_plain(ft)
         print("[bench] differentiable C4 / C5", file=sys.stderr, flush=True)

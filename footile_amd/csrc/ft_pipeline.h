@@ -37,6 +37,7 @@ struct Workspace {
     uint32_t lds_cells = 0;  // dynamic LDS cells needed by the raster kernel
     // raster blocks: (global band, row group) per block
     size_t n_rblocks = 0, cap_rblocks = 0;
+    uint32_t rb_first[3] = {0, 0, 0}, rb_count[3] = {0, 0, 0};  // raster blocks per pixel format
     uint4 *d_rblock = nullptr;  // raster blocks: (global band, row group, path, 0)
     // per-op scratch
     unsigned long long *d_op_cnt = nullptr, *d_op_off = nullptr;

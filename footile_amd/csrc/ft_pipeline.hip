@@ -279,10 +279,12 @@ __global__ __launch_bounds__(BLOCK) void k_top(Limits L0, const I2 *pts, const u
     uint32_t bad = 0;
     constexpr uint32_t TILE = TOP_ITEMS * BLOCK;
     for (uint32_t base = blockIdx.x * TILE; base < n; base += gridDim.x * TILE) {
+        // a thread's items are consecutive points, so its path rarely changes
+        // and the block reduction sees one key per path
         TopCand c[TOP_ITEMS];
 #pragma unroll
         for (uint32_t j = 0; j < TOP_ITEMS; ++j) {
-            const uint32_t i = base + j * BLOCK + threadIdx.x;
+            const uint32_t i = base + threadIdx.x * TOP_ITEMS + j;
             c[j] = top_cand(i < n ? i : 0, L, pts, pt_run, run_path, bad);
             if (i >= n) c[j].path = NO_IDX;
         }
@@ -729,6 +731,10 @@ __device__ __forceinline__ uint32_t cov_of(uint32_t sum) {
 // Graya8p: x = S0 | S1 << 16, y = 0, z = invw, w = S0 | S1 << 8.
 constexpr uint32_t LUT_ENTRIES = 256;
 constexpr uint32_t LUT_BYTES = LUT_ENTRIES * sizeof(uint4);
+constexpr uint32_t CELL_ROUND = 128;  // list entries per balanced cell-pass round (their slots share the table's LDS)
+constexpr uint32_t DIRECT_ROWS = 4;   // row groups up to this tall take the direct cell pass
+static_assert(CELL_ROUND * (sizeof(EdgeRec) + sizeof(uint32_t)) <= LUT_BYTES, "edge slots fit the table's LDS");
+static_assert(CELL_ROUND <= BLOCK, "one entry per thread");
 
 static inline size_t raster_lds_bytes(size_t cells) { return LUT_BYTES + cells * sizeof(int32_t); }
 
@@ -940,22 +946,29 @@ __device__ __forceinline__ void row_pass_rule(uint32_t rule, const int32_t *cell
         row_pass<FMT, CPT, 0, FRESH>(cells, xb, xe, row, lut, clr, carry, pp, loaded);
 }
 
-// One block per (band, row group): `rblock[b]` = (global band, group, path).
-// The block accumulates the band's listed edges for its G rows in LDS and
-// scans each row into the raster; the band's other groups are other blocks.
-// Dynamic LDS: the composite table (LUT_BYTES), then the rows' i32 cells.
-template <bool PF>
+// One block per (band, row group): `rblock[b]` = (global band, group, path);
+// one kernel per pixel format FMT, over the format's stretch of the table
+// from `first`.  The block accumulates the band's listed edges for its G rows
+// in LDS and scans each row into the raster; the band's other groups are
+// other blocks.  Dynamic LDS: LUT_BYTES for the cell pass's edge slots and
+// then the composite table (Rgba8p, Graya8p), followed by the rows' i32 cells.
+template <bool PF, int FMT>
 __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *paths, const uint4 *rblock,
-                                                  uint32_t band_shift, const uint32_t *band_off,
-                                                  const uint32_t *band_list, const EdgeRec *edges,
-                                                  const PathState *ps, DevStatus *st) {
+                                                  uint32_t first_block, uint32_t band_shift,
+                                                  const uint32_t *band_off, const uint32_t *band_list,
+                                                  const EdgeRec *edges, const PathState *ps, DevStatus *st) {
     extern __shared__ uint4 s_dyn[];
     __shared__ uint32_t s_segsum[BLOCK / 64];
+    constexpr bool HAS_LUT = FMT != FMT_MATTE8;
+    // The front LUT_BYTES hold the cell pass's edge slots, then the composite
+    // table; the rows' cells follow.
     uint4 *const lut = s_dyn;
+    EdgeRec *const s_edge = reinterpret_cast<EdgeRec *>(s_dyn);
+    uint32_t *const s_pair0 = reinterpret_cast<uint32_t *>(s_edge + CELL_ROUND);
     int32_t *const s_cells = reinterpret_cast<int32_t *>(s_dyn + LUT_ENTRIES);
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
-    const uint4 rb = rblock[blockIdx.x];
+    const uint4 rb = rblock[first_block + blockIdx.x];
     const uint32_t gb = rb.x, p = rb.z;
     if ((gb >= L.n_bands) | (p >= L.n_paths)) {
         if (threadIdx.x == 0) raise_err(st, FT_EDEVICE);
@@ -976,7 +989,11 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     const bool has_pts = S.topkey != NO_KEY && S.topidx != NO_IDX;
     const int32_t top_row = has_pts ? top_row_of(S.topkey) : 0;
     const int64_t first = !has_pts ? (int64_t)H : (top_row > 0 ? top_row : 0);
-    const uint32_t fmt = P.fmt;
+    constexpr uint32_t fmt = FMT;
+    if (P.fmt != fmt) {  // the block table groups paths by format
+        if (threadIdx.x == 0) raise_err(st, FT_EDEVICE);
+        return;
+    }
     const uint32_t bpp = fmt == FMT_RGBA8P ? 4 : (fmt == FMT_GRAYA8P ? 2 : 1);
     gbyte *const base = (gbyte *)(P.dst);
     const bool fresh = (P.flags & PATH_FRESH) != 0;
@@ -1013,10 +1030,10 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     pp.q0 = pp.q1 = pp.q2 = pp.q3 = make_uint4(0, 0, 0, 0);
     const bool rmw = !fresh & (fmt != FMT_MATTE8);
     const bool pre = PF & rmw & (xb < xe) & (r0 < rows);
-    if (pre) {
+    if constexpr (PF && FMT != FMT_MATTE8) if (pre) {
         const gbyte *row = base + (size_t)(ry0 + r0) * P.pitch;
         const uint32_t nck = (xe - xb + chunk - 1) / chunk;
-        if (fmt == FMT_RGBA8P) {
+        if constexpr (FMT == FMT_RGBA8P) {
             pp.q0 = dst_load<FMT_RGBA8P, 4>(row, xb, xe, 0, nck);
             pp.q1 = dst_load<FMT_RGBA8P, 4>(row, xb, xe, 1, nck);
             pp.q2 = dst_load<FMT_RGBA8P, 4>(row, xb, xe, 2, nck);
@@ -1028,8 +1045,6 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
             pp.q3 = dst_load<FMT_GRAYA8P, 8>(row, xb, xe, 3, nck);
         }
     }
-    if (fmt != FMT_MATTE8)
-        for (uint32_t a = threadIdx.x; a < LUT_ENTRIES; a += BLOCK) lut[a] = lut_entry(fmt, P.clr, a);
     for (uint32_t c = threadIdx.x; c < rows * Wp / 4; c += BLOCK) reinterpret_cast<int4 *>(s_cells)[c] = make_int4(0, 0, 0, 0);
     __syncthreads();
     const int fwd = (int)S.fwd;
@@ -1037,10 +1052,17 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     const bool list_ok = (off <= end) & (end <= L.n_pairs);
     const uint32_t cnt = list_ok ? end - off : 0u;
     if (!list_ok && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
-    // the band's edges, four per thread per round: list entries, then edge
-    // records, then their cells
+    // Cell pass, taller groups: the block takes CELL_ROUND list entries at a time; their
+    // (edge, row) pairs are laid out by a scan of the rows per edge and spread
+    // evenly over the threads (each finds its pair's edge by binary search in
+    // LDS), so an edge crossing many rows does not hold up a wavefront.  A
+    // pair's cells are counted in closed form (row_setup).
+    // Groups of at most DIRECT_ROWS rows (wide rasters) take four entries per
+    // thread and walk each edge's rows in the lane: an edge there crosses at
+    // most that many of the group's rows.
     constexpr int EB = 4;
-    for (uint32_t k0 = 0; list_ok && k0 < cnt; k0 += (uint32_t)BLOCK * EB) {
+    const bool direct = rows <= DIRECT_ROWS;
+    for (uint32_t k0 = 0; direct && list_ok && k0 < cnt; k0 += (uint32_t)BLOCK * EB) {
         uint32_t ei[EB];
         bool ok[EB];
 #pragma unroll
@@ -1056,21 +1078,63 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
             ok[j] = ok[j] & (ei[j] < L.cap_points);
             E[j] = edges[ok[j] ? ei[j] : 0u];
         }
-        auto cells_of = [&](const EdgeRec &e, bool on) {
-            if (!on) return;
-            const int32_t ed = ((int)((e.flags >> 1) & 1u) == fwd) ? 1 : -1;
-            int64_t a = (int64_t)fx_int(e.y_upper) - shift, b = (int64_t)fx_int(e.y_lower) - shift;
-            if (a < ry0) a = ry0;
-            if (b > (int64_t)y1 - 1) b = (int64_t)y1 - 1;
+#pragma unroll
+        for (int j = 0; j < EB; ++j) {
+            if (!ok[j]) continue;
+            const int32_t ed = ((int)((E[j].flags >> 1) & 1u) == fwd) ? 1 : -1;
+            int64_t a = (int64_t)fx_int(E[j].y_upper) - shift, b = (int64_t)fx_int(E[j].y_lower) - shift;
+            a = a < (int64_t)ry0 ? (int64_t)ry0 : a;
+            b = b > (int64_t)y1 - 1 ? (int64_t)y1 - 1 : b;
             for (int64_t r = a; r <= b; ++r)
-                edge_row_cells<DevAtom>(e, (int32_t)(r + shift), ed, (int32_t)W, s_cells + (size_t)(r - ry0) * Wp);
-        };
-        static_assert(EB == 4, "four explicit calls");
-        cells_of(E[0], ok[0]);
-        cells_of(E[1], ok[1]);
-        cells_of(E[2], ok[2]);
-        cells_of(E[3], ok[3]);
+                edge_row_cells<DevAtom>(E[j], (int32_t)(r + shift), ed, (int32_t)W,
+                                        s_cells + (size_t)(r - ry0) * Wp);
+        }
     }
+    for (uint32_t k0 = 0; !direct && list_ok && k0 < cnt; k0 += CELL_ROUND) {
+        const uint32_t k = k0 + threadIdx.x;
+        bool okk = (threadIdx.x < CELL_ROUND) & (k < cnt);
+        const uint32_t ei = band_list[off + (okk ? k : 0u)];
+        if (okk & (ei >= L.cap_points)) raise_err(st, FT_EDEVICE);
+        okk = okk & (ei < L.cap_points);
+        const EdgeRec E = edges[okk ? ei : 0u];
+        int64_t ra = (int64_t)fx_int(E.y_upper) - shift, rb = (int64_t)fx_int(E.y_lower) - shift;
+        ra = ra < (int64_t)ry0 ? (int64_t)ry0 : ra;
+        rb = rb > (int64_t)y1 - 1 ? (int64_t)y1 - 1 : rb;
+        const uint32_t nr = (okk & (ra <= rb)) ? (uint32_t)(rb - ra + 1) : 0u;
+        const uint32_t incl = wave_incl_u32(nr);
+        if (lane == 63) s_segsum[wid] = incl;
+        if (threadIdx.x < CELL_ROUND) s_edge[threadIdx.x] = E;
+        __syncthreads();
+        uint32_t before = 0, n_pairs = 0;
+#pragma unroll
+        for (int w = 0; w < (int)NW; ++w) {
+            before += w < wid ? s_segsum[w] : 0u;
+            n_pairs += s_segsum[w];
+        }
+        if (threadIdx.x < CELL_ROUND) s_pair0[threadIdx.x] = before + incl - nr;  // the edge's first pair
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < n_pairs; p += BLOCK) {
+            // the last slot whose first pair is at or before p (slots without
+            // rows share their successor's first pair)
+            uint32_t lo = 0;
+#pragma unroll
+            for (uint32_t step = CELL_ROUND / 2; step; step >>= 1) lo += s_pair0[lo + step] <= p ? step : 0u;
+            const EdgeRec Eo = s_edge[lo];
+            int32_t r_first = fx_int(Eo.y_upper) - shift;
+            r_first = r_first < (int32_t)ry0 ? (int32_t)ry0 : r_first;
+            const int32_t row = r_first + (int32_t)(p - s_pair0[lo]);  // raster row in [ry0, y1)
+            const int32_t ed = ((int)((Eo.flags >> 1) & 1u) == fwd) ? 1 : -1;
+            const RowCells c = row_setup(Eo, row + shift, (int32_t)W);
+            int32_t *const cr = s_cells + (size_t)(row - (int32_t)ry0) * Wp + c.x_s;
+            for (int32_t j = 0; j < c.n; ++j) {
+                const int32_t d = row_delta(c, j);
+                if (d) atomicAdd(&cr[j], d * ed);
+            }
+        }
+        __syncthreads();  // before the next entries overwrite the slots
+    }
+    if constexpr (HAS_LUT)
+        for (uint32_t a = threadIdx.x; a < LUT_ENTRIES; a += BLOCK) lut[a] = lut_entry(fmt, P.clr, a);
     __syncthreads();
     uint32_t carry = 0;
     if (seg > 1) {
@@ -1093,25 +1157,21 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
         if (xb >= xe) break;
         const int32_t *cells = s_cells + (size_t)r * Wp;
         gbyte *row = base + (size_t)(ry0 + r) * P.pitch;
-        switch (fmt) {
-        case FMT_MATTE8:
+        if constexpr (FMT == FMT_MATTE8) {
             if (cpt == 16)
                 row_pass_rule<FMT_MATTE8, 16, false>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, false);
             else
                 row_pass_rule<FMT_MATTE8, 4, false>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, false);
-            break;
-        case FMT_RGBA8P:
+        } else if constexpr (FMT == FMT_RGBA8P) {
             if (fresh)
                 row_pass_rule<FMT_RGBA8P, 4, true>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, false);
             else
                 row_pass_rule<FMT_RGBA8P, 4, false>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, loaded);
-            break;
-        default:
+        } else {
             if (fresh)
                 row_pass_rule<FMT_GRAYA8P, 8, true>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, false);
             else
                 row_pass_rule<FMT_GRAYA8P, 8, false>(P.rule, cells, xb, xe, row, lut, P.clr, carry, pp, loaded);
-            break;
         }
         loaded = false;
     }
@@ -1251,7 +1311,7 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     std::vector<uint32_t> op_path(n_ops, NO_IDX);
     std::vector<PathDev> pd(n_paths);
     std::vector<uint32_t> band_path;
-    std::vector<uint4> rblock;
+    std::vector<uint4> rblock_f[3];  // per pixel format (ft_format), then concatenated
     uint32_t lds = 0;
     for (size_t k = 0; k < n_paths; ++k) {
         const ft_fill_desc &d = paths[k];
@@ -1279,9 +1339,15 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
             for (uint32_t b = 0; b < nb; ++b) {
                 const uint32_t rows = std::min(P.band_rows, d.height - b * P.band_rows);
                 for (uint32_t g = 0; g < groups && g * G < rows; ++g)
-                    rblock.push_back(make_uint4(P.band_base + b, g, (uint32_t)k, 0u));
+                    rblock_f[d.fmt].push_back(make_uint4(P.band_base + b, g, (uint32_t)k, 0u));
             }
         }
+    }
+    std::vector<uint4> rblock;
+    for (int f = 0; f < 3; ++f) {
+        ws.rb_first[f] = (uint32_t)rblock.size();
+        ws.rb_count[f] = (uint32_t)rblock_f[f].size();
+        rblock.insert(rblock.end(), rblock_f[f].begin(), rblock_f[f].end());
     }
     if (rblock.size() >= 0x7FFFFFFFu) return FT_ENOTSUP;
     if (raster_lds_bytes(lds) > ws.lds_max) return FT_ENOTSUP;  // the tuning asks for more LDS than a block has
@@ -1506,11 +1572,35 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
                                        ws.cap_pairs, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end,
                                        ws.d_pstate, st);
             if (ws.n_rblocks) {
-                auto kr = ws.tuning.dst_prefetch ? k_raster<true> : k_raster<false>;
-                hipExtLaunchKernelGGL(kr, dim3((uint32_t)ws.n_rblocks), dim3(BLOCK),
-                                      (uint32_t)raster_lds_bytes(ws.lds_cells), s, ev ? ev[1] : nullptr,
-                                      ev ? ev[2] : nullptr, 0, L0, ws.d_paths, ws.d_rblock, ws.band_shift,
-                                      ws.d_band_off, ws.d_band_list, ws.d_edges, ws.d_pstate, st);
+                // one raster kernel per pixel format present; the events
+                // bracket the first and the last of them
+                int last = -1, first = -1;
+                for (int f = 0; f < 3; ++f)
+                    if (ws.rb_count[f]) {
+                        if (first < 0) first = f;
+                        last = f;
+                    }
+                const bool pf = ws.tuning.dst_prefetch != 0;
+                for (int f = 0; f < 3; ++f) {
+                    if (!ws.rb_count[f]) continue;
+                    hipEvent_t e0 = (ev && f == first) ? ev[1] : nullptr, e1 = (ev && f == last) ? ev[2] : nullptr;
+                    const dim3 grid(ws.rb_count[f]);
+                    const uint32_t lds = (uint32_t)raster_lds_bytes(ws.lds_cells);
+                    if (f == FT_MATTE8)
+                        hipExtLaunchKernelGGL(k_raster<false, FMT_MATTE8>, grid, dim3(BLOCK), lds, s, e0, e1, 0, L0,
+                                              ws.d_paths, ws.d_rblock, ws.rb_first[f], ws.band_shift, ws.d_band_off,
+                                              ws.d_band_list, ws.d_edges, ws.d_pstate, st);
+                    else if (f == FT_RGBA8P)
+                        hipExtLaunchKernelGGL(pf ? k_raster<true, FMT_RGBA8P> : k_raster<false, FMT_RGBA8P>, grid,
+                                              dim3(BLOCK), lds, s, e0, e1, 0, L0, ws.d_paths, ws.d_rblock,
+                                              ws.rb_first[f], ws.band_shift, ws.d_band_off, ws.d_band_list,
+                                              ws.d_edges, ws.d_pstate, st);
+                    else
+                        hipExtLaunchKernelGGL(pf ? k_raster<true, FMT_GRAYA8P> : k_raster<false, FMT_GRAYA8P>, grid,
+                                              dim3(BLOCK), lds, s, e0, e1, 0, L0, ws.d_paths, ws.d_rblock,
+                                              ws.rb_first[f], ws.band_shift, ws.d_band_off, ws.d_band_list,
+                                              ws.d_edges, ws.d_pstate, st);
+                }
             } else if (ev) {
                 FT_HIP(hipEventRecord(ev[1], s));
                 FT_HIP(hipEventRecord(ev[2], s));

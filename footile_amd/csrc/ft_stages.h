@@ -771,13 +771,43 @@ FT_HD inline bool path_forward(const PathState &s, const Limits &L, const I2 *pt
 // Edge::{starting_cov, continuing_cov, calculate_x_limits_*, scan_area,
 // first_cov, step_cov} (fig.rs:238-321).  Cells left of x = 0 land in cell 0;
 // cells at x >= W are dropped (fig.rs:292-295).
-template <class A>
-FT_HD inline void edge_row_cells(const EdgeRec &E, int32_t sr, int32_t ed, int32_t W, int32_t *cells) {
+//
+// The reference walks the row pixel by pixel: at step k the covered
+// fraction is x_pix(k) = min(pixel_cov(min(xc0 + k * st, ONE)), cov), and
+// pixel x = x_s + j receives x_pix(k_s + j) - x_pix(k_s + j - 1) until
+// x_pix reaches cov.  pixel_cov(u) = (u + 128) >> 8 for 0 <= u <= ONE, so
+// x_pix(k) >= cov exactly when xc0 + k * st >= 256 * cov - 128 (ONE is past
+// that threshold), which gives the number of cells in closed form.
+// row_setup describes the row; row_delta gives any one cell's delta, so the
+// cells of many rows can be spread over the lanes of a wavefront.
+struct RowCells {
+    int64_t xc0, st;  // x_cov at step k: min(xc0 + k * st, ONE)
+    int32_t cov;      // the row's coverage, 1..256
+    int32_t x_s;      // first cell
+    int32_t k_s;      // step of the first cell
+    int32_t n;        // cells (0: none)
+};
+
+FT_HD inline int32_t row_xpix(const RowCells &c, int64_t k) {
+    int64_t xc = c.xc0 + k * c.st;
+    if (xc > FX_ONE) xc = FX_ONE;
+    const int32_t xp = pixel_cov((int32_t)xc);
+    return xp < c.cov ? xp : c.cov;
+}
+
+FT_HD inline RowCells row_setup(const EdgeRec &E, int32_t sr, int32_t W) {
+    RowCells c;
+    c.xc0 = 0;
+    c.st = FX_ONE;
+    c.x_s = 0;
+    c.k_s = 0;
+    c.n = 0;
     const int32_t r0 = fx_int(E.y_upper), r1 = fx_int(E.y_lower);
     const bool starting = sr == r0, ending = sr == r1;
     int32_t cov = ending ? pixel_cov(fx_fract(E.y_lower)) : 256;
     if (starting) cov -= pixel_cov(fx_fract(E.y_upper));
-    if (cov <= 0) return;
+    c.cov = cov;
+    if (cov <= 0) return c;
     const int32_t x_bot = (int32_t)((uint32_t)E.x_bot0 + (uint32_t)(sr - r0) * (uint32_t)E.inv);
     const int32_t x0 = starting ? wsub(x_bot, fx_mul(E.inv, wsub(FX_ONE, fx_fract(E.y_upper)))) : wsub(x_bot, E.inv);
     const int32_t x1 = ending ? wsub(x_bot, fx_mul(E.inv, wsub(fx_ceil(E.y_lower), E.y_lower))) : x_bot;
@@ -786,34 +816,35 @@ FT_HD inline void edge_row_cells(const EdgeRec &E, int32_t sr, int32_t ed, int32
     const int32_t full = cov << 8;  // Fixed::from(cov as f32 / 256.0), exact
     const int32_t r = (min_pix == max_pix) ? fx_mul(wsub(FX_ONE, fx_fract(fx_avg(max_x, min_x))), full)
                                            : fx_mul(wsub(FX_ONE, fx_fract(min_x)), FX_HALF);
-    const int64_t xc0 = E.step > 0 ? fx_mul(r, E.step) : r;
-    const int64_t st = E.step > 0 ? E.step : FX_ONE;
-    // x_cov after k steps: min(xc0 + k * st, ONE); x_pix_k = min(pixel_cov, cov)
-    auto xp_at = [&](int64_t k) -> int32_t {
-        int64_t xc = xc0 + k * st;
-        if (xc > FX_ONE) xc = FX_ONE;
-        int32_t xp = pixel_cov((int32_t)xc);
-        return xp < cov ? xp : cov;
-    };
-    int32_t prev = 0;
-    int64_t k = 0;
-    int32_t x = min_pix;
-    if (min_pix < 0) {
-        // every step at x <= 0 adds to cell 0: their sum is x_pix at x = 0
-        const int64_t k0 = -(int64_t)min_pix;
-        const int32_t xp = xp_at(k0 < 65537 ? k0 : 65537);
-        if (xp) A::add_cell(&cells[0], xp * ed);
-        if (xp >= cov) return;
-        prev = xp;
-        k = k0 + 1;
-        x = 1;
-    }
-    for (; x < W; ++x, ++k) {
-        const int32_t xp = xp_at(k);
-        const int32_t d = xp - prev;
-        if (d) A::add_cell(&cells[x], d * ed);
-        if (xp >= cov) break;
-        prev = xp;
+    c.xc0 = E.step > 0 ? fx_mul(r, E.step) : r;
+    c.st = E.step > 0 ? E.step : FX_ONE;
+    if (min_pix >= W) return c;
+    // every step at x <= 0 adds to cell 0: their sum is x_pix at x = 0
+    const int64_t k0 = min_pix < 0 ? -(int64_t)min_pix : 0;
+    c.k_s = (int32_t)(k0 < 65537 ? k0 : 65537);
+    c.x_s = min_pix < 0 ? 0 : min_pix;
+    // first j >= 0 with xc0 + (k_s + j) * st >= 256 * cov - 128
+    // (left <= 256 * 256 and st <= ONE, so the division fits 32 bits)
+    const int64_t left = ((int64_t)cov * 256 - 128) - (c.xc0 + (int64_t)c.k_s * c.st);
+    const int64_t j_stop = left <= 0 ? 0 : (int64_t)(((uint32_t)left + (uint32_t)c.st - 1u) / (uint32_t)c.st);
+    const int64_t room = (int64_t)W - c.x_s;
+    c.n = (int32_t)(j_stop + 1 < room ? j_stop + 1 : room);
+    return c;
+}
+
+// Delta of cell x_s + j (0 <= j < n).
+FT_HD inline int32_t row_delta(const RowCells &c, int32_t j) {
+    const int32_t cur = row_xpix(c, (int64_t)c.k_s + j);
+    const int32_t prev = j > 0 ? row_xpix(c, (int64_t)c.k_s + j - 1) : 0;
+    return cur - prev;
+}
+
+template <class A>
+FT_HD inline void edge_row_cells(const EdgeRec &E, int32_t sr, int32_t ed, int32_t W, int32_t *cells) {
+    const RowCells c = row_setup(E, sr, W);
+    for (int32_t j = 0; j < c.n; ++j) {
+        const int32_t d = row_delta(c, j);
+        if (d) A::add_cell(&cells[c.x_s + j], d * ed);
     }
 }
 

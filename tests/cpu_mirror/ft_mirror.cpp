@@ -141,7 +141,78 @@ int stroke_outline(const F4 *ev, uint32_t n_ev, const PathDev &P, uint32_t join,
 
 }  // namespace
 
+// The reference's pixel walk along one scan row (fig.rs:285-321), step by
+// step, as a check of the closed form in row_setup / row_delta.
+static void walk_row(const EdgeRec &E, int32_t sr, int32_t ed, int32_t W, int32_t *cells) {
+    const int32_t r0 = fx_int(E.y_upper), r1 = fx_int(E.y_lower);
+    const bool starting = sr == r0, ending = sr == r1;
+    int32_t cov = ending ? pixel_cov(fx_fract(E.y_lower)) : 256;
+    if (starting) cov -= pixel_cov(fx_fract(E.y_upper));
+    if (cov <= 0) return;
+    const int32_t x_bot = (int32_t)((uint32_t)E.x_bot0 + (uint32_t)(sr - r0) * (uint32_t)E.inv);
+    const int32_t x0 = starting ? wsub(x_bot, fx_mul(E.inv, wsub(FX_ONE, fx_fract(E.y_upper)))) : wsub(x_bot, E.inv);
+    const int32_t x1 = ending ? wsub(x_bot, fx_mul(E.inv, wsub(fx_ceil(E.y_lower), E.y_lower))) : x_bot;
+    const int32_t min_x = x0 < x1 ? x0 : x1, max_x = x0 < x1 ? x1 : x0;
+    const int32_t min_pix = fx_int(min_x), max_pix = fx_int(max_x);
+    const int32_t full = cov << 8;
+    const int32_t r = (min_pix == max_pix) ? fx_mul(wsub(FX_ONE, fx_fract(fx_avg(max_x, min_x))), full)
+                                           : fx_mul(wsub(FX_ONE, fx_fract(min_x)), FX_HALF);
+    int32_t x_cov = E.step > 0 ? fx_mul(r, E.step) : r;  // first_cov, then step_cov
+    int32_t prev = 0;
+    for (int64_t x = min_pix; x < W; ++x) {
+        int32_t xp = pixel_cov(x_cov > FX_ONE ? FX_ONE : x_cov);
+        xp = xp < cov ? xp : cov;
+        const int32_t d = xp - prev;
+        if (d) cells[x < 0 ? 0 : x] += d * ed;
+        if (xp >= cov) break;
+        prev = xp;
+        const int32_t stp = E.step > 0 ? E.step : FX_ONE;
+        x_cov = x_cov > FX_ONE ? x_cov : wadd(x_cov, stp);
+    }
+}
+
 extern "C" {
+
+// Random edges and rows: cells from edge_row_cells (closed form) against
+// walk_row (step by step).  Returns the number of mismatching rows.
+int ftm_row_cells_check(uint64_t seed, int iters) {
+    uint64_t z = seed;
+    auto next = [&]() {
+        z += 0x9E3779B97F4A7C15ull;
+        uint64_t t = z;
+        t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ull;
+        t = (t ^ (t >> 27)) * 0x94D049BB133111EBull;
+        return t ^ (t >> 31);
+    };
+    int bad = 0;
+    std::vector<int32_t> a(64), b(64);
+    for (int it = 0; it < iters; ++it) {
+        const int32_t W = 1 + (int32_t)(next() % 48);
+        // a segment between two Fixed points within [-24, 72) px, mostly short
+        const int32_t span = (next() & 3) ? 6 : 80;
+        const int32_t ax = (int32_t)(next() % (uint64_t)(span << 16)) - (24 << 16);
+        const int32_t ay = (int32_t)(next() % (uint64_t)(span << 16));
+        const int32_t bx = (next() & 7) ? ax + (int32_t)(next() % (uint64_t)(span << 17)) - (span << 16)
+                                        : (int32_t)(next() % (uint64_t)(96 << 16)) - (24 << 16);
+        const int32_t by = (next() & 7) ? ay + (int32_t)(next() % (uint64_t)(span << 16)) + 1 : ay + 1;
+        I2 p, q;
+        p.x = ax;
+        p.y = ay;
+        q.x = bx;
+        q.y = by;
+        const EdgeRec E = make_edge(p, q);
+        if (!(E.flags & 1u)) continue;
+        const int32_t ed = (next() & 1) ? 1 : -1;
+        for (int32_t sr = fx_int(E.y_upper); sr <= fx_int(E.y_lower); ++sr) {
+            std::fill(a.begin(), a.end(), 0);
+            std::fill(b.begin(), b.end(), 0);
+            edge_row_cells<HostAtom>(E, sr, ed, W, a.data());
+            walk_row(E, sr, ed, W, b.data());
+            if (a != b) ++bad;
+        }
+    }
+    return bad;
+}
 
 // Run a batch (host raster pointers in desc.dst).  mode: MODE_FILL, or
 // MODE_STROKE_FLATTEN to only produce (x, y, w, kind) events into ev_out

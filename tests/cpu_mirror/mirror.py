@@ -30,6 +30,8 @@ def lib():
         L.ftm_run.argtypes = [vp, sz, vp, sz, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, sz,
                               ctypes.POINTER(sz), ctypes.c_uint32, ctypes.c_float]
         L.ftm_run.restype = ctypes.c_int
+        L.ftm_row_cells_check.argtypes = [ctypes.c_uint64, ctypes.c_int]
+        L.ftm_row_cells_check.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -83,3 +85,8 @@ def fill(width, height, fmt, rule, ops, clr=(255, 255, 255, 255), init=None, tra
     d.tolerance, d.pen_width, d.fresh = float(tolerance), float(pen_width), 0
     rc, off, st = run_batch(a, [d], raw=raw, mode=mode, join=join)
     return rc, px, (int(off[0]), int(st[0]))
+
+
+def row_cells_check(seed, iters):
+    """Rows whose closed-form cells differ from the step-by-step walk."""
+    return lib().ftm_row_cells_check(seed, iters)

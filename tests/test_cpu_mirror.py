@@ -51,3 +51,12 @@ def test_mirror_random_walk_4096(oracle):
         assert rc == 0
         assert counts == o.last_counts()
         assert np.array_equal(px, o.pixels())
+
+
+def test_row_cells_closed_form_matches_pixel_walk():
+    """The per-row cell count and deltas in closed form (row_setup / row_delta,
+    shared with the gfx950 raster) against the pixel-by-pixel walk of
+    fig.rs:285-321 on random edges, rows and raster widths, including edges
+    left of x = 0 and past the right edge."""
+    from tests.cpu_mirror import mirror
+    assert mirror.row_cells_check(12345, 200000) == 0
