@@ -701,15 +701,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) u32x4 gu32x4;
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
-__device__ inline uint32_t wave_incl_u32(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
+__device__ inline uint32_t wave_incl_u32(uint32_t v) { return wave_incl_dpp<false>(v); }
 
 // Coverage of a running cell sum under a fixed fill rule (0: NonZero,
 // 1: EvenOdd), as map_coverage.
@@ -810,6 +802,9 @@ template <int FMT, int CPT, uint32_t RULE, bool FRESH>
 __device__ __forceinline__ void row_chunk(const int32_t *cells, uint32_t x, uint32_t xe, gbyte *row,
                                           const uint4 *lut, uint32_t clr, uint32_t &carry, uint4 q) {
     constexpr int BPP = bpp_of<FMT>();
+    // A lane inside the segment holds CPT cells that are all in it: segment
+    // ends inside the row are multiples of the chunk, and cells in [W, Wp)
+    // are never written, so they read as 0.
     const bool in = x < xe;
     const uint32_t xr = in ? x : 0u;  // a readable address for lanes past the segment
     uint32_t c[CPT];
@@ -825,11 +820,12 @@ __device__ __forceinline__ void row_chunk(const int32_t *cells, uint32_t x, uint
     uint32_t run = 0;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-        run += (x + j < xe) ? c[j] : 0u;
+        run += c[j];
         v[j] = run;
     }
+    run = in ? run : 0u;
     const uint32_t incl = wave_incl_u32(run);
-    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     const uint32_t base = carry + incl - run;
     carry += total;
     if (!in) return;

@@ -74,15 +74,36 @@ __device__ inline T scan_op_t(T a, T b) {
         return a + b;
 }
 
+// Inclusive scan over the 64 lanes of a wavefront with DPP: row_shr 1, 2, 4
+// and 8 within each row of 16 lanes, then row_bcast:15 (rows 1 and 3 add the
+// last lane of the row before) and row_bcast:31 (rows 2 and 3 add lane 31).
+// Lanes without a source read 0, the identity of + and of unsigned max.
+// All 64 lanes must be active.
+template <bool IS_MAX>
+__device__ inline uint32_t wave_incl_dpp(uint32_t v) {
+    auto op = [](uint32_t a, uint32_t b) { return IS_MAX ? (a > b ? a : b) : a + b; };
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+    return v;
+}
+
 template <typename T, bool IS_MAX>
 __device__ inline T wave_incl_t(T v) {
-    const int lane = threadIdx.x & 63;
+    if constexpr (sizeof(T) == 4) {
+        return (T)wave_incl_dpp<IS_MAX>((uint32_t)v);
+    } else {
+        const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const T t = __shfl_up(v, o, 64);
-        if (lane >= o) v = scan_op_t<T, IS_MAX>(v, t);
+        for (int o = 1; o < 64; o <<= 1) {
+            const T t = __shfl_up(v, o, 64);
+            if (lane >= o) v = scan_op_t<T, IS_MAX>(v, t);
+        }
+        return v;
     }
-    return v;
 }
 
 // Host side: blocks of a scan over n_cap items and the tiles each block owns.
