@@ -311,8 +311,8 @@ __global__ __launch_bounds__(BLOCK) void k_top(Limits L0, const I2 *pts, const u
 // ---------------------------------------------------------------------------
 // Stage 5: edges and row-band binning.
 // ---------------------------------------------------------------------------
-// Band binning.  The (edge, band) pairs of a chunk of BLOCK consecutive
-// stored points are counted in an LDS histogram over the window of
+// Band binning.  The (edge, band) pairs of a chunk of consecutive stored
+// points are counted in an LDS histogram over the window of
 // BIN_WINDOW bands starting at the chunk's lowest band, and each nonzero bin
 // goes to global memory with one atomic.  The global counters are replicated:
 // band g, replica r sits at slot (g << band_shift) + r, and chunk c uses
@@ -321,6 +321,11 @@ __global__ __launch_bounds__(BLOCK) void k_top(Limits L0, const I2 *pts, const u
 // (band, replica) its list segment; a band's list is its replicas' segments
 // back to back, starting at slot g << band_shift.
 constexpr uint32_t BIN_WINDOW = 1024;
+// Points per thread of a binning chunk (a template parameter of k_edges and
+// k_bin, the same for both): 4 for batches of at least BIN_WIDE_MIN points,
+// whose kernels run many chunks per block; 1 below, where one chunk per block
+// keeps each thread's load chain short.
+constexpr size_t BIN_WIDE_MIN = (size_t)1 << 21;
 constexpr size_t BAND_SLOTS_MIN = 16384;
 constexpr uint32_t BAND_SHIFT_MAX = 5;
 static_assert(BIN_WINDOW == 4 * BLOCK, "the window scan gives each thread four bins");
@@ -344,26 +349,29 @@ __device__ inline uint32_t block_min_u32(uint32_t v, uint32_t *s_w) {
 // from a difference per edge and an inclusive scan over the window.  Returns
 // gmin (NO_IDX: no banded edge in the chunk).  Every thread of the block calls
 // it; the caller synchronises before s_h is written again.
-__device__ inline uint32_t bin_window_counts(bool on, uint32_t g0, uint32_t g1, uint32_t *s_h, uint32_t *s_w) {
-    const uint32_t gmin = block_min_u32(on ? g0 : NO_IDX, s_w);
+template <int NI>
+__device__ inline uint32_t bin_window_counts(const bool *on, const uint32_t *g0, const uint32_t *g1, uint32_t *s_h,
+                                             uint32_t *s_w) {
+    uint32_t m = NO_IDX;
+#pragma unroll
+    for (int j = 0; j < NI; ++j) m = (on[j] & (g0[j] < m)) ? g0[j] : m;
+    const uint32_t gmin = block_min_u32(m, s_w);
     if (gmin == NO_IDX) return NO_IDX;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     reinterpret_cast<uint4 *>(s_h)[t] = make_uint4(0, 0, 0, 0);
     __syncthreads();
-    if (on & (g0 - gmin < BIN_WINDOW)) {
-        const uint32_t hi = g1 - gmin < BIN_WINDOW - 1 ? g1 - gmin : BIN_WINDOW - 1;
-        atomicAdd(&s_h[g0 - gmin], 1u);
-        if (hi + 1 < BIN_WINDOW) atomicAdd(&s_h[hi + 1], 0xFFFFFFFFu);
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+        if (on[j] & (g0[j] - gmin < BIN_WINDOW)) {
+            const uint32_t hi = g1[j] - gmin < BIN_WINDOW - 1 ? g1[j] - gmin : BIN_WINDOW - 1;
+            atomicAdd(&s_h[g0[j] - gmin], 1u);
+            if (hi + 1 < BIN_WINDOW) atomicAdd(&s_h[hi + 1], 0xFFFFFFFFu);
+        }
     }
     __syncthreads();
     const uint4 q = reinterpret_cast<const uint4 *>(s_h)[t];
     const uint32_t a0 = q.x, a1 = a0 + q.y, a2 = a1 + q.z, a3 = a2 + q.w;
-    uint32_t x = a3;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(x, o, 64);
-        if (lane >= o) x += u;
-    }
+    const uint32_t x = wave_incl_dpp<false>(a3);
     if (lane == 63) s_w[wid] = x;
     __syncthreads();
     uint32_t pre = x - a3;
@@ -376,6 +384,7 @@ __device__ inline uint32_t bin_window_counts(bool on, uint32_t g0, uint32_t g1, 
 
 // One thread per stored point [0, n_kept): Edge::new, the point's band
 // record for k_bin, and the pairs per (band, replica).
+template <int BIN_ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, uint32_t band_shift, const uint32_t *pt_run,
                                                  const uint32_t *run_path, const uint32_t *run_start,
                                                  const uint32_t *run_end, const PathDev *paths, const I2 *pts,
@@ -387,21 +396,29 @@ __global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, uint32_t band_shift,
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     const uint32_t n = st->n_kept < L.cap_points ? st->n_kept : L.cap_points;
     const uint32_t rep_mask = (1u << band_shift) - 1u;
+    constexpr uint32_t BIN_CHUNK = BIN_ITEMS * BLOCK;  // points per chunk (one LDS window)
     uint32_t bad = 0;
-    for (uint32_t b = blockIdx.x * BLOCK; b < n; b += gridDim.x * BLOCK) {
-        const uint32_t i = b + threadIdx.x;
-        U2 rec;
-        rec.x = 1;
-        rec.y = 0;
-        if (i < n) {
-            const EdgeOut r = edge_point<DevAtom>(i, L, pt_run, run_path, run_start, run_end, paths, pts, edges, ps,
-                                                  bad);
-            rec = band_record(r);
-            pt_band[i] = rec;
+    for (uint32_t b = blockIdx.x * BIN_CHUNK; b < n; b += gridDim.x * BIN_CHUNK) {
+        bool on[BIN_ITEMS];
+        uint32_t g0[BIN_ITEMS], g1[BIN_ITEMS];
+#pragma unroll
+        for (int j = 0; j < (int)BIN_ITEMS; ++j) {
+            const uint32_t i = b + (uint32_t)j * BLOCK + threadIdx.x;
+            U2 rec;
+            rec.x = 1;
+            rec.y = 0;
+            if (i < n) {
+                const EdgeOut r = edge_point<DevAtom>(i, L, pt_run, run_path, run_start, run_end, paths, pts, edges,
+                                                      ps, bad);
+                rec = band_record(r);
+                pt_band[i] = rec;
+            }
+            on[j] = (rec.y >= rec.x) & (rec.y < L.n_bands);
+            g0[j] = rec.x;
+            g1[j] = rec.y;
         }
-        const bool on = (rec.y >= rec.x) & (rec.y < L.n_bands);
-        const uint32_t rep = (b / BLOCK) & rep_mask;
-        const uint32_t gmin = bin_window_counts(on, rec.x, rec.y, s_h, s_w);
+        const uint32_t rep = (b / BIN_CHUNK) & rep_mask;
+        const uint32_t gmin = bin_window_counts<BIN_ITEMS>(on, g0, g1, s_h, s_w);
         if (gmin == NO_IDX) continue;
         const uint4 q = reinterpret_cast<const uint4 *>(s_h)[threadIdx.x];
         const uint32_t c[4] = {q.x, q.y, q.z, q.w};
@@ -411,9 +428,11 @@ __global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, uint32_t band_shift,
             if (c[j] && g < L.n_bands) atomicAdd(&band_cnt[(g << band_shift) + rep], c[j]);
         }
         // bands past the window, one atomic each (edges far below the chunk's top band)
-        if (on)
-            for (uint32_t g = gmin + BIN_WINDOW > rec.x ? gmin + BIN_WINDOW : rec.x; g <= rec.y; ++g)
-                atomicAdd(&band_cnt[(g << band_shift) + rep], 1u);
+#pragma unroll
+        for (int j = 0; j < (int)BIN_ITEMS; ++j)
+            if (on[j])
+                for (uint32_t g = gmin + BIN_WINDOW > g0[j] ? gmin + BIN_WINDOW : g0[j]; g <= g1[j]; ++g)
+                    atomicAdd(&band_cnt[(g << band_shift) + rep], 1u);
         __syncthreads();
     }
     if (bad) raise_err(st, FT_EDEVICE);
@@ -423,6 +442,7 @@ __global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, uint32_t band_shift,
 // (chunk, band) reserves the chunk's range in the replica's segment, and LDS
 // cursors hand out slots within it.  Order within a list is not fixed; the
 // raster's integer sums do not depend on it.
+template <int BIN_ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, uint32_t band_shift, const U2 *pt_band,
                                                const uint32_t *band_off, uint32_t *band_cur, uint32_t *band_list,
                                                uint64_t cap_pairs, const I2 *pts, const uint32_t *pt_run,
@@ -444,6 +464,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, uint32_t band_shift, c
     const uint32_t n = st->n_kept < L.cap_points ? st->n_kept : L.cap_points;
     const uint32_t rep_mask = (1u << band_shift) - 1u;
     const uint32_t n_slots = L.n_bands << band_shift;
+    constexpr uint32_t BIN_CHUNK = BIN_ITEMS * BLOCK;  // points per chunk (one LDS window)
     uint32_t bad = 0;
     // winding direction per path (its top point is final after k_edges)
     for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < L.n_paths; p += gridDim.x * BLOCK) {
@@ -453,12 +474,19 @@ __global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, uint32_t band_shift, c
     }
     // list segment [begin, end) of a slot
     auto seg_end = [&](uint32_t slot) { return slot + 1 < n_slots ? band_off[slot + 1] : n_pairs; };
-    for (uint32_t b = blockIdx.x * BLOCK; b < n; b += gridDim.x * BLOCK) {
-        const uint32_t i = b + threadIdx.x;
-        const BinItem it = bin_item(i < n ? i : 0u, pt_band);
-        const bool on = it.banded & (i < n) & (it.g1 < L.n_bands);
-        const uint32_t rep = (b / BLOCK) & rep_mask;
-        const uint32_t gmin = bin_window_counts(on, it.g0, it.g1, s_h, s_w);
+    for (uint32_t b = blockIdx.x * BIN_CHUNK; b < n; b += gridDim.x * BIN_CHUNK) {
+        bool on[BIN_ITEMS];
+        uint32_t g0[BIN_ITEMS], g1[BIN_ITEMS];
+#pragma unroll
+        for (int j = 0; j < (int)BIN_ITEMS; ++j) {
+            const uint32_t i = b + (uint32_t)j * BLOCK + threadIdx.x;
+            const BinItem it = bin_item(i < n ? i : 0u, pt_band);
+            on[j] = it.banded & (i < n) & (it.g1 < L.n_bands);
+            g0[j] = it.g0;
+            g1[j] = it.g1;
+        }
+        const uint32_t rep = (b / BIN_CHUNK) & rep_mask;
+        const uint32_t gmin = bin_window_counts<BIN_ITEMS>(on, g0, g1, s_h, s_w);
         if (gmin == NO_IDX) continue;
         {
             const uint4 q = reinterpret_cast<const uint4 *>(s_h)[threadIdx.x];
@@ -480,15 +508,18 @@ __global__ __launch_bounds__(BLOCK) void k_bin(Limits L0, uint32_t band_shift, c
             reinterpret_cast<uint4 *>(s_h)[threadIdx.x] = make_uint4(0, 0, 0, 0);  // cursors
         }
         __syncthreads();
-        if (on) {
-            const uint32_t hi = it.g1 - gmin < BIN_WINDOW - 1 ? it.g1 : gmin + BIN_WINDOW - 1;
-            for (uint32_t g = it.g0; g <= hi; ++g) {
+#pragma unroll
+        for (int jj = 0; jj < (int)BIN_ITEMS; ++jj) {
+            if (!on[jj]) continue;
+            const uint32_t i = b + (uint32_t)jj * BLOCK + threadIdx.x;
+            const uint32_t hi = g1[jj] - gmin < BIN_WINDOW - 1 ? g1[jj] : gmin + BIN_WINDOW - 1;
+            for (uint32_t g = g0[jj]; g <= hi; ++g) {
                 const uint32_t k = g - gmin;
                 const uint32_t slot = atomicAdd(&s_h[k], 1u);
                 const uint32_t at = s_base[k];
                 if (at != NO_IDX) band_list[at + slot] = i;
             }
-            for (uint32_t g = gmin + BIN_WINDOW > it.g0 ? gmin + BIN_WINDOW : it.g0; g <= it.g1; ++g) {
+            for (uint32_t g = gmin + BIN_WINDOW > g0[jj] ? gmin + BIN_WINDOW : g0[jj]; g <= g1[jj]; ++g) {
                 const uint32_t slot = (g << band_shift) + rep;
                 const uint32_t at = band_off[slot] + atomicAdd(&band_cur[slot], 1u);
                 if (at < seg_end(slot))
@@ -1558,13 +1589,14 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         const size_t top_tile = (size_t)TOP_ITEMS * BLOCK;
         const size_t top_grid = std::max<size_t>(1, std::min<size_t>(GRID_MAX, (ws.cap_events + top_tile - 1) / top_tile));
         k_top<<<(uint32_t)top_grid, BLOCK, 0, s>>>(L0, ws.d_pts, ws.d_pt_run, ws.d_run_path, ws.d_pstate, st);
-        k_edges<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_run, ws.d_run_path, ws.d_run_start, ws.d_run_end,
+        const bool wide = ws.cap_events >= BIN_WIDE_MIN;
+        (wide ? k_edges<4> : k_edges<1>)<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_run, ws.d_run_path, ws.d_run_start, ws.d_run_end,
                                      ws.d_paths, ws.d_pts, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_pstate, st);
         if (ws.n_bands) {
             // pairs per (band, replica) -> list segment offsets and the pair total
             const uint32_t n_slots = (uint32_t)(ws.n_bands << ws.band_shift);
             scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_band_cnt, ws.d_band_off, nullptr, n_slots, n_slots, fl + 2 * tiles, ag + 2 * tiles, in + 2 * tiles, &st->n_pairs, st, true, false);
-            k_bin<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
+            (wide ? k_bin<4> : k_bin<1>)<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
                                        ws.cap_pairs, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end,
                                        ws.d_pstate, st);
             if (ws.n_rblocks) {
