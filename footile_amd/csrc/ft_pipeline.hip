@@ -826,6 +826,12 @@ __device__ __forceinline__ uint4 dst_load(const gbyte *row, uint32_t xb, uint32_
     return make_uint4(v.x, v.y, v.z, v.w);
 }
 
+// Coverage and the stored or composited pixels of CPT cells per lane at x,
+// whose running sums are base + v[j] (q: the destination, when loaded).
+template <int FMT, int CPT, uint32_t RULE, bool FRESH>
+__device__ __forceinline__ void chunk_out(const uint32_t *v, uint32_t base, uint32_t x, uint32_t xe, gbyte *row,
+                                          const uint4 *lut, uint32_t clr, uint4 q);
+
 // One chunk of a row pass: CPT cells per lane at x from LDS, their prefix
 // sum from `carry` (the sum of the cells left of the chunk), coverage, and
 // the pixels stored or composited (q: the chunk's destination, when loaded).
@@ -860,6 +866,52 @@ __device__ __forceinline__ void row_chunk(const int32_t *cells, uint32_t x, uint
     const uint32_t base = carry + incl - run;
     carry += total;
     if (!in) return;
+    chunk_out<FMT, CPT, RULE, FRESH>(v, base, x, xe, row, lut, clr, q);
+}
+
+// Rows of at most LPR * CPT pixels, LPR lanes per row (16 or 32): a
+// wavefront rasters 64 / LPR rows at once in one chunk each, with the prefix
+// sums segmented at LPR lanes.  `cells` and `row` are the lane's own row;
+// lanes with `active` false only take part in the scan.
+template <int FMT, int CPT, uint32_t RULE, bool FRESH, int LPR>
+__device__ __forceinline__ void row_multi(const int32_t *cells, uint32_t W, gbyte *row, const uint4 *lut,
+                                          uint32_t clr, bool active) {
+    constexpr int BPP = bpp_of<FMT>();
+    const uint32_t x = (uint32_t)(threadIdx.x & (LPR - 1)) * CPT;
+    const bool in = active & (x < W);
+    const uint32_t xr = in ? x : 0u;
+    uint32_t v[CPT];
+    uint32_t run = 0;
+#pragma unroll
+    for (int j = 0; j < CPT; j += 4) {
+        const int4 c4 = *reinterpret_cast<const int4 *>(cells + xr + j);
+        run += (uint32_t)c4.x;
+        v[j] = run;
+        run += (uint32_t)c4.y;
+        v[j + 1] = run;
+        run += (uint32_t)c4.z;
+        v[j + 2] = run;
+        run += (uint32_t)c4.w;
+        v[j + 3] = run;
+    }
+    uint4 q = make_uint4(0, 0, 0, 0);
+    if constexpr (FMT != FMT_MATTE8 && !FRESH) {
+        const gbyte *p = row + (size_t)xr * BPP;
+        if (in & (x + CPT <= W) & ((((uintptr_t)p) & 15) == 0)) {
+            const u32x4 d = *reinterpret_cast<const gu32x4 *>(p);
+            q = make_uint4(d.x, d.y, d.z, d.w);
+        }
+    }
+    run = in ? run : 0u;
+    const uint32_t incl = seg_incl_dpp<false, LPR>(run);
+    if (!in) return;
+    chunk_out<FMT, CPT, RULE, FRESH>(v, incl - run, x, W, row, lut, clr, q);
+}
+
+template <int FMT, int CPT, uint32_t RULE, bool FRESH>
+__device__ __forceinline__ void chunk_out(const uint32_t *v, uint32_t base, uint32_t x, uint32_t xe, gbyte *row,
+                                          const uint4 *lut, uint32_t clr, uint4 q) {
+    constexpr int BPP = bpp_of<FMT>();
     gbyte *dst = row + (size_t)x * BPP;
     constexpr uint32_t AL = CPT * BPP >= 16 ? 15 : CPT * BPP - 1;
     const bool full = (x + CPT <= xe) & ((((uintptr_t)dst) & AL) == 0);
@@ -973,6 +1025,23 @@ __device__ __forceinline__ void row_pass_rule(uint32_t rule, const int32_t *cell
         row_pass<FMT, CPT, 0, FRESH>(cells, xb, xe, row, lut, clr, carry, pp, loaded);
 }
 
+template <int FMT, int LPR>
+__device__ __forceinline__ void row_multi_any(uint32_t rule, bool fresh, const int32_t *cells, uint32_t W,
+                                              gbyte *row, const uint4 *lut, uint32_t clr, bool act) {
+    constexpr int CPT = FMT == FMT_GRAYA8P ? 8 : 4;
+    if (fresh & (FMT != FMT_MATTE8)) {
+        if (rule)
+            row_multi<FMT, CPT, 1, true, LPR>(cells, W, row, lut, clr, act);
+        else
+            row_multi<FMT, CPT, 0, true, LPR>(cells, W, row, lut, clr, act);
+    } else {
+        if (rule)
+            row_multi<FMT, CPT, 1, false, LPR>(cells, W, row, lut, clr, act);
+        else
+            row_multi<FMT, CPT, 0, false, LPR>(cells, W, row, lut, clr, act);
+    }
+}
+
 // One block per (band, row group): `rblock[b]` = (global band, group, path);
 // one kernel per pixel format FMT, over the format's stretch of the table
 // from `first`.  The block accumulates the band's listed edges for its G rows
@@ -1056,7 +1125,9 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     DstPipe pp;
     pp.q0 = pp.q1 = pp.q2 = pp.q3 = make_uint4(0, 0, 0, 0);
     const bool rmw = !fresh & (fmt != FMT_MATTE8);
-    const bool pre = PF & rmw & (xb < xe) & (r0 < rows);
+    // Narrow rows: LPR lanes per row, 64 / LPR rows per wavefront at once.
+    const uint32_t lpr = seg > 1 ? 64u : (W <= 16u * cpt ? 16u : (W <= 32u * cpt ? 32u : 64u));
+    const bool pre = PF & rmw & (xb < xe) & (r0 < rows) & (lpr == 64u);
     if constexpr (PF && FMT != FMT_MATTE8) if (pre) {
         const gbyte *row = base + (size_t)(ry0 + r0) * P.pitch;
         const uint32_t nck = (xe - xb + chunk - 1) / chunk;
@@ -1178,6 +1249,21 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
         if (lane == 0) s_segsum[wid] = sum;
         __syncthreads();
         for (uint32_t k = 0; k < sgi; ++k) carry += s_segsum[r0 * seg + k];
+    }
+    if (lpr < 64u) {
+        const uint32_t rpw = 64u / lpr, sub = (uint32_t)lane / lpr;
+        for (uint32_t rb = (uint32_t)wid * rpw; rb < rows; rb += NW * rpw) {
+            const uint32_t r = rb + sub;
+            const bool act = r < rows;
+            const uint32_t rr = act ? r : 0u;
+            const int32_t *cells = s_cells + (size_t)rr * Wp;
+            gbyte *row = base + (size_t)(ry0 + rr) * P.pitch;
+            if (lpr == 16u)
+                row_multi_any<FMT, 16>(P.rule, fresh, cells, W, row, lut, P.clr, act);
+            else
+                row_multi_any<FMT, 32>(P.rule, fresh, cells, W, row, lut, P.clr, act);
+        }
+        return;
     }
     bool loaded = pre;
     for (uint32_t r = r0; r < rows; r += rstep) {

@@ -79,16 +79,24 @@ __device__ inline T scan_op_t(T a, T b) {
 // last lane of the row before) and row_bcast:31 (rows 2 and 3 add lane 31).
 // Lanes without a source read 0, the identity of + and of unsigned max.
 // All 64 lanes must be active.
-template <bool IS_MAX>
-__device__ inline uint32_t wave_incl_dpp(uint32_t v) {
+// SEG = 16 or 32 stops after the row shifts or after row_bcast:15 and gives
+// independent scans of each 16- or 32-lane segment.
+template <bool IS_MAX, int SEG>
+__device__ inline uint32_t seg_incl_dpp(uint32_t v) {
+    static_assert(SEG == 16 || SEG == 32 || SEG == 64, "segments of 16, 32 or 64 lanes");
     auto op = [](uint32_t a, uint32_t b) { return IS_MAX ? (a > b ? a : b) : a + b; };
     v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
     v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
     v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
     v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
-    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
-    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+    if constexpr (SEG >= 32) v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+    if constexpr (SEG >= 64) v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
     return v;
+}
+
+template <bool IS_MAX>
+__device__ inline uint32_t wave_incl_dpp(uint32_t v) {
+    return seg_incl_dpp<IS_MAX, 64>(v);
 }
 
 template <typename T, bool IS_MAX>
