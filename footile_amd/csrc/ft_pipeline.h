@@ -32,6 +32,9 @@ struct Workspace {
     size_t cap_ops = 0, cap_paths = 0, cap_bands = 0;
     ft_path_op *d_ops = nullptr;
     uint32_t *d_op_path = nullptr;
+    uint32_t *d_op_order = nullptr;  // flattening order of the ops (a permutation)
+    std::vector<uint32_t> h_op_order;
+    bool op_ordered = false;
     PathDev *d_paths = nullptr;
     uint32_t *d_band_path = nullptr;
     uint32_t lds_cells = 0;  // dynamic LDS cells needed by the raster kernel

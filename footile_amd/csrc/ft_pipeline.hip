@@ -119,11 +119,17 @@ __global__ __launch_bounds__(BLOCK) void k_pen_width_mark(const ft_path_op *ops,
 // the op (a prefix over the op's lanes).
 __global__ __launch_bounds__(BLOCK) void k_count(const ft_path_op *ops, const uint32_t *op_path,
                                                  const PathDev *paths, uint32_t n_ops, uint32_t mode,
-                                                 const uint32_t *lastpw, unsigned long long *op_cnt,
-                                                 unsigned long long *sub_off, DevStatus *st) {
+                                                 const uint32_t *lastpw, const uint32_t *order,
+                                                 unsigned long long *op_cnt, unsigned long long *sub_off,
+                                                 DevStatus *st) {
     const uint64_t n_items = (uint64_t)n_ops * OP_LANES;
     for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_items; t += (uint64_t)gridDim.x * BLOCK) {
-        const uint32_t i = (uint32_t)(t >> FLAT_SPLIT), s = (uint32_t)t & (OP_LANES - 1);
+        const uint32_t oi = (uint32_t)(t >> FLAT_SPLIT), s = (uint32_t)t & (OP_LANES - 1);
+        const uint32_t i = order ? order[oi] : oi;  // item t is lane s of op i
+        if (i >= n_ops) {  // a corrupt order table
+            raise_err(st, FT_EDEVICE);
+            continue;
+        }
         const OpCount c = count_op(ops, op_path, paths, i, s, mode, lastpw);
         const unsigned long long v = pack_count(c);
         unsigned long long inc = v;
@@ -143,8 +149,9 @@ __global__ __launch_bounds__(BLOCK) void k_count(const ft_path_op *ops, const ui
 
 __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uint32_t *op_path,
                                                 const PathDev *paths, uint32_t n_ops, uint32_t mode,
-                                                const uint32_t *lastpw, const unsigned long long *op_off,
-                                                const unsigned long long *sub_off, EventSink sink, DevStatus *st) {
+                                                const uint32_t *lastpw, const uint32_t *order,
+                                                const unsigned long long *op_off, const unsigned long long *sub_off,
+                                                EventSink sink, DevStatus *st) {
     if (run_failed(st)) return;
     const unsigned long long tot = st->op_total;
     const uint32_t n_events = (uint32_t)(tot >> 32);
@@ -160,7 +167,12 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uin
     uint32_t bad = 0;
     const uint64_t n_items = (uint64_t)n_ops * OP_LANES;
     for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_items; t += (uint64_t)gridDim.x * BLOCK) {
-        const uint32_t i = (uint32_t)(t >> FLAT_SPLIT), s = (uint32_t)t & (OP_LANES - 1);
+        const uint32_t oi = (uint32_t)(t >> FLAT_SPLIT), s = (uint32_t)t & (OP_LANES - 1);
+        const uint32_t i = order ? order[oi] : oi;
+        if (i >= n_ops) {
+            bad = 1;
+            continue;
+        }
         bad |= emit_op(ops, op_path, paths, i, s, mode, lastpw, op_off[i] + sub_off[t], sink);
     }
     if (bad) raise_err(st, FT_EDEVICE);
@@ -320,6 +332,7 @@ __global__ __launch_bounds__(BLOCK) void k_top(Limits L0, const I2 *pts, const u
 // on the same cache lines.  An exclusive scan over the slots gives each
 // (band, replica) its list segment; a band's list is its replicas' segments
 // back to back, starting at slot g << band_shift.
+constexpr size_t OP_ORDER_MIN = 1024;  // batches from this many ops flatten in tag order
 constexpr uint32_t BIN_WINDOW = 1024;
 // Points per thread of a binning chunk (a template parameter of k_edges and
 // k_bin, the same for both): 4 for batches of at least BIN_WIDE_MIN points,
@@ -1343,7 +1356,7 @@ ft_status ws_init(Workspace &ws) {
 }
 
 void ws_free(Workspace &ws) {
-    void *bufs[] = {ws.d_ops, ws.d_op_path, ws.d_paths, ws.d_band_path, ws.d_op_cnt, ws.d_op_off, ws.d_sub_off,
+    void *bufs[] = {ws.d_ops, ws.d_op_path, ws.d_op_order, ws.d_paths, ws.d_band_path, ws.d_op_cnt, ws.d_op_off, ws.d_sub_off,
                     ws.d_op_pw,
                     ws.d_op_lastpw, ws.d_ev_xy, ws.d_ev_f, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_ev_last,
                     ws.d_st_flags, ws.d_st_off, ws.d_st_last, ws.d_spts, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
@@ -1468,7 +1481,8 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     const size_t old_ops = ws.cap_ops;
     if ((s = dev_grow(ws.d_ops, ws.cap_ops, n_ops))) return s;
     if (ws.cap_ops != old_ops || !ws.d_op_cnt) {  // per-op scratch follows the op capacity
-        if ((s = dev_alloc(ws.d_op_path, ws.cap_ops)) || (s = dev_alloc(ws.d_op_cnt, ws.cap_ops)) ||
+        if ((s = dev_alloc(ws.d_op_path, ws.cap_ops)) || (s = dev_alloc(ws.d_op_order, ws.cap_ops)) ||
+            (s = dev_alloc(ws.d_op_cnt, ws.cap_ops)) ||
             (s = dev_alloc(ws.d_op_off, ws.cap_ops)) || (s = dev_alloc(ws.d_op_pw, ws.cap_ops)) ||
             (s = dev_alloc(ws.d_sub_off, ws.cap_ops * OP_LANES)) ||
             (s = dev_alloc(ws.d_op_lastpw, ws.cap_ops)))
@@ -1517,6 +1531,21 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     if (n_ops) {
         FT_HIP(hipMemcpyAsync(ws.d_ops, ops, n_ops * sizeof(ft_path_op), hipMemcpyHostToDevice, ws.stream));
         FT_HIP(hipMemcpyAsync(ws.d_op_path, op_path.data(), n_ops * 4, hipMemcpyHostToDevice, ws.stream));
+    }
+    // Flattening visits the ops of a large batch cubics first, then quads,
+    // then the rest (stable within each), so that a wavefront's ops take the
+    // same branch.  Any order is correct: events land at their scanned offsets.
+    ws.op_ordered = n_ops >= OP_ORDER_MIN;
+    if (ws.op_ordered) {
+        ws.h_op_order.resize(n_ops);
+        size_t at = 0;
+        for (uint32_t pass = 0; pass < 3; ++pass)
+            for (size_t i = 0; i < n_ops; ++i) {
+                const uint32_t tag = ops[i].tag;
+                const uint32_t cls = tag == FT_OP_CUBIC ? 0u : (tag == FT_OP_QUAD ? 1u : 2u);
+                if (cls == pass) ws.h_op_order[at++] = (uint32_t)i;
+            }
+        FT_HIP(hipMemcpyAsync(ws.d_op_order, ws.h_op_order.data(), n_ops * 4, hipMemcpyHostToDevice, ws.stream));
     }
     if (n_paths)
         FT_HIP(hipMemcpyAsync(ws.d_paths, pd.data(), n_paths * sizeof(PathDev), hipMemcpyHostToDevice, ws.stream));
@@ -1603,8 +1632,9 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         lastpw = ws.d_op_lastpw;
     }
     if (n_ops) {
+        const uint32_t *order = ws.op_ordered ? ws.d_op_order : nullptr;
         k_count<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
-                                                                     lastpw, ws.d_op_cnt, ws.d_sub_off, st);
+                                                                     lastpw, order, ws.d_op_cnt, ws.d_sub_off, st);
         scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops, fl, ag, in, &st->op_total, st, false, false);
         EventSink em{};
         em.cap_events = ws.cap_events;
@@ -1619,7 +1649,7 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         em.run_start = ws.d_run_start;
         em.run_end = ws.d_run_end;
         k_emit<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
-                                                                    lastpw, ws.d_op_off, ws.d_sub_off, em, st);
+                                                                    lastpw, order, ws.d_op_off, ws.d_sub_off, em, st);
     }
     if (mode == MODE_STROKE) {
         const uint32_t ge = grid_for(ws.cap_events);
