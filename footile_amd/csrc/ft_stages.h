@@ -692,10 +692,18 @@ struct EdgeOut {
     bool banded;      // that edge touches a raster row
 };
 
-template <class A>
-FT_HD inline EdgeOut edge_point(uint32_t i, const Limits &L, const uint32_t *pt_run, const uint32_t *run_path,
+// edge_point in two parts: edge_eval only loads (so that a thread's several
+// points can have their loads in flight together), edge_commit stores.
+struct EdgeEval {
+    EdgeOut out;
+    EdgeRec E;
+    uint32_t top_local;  // local index of the point when it is at the path's top key
+    bool top;
+};
+
+FT_HD inline EdgeEval edge_eval(uint32_t i, const Limits &L, const uint32_t *pt_run, const uint32_t *run_path,
                                 const uint32_t *run_start, const uint32_t *run_end, const PathDev *paths,
-                                const I2 *pts, EdgeRec *edges, PathState *ps, uint32_t &bad) {
+                                const I2 *pts, const PathState *ps, uint32_t &bad) {
     // point i is stored iff pt_run[i] names a run (compact_event writes NO_IDX for capped points)
     const uint32_t run0 = pt_run[i];
     const bool rok = run0 < L.n_runs;
@@ -710,16 +718,31 @@ FT_HD inline EdgeOut edge_point(uint32_t i, const Limits &L, const uint32_t *pt_
     const I2 q = pts[i], q1 = pts[nok ? nx : i];
     const bool has = valid & nok & (nx != i) & (q1.y != q.y);
     bad |= (valid & !nok) ? 1u : 0u;
-    EdgeRec E = make_edge(q, q1);
-    if (!has) E.flags = 0;
-    if (valid & (point_key(q) == s.topkey)) A::min_u32(&ps[p].topidx, i - s.pt_base);
-    if (valid) edges[i] = E;
-    EdgeOut out;
-    out.banded = edge_bands(E, paths[p], s.topkey, L, out.g0, out.g1) & has;
-    out.path = valid ? p : NO_IDX;
-    out.valid = valid;
-    out.has = has;
-    return out;
+    EdgeEval r;
+    r.E = make_edge(q, q1);
+    if (!has) r.E.flags = 0;
+    r.top = valid & (point_key(q) == s.topkey);
+    r.top_local = i - s.pt_base;
+    r.out.banded = edge_bands(r.E, paths[p], s.topkey, L, r.out.g0, r.out.g1) & has;
+    r.out.path = valid ? p : NO_IDX;
+    r.out.valid = valid;
+    r.out.has = has;
+    return r;
+}
+
+template <class A>
+FT_HD inline void edge_commit(uint32_t i, const EdgeEval &r, EdgeRec *edges, PathState *ps) {
+    if (r.top) A::min_u32(&ps[r.out.path].topidx, r.top_local);
+    if (r.out.valid) edges[i] = r.E;
+}
+
+template <class A>
+FT_HD inline EdgeOut edge_point(uint32_t i, const Limits &L, const uint32_t *pt_run, const uint32_t *run_path,
+                                const uint32_t *run_start, const uint32_t *run_end, const PathDev *paths,
+                                const I2 *pts, EdgeRec *edges, PathState *ps, uint32_t &bad) {
+    const EdgeEval r = edge_eval(i, L, pt_run, run_path, run_start, run_end, paths, pts, ps, bad);
+    edge_commit<A>(i, r, edges, ps);
+    return r.out;
 }
 
 // Stage 5b: the band range k_edges recorded for point i, for scattering the
