@@ -237,13 +237,13 @@ __device__ inline void edge_at(const DScene &S, uint32_t v, uint32_t &k, float2 
     b = S.vtx[local + 1 < V ? v + 1 : vb];
 }
 
-__global__ __launch_bounds__(BLOCK) void k_clear_cells(DScene S, float2 *gvtx, float *gcol, uint32_t *flags,
-                                                       uint32_t n_flags, DevStatus *st) {
+__global__ __launch_bounds__(BLOCK) void k_clear_cells(DScene S, float2 *gvtx, float *gcol, unsigned long long *lb,
+                                                       uint32_t n_lb, DevStatus *st) {
     uint32_t n = S.n_cells > S.n_vtx ? S.n_cells : S.n_vtx;
     n = n > 4 * S.n_paths ? n : 4 * S.n_paths;
-    n = n > n_flags ? n : n_flags;
+    n = n > n_lb ? n : n_lb;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-        if (i < n_flags) flags[i] = 0;
+        if (i < n_lb) lb[i] = 0;
         if (i < S.n_cells) {
             S.cell_cnt[i] = 0;
             S.cell_cur[i] = 0;
@@ -510,8 +510,7 @@ struct ft_diff {
     uint32_t *d_cell_cnt = nullptr, *d_cell_off = nullptr, *d_cell_cur = nullptr, *d_cell_edge = nullptr;
     uint32_t cap_pairs = 0;
     size_t cap_tiles = 0;
-    uint32_t *d_flags = nullptr;
-    unsigned long long *d_agg = nullptr, *d_incl = nullptr;
+    unsigned long long *d_lb = nullptr;  // scan look-back words
     float4 *d_ck = nullptr;
     uint64_t ck_stride = 0;
     double *d_partial = nullptr, *h_partial = nullptr;
@@ -564,16 +563,16 @@ uint32_t grid_of(uint64_t n) {
 ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, float *gcol) {
     hipStream_t s = d->stream;
     DScene S = scene_of(d);
-    const uint32_t n_flags = (uint32_t)(2 * d->cap_tiles);  // look-back state of the two cell scans
+    const uint32_t n_lb = (uint32_t)(4 * d->cap_tiles);  // look-back words of the two cell scans
     const uint64_t n_clear = std::max<uint64_t>(std::max<uint64_t>(d->n_cells, d->n_vtx),
-                                                std::max<uint64_t>(4ull * d->n_paths, n_flags));
+                                                std::max<uint64_t>(4ull * d->n_paths, n_lb));
     k_clear_cells<<<grid_of(n_clear), BLOCK, 0, s>>>(S, clear_grads ? d->d_gvtx : nullptr,
-                                                     clear_grads ? gcol : nullptr, d->d_flags, n_flags, d->d_status);
+                                                     clear_grads ? gcol : nullptr, d->d_lb, n_lb, d->d_status);
     k_vertices<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, reinterpret_cast<const float2 *>(points), d->d_status);
     k_rows<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, d->d_status);
     const uint32_t nc = (uint32_t)d->n_cells;
-    scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_cnt, nullptr, nc, nc, d->d_flags, d->d_agg, d->d_incl, nullptr, d->d_status, false, true);
-    scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_off, nullptr, nc, nc, d->d_flags + d->cap_tiles, d->d_agg + d->cap_tiles, d->d_incl + d->cap_tiles, &d->d_status->n_pairs, d->d_status, false, false);
+    scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_cnt, nullptr, nc, nc, d->d_lb, nullptr, d->d_status, false, true);
+    scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_off, nullptr, nc, nc, d->d_lb + 2 * d->cap_tiles, &d->d_status->n_pairs, d->d_status, false, false);
     k_fill_cells<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, d->d_status);
     k_sort_cells<<<grid_of(d->n_cells), BLOCK, 0, s>>>(S, d->d_status);
     FD_HIP(hipGetLastError());
@@ -703,8 +702,7 @@ ft_status ft_diff_new(const ft_diff_layout *L, ft_diff **out) {
              (st = dalloc(d->d_img_first, L->n_images + 1)) || (st = dalloc(d->d_vtx, nv)) ||
              (st = dalloc(d->d_gvtx, nv)) || (st = dalloc(d->d_cell_cnt, cells)) ||
              (st = dalloc(d->d_cell_off, cells)) || (st = dalloc(d->d_cell_cur, cells)) ||
-             (st = dalloc(d->d_cell_edge, d->cap_pairs)) || (st = dalloc(d->d_flags, 2 * d->cap_tiles)) ||
-             (st = dalloc(d->d_agg, 2 * d->cap_tiles)) || (st = dalloc(d->d_incl, 2 * d->cap_tiles)) ||
+             (st = dalloc(d->d_cell_edge, d->cap_pairs)) || (st = dalloc(d->d_lb, 4 * d->cap_tiles)) ||
              (st = dalloc(d->d_ck, n_ck * d->ck_stride)) || (st = dalloc(d->d_partial, LOSS_GRID)) ||
              (st = dalloc(d->d_status, 1)))
         ;
@@ -732,8 +730,8 @@ ft_status ft_diff_new(const ft_diff_layout *L, ft_diff **out) {
 void ft_diff_free(ft_diff *d) {
     if (!d) return;
     void *bufs[] = {d->d_pt_off,   d->d_vtx_off,  d->d_vtx_path, d->d_pt_path,   d->d_img_first, d->d_vtx,
-                    d->d_gvtx,     d->d_cell_cnt, d->d_cell_off, d->d_cell_cur,  d->d_cell_edge, d->d_flags,
-                    d->d_agg,      d->d_incl,     d->d_ck,       d->d_partial,   d->d_status};
+                    d->d_gvtx,     d->d_cell_cnt, d->d_cell_off, d->d_cell_cur,  d->d_cell_edge, d->d_lb,
+                    d->d_ck,       d->d_partial,   d->d_status};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (d->h_partial) (void)hipHostFree(d->h_partial);

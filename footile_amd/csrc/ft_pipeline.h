@@ -78,8 +78,7 @@ struct Workspace {
     uint32_t *d_band_list = nullptr;
     // scan look-back state, SCAN_SLOTS instances
     size_t cap_tiles = 0;
-    uint32_t *d_scan_flags = nullptr;
-    unsigned long long *d_scan_agg = nullptr, *d_scan_incl = nullptr;
+    unsigned long long *d_scan_state = nullptr;  // scan look-back words, cleared by k_clear
     // results
     PathState *d_pstate = nullptr;
     DevStatus *d_status = nullptr;

@@ -80,10 +80,10 @@ __device__ inline void block_reduce_keyed(bool has, uint32_t key, T v, T identit
 // Stage 0: clear per-run state (a kernel, not hipMemsetAsync: graph-safe).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, uint32_t n_paths, uint32_t *band_cnt,
-                                                 uint32_t *band_cur, uint32_t n_slots, uint32_t *flags,
-                                                 uint32_t n_flags, DevStatus *st) {
+                                                 uint32_t *band_cur, uint32_t n_slots, unsigned long long *lb,
+                                                 uint32_t n_lb, DevStatus *st) {
     uint32_t n = n_paths > n_slots ? n_paths : n_slots;
-    n = n > n_flags ? n : n_flags;
+    n = n > n_lb ? n : n_lb;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
         if (i < n_paths) {
             PathState s;
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, uint32_t n_paths
             band_cnt[i] = 0;
             band_cur[i] = 0;
         }
-        if (i < n_flags) flags[i] = 0;
+        if (i < n_lb) lb[i] = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x < sizeof(DevStatus) / 4) reinterpret_cast<uint32_t *>(st)[threadIdx.x] = 0;
 }
@@ -195,27 +195,48 @@ __device__ inline Limits run_limits(const DevStatus *st, uint32_t n_ops, uint32_
 // ---------------------------------------------------------------------------
 // Stages 3-4: Fig ingest (runs, closing pop, cap, compaction, top point).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_ingest(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
-                                                  int32_t *run_first, const uint32_t *run_path, uint32_t *ev_last,
-                                                  DevStatus *st) {
-    if (run_failed(st)) return;
-    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
-    uint32_t bad = 0;
-    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
-        ingest_event(e, L, ev_meta, ev_xy, run_first, run_path, ev_last, bad);
-    if (bad) raise_err(st, FT_EDEVICE);
-}
+// Ingest as scan sources: the first pass of a scan computes its items from
+// the events (and stores the per-event side outputs), so no kernel writes the
+// items for the scan to read back.
+//
+// IngestSrc: ingest_event's pushed-point marks, whose running maximum is the
+// last pushed point so far; pass 1 also records each run's first point.
+struct IngestSrc {
+    static constexpr uint32_t ITEMS = 4;
+    Limits L;  // n_ops, n_paths, n_bands, cap_points set by the host; the rest by bind()
+    const uint32_t *meta;
+    const I2 *xy;
+    int32_t *run_first;
+    const uint32_t *run_path;
+    uint32_t bad;
+    __device__ void bind(const DevStatus *st) { L = run_limits(st, L.n_ops, L.n_paths, L.n_bands, L.cap_points); }
+    __device__ uint32_t value(uint64_t i, bool first) {
+        uint32_t fr;
+        const uint32_t v = ingest_mark((uint32_t)i, L, meta, xy, run_path, bad, fr);
+        if (first & (fr != NO_IDX)) run_first[fr] = (int32_t)i;
+        return v;
+    }
+};
 
-__global__ __launch_bounds__(BLOCK) void k_kept(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
-                                                const int32_t *run_first, const uint32_t *run_close,
-                                                const uint32_t *ev_last, uint32_t *ev_kept, DevStatus *st) {
-    if (run_failed(st)) return;
-    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
-    uint32_t bad = 0;
-    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
-        ev_kept[e] = kept_event(e, L, ev_meta, ev_xy, run_first, run_close, ev_last, bad);
-    if (bad) raise_err(st, FT_EDEVICE);
-}
+// KeptSrc: kept_event's flags (the point survives the closing pop), whose
+// exclusive sum is the kept point's index; pass 1 also stores the flags.
+struct KeptSrc {
+    static constexpr uint32_t ITEMS = 4;
+    Limits L;
+    const uint32_t *meta;
+    const I2 *xy;
+    const int32_t *run_first;
+    const uint32_t *run_close;
+    const uint32_t *last_scan;
+    uint32_t *kept;
+    uint32_t bad;
+    __device__ void bind(const DevStatus *st) { L = run_limits(st, L.n_ops, L.n_paths, L.n_bands, L.cap_points); }
+    __device__ uint32_t value(uint64_t i, bool first) {
+        const uint32_t v = kept_event((uint32_t)i, L, meta, xy, run_first, run_close, last_scan, bad);
+        if (first) kept[i] = v;
+        return v;
+    }
+};
 
 // One wavefront per path; the cap point is found by a 64-way search (each
 // round probes 64 events at once), a few round trips for any path length.
@@ -1363,7 +1384,7 @@ void ws_free(Workspace &ws) {
                     ws.d_sub_joined, ws.d_it_cnt, ws.d_it_off,
                     ws.d_run_first, ws.d_run_close, ws.d_run_path, ws.d_rblock, ws.d_run_start, ws.d_run_end, ws.d_pts,
                     ws.d_pt_run, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
-                    ws.d_scan_flags, ws.d_scan_agg, ws.d_scan_incl, ws.d_pstate, ws.d_status};
+                    ws.d_scan_state, ws.d_pstate, ws.d_status};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ws.h_status) (void)hipHostFree(ws.h_status);
@@ -1415,11 +1436,10 @@ static ft_status ensure_scan_capacity(Workspace &ws) {
     if (ws.n_ops > n) n = ws.n_ops;
     if ((ws.n_bands << ws.band_shift) > n) n = ws.n_bands << ws.band_shift;
     size_t tiles = (n + SCAN_TILE_MIN - 1) / SCAN_TILE_MIN + 1;
-    if (tiles <= ws.cap_tiles && ws.d_scan_flags) return FT_OK;
+    if (tiles < SCAN_GRID_MAX) tiles = SCAN_GRID_MAX;  // one state per scan block: any scan grid fits
+    if (tiles <= ws.cap_tiles && ws.d_scan_state) return FT_OK;
     ft_status s;
-    if ((s = dev_alloc(ws.d_scan_flags, tiles * SCAN_SLOTS)) || (s = dev_alloc(ws.d_scan_agg, tiles * SCAN_SLOTS)) ||
-        (s = dev_alloc(ws.d_scan_incl, tiles * SCAN_SLOTS)))
-        return s;
+    if ((s = dev_alloc(ws.d_scan_state, 2 * tiles * SCAN_SLOTS))) return s;
     ws.cap_tiles = tiles;
     return FT_OK;
 }
@@ -1614,28 +1634,27 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
     hipStream_t s = ws.stream;
     const uint32_t n_ops = (uint32_t)ws.n_ops;
     const size_t tiles = ws.cap_tiles;
-    uint32_t *fl = ws.d_scan_flags;
-    unsigned long long *ag = ws.d_scan_agg, *in = ws.d_scan_incl;
+    unsigned long long *lb = ws.d_scan_state;  // look-back words: 2 * tiles per scan slot
     DevStatus *st = ws.d_status;
-    const uint32_t n_flags = (uint32_t)(tiles * SCAN_SLOTS);
+    const uint32_t n_lb = (uint32_t)(2 * tiles * SCAN_SLOTS);
     // Timing: the first and the raster kernels carry the events in their
     // dispatch (ev[0] at k_clear's start, ev[1] / ev[2] around the raster), so
     // no marker packet stalls the stream between kernels.
     const size_t n_slots = ws.n_bands << ws.band_shift;
-    hipExtLaunchKernelGGL(k_clear, dim3(grid_for(std::max(std::max(ws.n_paths, n_slots), (size_t)n_flags))),
+    hipExtLaunchKernelGGL(k_clear, dim3(grid_for(std::max(std::max(ws.n_paths, n_slots), (size_t)n_lb))),
                           dim3(BLOCK), 0, s, ev ? ev[0] : nullptr, nullptr, 0, ws.d_pstate, (uint32_t)ws.n_paths,
-                          ws.d_band_cnt, ws.d_band_cur, (uint32_t)n_slots, fl, n_flags, st);
+                          ws.d_band_cnt, ws.d_band_cur, (uint32_t)n_slots, lb, n_lb, st);
     const uint32_t *lastpw = nullptr;
     if (mode != MODE_FILL && n_ops) {
         k_pen_width_mark<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, n_ops, ws.d_op_pw);
-        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_op_pw, ws.d_op_lastpw, nullptr, n_ops, n_ops, fl + 5 * tiles, ag + 5 * tiles, in + 5 * tiles, nullptr, st, false, false);
+        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_op_pw, ws.d_op_lastpw, nullptr, n_ops, n_ops, lb + 10 * tiles, nullptr, st, false, false);
         lastpw = ws.d_op_lastpw;
     }
     if (n_ops) {
         const uint32_t *order = ws.op_ordered ? ws.d_op_order : nullptr;
         k_count<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
                                                                      lastpw, order, ws.d_op_cnt, ws.d_sub_off, st);
-        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops, fl, ag, in, &st->op_total, st, false, false);
+        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops, lb, &st->op_total, st, false, false);
         EventSink em{};
         em.cap_events = ws.cap_events;
         em.cap_runs = (uint32_t)ws.cap_events;
@@ -1660,14 +1679,14 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         sp.miter = ws.miter;
         sp.tol_sq = ws.n_paths ? ws.stroke_tol_sq : 0.0f;
         k_st_flags<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_flags, ws.d_st_last, st);
-        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_st_flags, ws.d_st_off, &st->n_events, 0, cap, fl + 6 * tiles, ag + 6 * tiles, in + 6 * tiles, &st->st_total, st, true, false);
-        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_st_last, ws.d_st_last, &st->n_events, 0, cap, fl + 7 * tiles, ag + 7 * tiles, in + 7 * tiles, nullptr, st, true, true);
+        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_st_flags, ws.d_st_off, &st->n_events, 0, cap, lb + 12 * tiles, &st->st_total, st, true, false);
+        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_st_last, ws.d_st_last, &st->n_events, 0, cap, lb + 14 * tiles, nullptr, st, true, true);
         k_st_compact<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_off, ws.d_spts, ws.d_sub_start, ws.d_sub_event, cap, st);
         k_st_subs<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_last, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
                                        ws.d_sub_joined, cap, st);
         k_st_count<<<grid_for(cap_items), BLOCK, 0, s>>>(ws.d_spts, ws.d_sub_start, ws.d_sub_n, ws.d_sub_joined, sp,
                                                          ws.d_it_cnt, st);
-        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_it_cnt, ws.d_it_off, &st->st_items, 0, (uint32_t)cap_items, fl + 8 * tiles, ag + 8 * tiles, in + 8 * tiles, &st->stroke_total, st, true, false);
+        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_it_cnt, ws.d_it_off, &st->st_items, 0, (uint32_t)cap_items, lb + 16 * tiles, &st->stroke_total, st, true, false);
         EventSink em{};
         em.cap_events = ws.cap_events;
         em.cap_runs = cap;
@@ -1691,12 +1710,15 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         L0.n_bands = (uint32_t)ws.n_bands;
         L0.cap_points = (uint32_t)ws.cap_events;
         // pushed-point marks -> their running maximum (the last pushed point so far)
-        k_ingest<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, ws.d_ev_last, st);
-        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_ev_last, ws.d_ev_last, &st->n_events, 0, (uint32_t)ws.cap_events, fl + 4 * tiles, ag + 4 * tiles, in + 4 * tiles, nullptr, st, true, true);
+        const IngestSrc ing{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, 0u};
+        scan_launch_src<uint32_t, true>(s, ws.scan_resident, ing, ws.d_ev_last, &st->n_events, 0,
+                                        (uint32_t)ws.cap_events, lb + 8 * tiles,
+                                        nullptr, st, true, true);
         // kept flags -> point indices
-        k_kept<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_last,
-                                    ws.d_ev_kept, st);
-        scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_ev_kept, ws.d_ev_idx, &st->n_events, 0, (uint32_t)ws.cap_events, fl + tiles, ag + tiles, in + tiles, &st->n_kept, st, true, false);
+        const KeptSrc kp{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_last, ws.d_ev_kept, 0u};
+        scan_launch_src<uint32_t, false>(s, ws.scan_resident, kp, ws.d_ev_idx, &st->n_events, 0,
+                                         (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st,
+                                         true, false);
         k_cap<<<grid_for(ws.n_paths * 64), BLOCK, 0, s>>>(L0, ws.d_paths, ws.d_op_off, ws.d_ev_meta, ws.d_ev_xy,
                                                      ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first, ws.d_pstate, st);
         k_compact<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first,
@@ -1711,7 +1733,7 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         if (ws.n_bands) {
             // pairs per (band, replica) -> list segment offsets and the pair total
             const uint32_t n_slots = (uint32_t)(ws.n_bands << ws.band_shift);
-            scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_band_cnt, ws.d_band_off, nullptr, n_slots, n_slots, fl + 2 * tiles, ag + 2 * tiles, in + 2 * tiles, &st->n_pairs, st, true, false);
+            scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_band_cnt, ws.d_band_off, nullptr, n_slots, n_slots, lb + 4 * tiles, &st->n_pairs, st, true, false);
             (wide ? k_bin<4> : k_bin<1>)<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
                                        ws.cap_pairs, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end,
                                        ws.d_pstate, st);

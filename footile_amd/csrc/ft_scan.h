@@ -14,7 +14,7 @@ namespace ft {
 constexpr int BLOCK = 256;
 constexpr uint32_t GRID_MAX = 2048;     // grid-stride kernels: 8 blocks per CU
 // Items per thread of a scan tile: 16 for 32-bit items, 8 for 64-bit ones
-// (the same registers).  Look-back state is sized by the smallest tile.
+// (the same registers).
 template <typename T>
 constexpr uint32_t scan_items() {
     return sizeof(T) == 8 ? 8u : 16u;
@@ -62,9 +62,11 @@ __device__ inline bool run_failed(const DevStatus *st) {
 }
 
 // ---------------------------------------------------------------------------
-// Single-pass prefix scan with decoupled look-back.  Tiles are taken in
-// ticket order, so a tile only waits on tiles already owned by running blocks,
-// each of which publishes its aggregate before it waits on anything.
+// Single-pass prefix scan with decoupled look-back.  Block b publishes its
+// aggregate and then its inclusive prefix in two 64-bit words, each a value
+// below 2^63 with bit 63 as its ready mark, written and read with relaxed
+// agent-scope atomics: a reader that sees the mark has the value, so no
+// fence (and no L2 write-back or invalidate) is needed.
 // ---------------------------------------------------------------------------
 template <typename T, bool IS_MAX>
 __device__ inline T scan_op_t(T a, T b) {
@@ -121,9 +123,10 @@ struct ScanGrid {
     uint32_t blocks, tiles_per_block;
 };
 
-template <typename T>
+template <typename T, uint32_t ITEMS = scan_items<T>()>
 inline ScanGrid scan_grid_for(size_t n_cap, uint32_t resident) {
-    const uint64_t tiles = n_cap ? (n_cap + scan_tile<T>() - 1) / scan_tile<T>() : 1;
+    constexpr uint64_t TILE = (uint64_t)BLOCK * ITEMS;
+    const uint64_t tiles = n_cap ? (n_cap + TILE - 1) / TILE : 1;
     uint64_t g = resident ? resident : 64;  // unknown device: a grid any GPU holds
     if (g > SCAN_GRID_MAX) g = SCAN_GRID_MAX;
     if (g > tiles) g = tiles;
@@ -133,7 +136,20 @@ inline ScanGrid scan_grid_for(size_t n_cap, uint32_t resident) {
     return r;
 }
 
-// Prefix scan (sum or running maximum) of in[0, n) into out, n = *n_dev if
+// Items of a scan read from an array.  A scan's source gives item i's value;
+// `first` is true on the one visit of pass 1, where a source may also store
+// per-item side outputs.  bind() runs after the status check; `bad` set by a
+// source raises FT_EDEVICE.
+template <typename T>
+struct ArraySrc {
+    static constexpr uint32_t ITEMS = scan_items<T>();
+    const T *in;
+    uint32_t bad;
+    __device__ void bind(const DevStatus *) {}
+    __device__ T value(uint64_t i, bool) { return in[i]; }
+};
+
+// Prefix scan (sum or running maximum) of src's items [0, n) into out, n = *n_dev if
 // given, else n_host, at most n_cap.  Block b owns the tiles_per_block tiles
 // from b * tiles_per_block: it reduces them (coalesced loads), takes its
 // prefix by decoupled look-back over the blocks before it (lane l of the
@@ -141,12 +157,11 @@ inline ScanGrid scan_grid_for(size_t n_cap, uint32_t resident) {
 // scans the tiles again, transposing each through LDS so that both the loads
 // and the stores stay coalesced.  A single-tile block keeps its items in
 // registers between the passes.  `total` receives the whole reduction.
-template <typename T, bool IS_MAX>
-__global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint32_t *n_dev, uint32_t n_host,
-                                                uint32_t n_cap, uint32_t tiles_per_block, uint32_t *flags,
-                                                unsigned long long *agg, unsigned long long *incl, T *total,
-                                                DevStatus *st, bool check_failed, bool inclusive) {
-    constexpr uint32_t ITEMS = scan_items<T>(), TILE = scan_tile<T>(), ROW = ITEMS + 1;
+template <typename T, bool IS_MAX, class Src>
+__global__ __launch_bounds__(BLOCK) void k_scan(Src src, T *out, const uint32_t *n_dev, uint32_t n_host,
+                                                uint32_t n_cap, uint32_t tiles_per_block, unsigned long long *lb,
+                                                T *total, DevStatus *st, bool check_failed, bool inclusive) {
+    constexpr uint32_t ITEMS = Src::ITEMS, TILE = (uint32_t)BLOCK * ITEMS, ROW = ITEMS + 1;
     __shared__ T s_t[BLOCK * ROW];  // a tile, one padded row of ITEMS per thread
     __shared__ uint32_t s_fail, s_n;
     __shared__ T s_wave[BLOCK / 64];
@@ -161,6 +176,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint3
     }
     __syncthreads();
     if (s_fail) return;
+    src.bind(st);
     const uint64_t n = s_n < n_cap ? s_n : n_cap;
     const uint64_t n_tiles = (n + TILE - 1) / TILE;
     const uint64_t t0 = (uint64_t)blockIdx.x * tiles_per_block;
@@ -175,7 +191,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint3
 #pragma unroll
         for (uint32_t j = 0; j < ITEMS; ++j) {
             const uint64_t i = base + (uint64_t)j * BLOCK + tid;
-            xr[j] = i < n ? in[i] : (T)0;
+            xr[j] = i < n ? src.value(i, true) : (T)0;
         }
 #pragma unroll
         for (uint32_t j = 0; j < ITEMS; ++j) acc = scan_op_t<T, IS_MAX>(acc, xr[j]);
@@ -187,27 +203,33 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint3
     T btot = 0;
 #pragma unroll
     for (int w = 0; w < BLOCK / 64; ++w) btot = scan_op_t<T, IS_MAX>(btot, s_wave[w]);
-    // the chunk's prefix
+    // the chunk's prefix; lb[2 j] = block j's aggregate, lb[2 j + 1] its
+    // inclusive prefix, each READY | value (zero before the scan)
+    constexpr unsigned long long READY = 1ull << 63;
     const uint32_t b = blockIdx.x;
     if (wid == 0) {
         T prefix = 0;  // identity of + and of max over unsigned values
+        if (lane == 0 && ((unsigned long long)btot & READY)) raise_err(st, FT_ERANGE);  // 63-bit values only
         if (b == 0) {
-            if (lane == 0) {
-                __hip_atomic_store(&incl[0], (unsigned long long)btot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&flags[0], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
+            if (lane == 0)
+                __hip_atomic_store(&lb[1], READY | (unsigned long long)btot, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            if (lane == 0) {
-                __hip_atomic_store(&agg[b], (unsigned long long)btot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&flags[b], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
+            if (lane == 0)
+                __hip_atomic_store(&lb[2 * (size_t)b], READY | (unsigned long long)btot, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            // lane l reads block top - l: its inclusive word and its aggregate
             int64_t top = (int64_t)b - 1;
             uint32_t spins = 0;
             for (;;) {
                 const int64_t j = top - lane;
-                const uint32_t f =
-                    j >= 0 ? __hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 2u;
-                const unsigned long long im = __ballot(f == 2u), zm = __ballot(f == 0u);
+                unsigned long long wi = READY, wa = 0;  // past block 0: an inclusive zero
+                if (j >= 0) {
+                    wi = __hip_atomic_load(&lb[2 * (size_t)j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    wa = __hip_atomic_load(&lb[2 * (size_t)j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                const bool has_i = (wi & READY) != 0, has_a = (wa & READY) != 0;
+                const unsigned long long im = __ballot(has_i), zm = __ballot(!has_i & !has_a);
                 const int fi = im ? __ffsll((long long)im) - 1 : 64;  // nearest inclusive
                 const unsigned long long need = fi >= 63 ? ~0ull : ((2ull << fi) - 1ull);
                 if (zm & need) {
@@ -222,20 +244,16 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint3
                     continue;
                 }
                 T vv = 0;
-                if (lane <= fi && j >= 0)
-                    vv = (T)(f == 2u ? __hip_atomic_load(&incl[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                     : __hip_atomic_load(&agg[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                if (lane <= fi) vv = (T)((has_i ? wi : wa) & ~READY);
 #pragma unroll
                 for (int o = 32; o > 0; o >>= 1) vv = scan_op_t<T, IS_MAX>(vv, __shfl_xor(vv, o, 64));
                 prefix = scan_op_t<T, IS_MAX>(prefix, vv);
                 if (fi < 64) break;
                 top -= 64;
             }
-            if (lane == 0) {
-                __hip_atomic_store(&incl[b], (unsigned long long)scan_op_t<T, IS_MAX>(prefix, btot),
+            if (lane == 0)
+                __hip_atomic_store(&lb[2 * (size_t)b + 1], READY | (unsigned long long)scan_op_t<T, IS_MAX>(prefix, btot),
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&flags[b], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
         }
         if (lane == 0) {
             s_prefix = prefix;
@@ -251,7 +269,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint3
         for (uint32_t j = 0; j < ITEMS; ++j) {
             const uint32_t k = j * BLOCK + tid;
             const uint64_t i = base + k;
-            const T x = single ? xr[j] : (i < n ? in[i] : (T)0);
+            const T x = single ? xr[j] : (i < n ? src.value(i, false) : (T)0);
             s_t[(k / ITEMS) * ROW + k % ITEMS] = x;
         }
         __syncthreads();
@@ -287,16 +305,25 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T *in, T *out, const uint3
         running = scan_op_t<T, IS_MAX>(running, ttot);
         __syncthreads();
     }
+    if (src.bad) raise_err(st, FT_EDEVICE);
 }
 
 // Launches k_scan over a resident grid (see scan_grid_for).
+template <typename T, bool IS_MAX, class Src>
+inline void scan_launch_src(hipStream_t s, uint32_t resident, const Src &src, T *out, const uint32_t *n_dev,
+                            uint32_t n_host, uint32_t n_cap, unsigned long long *lb, T *total, DevStatus *st,
+                            bool check_failed, bool inclusive) {
+    const ScanGrid g = scan_grid_for<T, Src::ITEMS>(n_cap, resident);
+    k_scan<T, IS_MAX, Src><<<g.blocks, BLOCK, 0, s>>>(src, out, n_dev, n_host, n_cap, g.tiles_per_block, lb, total,
+                                                      st, check_failed, inclusive);
+}
+
 template <typename T, bool IS_MAX>
 inline void scan_launch(hipStream_t s, uint32_t resident, const T *in, T *out, const uint32_t *n_dev,
-                        uint32_t n_host, uint32_t n_cap, uint32_t *flags, unsigned long long *agg,
-                        unsigned long long *incl, T *total, DevStatus *st, bool check_failed, bool inclusive) {
-    const ScanGrid g = scan_grid_for<T>(n_cap, resident);
-    k_scan<T, IS_MAX><<<g.blocks, BLOCK, 0, s>>>(in, out, n_dev, n_host, n_cap, g.tiles_per_block, flags, agg, incl,
-                                                 total, st, check_failed, inclusive);
+                        uint32_t n_host, uint32_t n_cap, unsigned long long *lb, T *total, DevStatus *st,
+                        bool check_failed, bool inclusive) {
+    ArraySrc<T> src{in, 0u};
+    scan_launch_src<T, IS_MAX>(s, resident, src, out, n_dev, n_host, n_cap, lb, total, st, check_failed, inclusive);
 }
 
 }  // namespace ft
