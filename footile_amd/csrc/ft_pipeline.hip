@@ -238,6 +238,33 @@ struct KeptSrc {
     }
 };
 
+static_assert(IngestSrc::ITEMS == KeptSrc::ITEMS, "one tile size decides both ingest scans");
+
+// Plain passes for the pushed-point marks and the kept flags, for runs whose
+// scans have several tiles per block (where IngestSrc / KeptSrc would
+// evaluate every event twice).
+__global__ __launch_bounds__(BLOCK) void k_ingest(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
+                                                  int32_t *run_first, const uint32_t *run_path, uint32_t *ev_last,
+                                                  DevStatus *st) {
+    if (run_failed(st)) return;
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t bad = 0;
+    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
+        ingest_event(e, L, ev_meta, ev_xy, run_first, run_path, ev_last, bad);
+    if (bad) raise_err(st, FT_EDEVICE);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_kept(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
+                                                const int32_t *run_first, const uint32_t *run_close,
+                                                const uint32_t *ev_last, uint32_t *ev_kept, DevStatus *st) {
+    if (run_failed(st)) return;
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t bad = 0;
+    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
+        ev_kept[e] = kept_event(e, L, ev_meta, ev_xy, run_first, run_close, ev_last, bad);
+    if (bad) raise_err(st, FT_EDEVICE);
+}
+
 // One wavefront per path; the cap point is found by a 64-way search (each
 // round probes 64 events at once), a few round trips for any path length.
 __global__ __launch_bounds__(BLOCK) void k_cap(Limits L0, const PathDev *paths, const unsigned long long *op_off,
@@ -280,60 +307,28 @@ __global__ __launch_bounds__(BLOCK) void k_cap(Limits L0, const PathDev *paths, 
     if (bad) raise_err(st, FT_EDEVICE);
 }
 
+// Compaction of the stored points, and each path's minimum (y, x) key over
+// them (PathState::topkey): a block merges its points' keys per path and
+// flushes one atomic per path it holds.
 __global__ __launch_bounds__(BLOCK) void k_compact(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
                                                    const uint32_t *ev_kept, const uint32_t *ev_idx,
                                                    const int32_t *run_first, const uint32_t *run_path,
                                                    uint32_t *run_start, uint32_t *run_end, I2 *pts, uint32_t *pt_run,
-                                                   const PathState *ps, DevStatus *st) {
+                                                   PathState *ps, DevStatus *st) {
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     uint32_t bad = 0;
-    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
-        compact_event(e, L, ev_meta, ev_xy, ev_kept, ev_idx, run_first, run_path, run_start, run_end, pts, pt_run,
-                      ps, bad);
-    if (bad) raise_err(st, FT_EDEVICE);
-}
-
-// Per-path minimum (y, x) key over the stored points [0, n_kept): each thread
-// takes TOP_ITEMS points of a tile (loads first, then a running per-path
-// minimum), the block merges per path, and a path spanning many tiles gets
-// one atomic per tile.
-constexpr uint32_t TOP_ITEMS = 8;
-
-__global__ __launch_bounds__(BLOCK) void k_top(Limits L0, const I2 *pts, const uint32_t *pt_run,
-                                               const uint32_t *run_path, PathState *ps, DevStatus *st) {
-    if (run_failed(st)) return;
-    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
-    uint32_t n = st->n_kept;
-    if (n > L.cap_points) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
-        return;
-    }
-    uint32_t bad = 0;
-    constexpr uint32_t TILE = TOP_ITEMS * BLOCK;
-    for (uint32_t base = blockIdx.x * TILE; base < n; base += gridDim.x * TILE) {
-        // a thread's items are consecutive points, so its path rarely changes
-        // and the block reduction sees one key per path
-        TopCand c[TOP_ITEMS];
-#pragma unroll
-        for (uint32_t j = 0; j < TOP_ITEMS; ++j) {
-            const uint32_t i = base + threadIdx.x * TOP_ITEMS + j;
-            c[j] = top_cand(i < n ? i : 0, L, pts, pt_run, run_path, bad);
-            if (i >= n) c[j].path = NO_IDX;
-        }
-        uint32_t ck = NO_IDX;
-        unsigned long long cv = NO_KEY;
-#pragma unroll
-        for (uint32_t j = 0; j < TOP_ITEMS; ++j) {
-            const bool v = c[j].path != NO_IDX;
-            const bool change = v & (c[j].path != ck) & (ck != NO_IDX);
-            if (change) atomicMin(&ps[ck].topkey, cv);
-            const bool fresh = v & (c[j].path != ck);
-            cv = fresh ? c[j].key : (v ? (c[j].key < cv ? c[j].key : cv) : cv);
-            ck = v ? c[j].path : ck;
-        }
+    for (uint32_t base = blockIdx.x * BLOCK; base < L.n_events; base += gridDim.x * BLOCK) {
+        const uint32_t e = base + threadIdx.x;
+        TopCand c;
+        c.path = NO_IDX;
+        c.key = NO_KEY;
+        if (e < L.n_events)
+            compact_event(e, L, ev_meta, ev_xy, ev_kept, ev_idx, run_first, run_path, run_start, run_end, pts,
+                          pt_run, ps, bad, &c);
         block_reduce_keyed<unsigned long long>(
-            ck != NO_IDX, ck, cv, NO_KEY, [](unsigned long long a, unsigned long long b) { return a < b ? a : b; },
+            c.path != NO_IDX, c.path, c.key, NO_KEY,
+            [](unsigned long long a, unsigned long long b) { return a < b ? a : b; },
             [ps, &L](uint32_t k, unsigned long long v) {
                 if (k < L.n_paths) atomicMin(&ps[k].topkey, v);
             });
@@ -1709,24 +1704,36 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         L0.n_paths = (uint32_t)ws.n_paths;
         L0.n_bands = (uint32_t)ws.n_bands;
         L0.cap_points = (uint32_t)ws.cap_events;
+        // The event items of the two ingest scans are computed inside the scans
+        // while every scan block holds one tile, else by a pass of their own.
+        const bool in_scan = scan_grid_for<uint32_t, KeptSrc::ITEMS>(ws.cap_events, ws.scan_resident).tiles_per_block <= 1;
         // pushed-point marks -> their running maximum (the last pushed point so far)
-        const IngestSrc ing{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, 0u};
-        scan_launch_src<uint32_t, true>(s, ws.scan_resident, ing, ws.d_ev_last, &st->n_events, 0,
-                                        (uint32_t)ws.cap_events, lb + 8 * tiles,
-                                        nullptr, st, true, true);
+        if (in_scan) {
+            const IngestSrc ing{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, 0u};
+            scan_launch_src<uint32_t, true>(s, ws.scan_resident, ing, ws.d_ev_last, &st->n_events, 0,
+                                            (uint32_t)ws.cap_events, lb + 8 * tiles, nullptr, st, true, true);
+        } else {
+            k_ingest<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, ws.d_ev_last,
+                                          st);
+            scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_ev_last, ws.d_ev_last, &st->n_events, 0,
+                                        (uint32_t)ws.cap_events, lb + 8 * tiles, nullptr, st, true, true);
+        }
         // kept flags -> point indices
         const KeptSrc kp{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_last, ws.d_ev_kept, 0u};
-        scan_launch_src<uint32_t, false>(s, ws.scan_resident, kp, ws.d_ev_idx, &st->n_events, 0,
-                                         (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st,
-                                         true, false);
+        if (in_scan) {
+            scan_launch_src<uint32_t, false>(s, ws.scan_resident, kp, ws.d_ev_idx, &st->n_events, 0,
+                                             (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true, false);
+        } else {
+            k_kept<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_last,
+                                        ws.d_ev_kept, st);
+            scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_ev_kept, ws.d_ev_idx, &st->n_events, 0,
+                                         (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true, false);
+        }
         k_cap<<<grid_for(ws.n_paths * 64), BLOCK, 0, s>>>(L0, ws.d_paths, ws.d_op_off, ws.d_ev_meta, ws.d_ev_xy,
                                                      ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first, ws.d_pstate, st);
         k_compact<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first,
                                        ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_pts, ws.d_pt_run, ws.d_pstate,
                                        st);
-        const size_t top_tile = (size_t)TOP_ITEMS * BLOCK;
-        const size_t top_grid = std::max<size_t>(1, std::min<size_t>(GRID_MAX, (ws.cap_events + top_tile - 1) / top_tile));
-        k_top<<<(uint32_t)top_grid, BLOCK, 0, s>>>(L0, ws.d_pts, ws.d_pt_run, ws.d_run_path, ws.d_pstate, st);
         const bool wide = ws.cap_events >= BIN_WIDE_MIN;
         (wide ? k_edges<4> : k_edges<1>)<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_run, ws.d_run_path, ws.d_run_start, ws.d_run_end,
                                      ws.d_paths, ws.d_pts, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_pstate, st);

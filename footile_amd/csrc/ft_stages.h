@@ -569,10 +569,17 @@ FT_HD inline void cap_path(uint32_t p, const Limits &L, const PathDev *paths, co
 // Stage 4b: compaction into the per-path point arrays (the Fig's `points`).
 // A kept point beyond the cap gets pt_run = NO_IDX; a close event sets its
 // run's end (one past its last stored point).
+struct TopCand {
+    uint32_t path;  // NO_IDX: not a stored point
+    unsigned long long key;
+};
+
+// `top`, if given, receives the stored point's path and (y, x) key (path
+// NO_IDX when the event stores no point), for the per-path top-left point.
 FT_HD inline void compact_event(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy,
                                 const uint32_t *kept, const uint32_t *idx_of, const int32_t *run_first,
                                 const uint32_t *run_path, uint32_t *run_start, uint32_t *run_end, I2 *pts,
-                                uint32_t *pt_run, const PathState *ps, uint32_t &bad) {
+                                uint32_t *pt_run, const PathState *ps, uint32_t &bad, TopCand *top = nullptr) {
     const uint32_t m = meta[e];
     const uint32_t run = m & ~CLOSE_BIT;
     const bool rok = run < L.n_runs;
@@ -591,15 +598,14 @@ FT_HD inline void compact_event(uint32_t e, const Limits &L, const uint32_t *met
     if (keep & iok & ((int32_t)e == rf)) run_start[run] = idx;
     if (ok & ((m & CLOSE_BIT) != 0)) run_end[run] = idx < cap_end ? idx : cap_end;
     bad |= (ok ? 0u : 1u) | ((kp & !iok) ? 1u : 0u);
+    if (top) {
+        top->path = (keep & iok) ? p : NO_IDX;
+        top->key = point_key(q);
+    }
 }
 
 // Stage 4c: path and (y, x) key of stored point i, for the per-path top-left
 // point (Fig::sort_points' first vertex, fig.rs:467-478).
-struct TopCand {
-    uint32_t path;  // NO_IDX: not a stored point
-    unsigned long long key;
-};
-
 FT_HD inline TopCand top_cand(uint32_t i, const Limits &L, const I2 *pts, const uint32_t *pt_run,
                               const uint32_t *run_path, uint32_t &bad) {
     const uint32_t run = pt_run[i];

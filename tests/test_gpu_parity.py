@@ -341,3 +341,28 @@ def test_batch_rmw_formats_with_and_without_prefetch(ft, oracle):
                 assert o.fill(rule, ops, clr) == 0
                 want[y:y + h, x * bpp:(x + w) * bpp] = o.pixels().reshape(h, w * bpp)
             assert np.array_equal(got, want), (fmt, pf)
+
+
+def test_batch_over_a_million_events(ft, oracle):
+    """A batch past 2^20 events (its kept scan spans several tiles per block,
+    so the kept flags take a pass of their own): sampled tiles and their
+    point counts against the oracle."""
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    side = 4096
+    ops, tiles = workloads.atlas(12, n_paths=1700, atlas_side=side)
+    atlas = DeviceBuffer(side * side)
+    atlas.upload(np.full(side * side, 77, dtype=np.uint8))
+    descs = [FillDesc(int(b), int(c), atlas.ptr + int(y) * side + int(x), int(s), int(s), side,
+                      fresh=True) for b, c, x, y, s in tiles]
+    batch = Batch(ops, descs)
+    batch.run()
+    offered, stored, edges = batch.counts()
+    assert int(offered.sum()) > (1 << 20)
+    img = atlas.download().reshape(side, side)
+    rng = np.random.default_rng(5)
+    for t in sorted(rng.choice(len(tiles), size=40, replace=False)):
+        b, c, x, y, s = tiles[t]
+        o = oracle.Plotter(int(s), int(s))
+        assert o.fill(0, ops[b:b + c]) == 0
+        assert np.array_equal(img[y:y + s, x:x + s], o.pixels().reshape(s, s)), f"tile {t}"
+        assert (offered[t], stored[t]) == o.last_counts()
