@@ -1253,10 +1253,9 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
             const int32_t ed = ((int)((Eo.flags >> 1) & 1u) == fwd) ? 1 : -1;
             const RowCells c = row_setup(Eo, row + shift, (int32_t)W);
             int32_t *const cr = s_cells + (size_t)(row - (int32_t)ry0) * Wp + c.x_s;
-            for (int32_t j = 0; j < c.n; ++j) {
-                const int32_t d = row_delta(c, j);
+            row_walk(c, [&](int32_t j, int32_t d) {
                 if (d) atomicAdd(&cr[j], d * ed);
-            }
+            });
         }
         __syncthreads();  // before the next entries overwrite the slots
     }

@@ -807,8 +807,7 @@ FT_HD inline bool path_forward(const PathState &s, const Limits &L, const I2 *pt
 // x_pix reaches cov.  pixel_cov(u) = (u + 128) >> 8 for 0 <= u <= ONE, so
 // x_pix(k) >= cov exactly when xc0 + k * st >= 256 * cov - 128 (ONE is past
 // that threshold), which gives the number of cells in closed form.
-// row_setup describes the row; row_delta gives any one cell's delta, so the
-// cells of many rows can be spread over the lanes of a wavefront.
+// row_setup describes the row; row_walk produces its cells in order.
 struct RowCells {
     int64_t xc0, st;  // x_cov at step k: min(xc0 + k * st, ONE)
     int32_t cov;      // the row's coverage, 1..256
@@ -816,13 +815,6 @@ struct RowCells {
     int32_t k_s;      // step of the first cell
     int32_t n;        // cells (0: none)
 };
-
-FT_HD inline int32_t row_xpix(const RowCells &c, int64_t k) {
-    int64_t xc = c.xc0 + k * c.st;
-    if (xc > FX_ONE) xc = FX_ONE;
-    const int32_t xp = pixel_cov((int32_t)xc);
-    return xp < c.cov ? xp : c.cov;
-}
 
 FT_HD inline RowCells row_setup(const EdgeRec &E, int32_t sr, int32_t W) {
     RowCells c;
@@ -861,20 +853,33 @@ FT_HD inline RowCells row_setup(const EdgeRec &E, int32_t sr, int32_t W) {
     return c;
 }
 
-// Delta of cell x_s + j (0 <= j < n).
-FT_HD inline int32_t row_delta(const RowCells &c, int32_t j) {
-    const int32_t cur = row_xpix(c, (int64_t)c.k_s + j);
-    const int32_t prev = j > 0 ? row_xpix(c, (int64_t)c.k_s + j - 1) : 0;
-    return cur - prev;
+// The row's cells in order: f(j, delta of cell x_s + j) for j in [0, n).
+// x_cov = min(xc0 + k * st, ONE) is walked from k = k_s in 32 bits: it
+// starts at or below ONE (xc0 >= 0), stays at ONE once capped, and below the
+// cap one more step (0 < st <= ONE) fits; on [0, ONE], pixel_cov(u) is
+// (u + 128) >> 8.
+template <class F>
+FT_HD inline void row_walk(const RowCells &c, F &&f) {
+    const int64_t xk = c.xc0 + (int64_t)c.k_s * c.st;
+    int32_t xc = xk < FX_ONE ? (int32_t)xk : FX_ONE;
+    const int32_t st = (int32_t)c.st;
+    int32_t prev = 0;
+    for (int32_t j = 0; j < c.n; ++j) {
+        const int32_t xp = (xc + 128) >> 8;
+        const int32_t cur = xp < c.cov ? xp : c.cov;
+        xc = xc < FX_ONE - st ? xc + st : FX_ONE;
+        f(j, cur - prev);
+        prev = cur;
+    }
 }
 
 template <class A>
 FT_HD inline void edge_row_cells(const EdgeRec &E, int32_t sr, int32_t ed, int32_t W, int32_t *cells) {
     const RowCells c = row_setup(E, sr, W);
-    for (int32_t j = 0; j < c.n; ++j) {
-        const int32_t d = row_delta(c, j);
-        if (d) A::add_cell(&cells[c.x_s + j], d * ed);
-    }
+    int32_t *const at = cells + c.x_s;
+    row_walk(c, [&](int32_t j, int32_t d) {
+        if (d) A::add_cell(&at[j], d * ed);
+    });
 }
 
 FT_HD inline uint32_t map_coverage(int32_t sum, uint32_t rule) {
