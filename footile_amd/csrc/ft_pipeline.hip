@@ -783,12 +783,26 @@ __device__ __forceinline__ uint32_t cov_of(uint32_t sum) {
 // Graya8p: x = S0 | S1 << 16, y = 0, z = invw, w = S0 | S1 << 8.
 constexpr uint32_t LUT_ENTRIES = 256;
 constexpr uint32_t LUT_BYTES = LUT_ENTRIES * sizeof(uint4);
-constexpr uint32_t CELL_ROUND = 128;  // list entries per balanced cell-pass round (their slots share the table's LDS)
+// List entries per balanced cell-pass round: their slots share the front of
+// the block's LDS with the composite table (128 fit its 4 KiB); Matte8 has no
+// table and takes 256 in an 8 KiB front.
+constexpr uint32_t CELL_ROUND = 128;
+template <int FMT>
+constexpr uint32_t cell_round() {
+    return FMT == FMT_MATTE8 ? 2u * CELL_ROUND : CELL_ROUND;
+}
+template <int FMT>
+constexpr uint32_t front_bytes() {
+    return FMT == FMT_MATTE8 ? 2u * LUT_BYTES : LUT_BYTES;
+}
+static_assert(2 * CELL_ROUND * (sizeof(EdgeRec) + sizeof(uint32_t)) <= 2 * LUT_BYTES, "Matte8 slots fit its front");
 constexpr uint32_t DIRECT_ROWS = 4;   // row groups up to this tall take the direct cell pass
 static_assert(CELL_ROUND * (sizeof(EdgeRec) + sizeof(uint32_t)) <= LUT_BYTES, "edge slots fit the table's LDS");
-static_assert(CELL_ROUND <= BLOCK, "one entry per thread");
+static_assert(2 * CELL_ROUND <= BLOCK, "one entry per thread");
 
-static inline size_t raster_lds_bytes(size_t cells) { return LUT_BYTES + cells * sizeof(int32_t); }
+static inline size_t raster_lds_bytes(uint32_t fmt, size_t cells) {
+    return (fmt == FMT_MATTE8 ? front_bytes<FMT_MATTE8>() : front_bytes<FMT_RGBA8P>()) + cells * sizeof(int32_t);
+}
 
 __device__ inline uint4 lut_entry(uint32_t fmt, uint32_t clr, uint32_t a) {
     const uint32_t aw = ch8_widen(a);
@@ -1075,8 +1089,9 @@ __device__ __forceinline__ void row_multi_any(uint32_t rule, bool fresh, const i
 // one kernel per pixel format FMT, over the format's stretch of the table
 // from `first`.  The block accumulates the band's listed edges for its G rows
 // in LDS and scans each row into the raster; the band's other groups are
-// other blocks.  Dynamic LDS: LUT_BYTES for the cell pass's edge slots and
-// then the composite table (Rgba8p, Graya8p), followed by the rows' i32 cells.
+// other blocks.  Dynamic LDS: front_bytes<FMT>() for the cell pass's edge
+// slots and then the composite table (Rgba8p, Graya8p), followed by the
+// rows' i32 cells.
 template <bool PF, int FMT>
 __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *paths, const uint4 *rblock,
                                                   uint32_t first_block, uint32_t band_shift,
@@ -1087,10 +1102,11 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     constexpr bool HAS_LUT = FMT != FMT_MATTE8;
     // The front LUT_BYTES hold the cell pass's edge slots, then the composite
     // table; the rows' cells follow.
+    constexpr uint32_t CR = cell_round<FMT>();
     uint4 *const lut = s_dyn;
     EdgeRec *const s_edge = reinterpret_cast<EdgeRec *>(s_dyn);
-    uint32_t *const s_pair0 = reinterpret_cast<uint32_t *>(s_edge + CELL_ROUND);
-    int32_t *const s_cells = reinterpret_cast<int32_t *>(s_dyn + LUT_ENTRIES);
+    uint32_t *const s_pair0 = reinterpret_cast<uint32_t *>(s_edge + CR);
+    int32_t *const s_cells = reinterpret_cast<int32_t *>(s_dyn + front_bytes<FMT>() / sizeof(uint4));
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     const uint4 rb = rblock[first_block + blockIdx.x];
@@ -1179,7 +1195,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     const bool list_ok = (off <= end) & (end <= L.n_pairs);
     const uint32_t cnt = list_ok ? end - off : 0u;
     if (!list_ok && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
-    // Cell pass, taller groups: the block takes CELL_ROUND list entries at a time; their
+    // Cell pass, taller groups: the block takes CR list entries at a time; their
     // (edge, row) pairs are laid out by a scan of the rows per edge and spread
     // evenly over the threads (each finds its pair's edge by binary search in
     // LDS), so an edge crossing many rows does not hold up a wavefront.  A
@@ -1217,9 +1233,9 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
                                         s_cells + (size_t)(r - ry0) * Wp);
         }
     }
-    for (uint32_t k0 = 0; !direct && list_ok && k0 < cnt; k0 += CELL_ROUND) {
+    for (uint32_t k0 = 0; !direct && list_ok && k0 < cnt; k0 += CR) {
         const uint32_t k = k0 + threadIdx.x;
-        bool okk = (threadIdx.x < CELL_ROUND) & (k < cnt);
+        bool okk = (threadIdx.x < CR) & (k < cnt);
         const uint32_t ei = band_list[off + (okk ? k : 0u)];
         if (okk & (ei >= L.cap_points)) raise_err(st, FT_EDEVICE);
         okk = okk & (ei < L.cap_points);
@@ -1230,7 +1246,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
         const uint32_t nr = (okk & (ra <= rb)) ? (uint32_t)(rb - ra + 1) : 0u;
         const uint32_t incl = wave_incl_u32(nr);
         if (lane == 63) s_segsum[wid] = incl;
-        if (threadIdx.x < CELL_ROUND) s_edge[threadIdx.x] = E;
+        if (threadIdx.x < CR) s_edge[threadIdx.x] = E;
         __syncthreads();
         uint32_t before = 0, n_pairs = 0;
 #pragma unroll
@@ -1238,14 +1254,14 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
             before += w < wid ? s_segsum[w] : 0u;
             n_pairs += s_segsum[w];
         }
-        if (threadIdx.x < CELL_ROUND) s_pair0[threadIdx.x] = before + incl - nr;  // the edge's first pair
+        if (threadIdx.x < CR) s_pair0[threadIdx.x] = before + incl - nr;  // the edge's first pair
         __syncthreads();
         for (uint32_t p = threadIdx.x; p < n_pairs; p += BLOCK) {
             // the last slot whose first pair is at or before p (slots without
             // rows share their successor's first pair)
             uint32_t lo = 0;
 #pragma unroll
-            for (uint32_t step = CELL_ROUND / 2; step; step >>= 1) lo += s_pair0[lo + step] <= p ? step : 0u;
+            for (uint32_t step = CR / 2; step; step >>= 1) lo += s_pair0[lo + step] <= p ? step : 0u;
             const EdgeRec Eo = s_edge[lo];
             int32_t r_first = fx_int(Eo.y_upper) - shift;
             r_first = r_first < (int32_t)ry0 ? (int32_t)ry0 : r_first;
@@ -1490,7 +1506,9 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
         rblock.insert(rblock.end(), rblock_f[f].begin(), rblock_f[f].end());
     }
     if (rblock.size() >= 0x7FFFFFFFu) return FT_ENOTSUP;
-    if (raster_lds_bytes(lds) > ws.lds_max) return FT_ENOTSUP;  // the tuning asks for more LDS than a block has
+    // the tuning asks for more LDS than a block has
+    if (raster_lds_bytes(FMT_MATTE8, lds) > ws.lds_max || raster_lds_bytes(FMT_RGBA8P, lds) > ws.lds_max)
+        return FT_ENOTSUP;
     ft_status s;
     const size_t old_ops = ws.cap_ops;
     if ((s = dev_grow(ws.d_ops, ws.cap_ops, n_ops))) return s;
@@ -1757,7 +1775,7 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
                     if (!ws.rb_count[f]) continue;
                     hipEvent_t e0 = (ev && f == first) ? ev[1] : nullptr, e1 = (ev && f == last) ? ev[2] : nullptr;
                     const dim3 grid(ws.rb_count[f]);
-                    const uint32_t lds = (uint32_t)raster_lds_bytes(ws.lds_cells);
+                    const uint32_t lds = (uint32_t)raster_lds_bytes((uint32_t)f, ws.lds_cells);
                     if (f == FT_MATTE8)
                         hipExtLaunchKernelGGL(k_raster<false, FMT_MATTE8>, grid, dim3(BLOCK), lds, s, e0, e1, 0, L0,
                                               ws.d_paths, ws.d_rblock, ws.rb_first[f], ws.band_shift, ws.d_band_off,
