@@ -44,7 +44,8 @@ struct Workspace {
     uint4 *d_rblock = nullptr;  // raster blocks: (global band, row group, path, 0)
     // per-op scratch
     unsigned long long *d_op_cnt = nullptr, *d_op_off = nullptr;
-    unsigned long long *d_sub_off = nullptr;  // per (op, lane): offset within the op
+    unsigned long long *d_sub_off = nullptr;  // per (op, lane): offset within the op (+ leaf count)
+    I2 *d_leaf = nullptr;                      // per (op, lane): its first LEAF_CACHE points (fill runs)
     uint32_t *d_op_pw = nullptr, *d_op_lastpw = nullptr;
     // events and everything sized by them
     size_t cap_events = 0;

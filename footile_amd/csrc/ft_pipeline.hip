@@ -116,12 +116,15 @@ __global__ __launch_bounds__(BLOCK) void k_pen_width_mark(const ft_path_op *ops,
 
 // One thread per (op, lane); an op's OP_LANES lanes are adjacent lanes of one
 // wavefront.  Writes the op's event/close count and each lane's offset within
-// the op (a prefix over the op's lanes).
+// the op (a prefix over the op's lanes).  In fill runs a curve lane also
+// keeps its first LEAF_CACHE points in leaf[k * stride + t] (coalesced
+// across the lanes), and its leaf count rides in bits 16-31 of its offset
+// (the op's close count, at most 2, uses the low bits).
 __global__ __launch_bounds__(BLOCK) void k_count(const ft_path_op *ops, const uint32_t *op_path,
                                                  const PathDev *paths, uint32_t n_ops, uint32_t mode,
                                                  const uint32_t *lastpw, const uint32_t *order,
-                                                 unsigned long long *op_cnt, unsigned long long *sub_off,
-                                                 DevStatus *st) {
+                                                 unsigned long long *op_cnt, unsigned long long *sub_off, I2 *leaf,
+                                                 size_t stride, DevStatus *st) {
     const uint64_t n_items = (uint64_t)n_ops * OP_LANES;
     for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_items; t += (uint64_t)gridDim.x * BLOCK) {
         const uint32_t oi = (uint32_t)(t >> FLAT_SPLIT), s = (uint32_t)t & (OP_LANES - 1);
@@ -130,7 +133,7 @@ __global__ __launch_bounds__(BLOCK) void k_count(const ft_path_op *ops, const ui
             raise_err(st, FT_EDEVICE);
             continue;
         }
-        const OpCount c = count_op(ops, op_path, paths, i, s, mode, lastpw);
+        const OpCount c = count_op(ops, op_path, paths, i, s, mode, lastpw, leaf ? leaf + t : nullptr, stride);
         const unsigned long long v = pack_count(c);
         unsigned long long inc = v;
         uint32_t leaves = c.leaves;
@@ -140,7 +143,8 @@ __global__ __launch_bounds__(BLOCK) void k_count(const ft_path_op *ops, const ui
             if (s >= o) inc += u;
             leaves += __shfl_xor(leaves, o, OP_LANES);
         }
-        sub_off[t] = inc - v;
+        const unsigned long long lc = c.leaves < 0xFFFFu ? c.leaves : 0xFFFFu;
+        sub_off[t] = (inc - v) | (lc << 16);
         if (s == OP_LANES - 1) op_cnt[i] = inc;
         if (c.err) raise_err(st, c.err);
         if (s == 0 && leaves > FT_FLATTEN_MAX_LEAVES) raise_err(st, FT_ERANGE);
@@ -151,7 +155,7 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uin
                                                 const PathDev *paths, uint32_t n_ops, uint32_t mode,
                                                 const uint32_t *lastpw, const uint32_t *order,
                                                 const unsigned long long *op_off, const unsigned long long *sub_off,
-                                                EventSink sink, DevStatus *st) {
+                                                const I2 *leaf, size_t stride, EventSink sink, DevStatus *st) {
     if (run_failed(st)) return;
     const unsigned long long tot = st->op_total;
     const uint32_t n_events = (uint32_t)(tot >> 32);
@@ -173,7 +177,10 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uin
             bad = 1;
             continue;
         }
-        bad |= emit_op(ops, op_path, paths, i, s, mode, lastpw, op_off[i] + sub_off[t], sink);
+        const unsigned long long so = sub_off[t];
+        const uint32_t cached = (uint32_t)(so >> 16) & 0xFFFFu;
+        bad |= emit_op(ops, op_path, paths, i, s, mode, lastpw, op_off[i] + (so & ~(0xFFFFull << 16)), sink,
+                       leaf ? leaf + t : nullptr, cached, stride);
     }
     if (bad) raise_err(st, FT_EDEVICE);
 }
@@ -1388,7 +1395,7 @@ ft_status ws_init(Workspace &ws) {
 
 void ws_free(Workspace &ws) {
     void *bufs[] = {ws.d_ops, ws.d_op_path, ws.d_op_order, ws.d_paths, ws.d_band_path, ws.d_op_cnt, ws.d_op_off, ws.d_sub_off,
-                    ws.d_op_pw,
+                    ws.d_leaf, ws.d_op_pw,
                     ws.d_op_lastpw, ws.d_ev_xy, ws.d_ev_f, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_ev_last,
                     ws.d_st_flags, ws.d_st_off, ws.d_st_last, ws.d_spts, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
                     ws.d_sub_joined, ws.d_it_cnt, ws.d_it_off,
@@ -1517,6 +1524,7 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
             (s = dev_alloc(ws.d_op_cnt, ws.cap_ops)) ||
             (s = dev_alloc(ws.d_op_off, ws.cap_ops)) || (s = dev_alloc(ws.d_op_pw, ws.cap_ops)) ||
             (s = dev_alloc(ws.d_sub_off, ws.cap_ops * OP_LANES)) ||
+            (s = dev_alloc(ws.d_leaf, ws.cap_ops * OP_LANES * LEAF_CACHE)) ||
             (s = dev_alloc(ws.d_op_lastpw, ws.cap_ops)))
             return s;
     }
@@ -1664,8 +1672,10 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
     }
     if (n_ops) {
         const uint32_t *order = ws.op_ordered ? ws.d_op_order : nullptr;
+        I2 *const leaf = mode == MODE_FILL ? ws.d_leaf : nullptr;
         k_count<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
-                                                                     lastpw, order, ws.d_op_cnt, ws.d_sub_off, st);
+                                                                     lastpw, order, ws.d_op_cnt, ws.d_sub_off, leaf,
+                                                                     ws.cap_ops * OP_LANES, st);
         scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops, lb, &st->op_total, st, false, false);
         EventSink em{};
         em.cap_events = ws.cap_events;
@@ -1680,7 +1690,8 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         em.run_start = ws.d_run_start;
         em.run_end = ws.d_run_end;
         k_emit<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
-                                                                    lastpw, order, ws.d_op_off, ws.d_sub_off, em, st);
+                                                                    lastpw, order, ws.d_op_off, ws.d_sub_off, leaf,
+                                                                    ws.cap_ops * OP_LANES, em, st);
     }
     if (mode == MODE_STROKE) {
         const uint32_t ge = grid_for(ws.cap_events);

@@ -221,8 +221,14 @@ struct OpCount {
     uint32_t ne, nc, err, leaves;
 };
 
+// Fill runs may keep each curve lane's first LEAF_CACHE points (as the
+// Fixed points add_point stores) in `cache` (point k at cache[k * stride]),
+// so that emitting the lane is a copy unless it has more leaves.
+constexpr uint32_t LEAF_CACHE = 8;
+
 FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, const PathDev *paths, uint32_t i,
-                              uint32_t s, uint32_t mode, const uint32_t *lastpw) {
+                              uint32_t s, uint32_t mode, const uint32_t *lastpw, I2 *cache = nullptr,
+                              size_t stride = 0) {
     OpCount c = {0, 0, 0, 0};
     const uint32_t p = op_path[i];
     if (p == NO_IDX) return c;
@@ -237,7 +243,16 @@ FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, co
     case TAG_CUBIC: {
         const WP pen = pen_before(ops, lastpw, i, P);
         const float sw = width_before(ops, lastpw, i, P);
-        auto cnt = [&](WP) {};
+        I2 *const keep = mode == MODE_FILL ? cache : nullptr;
+        auto cnt = [&](WP q) {
+            const uint32_t k = c.leaves - 1;  // flatten counts a leaf before handing it over
+            if (keep && k < LEAF_CACHE) {
+                I2 v;
+                v.x = fx_from_f32(q.x);
+                v.y = fx_from_f32(q.y);
+                keep[k * stride] = v;
+            }
+        };
         const uint32_t rc = o.tag == TAG_QUAD ? flatten_share(quad_of(o, pen, sw, P), P.tol_sq, s, c.leaves, cnt)
                                               : flatten_share(cubic_of(o, pen, sw, P), P.tol_sq, s, c.leaves, cnt);
         if (rc)
@@ -297,6 +312,15 @@ struct EventSink {
         bad |= ok ? 0u : 1u;
         ++e;
     }
+    FT_HD void point_fx(I2 v) {  // a fill point already converted (count_op's cache)
+        const bool ok = e < cap_events;
+        if (ok) {
+            ev_xy[e] = v;
+            ev_meta[e] = run;
+        }
+        bad |= ok ? 0u : 1u;
+        ++e;
+    }
     FT_HD void close(bool joined) {
         const bool ok = (e < cap_events) & (run < cap_runs);
         if (ok & (mode == MODE_FILL)) {
@@ -323,9 +347,12 @@ struct EventSink {
 
 // Writes lane s's events of op i at offset `off` (events << 32 | closes
 // before them).
+// `cache` / `cached`: the lane's points kept by count_op (point k at
+// cache[k * stride]) and their number (used in fill runs when it is at most
+// LEAF_CACHE).
 FT_HD inline uint32_t emit_op(const ft_path_op *ops, const uint32_t *op_path, const PathDev *paths, uint32_t i,
                               uint32_t s, uint32_t mode, const uint32_t *lastpw, unsigned long long off,
-                              EventSink em) {
+                              EventSink em, const I2 *cache = nullptr, uint32_t cached = 0, size_t stride = 0) {
     const uint32_t p = op_path[i];
     if (p == NO_IDX) return 0;
     const PathDev &P = paths[p];
@@ -349,6 +376,10 @@ FT_HD inline uint32_t emit_op(const ft_path_op *ops, const uint32_t *op_path, co
         break;
     case TAG_QUAD:
     case TAG_CUBIC: {
+        if (cache && mode == MODE_FILL && cached <= LEAF_CACHE) {
+            for (uint32_t k = 0; k < cached; ++k) em.point_fx(cache[k * stride]);
+            break;
+        }
         const WP pen = pen_before(ops, lastpw, i, P);
         const float sw = width_before(ops, lastpw, i, P);
         uint32_t leaves = 0;
