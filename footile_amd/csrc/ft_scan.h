@@ -23,6 +23,9 @@ template <typename T>
 constexpr uint32_t scan_tile() {
     return (uint32_t)BLOCK * scan_items<T>();
 }
+// Sizes look-back state: a scan of n items over tiles of at least this many
+// items has at most n / SCAN_TILE_MIN + 1 blocks (sources with smaller tiles
+// run where the state holds SCAN_GRID_MAX blocks).
 constexpr uint32_t SCAN_TILE_MIN = (uint32_t)BLOCK * 8u;
 
 // Scan blocks certainly resident at once on the current device: four per
@@ -31,7 +34,7 @@ inline uint32_t scan_resident_blocks() {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        return 0;  // unknown: ticket order only
+        return 0;  // unknown: scans fall back to a 64-block grid
     return (uint32_t)cus * 4u;
 }
 constexpr uint32_t SCAN_GRID_MAX = 1024;  // largest scan grid
