@@ -82,6 +82,7 @@ struct Workspace {
     unsigned long long *d_scan_state = nullptr;  // scan look-back words, cleared by k_clear
     // results
     PathState *d_pstate = nullptr;
+    unsigned long long *d_top_rep = nullptr;  // per path: TOP_REPS partial top keys
     DevStatus *d_status = nullptr;
     DevStatus *h_status = nullptr;  // pinned
     std::vector<PathState> h_pstate;

@@ -79,10 +79,11 @@ __device__ inline void block_reduce_keyed(bool has, uint32_t key, T v, T identit
 // ---------------------------------------------------------------------------
 // Stage 0: clear per-run state (a kernel, not hipMemsetAsync: graph-safe).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, uint32_t n_paths, uint32_t *band_cnt,
+__global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, unsigned long long *top_rep, uint32_t n_paths,
+                                                 uint32_t *band_cnt,
                                                  uint32_t *band_cur, uint32_t n_slots, unsigned long long *lb,
                                                  uint32_t n_lb, DevStatus *st) {
-    uint32_t n = n_paths > n_slots ? n_paths : n_slots;
+    uint32_t n = n_paths * TOP_REPS > n_slots ? n_paths * TOP_REPS : n_slots;
     n = n > n_lb ? n : n_lb;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
         if (i < n_paths) {
@@ -97,6 +98,7 @@ __global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, uint32_t n_paths
             s.fwd = 0;
             ps[i] = s;
         }
+        if (i < n_paths * TOP_REPS) top_rep[i] = NO_KEY;
         if (i < n_slots) {
             band_cnt[i] = 0;
             band_cur[i] = 0;
@@ -315,13 +317,15 @@ __global__ __launch_bounds__(BLOCK) void k_cap(Limits L0, const PathDev *paths, 
 }
 
 // Compaction of the stored points, and each path's minimum (y, x) key over
-// them (PathState::topkey): a block merges its points' keys per path and
-// flushes one atomic per path it holds.
+// them: a block merges its points' keys per path and flushes one atomic per
+// path it holds into one of the path's TOP_REPS partial minima (block b into
+// b mod TOP_REPS), so that a one-path batch does not queue every block's
+// atomic on one address.  k_edges folds the partials into PathState::topkey.
 __global__ __launch_bounds__(BLOCK) void k_compact(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
                                                    const uint32_t *ev_kept, const uint32_t *ev_idx,
                                                    const int32_t *run_first, const uint32_t *run_path,
                                                    uint32_t *run_start, uint32_t *run_end, I2 *pts, uint32_t *pt_run,
-                                                   PathState *ps, DevStatus *st) {
+                                                   const PathState *ps, unsigned long long *top_rep, DevStatus *st) {
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     uint32_t bad = 0;
@@ -336,8 +340,8 @@ __global__ __launch_bounds__(BLOCK) void k_compact(Limits L0, const uint32_t *ev
         block_reduce_keyed<unsigned long long>(
             c.path != NO_IDX, c.path, c.key, NO_KEY,
             [](unsigned long long a, unsigned long long b) { return a < b ? a : b; },
-            [ps, &L](uint32_t k, unsigned long long v) {
-                if (k < L.n_paths) atomicMin(&ps[k].topkey, v);
+            [top_rep, &L](uint32_t k, unsigned long long v) {
+                if (k < L.n_paths) atomicMin(&top_rep[(size_t)k * TOP_REPS + (blockIdx.x & (TOP_REPS - 1))], v);
             });
     }
     if (bad) raise_err(st, FT_EDEVICE);
@@ -425,7 +429,7 @@ __global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, uint32_t band_shift,
                                                  const uint32_t *run_path, const uint32_t *run_start,
                                                  const uint32_t *run_end, const PathDev *paths, const I2 *pts,
                                                  EdgeRec *edges, U2 *pt_band, uint32_t *band_cnt, PathState *ps,
-                                                 DevStatus *st) {
+                                                 const unsigned long long *top_rep, DevStatus *st) {
     __shared__ __align__(16) uint32_t s_h[BIN_WINDOW];
     __shared__ uint32_t s_w[BLOCK / 64];
     if (run_failed(st)) return;
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, uint32_t band_shift,
             rec.y = 0;
             if (i < n) {
                 const EdgeOut r = edge_point<DevAtom>(i, L, pt_run, run_path, run_start, run_end, paths, pts, edges,
-                                                      ps, bad);
+                                                      ps, bad, top_rep);
                 rec = band_record(r);
                 pt_band[i] = rec;
             }
@@ -1401,7 +1405,7 @@ void ws_free(Workspace &ws) {
                     ws.d_sub_joined, ws.d_it_cnt, ws.d_it_off,
                     ws.d_run_first, ws.d_run_close, ws.d_run_path, ws.d_rblock, ws.d_run_start, ws.d_run_end, ws.d_pts,
                     ws.d_pt_run, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
-                    ws.d_scan_state, ws.d_pstate, ws.d_status};
+                    ws.d_scan_state, ws.d_pstate, ws.d_top_rep, ws.d_status};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ws.h_status) (void)hipHostFree(ws.h_status);
@@ -1531,7 +1535,8 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     const size_t old_paths = ws.cap_paths;
     if ((s = dev_grow(ws.d_paths, ws.cap_paths, n_paths))) return s;
     if (ws.cap_paths != old_paths || !ws.d_pstate)
-        if ((s = dev_alloc(ws.d_pstate, ws.cap_paths))) return s;
+        if ((s = dev_alloc(ws.d_pstate, ws.cap_paths)) || (s = dev_alloc(ws.d_top_rep, ws.cap_paths * TOP_REPS)))
+            return s;
     size_t nb_total = band_path.size();
     // replicas per band counter: enough that the slots spread over about
     // BAND_SLOTS_MIN counters, at most 2^BAND_SHIFT_MAX
@@ -1661,9 +1666,9 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
     // dispatch (ev[0] at k_clear's start, ev[1] / ev[2] around the raster), so
     // no marker packet stalls the stream between kernels.
     const size_t n_slots = ws.n_bands << ws.band_shift;
-    hipExtLaunchKernelGGL(k_clear, dim3(grid_for(std::max(std::max(ws.n_paths, n_slots), (size_t)n_lb))),
-                          dim3(BLOCK), 0, s, ev ? ev[0] : nullptr, nullptr, 0, ws.d_pstate, (uint32_t)ws.n_paths,
-                          ws.d_band_cnt, ws.d_band_cur, (uint32_t)n_slots, lb, n_lb, st);
+    hipExtLaunchKernelGGL(k_clear, dim3(grid_for(std::max(std::max(ws.n_paths * TOP_REPS, n_slots), (size_t)n_lb))),
+                          dim3(BLOCK), 0, s, ev ? ev[0] : nullptr, nullptr, 0, ws.d_pstate, ws.d_top_rep,
+                          (uint32_t)ws.n_paths, ws.d_band_cnt, ws.d_band_cur, (uint32_t)n_slots, lb, n_lb, st);
     const uint32_t *lastpw = nullptr;
     if (mode != MODE_FILL && n_ops) {
         k_pen_width_mark<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, n_ops, ws.d_op_pw);
@@ -1761,10 +1766,11 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
                                                      ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first, ws.d_pstate, st);
         k_compact<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first,
                                        ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_pts, ws.d_pt_run, ws.d_pstate,
-                                       st);
+                                       ws.d_top_rep, st);
         const bool wide = ws.cap_events >= BIN_WIDE_MIN;
         (wide ? k_edges<4> : k_edges<1>)<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_run, ws.d_run_path, ws.d_run_start, ws.d_run_end,
-                                     ws.d_paths, ws.d_pts, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_pstate, st);
+                                     ws.d_paths, ws.d_pts, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_pstate,
+                                     ws.d_top_rep, st);
         if (ws.n_bands) {
             // pairs per (band, replica) -> list segment offsets and the pair total
             const uint32_t n_slots = (uint32_t)(ws.n_bands << ws.band_shift);

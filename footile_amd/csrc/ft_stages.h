@@ -729,18 +729,32 @@ struct EdgeOut {
     bool banded;      // that edge touches a raster row
 };
 
+// A path's top key may be kept as TOP_REPS partial minima (the device
+// compaction spreads its atomics over them); top_fold is their minimum.
+constexpr uint32_t TOP_REPS = 8;
+FT_HD inline unsigned long long top_fold(const unsigned long long *rep) {
+    unsigned long long m = rep[0];
+    for (uint32_t k = 1; k < TOP_REPS; ++k) m = rep[k] < m ? rep[k] : m;
+    return m;
+}
+
 // edge_point in two parts: edge_eval only loads (so that a thread's several
 // points can have their loads in flight together), edge_commit stores.
+// With `top_rep` the path's top key is folded from its partial minima (and
+// stored into PathState::topkey by the points at it), else read from `ps`.
 struct EdgeEval {
     EdgeOut out;
     EdgeRec E;
-    uint32_t top_local;  // local index of the point when it is at the path's top key
+    unsigned long long topkey;  // the path's top key
+    uint32_t top_local;         // local index of the point when it is at the path's top key
     bool top;
+    bool fold;                  // topkey was folded from partial minima
 };
 
 FT_HD inline EdgeEval edge_eval(uint32_t i, const Limits &L, const uint32_t *pt_run, const uint32_t *run_path,
                                 const uint32_t *run_start, const uint32_t *run_end, const PathDev *paths,
-                                const I2 *pts, const PathState *ps, uint32_t &bad) {
+                                const I2 *pts, const PathState *ps, uint32_t &bad,
+                                const unsigned long long *top_rep = nullptr) {
     // point i is stored iff pt_run[i] names a run (compact_event writes NO_IDX for capped points)
     const uint32_t run0 = pt_run[i];
     const bool rok = run0 < L.n_runs;
@@ -758,9 +772,11 @@ FT_HD inline EdgeEval edge_eval(uint32_t i, const Limits &L, const uint32_t *pt_
     EdgeEval r;
     r.E = make_edge(q, q1);
     if (!has) r.E.flags = 0;
-    r.top = valid & (point_key(q) == s.topkey);
+    r.fold = top_rep != nullptr;
+    r.topkey = r.fold ? top_fold(top_rep + (size_t)p * TOP_REPS) : s.topkey;
+    r.top = valid & (point_key(q) == r.topkey);
     r.top_local = i - s.pt_base;
-    r.out.banded = edge_bands(r.E, paths[p], s.topkey, L, r.out.g0, r.out.g1) & has;
+    r.out.banded = edge_bands(r.E, paths[p], r.topkey, L, r.out.g0, r.out.g1) & has;
     r.out.path = valid ? p : NO_IDX;
     r.out.valid = valid;
     r.out.has = has;
@@ -770,14 +786,16 @@ FT_HD inline EdgeEval edge_eval(uint32_t i, const Limits &L, const uint32_t *pt_
 template <class A>
 FT_HD inline void edge_commit(uint32_t i, const EdgeEval &r, EdgeRec *edges, PathState *ps) {
     if (r.top) A::min_u32(&ps[r.out.path].topidx, r.top_local);
+    if (r.top & r.fold) ps[r.out.path].topkey = r.topkey;  // the same value from every point at it
     if (r.out.valid) edges[i] = r.E;
 }
 
 template <class A>
 FT_HD inline EdgeOut edge_point(uint32_t i, const Limits &L, const uint32_t *pt_run, const uint32_t *run_path,
                                 const uint32_t *run_start, const uint32_t *run_end, const PathDev *paths,
-                                const I2 *pts, EdgeRec *edges, PathState *ps, uint32_t &bad) {
-    const EdgeEval r = edge_eval(i, L, pt_run, run_path, run_start, run_end, paths, pts, ps, bad);
+                                const I2 *pts, EdgeRec *edges, PathState *ps, uint32_t &bad,
+                                const unsigned long long *top_rep = nullptr) {
+    const EdgeEval r = edge_eval(i, L, pt_run, run_path, run_start, run_end, paths, pts, ps, bad, top_rep);
     edge_commit<A>(i, r, edges, ps);
     return r.out;
 }

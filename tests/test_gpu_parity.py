@@ -366,3 +366,23 @@ def test_batch_over_a_million_events(ft, oracle):
         assert o.fill(0, ops[b:b + c]) == 0
         assert np.array_equal(img[y:y + s, x:x + s], o.pixels().reshape(s, s)), f"tile {t}"
         assert (offered[t], stored[t]) == o.last_counts()
+
+
+def test_long_curves_fine_tolerance(ft, oracle):
+    """Curves whose flattening lanes hold more than the eight points the count
+    pass keeps per lane (the emit pass walks them again), mixed with short
+    ones, at the finest tolerance (0.01) and the default."""
+    rng = np.random.default_rng(21)
+    ops = [(MOVE, (20.0, 20.0))]
+    for k in range(12):
+        pts = rng.uniform(0, 1000, size=6).astype(np.float32)
+        ops.append((CUBIC if k % 3 else QUAD, tuple(pts if k % 3 else pts[:4])))
+        ops.append((LINE, tuple(rng.uniform(0, 1000, size=2).astype(np.float32))))
+    ops.append((CLOSE, ()))
+    a = ops_array(ops)
+    stored = {}
+    for tol in (0.01, 0.3):
+        g, o = compare_fill(ft, oracle, 1000, 1000, MATTE8, [("fill", 1, a, (255,))], tol=tol, tag=f"tol {tol}")
+        assert g.last_counts() == o.last_counts()
+        stored[tol] = o.last_counts()[1]
+    assert stored[0.01] > 12 * 8 * 8  # far more leaves than 8 lanes x 8 kept points per curve
