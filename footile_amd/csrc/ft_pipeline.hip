@@ -248,6 +248,8 @@ struct KeptSrc {
 };
 
 static_assert(IngestSrc::ITEMS == KeptSrc::ITEMS, "one tile size decides both ingest scans");
+constexpr size_t TOP_FEW_PATHS = 64;  // batches below this many paths keep TOP_REPS partial top keys each
+static_assert((TOP_REPS & (TOP_REPS - 1)) == 0, "block residues by mask");
 
 // Plain passes for the pushed-point marks and the kept flags, for runs whose
 // scans have several tiles per block (where IngestSrc / KeptSrc would
@@ -318,14 +320,15 @@ __global__ __launch_bounds__(BLOCK) void k_cap(Limits L0, const PathDev *paths, 
 
 // Compaction of the stored points, and each path's minimum (y, x) key over
 // them: a block merges its points' keys per path and flushes one atomic per
-// path it holds into one of the path's TOP_REPS partial minima (block b into
-// b mod TOP_REPS), so that a one-path batch does not queue every block's
-// atomic on one address.  k_edges folds the partials into PathState::topkey.
+// path it holds, into PathState::topkey (top_rep null) or into one of the
+// path's TOP_REPS partial minima (block b into b mod TOP_REPS), so that a
+// batch of few paths does not queue every block's atomic on one address;
+// k_edges then folds the partials into PathState::topkey.
 __global__ __launch_bounds__(BLOCK) void k_compact(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
                                                    const uint32_t *ev_kept, const uint32_t *ev_idx,
                                                    const int32_t *run_first, const uint32_t *run_path,
                                                    uint32_t *run_start, uint32_t *run_end, I2 *pts, uint32_t *pt_run,
-                                                   const PathState *ps, unsigned long long *top_rep, DevStatus *st) {
+                                                   PathState *ps, unsigned long long *top_rep, DevStatus *st) {
     if (run_failed(st)) return;
     const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
     uint32_t bad = 0;
@@ -340,8 +343,9 @@ __global__ __launch_bounds__(BLOCK) void k_compact(Limits L0, const uint32_t *ev
         block_reduce_keyed<unsigned long long>(
             c.path != NO_IDX, c.path, c.key, NO_KEY,
             [](unsigned long long a, unsigned long long b) { return a < b ? a : b; },
-            [top_rep, &L](uint32_t k, unsigned long long v) {
-                if (k < L.n_paths) atomicMin(&top_rep[(size_t)k * TOP_REPS + (blockIdx.x & (TOP_REPS - 1))], v);
+            [ps, top_rep, &L](uint32_t k, unsigned long long v) {
+                if (k < L.n_paths)
+                    atomicMin(top_rep ? &top_rep[(size_t)k * TOP_REPS + (blockIdx.x & (TOP_REPS - 1))] : &ps[k].topkey, v);
             });
     }
     if (bad) raise_err(st, FT_EDEVICE);
@@ -449,7 +453,7 @@ __global__ __launch_bounds__(BLOCK) void k_edges(Limits L0, uint32_t band_shift,
             rec.y = 0;
             if (i < n) {
                 const EdgeOut r = edge_point<DevAtom>(i, L, pt_run, run_path, run_start, run_end, paths, pts, edges,
-                                                      ps, bad, top_rep);
+                                                      ps, bad, top_rep, TOP_REPS);
                 rec = band_record(r);
                 pt_band[i] = rec;
             }
@@ -1662,6 +1666,9 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
     unsigned long long *lb = ws.d_scan_state;  // look-back words: 2 * tiles per scan slot
     DevStatus *st = ws.d_status;
     const uint32_t n_lb = (uint32_t)(2 * tiles * SCAN_SLOTS);
+    // partial top keys per path: batches of few paths spread their compaction
+    // blocks' atomics; with many paths they are spread already
+    unsigned long long *const top_rep = ws.n_paths < TOP_FEW_PATHS ? ws.d_top_rep : nullptr;
     // Timing: the first and the raster kernels carry the events in their
     // dispatch (ev[0] at k_clear's start, ev[1] / ev[2] around the raster), so
     // no marker packet stalls the stream between kernels.
@@ -1766,11 +1773,11 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
                                                      ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first, ws.d_pstate, st);
         k_compact<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first,
                                        ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_pts, ws.d_pt_run, ws.d_pstate,
-                                       ws.d_top_rep, st);
+                                       top_rep, st);
         const bool wide = ws.cap_events >= BIN_WIDE_MIN;
         (wide ? k_edges<4> : k_edges<1>)<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_run, ws.d_run_path, ws.d_run_start, ws.d_run_end,
                                      ws.d_paths, ws.d_pts, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_pstate,
-                                     ws.d_top_rep, st);
+                                     top_rep, st);
         if (ws.n_bands) {
             // pairs per (band, replica) -> list segment offsets and the pair total
             const uint32_t n_slots = (uint32_t)(ws.n_bands << ws.band_shift);

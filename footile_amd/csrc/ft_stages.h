@@ -729,12 +729,12 @@ struct EdgeOut {
     bool banded;      // that edge touches a raster row
 };
 
-// A path's top key may be kept as TOP_REPS partial minima (the device
+// A path's top key may be kept as `reps` partial minima (the device
 // compaction spreads its atomics over them); top_fold is their minimum.
 constexpr uint32_t TOP_REPS = 8;
-FT_HD inline unsigned long long top_fold(const unsigned long long *rep) {
+FT_HD inline unsigned long long top_fold(const unsigned long long *rep, uint32_t reps) {
     unsigned long long m = rep[0];
-    for (uint32_t k = 1; k < TOP_REPS; ++k) m = rep[k] < m ? rep[k] : m;
+    for (uint32_t k = 1; k < reps; ++k) m = rep[k] < m ? rep[k] : m;
     return m;
 }
 
@@ -754,7 +754,7 @@ struct EdgeEval {
 FT_HD inline EdgeEval edge_eval(uint32_t i, const Limits &L, const uint32_t *pt_run, const uint32_t *run_path,
                                 const uint32_t *run_start, const uint32_t *run_end, const PathDev *paths,
                                 const I2 *pts, const PathState *ps, uint32_t &bad,
-                                const unsigned long long *top_rep = nullptr) {
+                                const unsigned long long *top_rep = nullptr, uint32_t top_reps = 1) {
     // point i is stored iff pt_run[i] names a run (compact_event writes NO_IDX for capped points)
     const uint32_t run0 = pt_run[i];
     const bool rok = run0 < L.n_runs;
@@ -773,7 +773,7 @@ FT_HD inline EdgeEval edge_eval(uint32_t i, const Limits &L, const uint32_t *pt_
     r.E = make_edge(q, q1);
     if (!has) r.E.flags = 0;
     r.fold = top_rep != nullptr;
-    r.topkey = r.fold ? top_fold(top_rep + (size_t)p * TOP_REPS) : s.topkey;
+    r.topkey = r.fold ? top_fold(top_rep + (size_t)p * top_reps, top_reps) : s.topkey;
     r.top = valid & (point_key(q) == r.topkey);
     r.top_local = i - s.pt_base;
     r.out.banded = edge_bands(r.E, paths[p], r.topkey, L, r.out.g0, r.out.g1) & has;
@@ -794,8 +794,9 @@ template <class A>
 FT_HD inline EdgeOut edge_point(uint32_t i, const Limits &L, const uint32_t *pt_run, const uint32_t *run_path,
                                 const uint32_t *run_start, const uint32_t *run_end, const PathDev *paths,
                                 const I2 *pts, EdgeRec *edges, PathState *ps, uint32_t &bad,
-                                const unsigned long long *top_rep = nullptr) {
-    const EdgeEval r = edge_eval(i, L, pt_run, run_path, run_start, run_end, paths, pts, ps, bad, top_rep);
+                                const unsigned long long *top_rep = nullptr, uint32_t top_reps = 1) {
+    const EdgeEval r =
+        edge_eval(i, L, pt_run, run_path, run_start, run_end, paths, pts, ps, bad, top_rep, top_reps);
     edge_commit<A>(i, r, edges, ps);
     return r.out;
 }

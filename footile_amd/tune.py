@@ -20,7 +20,7 @@ import numpy as np
 from ._lib import FootileError
 
 BAND_ROWS = (4, 8, 16, 32, 64)
-LDS_CELLS = (2048, 4096, 8192, 16384)
+LDS_CELLS = (2048, 3072, 4096, 8192, 16384)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "footile_amd", "tuning.json")
 
 
