@@ -571,8 +571,10 @@ ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, flo
     k_vertices<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, reinterpret_cast<const float2 *>(points), d->d_status);
     k_rows<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, d->d_status);
     const uint32_t nc = (uint32_t)d->n_cells;
-    scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_cnt, nullptr, nc, nc, d->d_lb, nullptr, d->d_status, false, true);
-    scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_off, nullptr, nc, nc, d->d_lb + 2 * d->cap_tiles, &d->d_status->n_pairs, d->d_status, false, false);
+    scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_cnt, nullptr, nc, nc, d->d_lb,
+                                 nullptr, d->d_status, false, true);
+    scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_off, nullptr, nc, nc,
+                                 d->d_lb + 2 * d->cap_tiles, &d->d_status->n_pairs, d->d_status, false, false);
     k_fill_cells<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, d->d_status);
     k_sort_cells<<<grid_of(d->n_cells), BLOCK, 0, s>>>(S, d->d_status);
     FD_HIP(hipGetLastError());

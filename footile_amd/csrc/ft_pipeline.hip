@@ -80,9 +80,8 @@ __device__ inline void block_reduce_keyed(bool has, uint32_t key, T v, T identit
 // Stage 0: clear per-run state (a kernel, not hipMemsetAsync: graph-safe).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, unsigned long long *top_rep, uint32_t n_paths,
-                                                 uint32_t *band_cnt,
-                                                 uint32_t *band_cur, uint32_t n_slots, unsigned long long *lb,
-                                                 uint32_t n_lb, DevStatus *st) {
+                                                 uint32_t *band_cnt, uint32_t *band_cur, uint32_t n_slots,
+                                                 unsigned long long *lb, uint32_t n_lb, DevStatus *st) {
     uint32_t n = n_paths * TOP_REPS > n_slots ? n_paths * TOP_REPS : n_slots;
     n = n > n_lb ? n : n_lb;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
@@ -344,8 +343,10 @@ __global__ __launch_bounds__(BLOCK) void k_compact(Limits L0, const uint32_t *ev
             c.path != NO_IDX, c.path, c.key, NO_KEY,
             [](unsigned long long a, unsigned long long b) { return a < b ? a : b; },
             [ps, top_rep, &L](uint32_t k, unsigned long long v) {
-                if (k < L.n_paths)
-                    atomicMin(top_rep ? &top_rep[(size_t)k * TOP_REPS + (blockIdx.x & (TOP_REPS - 1))] : &ps[k].topkey, v);
+                if (k >= L.n_paths) return;
+                unsigned long long *const at =
+                    top_rep ? &top_rep[(size_t)k * TOP_REPS + (blockIdx.x & (TOP_REPS - 1))] : &ps[k].topkey;
+                atomicMin(at, v);
             });
     }
     if (bad) raise_err(st, FT_EDEVICE);
@@ -1115,8 +1116,8 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     extern __shared__ uint4 s_dyn[];
     __shared__ uint32_t s_segsum[BLOCK / 64];
     constexpr bool HAS_LUT = FMT != FMT_MATTE8;
-    // The front LUT_BYTES hold the cell pass's edge slots, then the composite
-    // table; the rows' cells follow.
+    // The front front_bytes<FMT>() hold the cell pass's edge slots, then the
+    // composite table; the rows' cells follow.
     constexpr uint32_t CR = cell_round<FMT>();
     uint4 *const lut = s_dyn;
     EdgeRec *const s_edge = reinterpret_cast<EdgeRec *>(s_dyn);
@@ -1203,7 +1204,8 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
             pp.q3 = dst_load<FMT_GRAYA8P, 8>(row, xb, xe, 3, nck);
         }
     }
-    for (uint32_t c = threadIdx.x; c < rows * Wp / 4; c += BLOCK) reinterpret_cast<int4 *>(s_cells)[c] = make_int4(0, 0, 0, 0);
+    for (uint32_t c = threadIdx.x; c < rows * Wp / 4; c += BLOCK)
+        reinterpret_cast<int4 *>(s_cells)[c] = make_int4(0, 0, 0, 0);
     __syncthreads();
     const int fwd = (int)S.fwd;
     const int32_t shift = top_row < 0 ? top_row : 0;
@@ -1402,7 +1404,8 @@ ft_status ws_init(Workspace &ws) {
 }
 
 void ws_free(Workspace &ws) {
-    void *bufs[] = {ws.d_ops, ws.d_op_path, ws.d_op_order, ws.d_paths, ws.d_band_path, ws.d_op_cnt, ws.d_op_off, ws.d_sub_off,
+    void *bufs[] = {ws.d_ops, ws.d_op_path, ws.d_op_order, ws.d_paths, ws.d_band_path, ws.d_op_cnt, ws.d_op_off,
+                    ws.d_sub_off,
                     ws.d_leaf, ws.d_op_pw,
                     ws.d_op_lastpw, ws.d_ev_xy, ws.d_ev_f, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_ev_last,
                     ws.d_st_flags, ws.d_st_off, ws.d_st_last, ws.d_spts, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
@@ -1679,7 +1682,8 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
     const uint32_t *lastpw = nullptr;
     if (mode != MODE_FILL && n_ops) {
         k_pen_width_mark<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, n_ops, ws.d_op_pw);
-        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_op_pw, ws.d_op_lastpw, nullptr, n_ops, n_ops, lb + 10 * tiles, nullptr, st, false, false);
+        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_op_pw, ws.d_op_lastpw, nullptr, n_ops, n_ops,
+                                    lb + 10 * tiles, nullptr, st, false, false);
         lastpw = ws.d_op_lastpw;
     }
     if (n_ops) {
@@ -1688,7 +1692,8 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         k_count<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
                                                                      lastpw, order, ws.d_op_cnt, ws.d_sub_off, leaf,
                                                                      ws.cap_ops * OP_LANES, st);
-        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops, lb, &st->op_total, st, false, false);
+        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops,
+                                               lb, &st->op_total, st, false, false);
         EventSink em{};
         em.cap_events = ws.cap_events;
         em.cap_runs = (uint32_t)ws.cap_events;
@@ -1714,14 +1719,18 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         sp.miter = ws.miter;
         sp.tol_sq = ws.n_paths ? ws.stroke_tol_sq : 0.0f;
         k_st_flags<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_flags, ws.d_st_last, st);
-        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_st_flags, ws.d_st_off, &st->n_events, 0, cap, lb + 12 * tiles, &st->st_total, st, true, false);
-        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_st_last, ws.d_st_last, &st->n_events, 0, cap, lb + 14 * tiles, nullptr, st, true, true);
+        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_st_flags, ws.d_st_off, &st->n_events, 0, cap,
+                                               lb + 12 * tiles, &st->st_total, st, true, false);
+        scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_st_last, ws.d_st_last, &st->n_events, 0, cap,
+                                    lb + 14 * tiles, nullptr, st, true, true);
         k_st_compact<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_off, ws.d_spts, ws.d_sub_start, ws.d_sub_event, cap, st);
         k_st_subs<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_last, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
                                        ws.d_sub_joined, cap, st);
         k_st_count<<<grid_for(cap_items), BLOCK, 0, s>>>(ws.d_spts, ws.d_sub_start, ws.d_sub_n, ws.d_sub_joined, sp,
                                                          ws.d_it_cnt, st);
-        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_it_cnt, ws.d_it_off, &st->st_items, 0, (uint32_t)cap_items, lb + 16 * tiles, &st->stroke_total, st, true, false);
+        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_it_cnt, ws.d_it_off, &st->st_items, 0,
+                                               (uint32_t)cap_items, lb + 16 * tiles, &st->stroke_total, st, true,
+                                               false);
         EventSink em{};
         em.cap_events = ws.cap_events;
         em.cap_runs = cap;
@@ -1746,7 +1755,8 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         L0.cap_points = (uint32_t)ws.cap_events;
         // The event items of the two ingest scans are computed inside the scans
         // while every scan block holds one tile, else by a pass of their own.
-        const bool in_scan = scan_grid_for<uint32_t, KeptSrc::ITEMS>(ws.cap_events, ws.scan_resident).tiles_per_block <= 1;
+        const bool in_scan =
+            scan_grid_for<uint32_t, KeptSrc::ITEMS>(ws.cap_events, ws.scan_resident).tiles_per_block <= 1;
         // pushed-point marks -> their running maximum (the last pushed point so far)
         if (in_scan) {
             const IngestSrc ing{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, 0u};
@@ -1775,16 +1785,17 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
                                        ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_pts, ws.d_pt_run, ws.d_pstate,
                                        top_rep, st);
         const bool wide = ws.cap_events >= BIN_WIDE_MIN;
-        (wide ? k_edges<4> : k_edges<1>)<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_run, ws.d_run_path, ws.d_run_start, ws.d_run_end,
-                                     ws.d_paths, ws.d_pts, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_pstate,
-                                     top_rep, st);
+        (wide ? k_edges<4> : k_edges<1>)<<<ge, BLOCK, 0, s>>>(
+            L0, ws.band_shift, ws.d_pt_run, ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_paths, ws.d_pts,
+            ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_pstate, top_rep, st);
         if (ws.n_bands) {
             // pairs per (band, replica) -> list segment offsets and the pair total
             const uint32_t n_slots = (uint32_t)(ws.n_bands << ws.band_shift);
-            scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_band_cnt, ws.d_band_off, nullptr, n_slots, n_slots, lb + 4 * tiles, &st->n_pairs, st, true, false);
-            (wide ? k_bin<4> : k_bin<1>)<<<ge, BLOCK, 0, s>>>(L0, ws.band_shift, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
-                                       ws.cap_pairs, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end,
-                                       ws.d_pstate, st);
+            scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_band_cnt, ws.d_band_off, nullptr, n_slots, n_slots,
+                                         lb + 4 * tiles, &st->n_pairs, st, true, false);
+            (wide ? k_bin<4> : k_bin<1>)<<<ge, BLOCK, 0, s>>>(
+                L0, ws.band_shift, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list, ws.cap_pairs, ws.d_pts,
+                ws.d_pt_run, ws.d_run_start, ws.d_run_end, ws.d_pstate, st);
             if (ws.n_rblocks) {
                 // one raster kernel per pixel format present; the events
                 // bracket the first and the last of them

@@ -255,8 +255,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan(Src src, T *out, const uint32_t 
                 top -= 64;
             }
             if (lane == 0)
-                __hip_atomic_store(&lb[2 * (size_t)b + 1], READY | (unsigned long long)scan_op_t<T, IS_MAX>(prefix, btot),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&lb[2 * (size_t)b + 1],
+                                   READY | (unsigned long long)scan_op_t<T, IS_MAX>(prefix, btot), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) {
             s_prefix = prefix;
