@@ -386,3 +386,32 @@ def test_long_curves_fine_tolerance(ft, oracle):
         assert g.last_counts() == o.last_counts()
         stored[tol] = o.last_counts()[1]
     assert stored[0.01] > 12 * 8 * 8  # far more leaves than 8 lanes x 8 kept points per curve
+
+
+def test_degenerate_inputs(ft, oracle):
+    """No ops, a lone move, a flat (horizontal-only) polygon, 1x1 and 1-wide
+    rasters, figures wholly off the raster, and coordinates that saturate
+    Fixed: every format and both rules, against the oracle."""
+    cases = [
+        ("no ops", 16, 16, []),
+        ("lone move", 16, 16, [(MOVE, (3.0, 4.0)), (CLOSE, ())]),
+        ("flat", 16, 16, [(MOVE, (1.0, 5.0)), (LINE, (12.0, 5.0)), (LINE, (7.0, 5.0)), (CLOSE, ())]),
+        ("1x1", 1, 1, [(MOVE, (0.2, 0.1)), (LINE, (0.9, 0.6)), (LINE, (0.1, 0.95)), (CLOSE, ())]),
+        ("1 wide", 1, 40, [(MOVE, (-3.0, -2.0)), (CUBIC, (5.0, 10.0, -4.0, 30.0, 2.0, 44.0)), (CLOSE, ())]),
+        ("left of raster", 24, 24, [(MOVE, (-30.0, 2.0)), (LINE, (-5.0, 20.0)), (LINE, (-20.0, 22.0)), (CLOSE, ())]),
+        ("right of raster", 24, 24, [(MOVE, (30.0, 2.0)), (LINE, (50.0, 20.0)), (LINE, (40.0, 22.0)), (CLOSE, ())]),
+        ("below raster", 24, 24, [(MOVE, (2.0, 30.0)), (LINE, (20.0, 50.0)), (LINE, (4.0, 40.0)), (CLOSE, ())]),
+        ("above raster", 24, 24, [(MOVE, (2.0, -30.0)), (LINE, (20.0, -5.0)), (LINE, (4.0, -12.0)), (CLOSE, ())]),
+        ("saturating", 32, 32, [(MOVE, (-1.0e6, -1.0e6)), (LINE, (1.0e6, 3.0e4)), (LINE, (16.0, 1.0e7)),
+                                (QUAD, (-5.0e5, 10.0, 8.0, 8.0)), (CLOSE, ())]),
+    ]
+    rng = np.random.default_rng(3)
+    for name, w, h, ops in cases:
+        a = ops_array(ops)
+        for fmt in (MATTE8, RGBA8P, GRAYA8P):
+            init = rng.integers(0, 256, size=w * h * BPP[fmt]).astype(np.uint8)
+            clr = (90, 60, 30, 200) if fmt == RGBA8P else (200, 220, 0, 0)
+            for rule in (0, 1):
+                g, o = compare_fill(ft, oracle, w, h, fmt, [("fill", rule, a, clr)], init=init,
+                                    tag=f"{name} fmt {fmt} rule {rule}")
+                assert g.last_counts() == o.last_counts(), name
