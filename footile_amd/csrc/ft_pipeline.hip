@@ -186,6 +186,67 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uin
     if (bad) raise_err(st, FT_EDEVICE);
 }
 
+// Flattening in one pass, for fill runs whose (op, lane) items fit one
+// resident grid at one item per thread and are taken in op order: a block
+// counts its items (keeping up to LEAF_CACHE points per lane), takes its
+// prefix by look-back over the blocks before it, and emits them.  Writes each
+// op's offset (op_off, as the op scan does) and the run's event and close
+// totals (as k_emit does).
+__global__ __launch_bounds__(BLOCK) void k_flatten(const ft_path_op *ops, const uint32_t *op_path,
+                                                   const PathDev *paths, uint32_t n_ops,
+                                                   unsigned long long *op_off, I2 *leaf, size_t stride,
+                                                   unsigned long long *lb, EventSink sink, DevStatus *st) {
+    __shared__ unsigned long long s_wave[BLOCK / 64];
+    __shared__ unsigned long long s_prefix;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t n_items = (uint64_t)n_ops * OP_LANES;
+    const uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const bool live = t < n_items;
+    const uint32_t i = live ? (uint32_t)(t >> FLAT_SPLIT) : 0u, s = (uint32_t)t & (OP_LANES - 1);
+    OpCount c = {0, 0, 0, 0};
+    if (live) c = count_op(ops, op_path, paths, i, s, MODE_FILL, nullptr, leaf + t, stride);
+    const unsigned long long v = live ? pack_count(c) : 0ull;
+    uint32_t leaves = c.leaves;
+#pragma unroll
+    for (uint32_t o = 1; o < OP_LANES; o <<= 1) leaves += __shfl_xor(leaves, o, OP_LANES);
+    if (live & (c.err != 0u)) raise_err(st, c.err);
+    if (live & (s == 0) & (leaves > FT_FLATTEN_MAX_LEAVES)) raise_err(st, FT_ERANGE);
+    // the block's exclusive prefix of the packed (events, closes) counts
+    const unsigned long long winc = wave_incl_t<unsigned long long, false>(v);
+    if (lane == 63) s_wave[wid] = winc;
+    __syncthreads();
+    unsigned long long wpre = 0, btot = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        wpre += w < wid ? s_wave[w] : 0ull;
+        btot += s_wave[w];
+    }
+    if (wid == 0) {
+        const unsigned long long prefix = lookback_prefix<unsigned long long, false>(lb, blockIdx.x, btot, st);
+        if (lane == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+    const unsigned long long off = s_prefix + wpre + winc - v;
+    if (live & (s == 0)) op_off[i] = off;
+    const uint64_t last_t = n_items - 1;
+    if ((uint64_t)blockIdx.x == last_t / BLOCK && threadIdx.x == 0) {  // the run's totals
+        const unsigned long long tot = s_prefix + btot;
+        const uint32_t n_events = (uint32_t)(tot >> 32);
+        st->op_total = tot;
+        st->n_events = n_events;
+        st->n_closes = (uint32_t)tot;
+        if (n_events > sink.cap_events) {
+            atomicOr(&st->overflow, 1u);
+            st->need_events = n_events;
+        }
+    }
+    // a lane whose events would pass the capacity writes nothing: the host
+    // grows the buffers and runs again
+    if (!live || (off >> 32) + c.ne > sink.cap_events || (uint32_t)off + c.nc > sink.cap_runs) return;
+    if (emit_op(ops, op_path, paths, i, s, MODE_FILL, nullptr, off, sink, leaf + t, c.leaves, stride))
+        raise_err(st, FT_EDEVICE);
+}
+
 // Limits of the current run, from the counts the earlier stages produced.
 __device__ inline Limits run_limits(const DevStatus *st, uint32_t n_ops, uint32_t n_paths, uint32_t n_bands,
                                     uint32_t cap_points) {
@@ -1478,6 +1539,12 @@ ft_status ws_upload_ops(Workspace &ws, const ft_path_op *ops, size_t n_ops) {
     return FT_OK;
 }
 
+// One-pass flattening (k_flatten): one item per thread, every block resident.
+static inline bool flatten_in_one_pass(const Workspace &ws, size_t n_ops) {
+    const size_t blocks = (n_ops * OP_LANES + BLOCK - 1) / BLOCK;
+    return ws.scan_resident != 0 && blocks <= ws.scan_resident;
+}
+
 ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const ft_fill_desc *paths,
                        size_t n_paths, uint32_t extra_flags) {
     if (n_ops >= 0x7FFFFFFFu || n_paths >= 0x7FFFFFFFu) return FT_EINVAL;
@@ -1584,10 +1651,12 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
         FT_HIP(hipMemcpyAsync(ws.d_ops, ops, n_ops * sizeof(ft_path_op), hipMemcpyHostToDevice, ws.stream));
         FT_HIP(hipMemcpyAsync(ws.d_op_path, op_path.data(), n_ops * 4, hipMemcpyHostToDevice, ws.stream));
     }
-    // Flattening visits the ops of a large batch cubics first, then quads,
-    // then the rest (stable within each), so that a wavefront's ops take the
-    // same branch.  Any order is correct: events land at their scanned offsets.
-    ws.op_ordered = n_ops >= OP_ORDER_MIN;
+    // Fill runs whose (op, lane) items fit one resident grid flatten in one
+    // pass (k_flatten), which takes them in op order.  Larger batches flatten
+    // their ops cubics first, then quads, then the rest (stable within each),
+    // so that a wavefront's ops take the same branch.  Any order is correct:
+    // events land at their scanned offsets.
+    ws.op_ordered = n_ops >= OP_ORDER_MIN && !flatten_in_one_pass(ws, n_ops);
     if (ws.op_ordered) {
         ws.h_op_order.resize(n_ops);
         size_t at = 0;
@@ -1689,11 +1758,6 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
     if (n_ops) {
         const uint32_t *order = ws.op_ordered ? ws.d_op_order : nullptr;
         I2 *const leaf = mode == MODE_FILL ? ws.d_leaf : nullptr;
-        k_count<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
-                                                                     lastpw, order, ws.d_op_cnt, ws.d_sub_off, leaf,
-                                                                     ws.cap_ops * OP_LANES, st);
-        scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_op_cnt, ws.d_op_off, nullptr, n_ops, n_ops,
-                                               lb, &st->op_total, st, false, false);
         EventSink em{};
         em.cap_events = ws.cap_events;
         em.cap_runs = (uint32_t)ws.cap_events;
@@ -1706,9 +1770,18 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         em.run_path = ws.d_run_path;
         em.run_start = ws.d_run_start;
         em.run_end = ws.d_run_end;
-        k_emit<<<grid_for((size_t)n_ops * OP_LANES), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode,
-                                                                    lastpw, order, ws.d_op_off, ws.d_sub_off, leaf,
-                                                                    ws.cap_ops * OP_LANES, em, st);
+        const size_t n_items = (size_t)n_ops * OP_LANES;
+        if (mode == MODE_FILL && !order && flatten_in_one_pass(ws, n_ops)) {
+            k_flatten<<<(uint32_t)((n_items + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
+                ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, ws.d_op_off, leaf, ws.cap_ops * OP_LANES, lb, em, st);
+        } else {
+            k_count<<<grid_for(n_items), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode, lastpw, order,
+                                                        ws.d_op_cnt, ws.d_sub_off, leaf, ws.cap_ops * OP_LANES, st);
+            scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_op_cnt, ws.d_op_off, nullptr, n_ops,
+                                                   n_ops, lb, &st->op_total, st, false, false);
+            k_emit<<<grid_for(n_items), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode, lastpw, order,
+                                                       ws.d_op_off, ws.d_sub_off, leaf, ws.cap_ops * OP_LANES, em, st);
+        }
     }
     if (mode == MODE_STROKE) {
         const uint32_t ge = grid_for(ws.cap_events);

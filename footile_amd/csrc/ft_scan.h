@@ -139,6 +139,65 @@ inline ScanGrid scan_grid_for(size_t n_cap, uint32_t resident) {
     return r;
 }
 
+// Decoupled look-back for block b, called by one whole wavefront: publishes
+// the block's reduction btot, returns the reduction of blocks 0 .. b-1 (to
+// every lane) and publishes the block's inclusive prefix.  lb[2 j] is block
+// j's aggregate and lb[2 j + 1] its inclusive prefix, each LB_READY | value
+// (zero before the run); lane l reads block top - l, 64 blocks per round trip.
+constexpr unsigned long long LB_READY = 1ull << 63;
+template <typename T, bool IS_MAX>
+__device__ inline T lookback_prefix(unsigned long long *lb, uint32_t b, T btot, DevStatus *st) {
+    const int lane = threadIdx.x & 63;
+    T prefix = 0;  // identity of + and of max over unsigned values
+    if (lane == 0 && ((unsigned long long)btot & LB_READY)) raise_err(st, FT_ERANGE);  // 63-bit values only
+    if (b == 0) {
+        if (lane == 0)
+            __hip_atomic_store(&lb[1], LB_READY | (unsigned long long)btot, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        return prefix;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&lb[2 * (size_t)b], LB_READY | (unsigned long long)btot, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    int64_t top = (int64_t)b - 1;
+    uint32_t spins = 0;
+    for (;;) {
+        const int64_t j = top - lane;
+        unsigned long long wi = LB_READY, wa = 0;  // past block 0: an inclusive zero
+        if (j >= 0) {
+            wi = __hip_atomic_load(&lb[2 * (size_t)j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            wa = __hip_atomic_load(&lb[2 * (size_t)j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const bool has_i = (wi & LB_READY) != 0, has_a = (wa & LB_READY) != 0;
+        const unsigned long long im = __ballot(has_i), zm = __ballot(!has_i & !has_a);
+        const int fi = im ? __ffsll((long long)im) - 1 : 64;  // nearest inclusive
+        const unsigned long long need = fi >= 63 ? ~0ull : ((2ull << fi) - 1ull);
+        if (zm & need) {
+            if (++spins > SPIN_LIMIT) {  // never expected: report, do not hang
+                if (lane == 0) {
+                    atomicOr(&st->spin_fail, 1u);
+                    raise_err(st, FT_EDEVICE);
+                }
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        T vv = 0;
+        if (lane <= fi) vv = (T)((has_i ? wi : wa) & ~LB_READY);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) vv = scan_op_t<T, IS_MAX>(vv, __shfl_xor(vv, o, 64));
+        prefix = scan_op_t<T, IS_MAX>(prefix, vv);
+        if (fi < 64) break;
+        top -= 64;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&lb[2 * (size_t)b + 1],
+                           LB_READY | (unsigned long long)scan_op_t<T, IS_MAX>(prefix, btot), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    return prefix;
+}
+
 // Items of a scan read from an array.  A scan's source gives item i's value;
 // `first` is true on the one visit of pass 1, where a source may also store
 // per-item side outputs.  bind() runs after the status check; `bad` set by a
@@ -206,59 +265,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan(Src src, T *out, const uint32_t 
     T btot = 0;
 #pragma unroll
     for (int w = 0; w < BLOCK / 64; ++w) btot = scan_op_t<T, IS_MAX>(btot, s_wave[w]);
-    // the chunk's prefix; lb[2 j] = block j's aggregate, lb[2 j + 1] its
-    // inclusive prefix, each READY | value (zero before the scan)
-    constexpr unsigned long long READY = 1ull << 63;
-    const uint32_t b = blockIdx.x;
+    // the chunk's prefix
     if (wid == 0) {
-        T prefix = 0;  // identity of + and of max over unsigned values
-        if (lane == 0 && ((unsigned long long)btot & READY)) raise_err(st, FT_ERANGE);  // 63-bit values only
-        if (b == 0) {
-            if (lane == 0)
-                __hip_atomic_store(&lb[1], READY | (unsigned long long)btot, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            if (lane == 0)
-                __hip_atomic_store(&lb[2 * (size_t)b], READY | (unsigned long long)btot, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            // lane l reads block top - l: its inclusive word and its aggregate
-            int64_t top = (int64_t)b - 1;
-            uint32_t spins = 0;
-            for (;;) {
-                const int64_t j = top - lane;
-                unsigned long long wi = READY, wa = 0;  // past block 0: an inclusive zero
-                if (j >= 0) {
-                    wi = __hip_atomic_load(&lb[2 * (size_t)j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    wa = __hip_atomic_load(&lb[2 * (size_t)j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                const bool has_i = (wi & READY) != 0, has_a = (wa & READY) != 0;
-                const unsigned long long im = __ballot(has_i), zm = __ballot(!has_i & !has_a);
-                const int fi = im ? __ffsll((long long)im) - 1 : 64;  // nearest inclusive
-                const unsigned long long need = fi >= 63 ? ~0ull : ((2ull << fi) - 1ull);
-                if (zm & need) {
-                    if (++spins > SPIN_LIMIT) {  // never expected: report, do not hang
-                        if (lane == 0) {
-                            atomicOr(&st->spin_fail, 1u);
-                            raise_err(st, FT_EDEVICE);
-                        }
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                T vv = 0;
-                if (lane <= fi) vv = (T)((has_i ? wi : wa) & ~READY);
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) vv = scan_op_t<T, IS_MAX>(vv, __shfl_xor(vv, o, 64));
-                prefix = scan_op_t<T, IS_MAX>(prefix, vv);
-                if (fi < 64) break;
-                top -= 64;
-            }
-            if (lane == 0)
-                __hip_atomic_store(&lb[2 * (size_t)b + 1],
-                                   READY | (unsigned long long)scan_op_t<T, IS_MAX>(prefix, btot), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
+        const T prefix = lookback_prefix<T, IS_MAX>(lb, blockIdx.x, btot, st);
         if (lane == 0) {
             s_prefix = prefix;
             if (t1 == n_tiles && total) *total = scan_op_t<T, IS_MAX>(prefix, btot);
