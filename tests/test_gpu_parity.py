@@ -415,3 +415,27 @@ def test_degenerate_inputs(ft, oracle):
                 g, o = compare_fill(ft, oracle, w, h, fmt, [("fill", rule, a, clr)], init=init,
                                     tag=f"{name} fmt {fmt} rule {rule}")
                 assert g.last_counts() == o.last_counts(), name
+
+
+def test_batch_band_pair_overflow_grows_and_reruns(ft, oracle):
+    """Long near-vertical edges over one-row bands give far more (edge, band)
+    pairs than the first run's list capacity: the run reports the overflow,
+    the batch grows its lists and runs again, and the raster is the oracle's."""
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    w, h = 1024, 3000
+    ops = []
+    for k in range(6):
+        x = 40.0 + 160.0 * k
+        ops += [(MOVE, (x, 1.0)), (LINE, (x + 70.0, 2.0)), (LINE, (x + 90.0, 2999.0)), (LINE, (x + 5.0, 2990.0)),
+                (CLOSE, ())]
+    a = ops_array(ops)
+    buf = DeviceBuffer(w * h)
+    buf.upload(np.full(w * h, 9, dtype=np.uint8))
+    batch = Batch(a, [FillDesc(0, len(a), buf.ptr, w, h, w, MATTE8, rule=0, fresh=True)])
+    batch.set_tuning(1, 1024)
+    batch.run()
+    o = oracle.Plotter(w, h)
+    assert o.fill(0, a) == 0
+    assert np.array_equal(buf.download(), o.pixels())
+    offered, stored, _ = batch.counts()
+    assert (offered[0], stored[0]) == o.last_counts()
