@@ -115,12 +115,16 @@ __global__ __launch_bounds__(BLOCK) void k_pen_width_mark(const ft_path_op *ops,
         op_pw[i] = ops[i].tag == TAG_PEN_WIDTH ? i + 1 : 0u;
 }
 
+constexpr unsigned long long SUB_CACHED = 1ull << 30, SUB_LAST_CLOSE = 1ull << 31;
+
 // One thread per (op, lane); an op's OP_LANES lanes are adjacent lanes of one
 // wavefront.  Writes the op's event/close count and each lane's offset within
 // the op (a prefix over the op's lanes).  In fill runs a curve lane also
 // keeps its first LEAF_CACHE points in leaf[k * stride + t] (coalesced
-// across the lanes), and its leaf count rides in bits 16-31 of its offset
-// (the op's close count, at most 2, uses the low bits).
+// across the lanes).  The offset's low word carries the lane's facts for
+// k_emit above the op's close count (at most 2, bits 0-15): its leaf count
+// (bits 16-29), SUB_CACHED (its points are all in the cache) and
+// SUB_LAST_CLOSE (it emits the path's final close).
 __global__ __launch_bounds__(BLOCK) void k_count(const ft_path_op *ops, const uint32_t *op_path,
                                                  const PathDev *paths, uint32_t n_ops, uint32_t mode,
                                                  const uint32_t *lastpw, const uint32_t *order,
@@ -144,8 +148,10 @@ __global__ __launch_bounds__(BLOCK) void k_count(const ft_path_op *ops, const ui
             if (s >= o) inc += u;
             leaves += __shfl_xor(leaves, o, OP_LANES);
         }
-        const unsigned long long lc = c.leaves < 0xFFFFu ? c.leaves : 0xFFFFu;
-        sub_off[t] = (inc - v) | (lc << 16);
+        const unsigned long long lc = c.leaves < 0x3FFFu ? c.leaves : 0x3FFFu;
+        const bool cached = leaf && (c.flags & OPC_CURVE) && !c.err && c.leaves <= LEAF_CACHE;
+        sub_off[t] = (inc - v) | (lc << 16) | (cached ? SUB_CACHED : 0ull) |
+                     ((c.flags & OPC_LAST_CLOSE) ? SUB_LAST_CLOSE : 0ull);
         if (s == OP_LANES - 1) op_cnt[i] = inc;
         if (c.err) raise_err(st, c.err);
         if (s == 0 && leaves > FT_FLATTEN_MAX_LEAVES) raise_err(st, FT_ERANGE);
@@ -179,9 +185,28 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uin
             continue;
         }
         const unsigned long long so = sub_off[t];
-        const uint32_t cached = (uint32_t)(so >> 16) & 0xFFFFu;
-        bad |= emit_op(ops, op_path, paths, i, s, mode, lastpw, op_off[i] + (so & ~(0xFFFFull << 16)), sink,
-                       leaf ? leaf + t : nullptr, cached, stride);
+        const uint32_t cached = (uint32_t)(so >> 16) & 0x3FFFu;
+        const unsigned long long off = op_off[i] + (so & ~(0xFFFFull << 16));
+        if (so & SUB_CACHED) {  // a curve lane whose points are all cached: no op or path loads
+            EventSink em = sink;
+            em.e = off >> 32;
+            em.run = (uint32_t)off;
+            I2 q[LEAF_CACHE];
+#pragma unroll
+            for (uint32_t k = 0; k < LEAF_CACHE; ++k)
+                if (k < cached) q[k] = leaf[k * stride + t];
+#pragma unroll
+            for (uint32_t k = 0; k < LEAF_CACHE; ++k)
+                if (k < cached) em.point_fx(q[k]);
+            if (so & SUB_LAST_CLOSE) {
+                em.p = op_path[i];
+                em.close(true);
+            }
+            bad |= em.bad;
+            continue;
+        }
+        bad |= emit_op(ops, op_path, paths, i, s, mode, lastpw, off, sink, leaf ? leaf + t : nullptr, cached,
+                       stride);
     }
     if (bad) raise_err(st, FT_EDEVICE);
 }

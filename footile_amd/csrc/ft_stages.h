@@ -219,7 +219,9 @@ FT_HD inline Cubic cubic_of(const ft_path_op &o, WP pen, float sw, const PathDev
 // ---------------------------------------------------------------------------
 struct OpCount {
     uint32_t ne, nc, err, leaves;
+    uint32_t flags;  // OPC_CURVE: a fill's curve lane; OPC_LAST_CLOSE: it emits the path's final close
 };
+enum : uint32_t { OPC_CURVE = 1u, OPC_LAST_CLOSE = 2u };
 
 // Fill runs may keep each curve lane's first LEAF_CACHE points (as the
 // Fixed points add_point stores) in `cache` (point k at cache[k * stride]),
@@ -229,7 +231,7 @@ constexpr uint32_t LEAF_CACHE = 8;
 FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, const PathDev *paths, uint32_t i,
                               uint32_t s, uint32_t mode, const uint32_t *lastpw, I2 *cache = nullptr,
                               size_t stride = 0) {
-    OpCount c = {0, 0, 0, 0};
+    OpCount c = {0, 0, 0, 0, 0};
     const uint32_t p = op_path[i];
     if (p == NO_IDX) return c;
     const PathDev &P = paths[p];
@@ -259,6 +261,7 @@ FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, co
             c.err = rc;
         else
             c.ne = c.leaves;
+        c.flags |= mode == MODE_FILL ? OPC_CURVE : 0u;
         break;
     }
     case TAG_PEN_WIDTH: break;
@@ -267,6 +270,7 @@ FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, co
     if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count && s == OP_LANES - 1) {
         ++c.ne;
         ++c.nc;
+        c.flags |= OPC_LAST_CLOSE;
     }
     return c;
 }
