@@ -34,6 +34,12 @@ struct Workspace {
     uint32_t *d_op_path = nullptr;
     uint32_t *d_op_order = nullptr;  // flattening order of the ops (a permutation)
     std::vector<uint32_t> h_op_order;
+    // host staging of the layout's uploads (op -> path, paths, band -> path,
+    // raster blocks), alive until the next layout
+    std::vector<uint32_t> h_op_path, h_band_path;
+    std::vector<PathDev> h_paths;
+    std::vector<uint4> h_rblock;
+    bool uploads_pending = false;  // a layout's uploads may not have run yet (cleared by a stream sync)
     bool op_ordered = false;
     PathDev *d_paths = nullptr;
     uint32_t *d_band_path = nullptr;

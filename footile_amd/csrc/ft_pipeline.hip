@@ -1574,9 +1574,19 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
                        size_t n_paths, uint32_t extra_flags) {
     if (n_ops >= 0x7FFFFFFFu || n_paths >= 0x7FFFFFFFu) return FT_EINVAL;
     graph_drop(ws);
-    std::vector<uint32_t> op_path(n_ops, NO_IDX);
-    std::vector<PathDev> pd(n_paths);
-    std::vector<uint32_t> band_path;
+    // host staging kept in the workspace: the uploads below need not complete
+    // before this returns (the run that follows is ordered after them); the
+    // staging is only rewritten once the previous layout's uploads are done
+    if (ws.uploads_pending) FT_HIP(hipStreamSynchronize(ws.stream));
+    ws.uploads_pending = false;
+    std::vector<uint32_t> &op_path = ws.h_op_path;
+    std::vector<PathDev> &pd = ws.h_paths;
+    std::vector<uint32_t> &band_path = ws.h_band_path;
+    std::vector<uint4> &rblock = ws.h_rblock;
+    op_path.assign(n_ops, NO_IDX);
+    pd.assign(n_paths, PathDev());
+    band_path.clear();
+    rblock.clear();
     std::vector<uint4> rblock_f[3];  // per pixel format (ft_format), then concatenated
     uint32_t lds = 0;
     for (size_t k = 0; k < n_paths; ++k) {
@@ -1609,7 +1619,6 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
             }
         }
     }
-    std::vector<uint4> rblock;
     for (int f = 0; f < 3; ++f) {
         ws.rb_first[f] = (uint32_t)rblock.size();
         ws.rb_count[f] = (uint32_t)rblock_f[f].size();
@@ -1700,7 +1709,7 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     if (!rblock.empty())
         FT_HIP(hipMemcpyAsync(ws.d_rblock, rblock.data(), rblock.size() * sizeof(uint4), hipMemcpyHostToDevice,
                               ws.stream));
-    FT_HIP(hipStreamSynchronize(ws.stream));  // host vectors go out of scope
+    ws.uploads_pending = true;
     return FT_OK;
 }
 
@@ -1940,6 +1949,7 @@ ft_status ws_finish(Workspace &ws, bool read_pstate) {
         FT_HIP(hipMemcpyAsync(ws.h_pstate.data(), ws.d_pstate, ws.n_paths * sizeof(PathState),
                               hipMemcpyDeviceToHost, ws.stream));
     FT_HIP(hipStreamSynchronize(ws.stream));
+    ws.uploads_pending = false;
     for (int k = 0; k < ws.n_pending; ++k) {
         float total = 0.0f, raster = 0.0f;
         hipEvent_t *ev = ws.ev + 3 * k;
