@@ -245,16 +245,24 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
             if (ops[i].tag == TAG_PEN_WIDTH) m = (uint32_t)i + 1;
         }
     }
-    // stage 1: counts + exclusive scan
-    std::vector<unsigned long long> op_off(n_ops), sub_off(n_ops * OP_LANES);
+    // stage 1: counts + exclusive scan; fill runs keep each lane's first
+    // LEAF_CACHE points (point k of item t at leaf[k * n_items + t]), as the
+    // device count pass does, and the emit copies lanes with no more leaves
+    const size_t n_items = n_ops * OP_LANES;
+    auto leaf = buf<I2>(mode == MODE_FILL ? n_items * LEAF_CACHE : 0);
+    std::vector<uint32_t> lane_leaves(n_items);
+    std::vector<unsigned long long> op_off(n_ops), sub_off(n_items);
     unsigned long long tot = 0;
     for (size_t i = 0; i < n_ops; ++i) {
         op_off[i] = tot;
         uint32_t leaves = 0;
         for (uint32_t s = 0; s < OP_LANES; ++s) {
+            const size_t t = i * OP_LANES + s;
             OpCount c = count_op(ops, op_path.data(), paths.data(), (uint32_t)i, s, mode,
-                                 mode == MODE_FILL ? nullptr : lastpw.data());
+                                 mode == MODE_FILL ? nullptr : lastpw.data(),
+                                 mode == MODE_FILL ? leaf.data() + t : nullptr, n_items);
             if (c.err) return (int)c.err;
+            lane_leaves[t] = c.leaves;
             sub_off[i * OP_LANES + s] = tot - op_off[i];
             tot += pack_count(c);
             leaves += c.leaves;
@@ -285,7 +293,9 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
     for (size_t i = 0; i < n_ops; ++i)
         for (uint32_t s = 0; s < OP_LANES; ++s)
             bad |= emit_op(ops, op_path.data(), paths.data(), (uint32_t)i, s, mode,
-                           mode == MODE_FILL ? nullptr : lastpw.data(), op_off[i] + sub_off[i * OP_LANES + s], sink);
+                           mode == MODE_FILL ? nullptr : lastpw.data(), op_off[i] + sub_off[i * OP_LANES + s], sink,
+                           mode == MODE_FILL ? leaf.data() + i * OP_LANES + s : nullptr, lane_leaves[i * OP_LANES + s],
+                           n_items);
     if (bad) return FT_EDEVICE;
     if (mode == MODE_STROKE_FLATTEN) {
         if (n_ev) *n_ev = n_events;
