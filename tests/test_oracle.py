@@ -7,6 +7,9 @@ Checks, in order of strength:
   * closed forms: integer-aligned rectangles, half-pixel edges, holes by
     opposite winding, exact-integer translation invariance, and agreement
     with FP64 exact-area coverage;
+  * the Rgba8p/Graya8p composite: pix Ch8 multiplication over all pairs, and
+    SrcOver properties on partial coverage (premultiplied stays
+    premultiplied, zero and full coverage, a real-valued bound);
   * flattening: every emitted point is the Bezier curve at an increasing
     dyadic parameter, checked against an independent FP64 Bernstein
     evaluation of the same subdivision rule.
@@ -258,6 +261,81 @@ def test_coverage_agrees_with_exact_area(oracle):
         assert (img[nb_max <= 0.0] == 0).all()
         total = _clip_area(pts, -1, -1, 33, 33) * 255
         assert abs(img.sum() - total) <= 0.10 * total
+
+
+def test_ch8_mul_is_a_rounded_product(oracle):
+    """pix's Ch8 multiplication, which SrcOver applies at src/fig.rs:642,663,
+    approximates x*y/255.  Over all 65,536 pairs it is zero at 0, exact at
+    255*255, commutative, monotone in each operand, and within one step of
+    the real product (x*255 rounds down by at most 1).  A wrong widening or
+    shift fails these.  The exact rounding of pix 0.14 is what they leave
+    open."""
+    L = oracle.lib()
+    m = np.array([[L.or_ch8_mul(a, b) for b in range(256)] for a in range(256)], dtype=np.int64)
+    x = np.arange(256)
+    exact = np.outer(x, x) / 255.0
+    assert (m[0] == 0).all() and (m[:, 0] == 0).all() and m[255, 255] == 255
+    assert (m == m.T).all()
+    assert (np.diff(m, axis=0) >= 0).all() and (np.diff(m, axis=1) >= 0).all()
+    assert np.abs(m - exact).max() <= 1.0
+    assert ((m[:, 255] == x) | (m[:, 255] == x - 1)).all()
+
+
+def _convex_polygon(rng, w, h):
+    n = int(rng.integers(3, 9))
+    cx, cy = rng.uniform(7, w - 7), rng.uniform(7, h - 7)
+    r, t0 = rng.uniform(3.5, 6.5), rng.uniform(0, 2 * np.pi)
+    return [(float(np.float32(cx + r * np.cos(t0 + 2 * np.pi * k / n))),
+             float(np.float32(cy + r * np.sin(t0 + 2 * np.pi * k / n)))) for k in range(n)]
+
+
+def _premultiplied(rng, n, nch):
+    a = rng.integers(0, 256, size=(n, 1))
+    c = np.minimum((rng.random((n, nch - 1)) * (a + 1)).astype(np.int64), a)
+    return np.concatenate([c, a], axis=1).astype(np.uint8)
+
+
+@pytest.mark.parametrize("fmt", ["rgba8p", "graya8p"])
+@pytest.mark.parametrize("rule", [0, 1])
+def test_src_over_properties(oracle, fmt, rule):
+    """SrcOver of a premultiplied colour at the scan's coverage
+    (src/fig.rs:637-664), over random premultiplied destinations.  The
+    coverage of each pixel is read from a Matte8 fill of the same convex
+    polygon on a fresh raster (the same accumulation).  Then:
+    * channels never exceed alpha (a wrong alpha index breaks this);
+    * no coverage leaves the destination, at most one step darker;
+    * full coverage of an opaque colour gives the colour, at most one step
+      darker (a missing coverage factor or swapped operands breaks these);
+    * every pixel is within 3 of the real-valued s*c + d*(1 - sa*c),
+      saturated at 255 (a dropped term breaks this)."""
+    F, nch = (oracle.RGBA8P, 4) if fmt == "rgba8p" else (oracle.GRAYA8P, 2)
+    rng = np.random.default_rng(29 + rule)
+    w = h = 24
+    for trial in range(16):
+        pts = _convex_polygon(rng, w, h)
+        m = oracle.Plotter(w, h)
+        assert m.fill(rule, poly_ops(pts)) == 0
+        cov = m.pixels().reshape(h, w).astype(np.int64)
+        clr = _premultiplied(rng, 1, nch)[0]
+        if trial % 2 == 0:
+            clr[-1] = 255
+        dst = _premultiplied(rng, w * h, nch).reshape(h, w, nch)
+        p = oracle.Plotter(w, h, F)
+        p.set_pixels(dst)
+        assert p.fill(rule, poly_ops(pts), tuple(int(v) for v in clr)) == 0
+        out = p.pixels().reshape(h, w, nch).astype(np.int64)
+        d = dst.astype(np.int64)
+        s = clr.astype(np.int64)
+        assert (cov == 255).any() and ((cov > 0) & (cov < 255)).any()
+        assert (out[..., :-1] <= out[..., -1:]).all()
+        z = cov == 0
+        assert ((out[z] <= d[z]) & (out[z] >= d[z] - 1)).all()
+        if clr[-1] == 255:
+            f = cov == 255
+            assert ((out[f] <= s) & (out[f] >= s - 1)).all()
+        a = cov[..., None] / 255.0
+        real = np.minimum(s * a + d * (1.0 - s[-1] / 255.0 * a), 255.0)
+        assert np.abs(out - real).max() <= 3.0, (trial, np.abs(out - real).max())
 
 
 def _bezier(ctrl, t):
