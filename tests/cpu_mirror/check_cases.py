@@ -143,6 +143,20 @@ def main(quick=False):
         join = [(0, 4.0), (1, 0.0), (2, 0.0), (0, 1.5)][t % 4]
         check_stroke(f"stroke{t} join{join}", 96, 96, MATTE8, ops, join=join)
     check_stroke("empty stroke", 32, 32, MATTE8, ops_array([(MOVE, (3.0, 3.0)), (CLOSE, ())]))
+    # rotations, shears, reflections and translations
+    rng = np.random.default_rng(91)
+    for t in range(4 if quick else 8):
+        a = rng.uniform(0, 2 * np.pi)
+        sx, sy = rng.uniform(0.6, 1.4, size=2) * rng.choice([-1.0, 1.0], size=2)
+        sh = rng.uniform(-0.5, 0.5)
+        m = np.array([[np.cos(a), -np.sin(a)], [np.sin(a), np.cos(a)]]) @ np.array([[sx, sh], [0.0, sy]])
+        off = np.array([64.0, 64.0]) - m @ np.array([48.0, 48.0]) + rng.uniform(-8, 8, size=2)
+        e = tuple(float(v) for v in np.float32([m[0, 0], m[0, 1], off[0], m[1, 0], m[1, 1], off[1]]))
+        ops = random_path(rng, 96, 96, n_sub=2, kinds=(LINE, QUAD, CUBIC), margin=-8.0, close_prob=0.5)
+        ops = np.concatenate([ops_array([(PEN_WIDTH, (float(rng.uniform(1, 6)),))]), ops])
+        fmt, clr = (MATTE8, (255, 255, 255, 255)) if t % 2 == 0 else (RGBA8P, (150, 90, 40, 200))
+        check(f"affine{t}", 128, 128, fmt, t % 2, ops, clr, transform=e)
+        check_stroke(f"affine stroke{t}", 128, 128, fmt, ops, clr, transform=e)
     # the 65,535-point cap and its close-pop interplay
     rng = np.random.default_rng(8)
     n = 66000

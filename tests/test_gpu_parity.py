@@ -161,6 +161,30 @@ def test_transform_tolerance_and_state(ft, oracle):
                  tol=0.05, tag="fishyp")
 
 
+def test_general_affine_transforms(ft, oracle):
+    """Rotations, shears, reflections and translations (src/plotter.rs:168-172)
+    applied to curve paths: the device evaluates e0*x + e1*y + e2 in the same
+    f32 order as the oracle, so fills and strokes stay bit-exact."""
+    rng = np.random.default_rng(91)
+    for t in range(8):
+        a = rng.uniform(0, 2 * np.pi)
+        sx, sy = rng.uniform(0.6, 1.4, size=2) * rng.choice([-1.0, 1.0], size=2)
+        sh = rng.uniform(-0.5, 0.5)
+        c, s = np.cos(a), np.sin(a)
+        m = np.array([[c, -s], [s, c]]) @ np.array([[sx, sh], [0.0, sy]])
+        off = np.array([64.0, 64.0]) - m @ np.array([48.0, 48.0]) + rng.uniform(-8, 8, size=2)
+        e = tuple(float(v) for v in np.float32([m[0, 0], m[0, 1], off[0], m[1, 0], m[1, 1], off[1]]))
+        ops = random_path(rng, 96, 96, n_sub=2, kinds=(LINE, QUAD, CUBIC), margin=-8.0, close_prob=0.5)
+        ops = np.concatenate([ops_array([(PEN_WIDTH, (float(rng.uniform(1, 6)),))]), ops])
+        if t % 2 == 0:
+            fmt, clr = MATTE8, (255,)
+        else:
+            fmt, clr = RGBA8P, (150, 90, 40, 200)
+        compare_fill(ft, oracle, 128, 128, fmt,
+                     [("fill", t % 2, ops, clr), ("stroke", ops, clr), ("fill", 1 - t % 2, ops, clr)],
+                     transform=e, tag=f"affine{t} {e}")
+
+
 def test_stroke_joins(ft, oracle):
     rng = np.random.default_rng(5)
     for t in range(12):

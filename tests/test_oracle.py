@@ -263,6 +263,45 @@ def test_coverage_agrees_with_exact_area(oracle):
         assert abs(img.sum() - total) <= 0.10 * total
 
 
+def test_transform_layout(oracle):
+    """The user-to-pixel transform (src/plotter.rs:168-172, pointy
+    `Transform * Pt`) is x' = e0*x + e1*y + e2, y' = e3*x + e4*y + e5.
+    Maps exact in f32 fix the element layout: a dyadic translation, a quarter
+    turn, the diagonal mirror and a reflecting scale.  A transposed matrix or
+    a translation in the wrong slot fails them.  Random affine maps agree
+    with FP64 to f32 rounding.  The last check renders an integer rectangle
+    under a quarter turn, which must land as the rotated rectangle, exactly."""
+    rng = np.random.default_rng(41)
+    pts = [(float(x), float(y)) for x, y in rng.integers(-160, 160, size=(9, 2)) / 8.0]
+    exact = [((1, 0, 3.25, 0, 1, -5.5), lambda x, y: (x + 3.25, y - 5.5)),
+             ((0, -1, 20, 1, 0, 0), lambda x, y: (20 - y, x)),
+             ((0, 1, 0, 1, 0, 0), lambda x, y: (y, x)),
+             ((2, 0, 0.5, 0, -1, 4), lambda x, y: (2 * x + 0.5, 4 - y))]
+    for e, f in exact:
+        p = oracle.Plotter(64, 64)
+        p.set_transform(e)
+        rc, out = p.flatten(poly_ops(pts))
+        assert rc == 0
+        got = out[~np.isnan(out[:, 0]), :2]
+        assert np.array_equal(got, np.array([f(x, y) for x, y in pts], dtype=np.float32)), e
+    for _ in range(20):
+        e = rng.uniform(-3, 3, size=6).astype(np.float32)
+        p = oracle.Plotter(64, 64)
+        p.set_transform(e)
+        rc, out = p.flatten(poly_ops(pts))
+        got = out[~np.isnan(out[:, 0]), :2].astype(np.float64)
+        q = np.array(pts)
+        E = e.astype(np.float64)
+        want = np.stack([E[0] * q[:, 0] + E[1] * q[:, 1] + E[2], E[3] * q[:, 0] + E[4] * q[:, 1] + E[5]], 1)
+        assert np.abs(got - want).max() <= 1e-5 * (1 + np.abs(want).max())
+    p = oracle.Plotter(32, 24)
+    p.set_transform((0, -1, 24, 1, 0, 0))  # (x, y) -> (24 - y, x)
+    assert p.fill(0, ops_array(_rect(3, 4, 17, 9))) == 0
+    want = np.zeros((24, 32), np.uint8)
+    want[3:17, 15:20] = 255
+    assert (p.pixels().reshape(24, 32) == want).all()
+
+
 def test_ch8_mul_is_a_rounded_product(oracle):
     """pix's Ch8 multiplication, which SrcOver applies at src/fig.rs:642,663,
     approximates x*y/255.  Over all 65,536 pairs it is zero at 0, exact at
