@@ -195,6 +195,23 @@ def test_stroke_joins(ft, oracle):
         compare_fill(ft, oracle, 96, 96, MATTE8, [("stroke", ops, (255,))], join=join, tag=f"stroke{t}")
 
 
+def test_stroke_join_geometry(ft, oracle):
+    """The geometric stroke cases that pin the oracle's stroker (closed
+    squares, an acute V either side of its miter limit 1.8028, random round
+    turns): the device stroker gives the same bytes."""
+    square = [(MOVE, (10.0, 10.0)), (LINE, (40.0, 10.0)), (LINE, (40.0, 30.0)), (LINE, (10.0, 30.0)), (CLOSE, ())]
+    vee = [(MOVE, (4.0, 30.0)), (LINE, (20.0, 6.0)), (LINE, (36.0, 30.0))]
+    for shape, width, w, h in [(square, 10.0, 52, 42), (vee, 4.0, 40, 36)]:
+        ops = ops_array([(PEN_WIDTH, (width,))] + shape)
+        for join in [(0, 4.0), (0, 1.85), (0, 1.75), (0, 1.5), (0, 1.3), (1, 0.0), (2, 0.0)]:
+            compare_fill(ft, oracle, w, h, MATTE8, [("stroke", ops, (255,))], join=join, tag=f"join {join}")
+    rng = np.random.default_rng(3)
+    for t in range(24):
+        pts = [tuple(float(v) for v in rng.integers(6, 42, size=2)) for _ in range(3)]
+        ops = ops_array([(PEN_WIDTH, (float(rng.integers(4, 14)),)), (MOVE, pts[0]), (LINE, pts[1]), (LINE, pts[2])])
+        compare_fill(ft, oracle, 48, 48, MATTE8, [("stroke", ops, (255,))], join=(2, 0.0), tag=f"turn{t} {pts}")
+
+
 def test_sequential_fills_keep_state(ft, oracle):
     rng = np.random.default_rng(77)
     for fmt in (MATTE8, RGBA8P):
