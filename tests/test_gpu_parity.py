@@ -195,6 +195,22 @@ def test_stroke_joins(ft, oracle):
         compare_fill(ft, oracle, 96, 96, MATTE8, [("stroke", ops, (255,))], join=join, tag=f"stroke{t}")
 
 
+def test_maximum_width(ft, oracle):
+    """The widest raster the device path takes, FT_MAX_WIDTH = 16,384 px (one
+    row of i32 cells is 64 KiB of LDS), in every format with both rules and a
+    stroke; one pixel wider is FT_ENOTSUP, returned before any device work."""
+    rng = np.random.default_rng(16384)
+    w, h = 16384, 6
+    for t, fmt in enumerate((MATTE8, RGBA8P, GRAYA8P)):
+        ops = random_path(rng, w, h, n_sub=2, kinds=(LINE, QUAD, CUBIC), margin=20.0, close_prob=0.7)
+        clr = {MATTE8: (255,), RGBA8P: (200, 120, 60, 230), GRAYA8P: (90, 200)}[fmt]
+        compare_fill(ft, oracle, w, h, fmt, [("fill", 0, ops, clr), ("stroke", ops, clr), ("fill", 1, ops, clr)],
+                     tag=f"width {w} fmt {fmt}")
+    with pytest.raises(ft.FootileError) as e:
+        ft.Plotter(ft.Raster.with_clear(w + 1, 4))
+    assert e.value.code == 5
+
+
 def test_stroke_join_geometry(ft, oracle):
     """The geometric stroke cases that pin the oracle's stroker (closed
     squares, an acute V either side of its miter limit 1.8028, random round
