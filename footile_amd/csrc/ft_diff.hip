@@ -30,6 +30,7 @@ constexpr int DSEG = 16;          // segments per cubic
 constexpr float EPS_W = 1e-5f;    // narrower pieces use the midpoint value
 constexpr int DBLOCK = 128;       // pixels of one row per block
 constexpr int KSEG = 32;          // paths per checkpoint segment
+constexpr uint32_t VCACHE = 256;  // vertex gradients a backward block gathers in LDS per path
 
 struct DScene {
     uint32_t n_images, W, H, n_paths;
@@ -365,8 +366,11 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                                                      float4 *ck, uint64_t ck_stride, float2 *gvtx, float *gcol,
                                                      DevStatus *st) {
     __shared__ float s_A[KSEG][DBLOCK];  // coverage of the segment's paths; each thread uses its own column
+    __shared__ float2 s_gv[VCACHE];      // the current path's vertex gradients from this block's pixels
     if (run_failed(st)) return;
     const int tid = threadIdx.x, lane = tid & 63;
+    for (uint32_t v = tid; v < VCACHE; v += DBLOCK) s_gv[v] = make_float2(0.0f, 0.0f);
+    __syncthreads();
     const int X = (int)(blockIdx.x * DBLOCK + tid), Y = (int)blockIdx.y;
     const uint32_t im = blockIdx.z;
     const bool active = (uint32_t)X < S.W;
@@ -426,29 +430,52 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
             const float keep = 1.0f - cc.w * al;
             g = make_float4(g.x * keep, g.y * keep, g.z * keep, g.w * keep);
             const float dA = active ? dal * dadA : 0.0f;
-            if (!__ballot(dA != 0.0f)) continue;
+            // k and Y are the same for the whole block, so cnt, V and
+            // `gather` are too: the barriers below are reached by every thread
             const uint64_t c = (uint64_t)k * S.H + (uint32_t)Y;
             const uint32_t cnt = S.cell_cnt[c], off = S.cell_off[c];
             const uint32_t vb = S.vtx_off[k], V = S.vtx_off[k + 1] - vb;
-            for (uint32_t q = 0; q < cnt; ++q) {
-                const uint32_t e = S.cell_edge[off + q];
-                bad |= e < V ? 0u : 1u;
-                const uint32_t i = e < V ? e : 0u;
-                const uint32_t i1 = i + 1 < V ? i + 1 : 0;
-                const float2 p = S.vtx[vb + i], pq = S.vtx[vb + i1];
-                const Piece r = piece_of(p, pq, Y);
-                if (!r.live) continue;
-                float mua, mub;
-                const float M = mean_clamp((float)X + 1.0f - r.xa, (float)X + 1.0f - r.xb, mua, mub);
-                const float S1 = wave_sum(dA * M), S2 = wave_sum(dA * mua), S3 = wave_sum(dA * mub);
-                if (lane == 0) {
-                    float gr[4];
-                    piece_grad(p, pq, Y, S1, S2, S3, gr);
-                    atomicAdd(&gvtx[vb + i].x, gr[0]);
-                    atomicAdd(&gvtx[vb + i].y, gr[1]);
-                    atomicAdd(&gvtx[vb + i1].x, gr[2]);
-                    atomicAdd(&gvtx[vb + i1].y, gr[3]);
+            const bool gather = V <= VCACHE;
+            if (__ballot(dA != 0.0f)) {
+                for (uint32_t q = 0; q < cnt; ++q) {
+                    const uint32_t e = S.cell_edge[off + q];
+                    bad |= e < V ? 0u : 1u;
+                    const uint32_t i = e < V ? e : 0u;
+                    const uint32_t i1 = i + 1 < V ? i + 1 : 0;
+                    const float2 p = S.vtx[vb + i], pq = S.vtx[vb + i1];
+                    const Piece r = piece_of(p, pq, Y);
+                    if (!r.live) continue;
+                    float mua, mub;
+                    const float M = mean_clamp((float)X + 1.0f - r.xa, (float)X + 1.0f - r.xb, mua, mub);
+                    const float S1 = wave_sum(dA * M), S2 = wave_sum(dA * mua), S3 = wave_sum(dA * mub);
+                    if (lane == 0) {
+                        float gr[4];
+                        piece_grad(p, pq, Y, S1, S2, S3, gr);
+                        if (gather) {  // i, i1 < V <= VCACHE
+                            atomicAdd(&s_gv[i].x, gr[0]);
+                            atomicAdd(&s_gv[i].y, gr[1]);
+                            atomicAdd(&s_gv[i1].x, gr[2]);
+                            atomicAdd(&s_gv[i1].y, gr[3]);
+                        } else {
+                            atomicAdd(&gvtx[vb + i].x, gr[0]);
+                            atomicAdd(&gvtx[vb + i].y, gr[1]);
+                            atomicAdd(&gvtx[vb + i1].x, gr[2]);
+                            atomicAdd(&gvtx[vb + i1].y, gr[3]);
+                        }
+                    }
                 }
+            }
+            if (gather & (cnt != 0)) {  // one global atomic per touched vertex and block
+                __syncthreads();
+                for (uint32_t v = tid; v < V; v += DBLOCK) {
+                    const float2 a = s_gv[v];
+                    if ((a.x != 0.0f) | (a.y != 0.0f)) {
+                        atomicAdd(&gvtx[vb + v].x, a.x);
+                        atomicAdd(&gvtx[vb + v].y, a.y);
+                        s_gv[v] = make_float2(0.0f, 0.0f);
+                    }
+                }
+                __syncthreads();
             }
         }
     }
