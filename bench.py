@@ -316,7 +316,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
-              "(the rasteriser is measured on one GPU's share; see DESIGN.md for sharding)",
+              "(the rasteriser is measured on one GPU's share of a sharded job)",
               file=sys.stderr)
         sys.exit(2)
 
