@@ -367,9 +367,11 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                                                      DevStatus *st) {
     __shared__ float s_A[KSEG][DBLOCK];  // coverage of the segment's paths; each thread uses its own column
     __shared__ float2 s_gv[VCACHE];      // the current path's vertex gradients from this block's pixels
+    __shared__ float s_gc[4];            // and its colour gradient
     if (run_failed(st)) return;
     const int tid = threadIdx.x, lane = tid & 63;
     for (uint32_t v = tid; v < VCACHE; v += DBLOCK) s_gv[v] = make_float2(0.0f, 0.0f);
+    if (tid < 4) s_gc[tid] = 0.0f;
     __syncthreads();
     const int X = (int)(blockIdx.x * DBLOCK + tid), Y = (int)blockIdx.y;
     const uint32_t im = blockIdx.z;
@@ -420,11 +422,11 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
             if (__ballot(al != 0.0f)) {
                 const float r0 = wave_sum(g.x * al), r1 = wave_sum(g.y * al), r2 = wave_sum(g.z * al);
                 const float r3 = wave_sum(g.w * al - al * gdotC);
-                if (lane == 0) {
-                    atomicAdd(&gcol[4 * (uint64_t)k + 0], r0);
-                    atomicAdd(&gcol[4 * (uint64_t)k + 1], r1);
-                    atomicAdd(&gcol[4 * (uint64_t)k + 2], r2);
-                    atomicAdd(&gcol[4 * (uint64_t)k + 3], r3);
+                if (lane == 0) {  // al != 0 needs edges in this row: flushed below
+                    atomicAdd(&s_gc[0], r0);
+                    atomicAdd(&s_gc[1], r1);
+                    atomicAdd(&s_gc[2], r2);
+                    atomicAdd(&s_gc[3], r3);
                 }
             }
             const float keep = 1.0f - cc.w * al;
@@ -465,9 +467,16 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                     }
                 }
             }
-            if (gather & (cnt != 0)) {  // one global atomic per touched vertex and block
+            if (cnt != 0) {  // one global atomic per colour channel and touched vertex, per block
                 __syncthreads();
-                for (uint32_t v = tid; v < V; v += DBLOCK) {
+                if (tid < 4) {
+                    const float a = s_gc[tid];
+                    if (a != 0.0f) {
+                        atomicAdd(&gcol[4 * (uint64_t)k + tid], a);
+                        s_gc[tid] = 0.0f;
+                    }
+                }
+                for (uint32_t v = tid; gather && v < V; v += DBLOCK) {
                     const float2 a = s_gv[v];
                     if ((a.x != 0.0f) | (a.y != 0.0f)) {
                         atomicAdd(&gvtx[vb + v].x, a.x);
