@@ -29,7 +29,7 @@ namespace diff {
 constexpr int DSEG = 16;          // segments per cubic
 constexpr float EPS_W = 1e-5f;    // narrower pieces use the midpoint value
 constexpr int DBLOCK = 128;       // pixels of one row per block
-constexpr int KSEG = 32;          // paths per checkpoint segment
+constexpr int KSEG = 8;           // paths per checkpoint segment
 constexpr uint32_t VCACHE = 256;  // vertex gradients a backward block gathers in LDS per path
 
 struct DScene {
