@@ -31,6 +31,7 @@ constexpr float EPS_W = 1e-5f;    // narrower pieces use the midpoint value
 constexpr int DBLOCK = 128;       // pixels of one row per block
 constexpr int KSEG = 8;           // paths per checkpoint segment
 constexpr uint32_t VCACHE = 256;  // vertex gradients a backward block gathers in LDS per path
+constexpr uint64_t COV_MAX_BYTES = 16ull << 30;  // largest kept-coverage buffer
 
 struct DScene {
     uint32_t n_images, W, H, n_paths;
@@ -363,7 +364,8 @@ __global__ __launch_bounds__(BLOCK) void k_loss(const float *canvas, const float
 }
 
 __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *colours, const float4 *grad_canvas,
-                                                     float4 *ck, uint64_t ck_stride, float2 *gvtx, float *gcol,
+                                                     float4 *ck, uint64_t ck_stride, float *cov, float2 *gvtx,
+                                                     float *gcol,
                                                      DevStatus *st) {
     __shared__ float s_A[KSEG][DBLOCK];  // coverage of the segment's paths; each thread uses its own column
     __shared__ float2 s_gv[VCACHE];      // the current path's vertex gradients from this block's pixels
@@ -389,7 +391,9 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
         const uint32_t cnt = S.cell_cnt[c];
         if (!cnt) continue;
         float dadA;
-        const float al = alpha_of(path_area(S, k, S.cell_off[c], cnt, X, Y, bad), dadA);
+        const float A = path_area(S, k, S.cell_off[c], cnt, X, Y, bad);
+        if (cov) cov[(uint64_t)j * ck_stride + slot] = A;  // j < max_paths, slot < ck_stride
+        const float al = alpha_of(A, dadA);
         C = over(colours[k], al, C);
     }
     float4 g = active ? grad_canvas[((uint64_t)im * S.H + (uint32_t)Y) * S.W + (uint32_t)X]
@@ -402,7 +406,9 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
             const uint32_t k = k0 + j;
             const uint64_t c = (uint64_t)k * S.H + (uint32_t)Y;
             const uint32_t cnt = S.cell_cnt[c];
-            s_A[j - j0][tid] = cnt ? path_area(S, k, S.cell_off[c], cnt, X, Y, bad) : 0.0f;
+            s_A[j - j0][tid] = !cnt ? 0.0f
+                               : cov  ? cov[(uint64_t)j * ck_stride + slot]
+                                      : path_area(S, k, S.cell_off[c], cnt, X, Y, bad);
         }
         const float4 Cseg = ck[(uint64_t)seg * ck_stride + slot];
         for (uint32_t j = j1; j-- > j0;) {
@@ -549,6 +555,7 @@ struct ft_diff {
     unsigned long long *d_lb = nullptr;  // scan look-back words
     float4 *d_ck = nullptr;
     uint64_t ck_stride = 0;
+    float *d_cov = nullptr;  // [max_paths][ck_stride] coverage from the backward's replay, or null: recomputed
     double *d_partial = nullptr, *h_partial = nullptr;
     DevStatus *d_status = nullptr, *h_status = nullptr;
 };
@@ -670,7 +677,7 @@ ft_status run_backward(ft_diff *d, const float *points, const float *colours, co
         DScene S = scene_of(d);
         k_backward<<<raster_grid(d), DBLOCK, 0, d->stream>>>(
             S, reinterpret_cast<const float4 *>(colours), reinterpret_cast<const float4 *>(grad_canvas), d->d_ck,
-            d->ck_stride, d->d_gvtx, grad_colours, d->d_status);
+            d->ck_stride, d->d_cov, d->d_gvtx, grad_colours, d->d_status);
         k_point_grad<<<grid_of(d->n_pts), BLOCK, 0, d->stream>>>(S, d->d_gvtx, reinterpret_cast<float2 *>(grad_points),
                                                                  d->n_pts, d->d_pt_path);
         FD_HIP(hipGetLastError());
@@ -757,6 +764,14 @@ ft_status ft_diff_new(const ft_diff_layout *L, ft_diff **out) {
         st = FT_EDEVICE;
     else
         st = FT_OK;
+    // Coverage kept from the backward's forward replay (4 B per path and
+    // pixel) spares its second evaluation; without room for it the
+    // backward recomputes.
+    const uint64_t cov_elems = (uint64_t)max_paths * d->ck_stride;
+    if (!st && cov_elems && cov_elems <= COV_MAX_BYTES / sizeof(float) && dalloc(d->d_cov, cov_elems)) {
+        d->d_cov = nullptr;
+        (void)hipGetLastError();  // the failed optional allocation is not an error of a later call
+    }
     if (st) {
         ft_diff_free(d);
         return st;
@@ -769,7 +784,7 @@ void ft_diff_free(ft_diff *d) {
     if (!d) return;
     void *bufs[] = {d->d_pt_off,   d->d_vtx_off,  d->d_vtx_path, d->d_pt_path,   d->d_img_first, d->d_vtx,
                     d->d_gvtx,     d->d_cell_cnt, d->d_cell_off, d->d_cell_cur,  d->d_cell_edge, d->d_lb,
-                    d->d_ck,       d->d_partial,   d->d_status};
+                    d->d_ck,       d->d_cov,       d->d_partial,   d->d_status};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (d->h_partial) (void)hipHostFree(d->h_partial);
