@@ -322,19 +322,29 @@ __global__ __launch_bounds__(BLOCK) void k_sort_cells(DScene S, DevStatus *st) {
 // ---------------------------------------------------------------------------
 // Forward, loss, backward
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(DBLOCK) void k_forward(DScene S, const float4 *colours, float4 *canvas, DevStatus *st) {
+// With ck non-null it also leaves what k_backward's replay would write: the
+// canvas before every KSEG-th path, and each path's coverage when cov is
+// non-null (same slots as k_backward).
+__global__ __launch_bounds__(DBLOCK) void k_forward(DScene S, const float4 *colours, float4 *canvas, float4 *ck,
+                                                    uint64_t ck_stride, float *cov, DevStatus *st) {
     if (run_failed(st)) return;
     const int X = (int)(blockIdx.x * DBLOCK + threadIdx.x), Y = (int)blockIdx.y;
     const uint32_t im = blockIdx.z;
     const uint32_t k0 = S.img_first[im], k1 = S.img_first[im + 1];
+    const uint64_t slot = ((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * (gridDim.x * DBLOCK) +
+                          (uint64_t)blockIdx.x * DBLOCK + threadIdx.x;
     float4 C = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t bad = 0;
     for (uint32_t k = k0; k < k1; ++k) {
+        const uint32_t j = k - k0;
+        if (ck && j % KSEG == 0) ck[(uint64_t)(j / KSEG) * ck_stride + slot] = C;
         const uint64_t c = (uint64_t)k * S.H + (uint32_t)Y;
         const uint32_t cnt = S.cell_cnt[c];
         if (!cnt) continue;
         float dadA;
-        const float al = alpha_of(path_area(S, k, S.cell_off[c], cnt, X, Y, bad), dadA);
+        const float A = path_area(S, k, S.cell_off[c], cnt, X, Y, bad);
+        if (ck && cov) cov[(uint64_t)j * ck_stride + slot] = A;  // j < max_paths, slot < ck_stride
+        const float al = alpha_of(A, dadA);
         C = over(colours[k], al, C);
     }
     if (bad) raise_err(st, FT_EDEVICE);
@@ -364,8 +374,8 @@ __global__ __launch_bounds__(BLOCK) void k_loss(const float *canvas, const float
 }
 
 __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *colours, const float4 *grad_canvas,
-                                                     float4 *ck, uint64_t ck_stride, float *cov, float2 *gvtx,
-                                                     float *gcol,
+                                                     float4 *ck, uint64_t ck_stride, float *cov, uint32_t replayed,
+                                                     float2 *gvtx, float *gcol,
                                                      DevStatus *st) {
     __shared__ float s_A[KSEG][DBLOCK];  // coverage of the segment's paths; each thread uses its own column
     __shared__ float2 s_gv[VCACHE];      // the current path's vertex gradients from this block's pixels
@@ -382,9 +392,10 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                           (uint64_t)blockIdx.x * DBLOCK + tid;
     const uint32_t k0 = S.img_first[im], np = S.img_first[im + 1] - k0;
     // pass 1: forward, checkpointing the canvas before every KSEG-th path
+    // (skipped when k_forward of the same inputs has written both: `replayed`)
     float4 C = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t bad = 0;
-    for (uint32_t j = 0; j < np; ++j) {
+    for (uint32_t j = 0; !replayed && j < np; ++j) {
         if (j % KSEG == 0) ck[(uint64_t)(j / KSEG) * ck_stride + slot] = C;
         const uint32_t k = k0 + j;
         const uint64_t c = (uint64_t)k * S.H + (uint32_t)Y;
@@ -644,12 +655,14 @@ ft_status grow_pairs(ft_diff *d) {
 
 dim3 raster_grid(const ft_diff *d) { return dim3((d->W + DBLOCK - 1) / DBLOCK, d->H, d->n_images); }
 
-ft_status run_forward(ft_diff *d, const float *points, const float *colours, float *canvas) {
+// keep: also write the backward's checkpoints and coverage (a step's forward)
+ft_status run_forward(ft_diff *d, const float *points, const float *colours, float *canvas, bool keep) {
     for (int attempt = 0; attempt < 3; ++attempt) {
         ft_status st = launch_geometry(d, points, false, nullptr);
         if (st) return st;
         k_forward<<<raster_grid(d), DBLOCK, 0, d->stream>>>(scene_of(d), reinterpret_cast<const float4 *>(colours),
-                                                             reinterpret_cast<float4 *>(canvas), d->d_status);
+                                                             reinterpret_cast<float4 *>(canvas), keep ? d->d_ck : nullptr,
+                                                             d->ck_stride, d->d_cov, d->d_status);
         FD_HIP(hipGetLastError());
         st = finish(d);
         if (st != (ft_status)0x100) return st;
@@ -669,15 +682,16 @@ ft_status run_loss(ft_diff *d, const float *canvas, const float *target, float *
     return FT_OK;
 }
 
+// replayed: run_forward(keep) of the same points and colours came just before
 ft_status run_backward(ft_diff *d, const float *points, const float *colours, const float *grad_canvas,
-                       float *grad_points, float *grad_colours) {
+                       float *grad_points, float *grad_colours, bool replayed) {
     for (int attempt = 0; attempt < 3; ++attempt) {
         ft_status st = launch_geometry(d, points, true, grad_colours);
         if (st) return st;
         DScene S = scene_of(d);
         k_backward<<<raster_grid(d), DBLOCK, 0, d->stream>>>(
             S, reinterpret_cast<const float4 *>(colours), reinterpret_cast<const float4 *>(grad_canvas), d->d_ck,
-            d->ck_stride, d->d_cov, d->d_gvtx, grad_colours, d->d_status);
+            d->ck_stride, d->d_cov, replayed ? 1u : 0u, d->d_gvtx, grad_colours, d->d_status);
         k_point_grad<<<grid_of(d->n_pts), BLOCK, 0, d->stream>>>(S, d->d_gvtx, reinterpret_cast<float2 *>(grad_points),
                                                                  d->n_pts, d->d_pt_path);
         FD_HIP(hipGetLastError());
@@ -803,7 +817,7 @@ ft_status ft_diff_info(const ft_diff *d, uint64_t *n_points, uint64_t *n_paths, 
 
 ft_status ft_diff_forward(ft_diff *d, const float *points, const float *colours, float *canvas) {
     if (!d || !points || !colours || !canvas) return FT_EINVAL;
-    return run_forward(d, points, colours, canvas);
+    return run_forward(d, points, colours, canvas, false);
 }
 
 ft_status ft_diff_loss(ft_diff *d, const float *canvas, const float *target, float *grad_canvas, double *loss) {
@@ -814,16 +828,16 @@ ft_status ft_diff_loss(ft_diff *d, const float *canvas, const float *target, flo
 ft_status ft_diff_backward(ft_diff *d, const float *points, const float *colours, const float *grad_canvas,
                            float *grad_points, float *grad_colours) {
     if (!d || !points || !colours || !grad_canvas || !grad_points || !grad_colours) return FT_EINVAL;
-    return run_backward(d, points, colours, grad_canvas, grad_points, grad_colours);
+    return run_backward(d, points, colours, grad_canvas, grad_points, grad_colours, false);
 }
 
 ft_status ft_diff_step(ft_diff *d, const float *points, const float *colours, const float *target, float *canvas,
                        float *grad_canvas, float *grad_points, float *grad_colours, double *loss) {
     if (!d || !points || !colours || !target || !canvas || !grad_canvas || !grad_points || !grad_colours)
         return FT_EINVAL;
-    ft_status st = run_forward(d, points, colours, canvas);
+    ft_status st = run_forward(d, points, colours, canvas, true);
     if (!st) st = run_loss(d, canvas, target, grad_canvas, loss);
-    if (!st) st = run_backward(d, points, colours, grad_canvas, grad_points, grad_colours);
+    if (!st) st = run_backward(d, points, colours, grad_canvas, grad_points, grad_colours, true);
     return st;
 }
 
