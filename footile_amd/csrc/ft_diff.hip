@@ -239,14 +239,16 @@ __device__ inline void edge_at(const DScene &S, uint32_t v, uint32_t &k, float2 
     b = S.vtx[local + 1 < V ? v + 1 : vb];
 }
 
+// geometry = 0 keeps the cells and look-back words (a backward whose step's
+// forward has just built them) and clears only the gradients and the status.
 __global__ __launch_bounds__(BLOCK) void k_clear_cells(DScene S, float2 *gvtx, float *gcol, unsigned long long *lb,
-                                                       uint32_t n_lb, DevStatus *st) {
+                                                       uint32_t n_lb, uint32_t geometry, DevStatus *st) {
     uint32_t n = S.n_cells > S.n_vtx ? S.n_cells : S.n_vtx;
     n = n > 4 * S.n_paths ? n : 4 * S.n_paths;
     n = n > n_lb ? n : n_lb;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-        if (i < n_lb) lb[i] = 0;
-        if (i < S.n_cells) {
+        if (geometry && i < n_lb) lb[i] = 0;
+        if (geometry && i < S.n_cells) {
             S.cell_cnt[i] = 0;
             S.cell_cur[i] = 0;
         }
@@ -614,14 +616,20 @@ uint32_t grid_of(uint64_t n) {
 }
 
 // Geometry for one parameter set: vertices, per-(path, row) edge lists.
-ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, float *gcol) {
+// rebuild = false: the geometry of the same points is already in place.
+ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, float *gcol, bool rebuild = true) {
     hipStream_t s = d->stream;
     DScene S = scene_of(d);
     const uint32_t n_lb = (uint32_t)(4 * d->cap_tiles);  // look-back words of the two cell scans
     const uint64_t n_clear = std::max<uint64_t>(std::max<uint64_t>(d->n_cells, d->n_vtx),
                                                 std::max<uint64_t>(4ull * d->n_paths, n_lb));
     k_clear_cells<<<grid_of(n_clear), BLOCK, 0, s>>>(S, clear_grads ? d->d_gvtx : nullptr,
-                                                     clear_grads ? gcol : nullptr, d->d_lb, n_lb, d->d_status);
+                                                     clear_grads ? gcol : nullptr, d->d_lb, n_lb, rebuild ? 1u : 0u,
+                                                     d->d_status);
+    if (!rebuild) {
+        FD_HIP(hipGetLastError());
+        return FT_OK;
+    }
     k_vertices<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, reinterpret_cast<const float2 *>(points), d->d_status);
     k_rows<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, d->d_status);
     const uint32_t nc = (uint32_t)d->n_cells;
@@ -686,7 +694,7 @@ ft_status run_loss(ft_diff *d, const float *canvas, const float *target, float *
 ft_status run_backward(ft_diff *d, const float *points, const float *colours, const float *grad_canvas,
                        float *grad_points, float *grad_colours, bool replayed) {
     for (int attempt = 0; attempt < 3; ++attempt) {
-        ft_status st = launch_geometry(d, points, true, grad_colours);
+        ft_status st = launch_geometry(d, points, true, grad_colours, !replayed || attempt > 0);
         if (st) return st;
         DScene S = scene_of(d);
         k_backward<<<raster_grid(d), DBLOCK, 0, d->stream>>>(
