@@ -11,11 +11,12 @@
 //   k_forward    one lane per pixel: coverage of every path of the image from
 //                its cell's edges, composite, write the canvas
 //   k_loss       L2 against a target: loss partials and d loss / d canvas
-//   k_backward   one lane per pixel: replays the forward with canvas
-//                checkpoints every KSEG paths (coverage is recomputed, never
-//                stored per path), walks the paths backwards, and reduces the
-//                per-edge and per-colour gradients over the wavefront before
-//                one atomic add each
+//   k_backward   one lane per pixel: replays the forward into canvas
+//                checkpoints every KSEG paths and per-path coverage (inside
+//                ft_diff_step k_forward has written both), walks the paths
+//                backwards, reduces the per-edge and per-colour gradients over
+//                the wavefront and gathers them per block in LDS before one
+//                atomic add per touched vertex and colour channel
 //   k_point_grad vertex gradients -> control point gradients
 #include <algorithm>
 #include <cstring>

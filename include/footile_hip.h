@@ -214,11 +214,15 @@ ft_status ft_diff_info(const ft_diff *d, uint64_t *n_points, uint64_t *n_paths, 
 ft_status ft_diff_forward(ft_diff *d, const float *points, const float *colours, float *canvas);
 /* loss = mean((canvas - target)^2); grad_canvas (may be NULL) = d loss / d canvas */
 ft_status ft_diff_loss(ft_diff *d, const float *canvas, const float *target, float *grad_canvas, double *loss);
-/* Vector-Jacobian product: grad_canvas -> d/d points, d/d colours.  Coverage
- * is recomputed, never stored per path. */
+/* Vector-Jacobian product: grad_canvas -> d/d points, d/d colours.  Replays
+ * the forward into canvas checkpoints every 8 paths and a per-path coverage
+ * buffer (4 B per path and pixel, allocated by ft_diff_new when it fits in
+ * 16 GiB; otherwise coverage is recomputed). */
 ft_status ft_diff_backward(ft_diff *d, const float *points, const float *colours, const float *grad_canvas,
                            float *grad_points, float *grad_colours);
-/* forward + loss + backward (one optimisation step's gradient) */
+/* forward + loss + backward (one optimisation step's gradient); the
+ * forward leaves the checkpoints, coverage and geometry the backward needs,
+ * so the backward does not replay it */
 ft_status ft_diff_step(ft_diff *d, const float *points, const float *colours, const float *target, float *canvas,
                        float *grad_canvas, float *grad_points, float *grad_colours, double *loss);
 
