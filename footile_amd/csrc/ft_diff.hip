@@ -19,6 +19,7 @@
 //                atomic add per touched vertex and colour channel
 //   k_point_grad vertex gradients -> control point gradients
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -790,7 +791,10 @@ ft_status ft_diff_new(const ft_diff_layout *L, ft_diff **out) {
     // Coverage kept from the backward's forward replay (4 B per path and
     // pixel) spares its second evaluation; without room for it the
     // backward recomputes.
-    const uint64_t cov_elems = (uint64_t)max_paths * d->ck_stride;
+    // FT_DIFF_RECOMPUTE_COVERAGE=1 selects the recomputing path without the
+    // buffer (the path large layouts take), for tests and memory-tight callers.
+    const char *no_cov = std::getenv("FT_DIFF_RECOMPUTE_COVERAGE");
+    const uint64_t cov_elems = (no_cov && no_cov[0] == '1') ? 0 : (uint64_t)max_paths * d->ck_stride;
     if (!st && cov_elems && cov_elems <= COV_MAX_BYTES / sizeof(float) && dalloc(d->d_cov, cov_elems)) {
         d->d_cov = nullptr;
         (void)hipGetLastError();  // the failed optional allocation is not an error of a later call

@@ -124,6 +124,33 @@ def test_step_equals_forward_loss_backward(oracle):
     assert np.allclose(a, b, rtol=1e-5, atol=1e-7 * np.abs(b).max())
 
 
+def test_recomputed_coverage_matches_kept(oracle, monkeypatch):
+    """Without the per-path coverage buffer (the path layouts take when it
+    would exceed 16 GiB, forced here by FT_DIFF_RECOMPUTE_COVERAGE=1 at
+    ft_diff_new) the step and a separate backward give the same canvas, loss
+    and gradients as with it."""
+    from footile_amd.diff import device_array
+    from footile_amd.plotter import DeviceBuffer
+    sc = _scene(12)
+    out = []
+    for env in ("0", "1"):
+        monkeypatch.setenv("FT_DIFF_RECOMPUTE_COVERAGE", env)
+        r, P, C = _gpu(sc)
+        T = device_array(np.full(r.canvas_shape, 0.25, np.float32))
+        c, g = r.new_canvas(), r.new_canvas()
+        gp, gc = DeviceBuffer(8 * r.n_points), DeviceBuffer(16 * r.n_paths)
+        loss = r.step(P, C, T, c, g, gp, gc)
+        gp2, gc2 = DeviceBuffer(8 * r.n_points), DeviceBuffer(16 * r.n_paths)
+        r.backward(P, C, g, gp2, gc2)  # replays the forward
+        out.append((loss, c.download(), gp.download(dtype=np.float32), gc.download(dtype=np.float32),
+                    gp2.download(dtype=np.float32), gc2.download(dtype=np.float32)))
+    (l1, c1, *g1), (l2, c2, *g2) = out
+    assert l1 == l2 and np.array_equal(c1, c2)
+    for x in g1 + g2[2:]:
+        y = g2[0] if x.size == g2[0].size else g2[1]
+        assert np.allclose(x, y, rtol=1e-5, atol=1e-7 * np.abs(y).max())
+
+
 def test_empty_image_and_bad_layout(oracle):
     from footile_amd._lib import FootileError
     from footile_amd.diff import DiffRenderer, device_array
