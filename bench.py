@@ -220,10 +220,20 @@ def _diff_time(r, b, steps, warmup):
     return (time.perf_counter() - t0) / steps, loss
 
 
+def _grad_errors(got, want):
+    """Normwise ||g - g64|| / ||g64||, and the largest and mean elementwise
+    |g - g64| relative to max |g64| (an element's own |g64| can be ~0)."""
+    d = np.abs(got - want) / max(float(np.abs(want).max()), 1e-30)
+    return {"normwise": _grad_rel(got, want), "max_elementwise": float(d.max()),
+            "mean_elementwise": float(d.mean())}
+
+
 def bench_diff(steps, warmup, check=True):
     """Differentiable configs: C4 (one 512^2 image, 256 paths x 8 cubics) and
     C5 (64 images of 256^2, U[32,512] paths x U[4,16] cubics), fwd + loss +
-    bwd per step; gradients against the FP64 oracle."""
+    bwd per step (one ft_diff_step call, coverage recomputed in the backward);
+    gradients against the FP64 oracle.  The opt-in kept-coverage handle is
+    timed beside each as its own row."""
     from footile_amd import diffscene as ds
     from footile_amd.diff import DiffRenderer
     out = {}
@@ -233,8 +243,15 @@ def bench_diff(steps, warmup, check=True):
     b = _diff_buffers(r, sc)
     dt, loss = _diff_time(r, b, steps, warmup)
     c4 = {"workload": "C4: 512x512 fp32 RGBA, 256 closed paths x 8 cubics (seed 3), L2 to a target rendered "
-                      "from params + N(0,2px) / colours + N(0,0.05)",
+                      "from params + N(0,2px) / colours + N(0,0.05); one ft_diff_step, coverage recomputed in "
+                      "the backward",
           "grad_steps_per_s": 1.0 / dt, "ms_per_step": dt * 1e3, "loss": loss}
+    rk = DiffRenderer.for_scene(sc, keep_coverage=True)
+    bk = _diff_buffers(rk, sc)
+    dtk, _ = _diff_time(rk, bk, steps, warmup)
+    c4["kept_coverage_variant"] = {"grad_steps_per_s": 1.0 / dtk, "ms_per_step": dtk * 1e3,
+                                   "extra_bytes": 4 * sc.n_paths * sc.width * sc.height}
+    del rk, bk
     if check:
         from oracle import pydiff as pd
         target = b["T"].download(dtype=np.float32).reshape(r.canvas_shape)
@@ -244,18 +261,23 @@ def bench_diff(steps, warmup, check=True):
         c4["fp64_oracle_s"] = time.perf_counter() - t0
         gp = b["gp"].download(dtype=np.float32).reshape(-1, 2)
         gc = b["gc"].download(dtype=np.float32).reshape(-1, 4)
-        c4["grad_rel_err_points"] = _grad_rel(gp, want_gp)
-        c4["grad_rel_err_colours"] = _grad_rel(gc, want_gc)
-        c4["grad_max_abs_err_over_max_points"] = float(np.abs(gp - want_gp).max() / np.abs(want_gp).max())
+        c4["grad_err_points"] = _grad_errors(gp, want_gp)
+        c4["grad_err_colours"] = _grad_errors(gc, want_gc)
         c4["loss_rel_err"] = abs(loss - want_loss) / abs(want_loss)
     out["c4"] = c4
-    # C5: the batched launch vs one launch per image
+    # C5: the batched step vs one step per image
     sc = ds.scene_c5()
     r = DiffRenderer.for_scene(sc)
     b = _diff_buffers(r, sc)
     dt, loss = _diff_time(r, b, max(2, steps // 2), 1)
-    c5 = {"workload": "C5: 64 images 256x256 fp32 RGBA, U[32,512] paths x U[4,16] cubics (seed 4), one launch",
+    c5 = {"workload": "C5: 64 images 256x256 fp32 RGBA, U[32,512] paths x U[4,16] cubics (seed 4); all images "
+                      "in one ft_diff_step (one host wait), coverage recomputed in the backward",
           "grad_steps_per_s": 1.0 / dt, "ms_per_step": dt * 1e3, "paths": int(sc.n_paths)}
+    rk = DiffRenderer.for_scene(sc, keep_coverage=True)
+    bk = _diff_buffers(rk, sc)
+    dtk, _ = _diff_time(rk, bk, max(2, steps // 2), 1)
+    c5["kept_coverage_variant"] = {"grad_steps_per_s": 1.0 / dtk, "ms_per_step": dtk * 1e3}
+    del rk, bk
     # per-image launches of the same scenes
     per_image = 0.0
     tp, tc = ds.perturbed(sc)
@@ -277,23 +299,29 @@ def bench_diff(steps, warmup, check=True):
         for rs, bs, *_ in subs:
             rs.step(bs["P"], bs["C"], bs["T"], bs["canvas"], bs["gcan"], bs["gp"], bs["gc"])
     per_image = (time.perf_counter() - t0) / reps
-    c5["per_image_launch_ms_per_step"] = per_image * 1e3
-    c5["speedup_vs_per_image_launch"] = per_image / dt
+    c5["per_image_step_ms"] = per_image * 1e3
+    c5["speedup_vs_per_image_step"] = per_image / dt
     if check:
+        from concurrent.futures import ThreadPoolExecutor
         from oracle import pydiff as pd
         gp_all = b["gp"].download(dtype=np.float32).reshape(-1, 2)
         gc_all = b["gc"].download(dtype=np.float32).reshape(-1, 4)
         target = b["T"].download(dtype=np.float32).reshape(r.canvas_shape)
-        errs_p, errs_c = [], []
         scale = 1.0 / sc.n_images  # the batched loss is the mean over every image's channels
-        for i, (rs, bs, sub, k0, k1) in enumerate(subs[:2]):  # two images against FP64 (own params each)
+        checked = list(range(0, sc.n_images, 8))  # 8 images against FP64 (own params each)
+
+        def grads(i):
+            rs, bs, sub, k0, k1 = subs[i]
             _, want_gp, want_gc = pd.Scene(1, sub.width, sub.height, sub.path_image, sub.path_cubics).backward(
                 sub.points, sub.colours, target[i:i + 1])
-            want_gp, want_gc = want_gp * scale, want_gc * scale
-            errs_p.append(_grad_rel(gp_all[pt_first[k0]:pt_first[k1]], want_gp))
-            errs_c.append(_grad_rel(gc_all[k0:k1], want_gc))
-        c5["grad_rel_err_points_images_0_1"] = max(errs_p)
-        c5["grad_rel_err_colours_images_0_1"] = max(errs_c)
+            return (gp_all[pt_first[k0]:pt_first[k1]], want_gp * scale, gc_all[k0:k1], want_gc * scale)
+
+        with ThreadPoolExecutor(min(8, len(os.sched_getaffinity(0)))) as ex:
+            res = list(ex.map(grads, checked))
+        c5["grad_err_points"] = _grad_errors(np.concatenate([g[0] for g in res]), np.concatenate([g[1] for g in res]))
+        c5["grad_err_colours"] = _grad_errors(np.concatenate([g[2] for g in res]),
+                                              np.concatenate([g[3] for g in res]))
+        c5["grad_checked_images"] = checked
     out["c5"] = c5
     return out
 

@@ -10,16 +10,18 @@
 //   k_fill_cells, k_sort_cells   edge lists per cell, in edge order
 //   k_forward    one lane per pixel: coverage of every path of the image from
 //                its cell's edges, composite, write the canvas
-//   k_loss       L2 against a target: loss partials and d loss / d canvas
+//   k_loss, k_loss_sum   L2 against a target: d loss / d canvas and the loss,
+//                summed on the device in a fixed order
 //   k_backward   one lane per pixel: replays the forward into canvas
-//                checkpoints every KSEG paths and per-path coverage (inside
-//                ft_diff_step k_forward has written both), walks the paths
-//                backwards, reduces the per-edge and per-colour gradients over
-//                the wavefront and gathers them per block in LDS before one
-//                atomic add per touched vertex and colour channel
+//                checkpoints every KSEG paths (inside ft_diff_step k_forward
+//                has written them), walks each segment's paths backwards with
+//                their coverage recomputed into LDS (or read from the kept
+//                coverage buffer, FT_DIFF_KEEP_COVERAGE), reduces the per-edge
+//                and per-colour gradients over the wavefront and gathers them
+//                per block in LDS before one atomic add per touched vertex and
+//                colour channel
 //   k_point_grad vertex gradients -> control point gradients
 #include <algorithm>
-#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -241,16 +243,16 @@ __device__ inline void edge_at(const DScene &S, uint32_t v, uint32_t &k, float2 
     b = S.vtx[local + 1 < V ? v + 1 : vb];
 }
 
-// geometry = 0 keeps the cells and look-back words (a backward whose step's
-// forward has just built them) and clears only the gradients and the status.
+// Clears the cell counters, the scans' look-back words, the status and (when
+// given) the gradients.
 __global__ __launch_bounds__(BLOCK) void k_clear_cells(DScene S, float2 *gvtx, float *gcol, unsigned long long *lb,
-                                                       uint32_t n_lb, uint32_t geometry, DevStatus *st) {
+                                                       uint32_t n_lb, DevStatus *st) {
     uint32_t n = S.n_cells > S.n_vtx ? S.n_cells : S.n_vtx;
     n = n > 4 * S.n_paths ? n : 4 * S.n_paths;
     n = n > n_lb ? n : n_lb;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-        if (geometry && i < n_lb) lb[i] = 0;
-        if (geometry && i < S.n_cells) {
+        if (i < n_lb) lb[i] = 0;
+        if (i < S.n_cells) {
             S.cell_cnt[i] = 0;
             S.cell_cur[i] = 0;
         }
@@ -512,6 +514,23 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
     if (bad) raise_err(st, FT_EDEVICE);
 }
 
+// The loss partials summed in a fixed order by one block: the same value for
+// a step and for a separate ft_diff_loss of the same canvas.
+__global__ __launch_bounds__(BLOCK) void k_loss_sum(const double *partial, uint32_t n, double *out) {
+    __shared__ double s_w[BLOCK / 64];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) acc += partial[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < BLOCK / 64; ++w) t += s_w[w];
+        out[0] = t;
+    }
+}
+
 // Control point gradients from vertex gradients (one thread per point).
 __global__ __launch_bounds__(BLOCK) void k_point_grad(DScene S, const float2 *gvtx, float2 *gpts, uint32_t n_pts,
                                                       const uint32_t *pt_path) {
@@ -570,8 +589,9 @@ struct ft_diff {
     unsigned long long *d_lb = nullptr;  // scan look-back words
     float4 *d_ck = nullptr;
     uint64_t ck_stride = 0;
-    float *d_cov = nullptr;  // [max_paths][ck_stride] coverage from the backward's replay, or null: recomputed
-    double *d_partial = nullptr, *h_partial = nullptr;
+    float *d_cov = nullptr;  // [max_paths][ck_stride] kept coverage (FT_DIFF_KEEP_COVERAGE), or null: recomputed
+    double *d_partial = nullptr;  // [LOSS_GRID] loss partials, then [LOSS_GRID] their sum
+    double *h_loss = nullptr;     // pinned: the summed loss
     DevStatus *d_status = nullptr, *h_status = nullptr;
 };
 
@@ -618,20 +638,14 @@ uint32_t grid_of(uint64_t n) {
 }
 
 // Geometry for one parameter set: vertices, per-(path, row) edge lists.
-// rebuild = false: the geometry of the same points is already in place.
-ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, float *gcol, bool rebuild = true) {
+ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, float *gcol) {
     hipStream_t s = d->stream;
     DScene S = scene_of(d);
     const uint32_t n_lb = (uint32_t)(4 * d->cap_tiles);  // look-back words of the two cell scans
     const uint64_t n_clear = std::max<uint64_t>(std::max<uint64_t>(d->n_cells, d->n_vtx),
                                                 std::max<uint64_t>(4ull * d->n_paths, n_lb));
     k_clear_cells<<<grid_of(n_clear), BLOCK, 0, s>>>(S, clear_grads ? d->d_gvtx : nullptr,
-                                                     clear_grads ? gcol : nullptr, d->d_lb, n_lb, rebuild ? 1u : 0u,
-                                                     d->d_status);
-    if (!rebuild) {
-        FD_HIP(hipGetLastError());
-        return FT_OK;
-    }
+                                                     clear_grads ? gcol : nullptr, d->d_lb, n_lb, d->d_status);
     k_vertices<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, reinterpret_cast<const float2 *>(points), d->d_status);
     k_rows<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, d->d_status);
     const uint32_t nc = (uint32_t)d->n_cells;
@@ -645,8 +659,11 @@ ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, flo
     return FT_OK;
 }
 
-ft_status finish(ft_diff *d) {
+// The run's one host wait: status (and, with loss, the device-summed loss).
+ft_status finish(ft_diff *d, bool loss = false) {
     FD_HIP(hipMemcpyAsync(d->h_status, d->d_status, sizeof(DevStatus), hipMemcpyDeviceToHost, d->stream));
+    if (loss)
+        FD_HIP(hipMemcpyAsync(d->h_loss, d->d_partial + LOSS_GRID, sizeof(double), hipMemcpyDeviceToHost, d->stream));
     FD_HIP(hipStreamSynchronize(d->stream));
     if (d->h_status->err) return (ft_status)d->h_status->err;
     if (d->h_status->overflow) return (ft_status)0x100;
@@ -665,14 +682,13 @@ ft_status grow_pairs(ft_diff *d) {
 
 dim3 raster_grid(const ft_diff *d) { return dim3((d->W + DBLOCK - 1) / DBLOCK, d->H, d->n_images); }
 
-// keep: also write the backward's checkpoints and coverage (a step's forward)
-ft_status run_forward(ft_diff *d, const float *points, const float *colours, float *canvas, bool keep) {
+ft_status run_forward(ft_diff *d, const float *points, const float *colours, float *canvas) {
     for (int attempt = 0; attempt < 3; ++attempt) {
         ft_status st = launch_geometry(d, points, false, nullptr);
         if (st) return st;
         k_forward<<<raster_grid(d), DBLOCK, 0, d->stream>>>(scene_of(d), reinterpret_cast<const float4 *>(colours),
-                                                             reinterpret_cast<float4 *>(canvas), keep ? d->d_ck : nullptr,
-                                                             d->ck_stride, d->d_cov, d->d_status);
+                                                             reinterpret_cast<float4 *>(canvas), nullptr,
+                                                             d->ck_stride, nullptr, d->d_status);
         FD_HIP(hipGetLastError());
         st = finish(d);
         if (st != (ft_status)0x100) return st;
@@ -681,31 +697,68 @@ ft_status run_forward(ft_diff *d, const float *points, const float *colours, flo
     return FT_ENOMEM;
 }
 
-ft_status run_loss(ft_diff *d, const float *canvas, const float *target, float *grad, double *loss) {
+// Enqueues the loss partials and their fixed-order sum (at d_partial[LOSS_GRID]).
+ft_status launch_loss(ft_diff *d, const float *canvas, const float *target, float *grad) {
     k_loss<<<LOSS_GRID, BLOCK, 0, d->stream>>>(canvas, target, grad, d->canvas_floats, d->d_partial);
+    k_loss_sum<<<1, BLOCK, 0, d->stream>>>(d->d_partial, LOSS_GRID, d->d_partial + LOSS_GRID);
     FD_HIP(hipGetLastError());
-    FD_HIP(hipMemcpyAsync(d->h_partial, d->d_partial, LOSS_GRID * sizeof(double), hipMemcpyDeviceToHost, d->stream));
-    FD_HIP(hipStreamSynchronize(d->stream));
-    double t = 0.0;
-    for (uint32_t i = 0; i < LOSS_GRID; ++i) t += d->h_partial[i];
-    if (loss) *loss = t / (double)d->canvas_floats;
     return FT_OK;
 }
 
-// replayed: run_forward(keep) of the same points and colours came just before
+ft_status run_loss(ft_diff *d, const float *canvas, const float *target, float *grad, double *loss) {
+    ft_status st = launch_loss(d, canvas, target, grad);
+    if (!st) {
+        FD_HIP(hipMemcpyAsync(d->h_loss, d->d_partial + LOSS_GRID, sizeof(double), hipMemcpyDeviceToHost,
+                              d->stream));
+        FD_HIP(hipStreamSynchronize(d->stream));
+        if (loss) *loss = *d->h_loss / (double)d->canvas_floats;
+    }
+    return st;
+}
+
+// Enqueues the backward kernels.  replayed: k_forward with checkpoints of the
+// same points and colours ran just before on the same geometry.
+ft_status launch_backward(ft_diff *d, const float *colours, const float *grad_canvas, float *grad_points,
+                          float *grad_colours, bool replayed) {
+    DScene S = scene_of(d);
+    k_backward<<<raster_grid(d), DBLOCK, 0, d->stream>>>(
+        S, reinterpret_cast<const float4 *>(colours), reinterpret_cast<const float4 *>(grad_canvas), d->d_ck,
+        d->ck_stride, d->d_cov, replayed ? 1u : 0u, d->d_gvtx, grad_colours, d->d_status);
+    k_point_grad<<<grid_of(d->n_pts), BLOCK, 0, d->stream>>>(S, d->d_gvtx, reinterpret_cast<float2 *>(grad_points),
+                                                             d->n_pts, d->d_pt_path);
+    FD_HIP(hipGetLastError());
+    return FT_OK;
+}
+
 ft_status run_backward(ft_diff *d, const float *points, const float *colours, const float *grad_canvas,
-                       float *grad_points, float *grad_colours, bool replayed) {
+                       float *grad_points, float *grad_colours) {
     for (int attempt = 0; attempt < 3; ++attempt) {
-        ft_status st = launch_geometry(d, points, true, grad_colours, !replayed || attempt > 0);
+        ft_status st = launch_geometry(d, points, true, grad_colours);
+        if (!st) st = launch_backward(d, colours, grad_canvas, grad_points, grad_colours, false);
         if (st) return st;
-        DScene S = scene_of(d);
-        k_backward<<<raster_grid(d), DBLOCK, 0, d->stream>>>(
-            S, reinterpret_cast<const float4 *>(colours), reinterpret_cast<const float4 *>(grad_canvas), d->d_ck,
-            d->ck_stride, d->d_cov, replayed ? 1u : 0u, d->d_gvtx, grad_colours, d->d_status);
-        k_point_grad<<<grid_of(d->n_pts), BLOCK, 0, d->stream>>>(S, d->d_gvtx, reinterpret_cast<float2 *>(grad_points),
-                                                                 d->n_pts, d->d_pt_path);
-        FD_HIP(hipGetLastError());
         st = finish(d);
+        if (st != (ft_status)0x100) return st;
+        if ((st = grow_pairs(d))) return st;
+    }
+    return FT_ENOMEM;
+}
+
+// forward (leaving checkpoints), loss and backward on one geometry, with one
+// host wait; a pair overflow turns the later kernels into no-ops, and the
+// step is run again with room for the pairs.
+ft_status run_step(ft_diff *d, const float *points, const float *colours, const float *target, float *canvas,
+                   float *grad_canvas, float *grad_points, float *grad_colours, double *loss) {
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        ft_status st = launch_geometry(d, points, true, grad_colours);
+        if (st) return st;
+        k_forward<<<raster_grid(d), DBLOCK, 0, d->stream>>>(scene_of(d), reinterpret_cast<const float4 *>(colours),
+                                                             reinterpret_cast<float4 *>(canvas), d->d_ck,
+                                                             d->ck_stride, d->d_cov, d->d_status);
+        FD_HIP(hipGetLastError());
+        if ((st = launch_loss(d, canvas, target, grad_canvas))) return st;
+        if ((st = launch_backward(d, colours, grad_canvas, grad_points, grad_colours, true))) return st;
+        st = finish(d, true);
+        if (st == FT_OK && loss) *loss = *d->h_loss / (double)d->canvas_floats;
         if (st != (ft_status)0x100) return st;
         if ((st = grow_pairs(d))) return st;
     }
@@ -717,7 +770,8 @@ ft_status run_backward(ft_diff *d, const float *points, const float *colours, co
 extern "C" {
 
 ft_status ft_diff_new(const ft_diff_layout *L, ft_diff **out) {
-    if (!L || !out || !L->path_image || !L->path_cubics || !L->n_images || !L->width || !L->height)
+    if (!L || !out || !L->path_image || !L->path_cubics || !L->n_images || !L->width || !L->height ||
+        (L->flags & ~FT_DIFF_KEEP_COVERAGE))
         return FT_EINVAL;
     if (L->width > FT_MAX_WIDTH || L->height > FT_MAX_WIDTH) return FT_ENOTSUP;
     std::vector<uint32_t> pt_off(L->n_paths + 1), vtx_off(L->n_paths + 1), img_first(L->n_images + 1);
@@ -772,11 +826,11 @@ ft_status ft_diff_new(const ft_diff_layout *L, ft_diff **out) {
              (st = dalloc(d->d_gvtx, nv)) || (st = dalloc(d->d_cell_cnt, cells)) ||
              (st = dalloc(d->d_cell_off, cells)) || (st = dalloc(d->d_cell_cur, cells)) ||
              (st = dalloc(d->d_cell_edge, d->cap_pairs)) || (st = dalloc(d->d_lb, 4 * d->cap_tiles)) ||
-             (st = dalloc(d->d_ck, n_ck * d->ck_stride)) || (st = dalloc(d->d_partial, LOSS_GRID)) ||
+             (st = dalloc(d->d_ck, n_ck * d->ck_stride)) || (st = dalloc(d->d_partial, LOSS_GRID + 1)) ||
              (st = dalloc(d->d_status, 1)))
         ;
-    else if (hipHostMalloc(reinterpret_cast<void **>(&d->h_partial), LOSS_GRID * sizeof(double),
-                           hipHostMallocDefault) != hipSuccess ||
+    else if (hipHostMalloc(reinterpret_cast<void **>(&d->h_loss), sizeof(double), hipHostMallocDefault) !=
+                 hipSuccess ||
              hipHostMalloc(reinterpret_cast<void **>(&d->h_status), sizeof(DevStatus), hipHostMallocDefault) !=
                  hipSuccess)
         st = FT_ENOMEM;
@@ -788,13 +842,11 @@ ft_status ft_diff_new(const ft_diff_layout *L, ft_diff **out) {
         st = FT_EDEVICE;
     else
         st = FT_OK;
-    // Coverage kept from the backward's forward replay (4 B per path and
-    // pixel) spares its second evaluation; without room for it the
-    // backward recomputes.
-    // FT_DIFF_RECOMPUTE_COVERAGE=1 selects the recomputing path without the
-    // buffer (the path large layouts take), for tests and memory-tight callers.
-    const char *no_cov = std::getenv("FT_DIFF_RECOMPUTE_COVERAGE");
-    const uint64_t cov_elems = (no_cov && no_cov[0] == '1') ? 0 : (uint64_t)max_paths * d->ck_stride;
+    // By default the backward recomputes every path's coverage.  On request
+    // (FT_DIFF_KEEP_COVERAGE) the forward keeps it, 4 B per path and pixel,
+    // which spares the second evaluation; without room for the buffer the
+    // handle recomputes anyway.
+    const uint64_t cov_elems = (L->flags & FT_DIFF_KEEP_COVERAGE) ? (uint64_t)max_paths * d->ck_stride : 0;
     if (!st && cov_elems && cov_elems <= COV_MAX_BYTES / sizeof(float) && dalloc(d->d_cov, cov_elems)) {
         d->d_cov = nullptr;
         (void)hipGetLastError();  // the failed optional allocation is not an error of a later call
@@ -814,7 +866,7 @@ void ft_diff_free(ft_diff *d) {
                     d->d_ck,       d->d_cov,       d->d_partial,   d->d_status};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    if (d->h_partial) (void)hipHostFree(d->h_partial);
+    if (d->h_loss) (void)hipHostFree(d->h_loss);
     if (d->h_status) (void)hipHostFree(d->h_status);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
@@ -830,7 +882,7 @@ ft_status ft_diff_info(const ft_diff *d, uint64_t *n_points, uint64_t *n_paths, 
 
 ft_status ft_diff_forward(ft_diff *d, const float *points, const float *colours, float *canvas) {
     if (!d || !points || !colours || !canvas) return FT_EINVAL;
-    return run_forward(d, points, colours, canvas, false);
+    return run_forward(d, points, colours, canvas);
 }
 
 ft_status ft_diff_loss(ft_diff *d, const float *canvas, const float *target, float *grad_canvas, double *loss) {
@@ -841,17 +893,14 @@ ft_status ft_diff_loss(ft_diff *d, const float *canvas, const float *target, flo
 ft_status ft_diff_backward(ft_diff *d, const float *points, const float *colours, const float *grad_canvas,
                            float *grad_points, float *grad_colours) {
     if (!d || !points || !colours || !grad_canvas || !grad_points || !grad_colours) return FT_EINVAL;
-    return run_backward(d, points, colours, grad_canvas, grad_points, grad_colours, false);
+    return run_backward(d, points, colours, grad_canvas, grad_points, grad_colours);
 }
 
 ft_status ft_diff_step(ft_diff *d, const float *points, const float *colours, const float *target, float *canvas,
                        float *grad_canvas, float *grad_points, float *grad_colours, double *loss) {
     if (!d || !points || !colours || !target || !canvas || !grad_canvas || !grad_points || !grad_colours)
         return FT_EINVAL;
-    ft_status st = run_forward(d, points, colours, canvas, true);
-    if (!st) st = run_loss(d, canvas, target, grad_canvas, loss);
-    if (!st) st = run_backward(d, points, colours, grad_canvas, grad_points, grad_colours, true);
-    return st;
+    return run_step(d, points, colours, target, canvas, grad_canvas, grad_points, grad_colours, loss);
 }
 
 }  // extern "C"

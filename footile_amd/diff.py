@@ -17,9 +17,13 @@ from .plotter import DeviceBuffer
 _vp = ctypes.c_void_p
 
 
+KEEP_COVERAGE = 1  # FT_DIFF_KEEP_COVERAGE
+
+
 class _Layout(ctypes.Structure):
     _fields_ = [("n_images", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
-                ("n_paths", ctypes.c_uint32), ("path_image", ctypes.c_void_p), ("path_cubics", ctypes.c_void_p)]
+                ("n_paths", ctypes.c_uint32), ("path_image", ctypes.c_void_p), ("path_cubics", ctypes.c_void_p),
+                ("flags", ctypes.c_uint32)]
 
 
 def device_array(a):
@@ -31,14 +35,18 @@ def device_array(a):
 
 
 class DiffRenderer:
-    """A scene layout and its device workspace (ft_diff)."""
+    """A scene layout and its device workspace (ft_diff).
 
-    def __init__(self, n_images, width, height, path_image, path_cubics):
+    By default the backward recomputes every path's coverage; keep_coverage
+    (FT_DIFF_KEEP_COVERAGE) keeps it from the forward instead, at 4 B per path
+    and pixel."""
+
+    def __init__(self, n_images, width, height, path_image, path_cubics, keep_coverage=False):
         self.n_images, self.width, self.height = int(n_images), int(width), int(height)
         self.path_image = np.ascontiguousarray(path_image, np.uint32)
         self.path_cubics = np.ascontiguousarray(path_cubics, np.uint32)
         L = _Layout(self.n_images, self.width, self.height, len(self.path_image), self.path_image.ctypes.data,
-                    self.path_cubics.ctypes.data)
+                    self.path_cubics.ctypes.data, KEEP_COVERAGE if keep_coverage else 0)
         h = _vp()
         check(lib().ft_diff_new(ctypes.byref(L), ctypes.byref(h)), "ft_diff_new")
         self._h = h.value
@@ -48,8 +56,8 @@ class DiffRenderer:
         self.n_points, self.n_paths, self.canvas_floats = npts.value, npaths.value, nfl.value
 
     @classmethod
-    def for_scene(cls, scene):
-        return cls(scene.n_images, scene.width, scene.height, scene.path_image, scene.path_cubics)
+    def for_scene(cls, scene, keep_coverage=False):
+        return cls(scene.n_images, scene.width, scene.height, scene.path_image, scene.path_cubics, keep_coverage)
 
     def __del__(self):
         self.close()

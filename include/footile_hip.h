@@ -201,11 +201,17 @@ ft_status ft_memset_d(void *dst, int value, size_t bytes);
  * [n_images][height][width][4].  Every call is synchronous.
  * ------------------------------------------------------------------------ */
 typedef struct ft_diff ft_diff;
+/* ft_diff_layout.flags */
+#define FT_DIFF_KEEP_COVERAGE 1u /* opt-in: keep each path's coverage from the
+                                    forward for the backward (4 B per path and
+                                    pixel, see ft_diff_backward) instead of
+                                    recomputing it */
 typedef struct ft_diff_layout {
     uint32_t n_images, width, height;
     uint32_t n_paths;
     const uint32_t *path_image;  /* host [n_paths] */
     const uint32_t *path_cubics; /* host [n_paths] */
+    uint32_t flags;              /* 0 or FT_DIFF_KEEP_COVERAGE; other bits: FT_EINVAL */
 } ft_diff_layout;
 ft_status ft_diff_new(const ft_diff_layout *layout, ft_diff **out);
 void ft_diff_free(ft_diff *d);
@@ -215,14 +221,17 @@ ft_status ft_diff_forward(ft_diff *d, const float *points, const float *colours,
 /* loss = mean((canvas - target)^2); grad_canvas (may be NULL) = d loss / d canvas */
 ft_status ft_diff_loss(ft_diff *d, const float *canvas, const float *target, float *grad_canvas, double *loss);
 /* Vector-Jacobian product: grad_canvas -> d/d points, d/d colours.  Replays
- * the forward into canvas checkpoints every 8 paths and a per-path coverage
- * buffer (4 B per path and pixel, allocated by ft_diff_new when it fits in
- * 16 GiB; otherwise coverage is recomputed). */
+ * the forward into canvas checkpoints every 8 paths (16 B per pixel of every
+ * image per checkpoint, max_paths / 8 rounded up of them, max_paths the most
+ * paths of one image) and recomputes each path's coverage while it walks the
+ * paths backwards.  With FT_DIFF_KEEP_COVERAGE (and when the buffer fits in
+ * 16 GiB) the replay also keeps every path's coverage, max_paths x 4 B per
+ * pixel of every image, and the walk reads it instead. */
 ft_status ft_diff_backward(ft_diff *d, const float *points, const float *colours, const float *grad_canvas,
                            float *grad_points, float *grad_colours);
-/* forward + loss + backward (one optimisation step's gradient); the
- * forward leaves the checkpoints, coverage and geometry the backward needs,
- * so the backward does not replay it */
+/* forward + loss + backward (one optimisation step's gradient) with one host
+ * wait: the forward leaves the checkpoints and geometry the backward needs,
+ * so the backward does not replay it, and the loss is reduced on the device */
 ft_status ft_diff_step(ft_diff *d, const float *points, const float *colours, const float *target, float *canvas,
                        float *grad_canvas, float *grad_points, float *grad_colours, double *loss);
 

@@ -48,11 +48,12 @@ def test_struct_layouts_match_c(tmp_path):
 #include <stddef.h>
 #include "{HEADER}"
 int main(void) {{
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ft_path_op), sizeof(ft_fill_desc),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ft_path_op), sizeof(ft_fill_desc),
          offsetof(ft_fill_desc, dst), offsetof(ft_fill_desc, fmt), offsetof(ft_fill_desc, clr),
          offsetof(ft_fill_desc, transform), offsetof(ft_fill_desc, tolerance),
          offsetof(ft_fill_desc, pen_width), offsetof(ft_fill_desc, fresh), sizeof(ft_diff_layout),
-         offsetof(ft_diff_layout, path_image), offsetof(ft_diff_layout, path_cubics));
+         offsetof(ft_diff_layout, path_image), offsetof(ft_diff_layout, path_cubics),
+         offsetof(ft_diff_layout, flags));
   return 0;
 }}
 ''')
@@ -63,7 +64,7 @@ int main(void) {{
     want = [OP_DTYPE.itemsize, ctypes.sizeof(FillDescC), FillDescC.dst.offset, FillDescC.fmt.offset,
             FillDescC.clr.offset, FillDescC.transform.offset, FillDescC.tolerance.offset,
             FillDescC.pen_width.offset, FillDescC.fresh.offset, ctypes.sizeof(_Layout), _Layout.path_image.offset,
-            _Layout.path_cubics.offset]
+            _Layout.path_cubics.offset, _Layout.flags.offset]
     assert got == want
 
 

@@ -37,9 +37,9 @@ def _oracle_scene(sc):
     return pd.Scene(sc.n_images, sc.width, sc.height, sc.path_image, sc.path_cubics)
 
 
-def _gpu(sc):
+def _gpu(sc, keep_coverage=False):
     from footile_amd.diff import DiffRenderer, device_array
-    r = DiffRenderer.for_scene(sc)
+    r = DiffRenderer.for_scene(sc, keep_coverage)
     return r, device_array(sc.points), device_array(sc.colours)
 
 
@@ -124,18 +124,17 @@ def test_step_equals_forward_loss_backward(oracle):
     assert np.allclose(a, b, rtol=1e-5, atol=1e-7 * np.abs(b).max())
 
 
-def test_recomputed_coverage_matches_kept(oracle, monkeypatch):
-    """Without the per-path coverage buffer (the path layouts take when it
-    would exceed 16 GiB, forced here by FT_DIFF_RECOMPUTE_COVERAGE=1 at
-    ft_diff_new) the step and a separate backward give the same canvas, loss
-    and gradients as with it."""
+def test_kept_coverage_matches_recompute(oracle):
+    """The opt-in kept-coverage handle (FT_DIFF_KEEP_COVERAGE) and the default
+    one, which recomputes coverage in the backward, give the same canvas and
+    loss and gradients within float32 rounding, through a step and through a
+    separate backward."""
     from footile_amd.diff import device_array
     from footile_amd.plotter import DeviceBuffer
     sc = _scene(12)
     out = []
-    for env in ("0", "1"):
-        monkeypatch.setenv("FT_DIFF_RECOMPUTE_COVERAGE", env)
-        r, P, C = _gpu(sc)
+    for keep in (False, True):
+        r, P, C = _gpu(sc, keep)
         T = device_array(np.full(r.canvas_shape, 0.25, np.float32))
         c, g = r.new_canvas(), r.new_canvas()
         gp, gc = DeviceBuffer(8 * r.n_points), DeviceBuffer(16 * r.n_paths)
@@ -149,6 +148,127 @@ def test_recomputed_coverage_matches_kept(oracle, monkeypatch):
     for x in g1 + g2[2:]:
         y = g2[0] if x.size == g2[0].size else g2[1]
         assert np.allclose(x, y, rtol=1e-5, atol=1e-7 * np.abs(y).max())
+
+
+@pytest.mark.parametrize("keep", [False, True])
+def test_many_paths_long_cubics_vs_fp64(oracle, keep):
+    """Layouts past the backward's fast cases against FP64: 21 paths per image
+    (three checkpoint segments of 8, the last one partial) and paths of up to
+    16 cubics, whose 257 vertices are past the 256 a block gathers in LDS (so
+    their gradients go straight to global memory).  Recomputed and kept
+    coverage, through ft_diff_step and through separate forward, loss and
+    backward calls."""
+    from footile_amd.diff import device_array
+    from footile_amd.plotter import DeviceBuffer
+    sc = _scene(21, n_images=2, w=64, h=48, paths=21, cubics=(12, 17))
+    sc.path_cubics[::5] = 16  # several paths of exactly 257 vertices
+    n_pts = int(np.sum(1 + 3 * sc.path_cubics.astype(np.int64)))
+    rng = np.random.default_rng(22)
+    sc.points = rng.uniform(-3.0, [64 + 3.0, 48 + 3.0], size=(n_pts, 2)).astype(np.float32)
+    assert sc.n_paths == 42 and int(sc.path_cubics.max()) == 16
+    target = rng.uniform(0, 1, size=(2, 48, 64, 4)).astype(np.float32)
+    o = _oracle_scene(sc)
+    want_loss, want_gp, want_gc = o.backward(sc.points, sc.colours, target)
+    _, want_canvas = o.forward(sc.points, sc.colours)
+    r, P, C = _gpu(sc, keep)
+    T = device_array(target)
+    canvas, gcan = r.new_canvas(), r.new_canvas()
+    gp, gc = DeviceBuffer(8 * r.n_points), DeviceBuffer(16 * r.n_paths)
+    loss = r.step(P, C, T, canvas, gcan, gp, gc)
+    assert abs(loss - want_loss) <= 1e-5 * abs(want_loss)
+    assert np.abs(canvas.download(dtype=np.float32).reshape(r.canvas_shape) - want_canvas).max() < 2e-5
+    assert _rel(gp.download(dtype=np.float32).reshape(-1, 2), want_gp) < 1e-4
+    assert _rel(gc.download(dtype=np.float32).reshape(-1, 4), want_gc) < 1e-4
+    # the same through the separate entry points
+    c2, g2 = r.new_canvas(), r.new_canvas()
+    r.forward(P, C, c2)
+    assert r.loss(c2, T, g2) == loss
+    gp2, gc2 = DeviceBuffer(8 * r.n_points), DeviceBuffer(16 * r.n_paths)
+    r.backward(P, C, g2, gp2, gc2)
+    assert _rel(gp2.download(dtype=np.float32).reshape(-1, 2), want_gp) < 1e-4
+    assert _rel(gc2.download(dtype=np.float32).reshape(-1, 4), want_gc) < 1e-4
+
+
+def _elementwise(got, want):
+    """Largest and mean |got - want| / max|want| over the elements."""
+    d = np.abs(got - want) / max(float(np.abs(want).max()), 1e-30)
+    return float(d.max()), float(d.mean())
+
+
+def test_c4_full_size_vs_fp64(oracle):
+    """The differentiable single-image scene at its full size: 512x512, 256
+    paths of 8 cubics (seed 3), target rendered from perturbed parameters;
+    canvas, loss and all gradients of one step against FP64, with coverage
+    recomputed in the backward."""
+    from footile_amd import diffscene as ds
+    from footile_amd.diff import device_array
+    from footile_amd.plotter import DeviceBuffer
+    sc = ds.scene_c4()
+    r, P, C = _gpu(sc)
+    tp, tc = ds.perturbed(sc)
+    T = r.new_canvas()
+    r.forward(device_array(tp), device_array(tc), T)
+    target = T.download(dtype=np.float32).reshape(r.canvas_shape)
+    canvas, gcan = r.new_canvas(), r.new_canvas()
+    gp, gc = DeviceBuffer(8 * r.n_points), DeviceBuffer(16 * r.n_paths)
+    loss = r.step(P, C, T, canvas, gcan, gp, gc)
+    o = _oracle_scene(sc)
+    want_loss, want_gp, want_gc = o.backward(sc.points, sc.colours, target)
+    _, want_canvas = o.forward(sc.points, sc.colours)
+    # 256 float32 composites deep: within 1e-4 per channel (measured 3.3e-5)
+    assert np.abs(canvas.download(dtype=np.float32).reshape(r.canvas_shape) - want_canvas).max() < 1e-4
+    assert abs(loss - want_loss) <= 1e-5 * abs(want_loss)
+    got_gp = gp.download(dtype=np.float32).reshape(-1, 2)
+    got_gc = gc.download(dtype=np.float32).reshape(-1, 4)
+    assert _rel(got_gp, want_gp) < 1e-4 and _rel(got_gc, want_gc) < 1e-4
+    assert _elementwise(got_gp, want_gp)[0] < 1e-4 and _elementwise(got_gc, want_gc)[0] < 1e-4
+
+
+def test_c5_full_layout_vs_fp64(oracle):
+    """The batched differentiable scene at its full size: 64 images of 256x256
+    with U[32, 512] paths of U[4, 16] cubics (seed 4), one ft_diff_step with
+    coverage recomputed in the backward.  The canvas of every image against
+    FP64, and the gradients of 8 images (each image's parameters only touch
+    its own pixels; the batched loss is the mean over all images)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from footile_amd import diffscene as ds
+    from footile_amd.diff import device_array
+    from footile_amd.plotter import DeviceBuffer
+    from oracle import pydiff as pd
+    sc = ds.scene_c5()
+    r, P, C = _gpu(sc)
+    tp, tc = ds.perturbed(sc)
+    T = r.new_canvas()
+    r.forward(device_array(tp), device_array(tc), T)
+    target = T.download(dtype=np.float32).reshape(r.canvas_shape)
+    canvas, gcan = r.new_canvas(), r.new_canvas()
+    gp, gc = DeviceBuffer(8 * r.n_points), DeviceBuffer(16 * r.n_paths)
+    r.step(P, C, T, canvas, gcan, gp, gc)
+    got_canvas = canvas.download(dtype=np.float32).reshape(r.canvas_shape)
+    got_gp = gp.download(dtype=np.float32).reshape(-1, 2)
+    got_gc = gc.download(dtype=np.float32).reshape(-1, 4)
+    first = np.searchsorted(sc.path_image, np.arange(sc.n_images + 1))
+    pt_first = np.concatenate([[0], np.cumsum(1 + 3 * sc.path_cubics.astype(np.int64))])
+
+    def image(i):
+        k0, k1 = int(first[i]), int(first[i + 1])
+        o = pd.Scene(1, sc.width, sc.height, np.zeros(k1 - k0, np.uint32), sc.path_cubics[k0:k1])
+        pts, col = sc.points[pt_first[k0]:pt_first[k1]], sc.colours[k0:k1]
+        _, want_canvas = o.forward(pts, col)
+        err = float(np.abs(got_canvas[i] - want_canvas[0]).max())
+        if i % 8:
+            return err, None
+        _, want_gp, want_gc = o.backward(pts, col, target[i:i + 1])
+        scale = 1.0 / sc.n_images
+        return err, (_rel(got_gp[pt_first[k0]:pt_first[k1]], want_gp * scale),
+                     _rel(got_gc[k0:k1], want_gc * scale))
+
+    with ThreadPoolExecutor(16) as ex:
+        res = list(ex.map(image, range(sc.n_images)))
+    assert max(e for e, _ in res) < 1e-4  # up to 512 float32 composites deep
+    grads = [g for _, g in res if g is not None]
+    assert len(grads) == 8
+    assert max(g[0] for g in grads) < 1e-4 and max(g[1] for g in grads) < 1e-4
 
 
 def test_empty_image_and_bad_layout(oracle):

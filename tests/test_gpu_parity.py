@@ -496,3 +496,55 @@ def test_batch_band_pair_overflow_grows_and_reruns(ft, oracle):
     assert np.array_equal(buf.download(), o.pixels())
     offered, stored, _ = batch.counts()
     assert (offered[0], stored[0]) == o.last_counts()
+
+
+def _oracle_tiles(oracle, ops, tiles, idx, threads=16):
+    """The oracle's fresh s x s Matte8 tile and (offered, stored) counts for
+    each tile index, on a thread pool (the oracle's calls release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(t):
+        b, c, _, _, s = (int(v) for v in tiles[t])
+        o = oracle.Plotter(s, s)
+        assert o.fill(0, ops[b:b + c]) == 0
+        return o.pixels().reshape(s, s), o.last_counts()
+
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(one, idx))
+
+
+def test_atlas_full_size_every_tile(ft, oracle):
+    """The batched atlas at its full size: 16,384 paths (lognormal(3.5, 1)
+    segments, seed 2) into fresh tiles of a 16384x16384 Matte8 atlas, with the
+    autotuned raster geometry and the launch sequence replayed as a HIP graph.
+    Every tile and its offered / stored point counts against the oracle, and
+    the pixels between tiles untouched."""
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    from footile_amd.tune import autotune
+    side = 16384
+    ops, tiles = workloads.atlas(2, n_paths=16384, atlas_side=side)
+    atlas = DeviceBuffer(side * side)
+    atlas.upload(np.full(side * side, 77, dtype=np.uint8))
+    descs = [FillDesc(int(b), int(c), atlas.ptr + int(y) * side + int(x), int(s), int(s), side,
+                      fresh=True) for b, c, x, y, s in tiles]
+    batch = Batch(ops, descs)
+    autotune(batch, descs, len(ops), cache=None)
+    atlas.upload(np.full(side * side, 77, dtype=np.uint8))
+    batch.set_graph(True)
+    for _ in range(2):  # capture, then a replay of the captured graph
+        batch.launch()
+        batch.sync()
+    batch.set_graph(False)
+    img = atlas.download().reshape(side, side)
+    offered, stored, _ = batch.counts()
+    covered = np.zeros((side, side), bool)
+    for k0 in range(0, len(tiles), 2048):
+        idx = list(range(k0, min(k0 + 2048, len(tiles))))
+        for t, (want, counts) in zip(idx, _oracle_tiles(oracle, ops, tiles, idx)):
+            b, c, x, y, s = (int(v) for v in tiles[t])
+            if not np.array_equal(img[y:y + s, x:x + s], want):
+                bad = np.argwhere(img[y:y + s, x:x + s] != want)
+                raise AssertionError(f"tile {t} (side {s}): {len(bad)} pixels differ, first at {bad[:4].tolist()}")
+            assert (int(offered[t]), int(stored[t])) == counts, f"tile {t}"
+            covered[y:y + s, x:x + s] = True
+    assert np.all(img[~covered] == 77)
