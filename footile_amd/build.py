@@ -29,7 +29,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
               f"-I{os.path.join(ROOT, 'include')}"]
 
-HIP_SOURCES = ["ft_pipeline.hip", "ft_diff.hip", "ft_api.cpp"]
+HIP_SOURCES = ["ft_pipeline.hip", "ft_diff.hip", "ft_noalloc.hip", "ft_api.cpp"]
 HOST_SOURCES = []
 
 

@@ -107,10 +107,21 @@ struct Workspace {
     hipEvent_t *ev = nullptr;  // TIMING_RING x 3 events (start, raster start, raster end), created on first use
     double t_total_ms = 0.0, t_raster_ms = 0.0;
     uint64_t n_timed = 0;
+    // Arena mode (ws_init_arena): every device buffer is carved from
+    // [arena, arena + arena_bytes), nothing is allocated or freed, each
+    // ws_set_batch carves afresh, and a run that outgrows its carved
+    // capacities fails with FT_ENOMEM instead of growing.  The stream is the
+    // caller's and the status is read back into h_status_store.
+    uint8_t *arena = nullptr;
+    size_t arena_bytes = 0, arena_used = 0;
+    DevStatus h_status_store{};
 };
 
 // Allocate the stream and status buffers.
 ft_status ws_init(Workspace &ws);
+// Arena mode: no allocation; device buffers come from `arena` (256-byte
+// aligned, `bytes` long), work goes to `stream`.
+ft_status ws_init_arena(Workspace &ws, void *arena, size_t bytes, hipStream_t stream);
 void ws_free(Workspace &ws);
 // Upload a batch layout: ops + path descriptors (host arrays).
 // extra_flags (PATH_RAW, ...) are or-ed into every path's flags.

@@ -1450,14 +1450,35 @@ static void graph_drop(Workspace &ws) {
     ws.gexec = nullptr;
 }
 
+// Arena mode: the next n elements of the arena (256-byte aligned); the arena
+// is only rewound by ws_set_batch.
 template <typename T>
-static ft_status dev_grow(T *&ptr, size_t &cap, size_t need) {
+static ft_status arena_carve(Workspace &ws, T *&ptr, size_t n) {
+    constexpr size_t ALIGN = 256;
+    const size_t at = (ws.arena_used + ALIGN - 1) / ALIGN * ALIGN;
+    const size_t bytes = (n ? n : 1) * sizeof(T);
+    if (at > ws.arena_bytes || bytes > ws.arena_bytes - at) {
+        ptr = nullptr;
+        return FT_ENOMEM;
+    }
+    ptr = reinterpret_cast<T *>(ws.arena + at);
+    ws.arena_used = at + bytes;
+    return FT_OK;
+}
+
+template <typename T>
+static ft_status dev_grow(Workspace &ws, T *&ptr, size_t &cap, size_t need) {
     if (need <= cap && ptr) return FT_OK;
+    size_t n = need < 1 ? 1 : need;
+    if (ws.arena) {
+        const ft_status s = arena_carve(ws, ptr, n);
+        cap = s ? 0 : n;
+        return s;
+    }
     if (ptr) {
         (void)hipFree(ptr);
         ptr = nullptr;
     }
-    size_t n = need < 1 ? 1 : need;
     if (hipMalloc(reinterpret_cast<void **>(&ptr), n * sizeof(T)) != hipSuccess) {
         cap = 0;
         return FT_ENOMEM;
@@ -1467,12 +1488,28 @@ static ft_status dev_grow(T *&ptr, size_t &cap, size_t need) {
 }
 
 template <typename T>
-static ft_status dev_alloc(T *&ptr, size_t n) {
+static ft_status dev_alloc(Workspace &ws, T *&ptr, size_t n) {
+    if (ws.arena) return arena_carve(ws, ptr, n);
     if (ptr) {
         (void)hipFree(ptr);
         ptr = nullptr;
     }
     if (hipMalloc(reinterpret_cast<void **>(&ptr), (n ? n : 1) * sizeof(T)) != hipSuccess) return FT_ENOMEM;
+    return FT_OK;
+}
+
+ft_status ws_init_arena(Workspace &ws, void *arena, size_t bytes, hipStream_t stream) {
+    if (!arena || (reinterpret_cast<uintptr_t>(arena) & 255u)) return FT_EINVAL;
+    FT_HIP(hipGetDevice(&ws.device));
+    ws.scan_resident = scan_resident_blocks();
+    int lds_max = 0;
+    FT_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ws.device));
+    ws.lds_max = (size_t)lds_max;
+    ws.stream = stream;
+    ws.arena = reinterpret_cast<uint8_t *>(arena);
+    ws.arena_bytes = bytes;
+    ws.arena_used = 0;
+    ws.h_status = &ws.h_status_store;
     return FT_OK;
 }
 
@@ -1490,6 +1527,11 @@ ft_status ws_init(Workspace &ws) {
 }
 
 void ws_free(Workspace &ws) {
+    if (ws.arena) {  // nothing was allocated; the stream is the caller's
+        graph_drop(ws);
+        ws = Workspace();
+        return;
+    }
     void *bufs[] = {ws.d_ops, ws.d_op_path, ws.d_op_order, ws.d_paths, ws.d_band_path, ws.d_op_cnt, ws.d_op_off,
                     ws.d_sub_off,
                     ws.d_leaf, ws.d_op_pw,
@@ -1516,12 +1558,14 @@ static ft_status ensure_event_capacity(Workspace &ws, size_t need) {
     if (need <= ws.cap_events && ws.d_ev_meta) return FT_OK;
     size_t cap = need < 4096 ? 4096 : need;
     ft_status s;
-    if ((s = dev_alloc(ws.d_ev_xy, cap)) || (s = dev_alloc(ws.d_ev_f, cap)) || (s = dev_alloc(ws.d_ev_meta, cap)) ||
-        (s = dev_alloc(ws.d_ev_kept, cap)) || (s = dev_alloc(ws.d_ev_idx, cap)) || (s = dev_alloc(ws.d_ev_last, cap)) ||
-        (s = dev_alloc(ws.d_run_first, cap)) || (s = dev_alloc(ws.d_run_close, cap)) ||
-        (s = dev_alloc(ws.d_run_path, cap)) || (s = dev_alloc(ws.d_run_start, cap)) ||
-        (s = dev_alloc(ws.d_run_end, cap)) || (s = dev_alloc(ws.d_pts, cap)) || (s = dev_alloc(ws.d_pt_run, cap)) ||
-        (s = dev_alloc(ws.d_edges, cap)) || (s = dev_alloc(ws.d_pt_band, cap))) {
+    if (ws.arena) ws.d_ev_f = nullptr;  // arena workspaces only fill: no stroke-mode events
+    if ((s = dev_alloc(ws, ws.d_ev_xy, cap)) || (!ws.arena && (s = dev_alloc(ws, ws.d_ev_f, cap))) ||
+        (s = dev_alloc(ws, ws.d_ev_meta, cap)) ||
+        (s = dev_alloc(ws, ws.d_ev_kept, cap)) || (s = dev_alloc(ws, ws.d_ev_idx, cap)) || (s = dev_alloc(ws, ws.d_ev_last, cap)) ||
+        (s = dev_alloc(ws, ws.d_run_first, cap)) || (s = dev_alloc(ws, ws.d_run_close, cap)) ||
+        (s = dev_alloc(ws, ws.d_run_path, cap)) || (s = dev_alloc(ws, ws.d_run_start, cap)) ||
+        (s = dev_alloc(ws, ws.d_run_end, cap)) || (s = dev_alloc(ws, ws.d_pts, cap)) || (s = dev_alloc(ws, ws.d_pt_run, cap)) ||
+        (s = dev_alloc(ws, ws.d_edges, cap)) || (s = dev_alloc(ws, ws.d_pt_band, cap))) {
         ws.cap_events = 0;
         return s;
     }
@@ -1533,11 +1577,11 @@ static ft_status ensure_stroke_capacity(Workspace &ws) {
     if (ws.cap_stroke == ws.cap_events && ws.d_st_flags) return FT_OK;
     const size_t cap = ws.cap_events, items = 4 * cap + 1;
     ft_status s;
-    if ((s = dev_alloc(ws.d_st_flags, cap)) || (s = dev_alloc(ws.d_st_off, cap)) ||
-        (s = dev_alloc(ws.d_st_last, cap)) || (s = dev_alloc(ws.d_spts, cap)) || (s = dev_alloc(ws.d_sub_start, cap)) ||
-        (s = dev_alloc(ws.d_sub_event, cap)) || (s = dev_alloc(ws.d_sub_n, cap)) ||
-        (s = dev_alloc(ws.d_sub_joined, cap)) || (s = dev_alloc(ws.d_it_cnt, items)) ||
-        (s = dev_alloc(ws.d_it_off, items))) {
+    if ((s = dev_alloc(ws, ws.d_st_flags, cap)) || (s = dev_alloc(ws, ws.d_st_off, cap)) ||
+        (s = dev_alloc(ws, ws.d_st_last, cap)) || (s = dev_alloc(ws, ws.d_spts, cap)) || (s = dev_alloc(ws, ws.d_sub_start, cap)) ||
+        (s = dev_alloc(ws, ws.d_sub_event, cap)) || (s = dev_alloc(ws, ws.d_sub_n, cap)) ||
+        (s = dev_alloc(ws, ws.d_sub_joined, cap)) || (s = dev_alloc(ws, ws.d_it_cnt, items)) ||
+        (s = dev_alloc(ws, ws.d_it_off, items))) {
         ws.cap_stroke = 0;
         return s;
     }
@@ -1553,7 +1597,7 @@ static ft_status ensure_scan_capacity(Workspace &ws) {
     if (tiles < SCAN_GRID_MAX) tiles = SCAN_GRID_MAX;  // one state per scan block: any scan grid fits
     if (tiles <= ws.cap_tiles && ws.d_scan_state) return FT_OK;
     ft_status s;
-    if ((s = dev_alloc(ws.d_scan_state, 2 * tiles * SCAN_SLOTS))) return s;
+    if ((s = dev_alloc(ws, ws.d_scan_state, 2 * tiles * SCAN_SLOTS))) return s;
     ws.cap_tiles = tiles;
     return FT_OK;
 }
@@ -1561,6 +1605,56 @@ static ft_status ensure_scan_capacity(Workspace &ws) {
 ft_status ws_upload_ops(Workspace &ws, const ft_path_op *ops, size_t n_ops) {
     if (n_ops != ws.n_ops) return FT_EINVAL;
     if (n_ops) FT_HIP(hipMemcpyAsync(ws.d_ops, ops, n_ops * sizeof(ft_path_op), hipMemcpyHostToDevice, ws.stream));
+    return FT_OK;
+}
+
+// Arena mode: forget every carved buffer (the next layout carves afresh) and
+// carve the status block first.
+static ft_status arena_rewind(Workspace &ws) {
+    ws.arena_used = 0;
+    void **ptrs[] = {(void **)&ws.d_ops, (void **)&ws.d_op_path, (void **)&ws.d_op_order, (void **)&ws.d_paths,
+                     (void **)&ws.d_band_path, (void **)&ws.d_op_cnt, (void **)&ws.d_op_off, (void **)&ws.d_sub_off,
+                     (void **)&ws.d_leaf, (void **)&ws.d_op_pw, (void **)&ws.d_op_lastpw, (void **)&ws.d_ev_xy,
+                     (void **)&ws.d_ev_f, (void **)&ws.d_ev_meta, (void **)&ws.d_ev_kept, (void **)&ws.d_ev_idx,
+                     (void **)&ws.d_ev_last, (void **)&ws.d_st_flags, (void **)&ws.d_st_off, (void **)&ws.d_st_last,
+                     (void **)&ws.d_spts, (void **)&ws.d_sub_start, (void **)&ws.d_sub_event, (void **)&ws.d_sub_n,
+                     (void **)&ws.d_sub_joined, (void **)&ws.d_it_cnt, (void **)&ws.d_it_off,
+                     (void **)&ws.d_run_first, (void **)&ws.d_run_close, (void **)&ws.d_run_path,
+                     (void **)&ws.d_rblock, (void **)&ws.d_run_start, (void **)&ws.d_run_end, (void **)&ws.d_pts,
+                     (void **)&ws.d_pt_run, (void **)&ws.d_edges, (void **)&ws.d_pt_band, (void **)&ws.d_band_cnt,
+                     (void **)&ws.d_band_off, (void **)&ws.d_band_cur, (void **)&ws.d_band_list,
+                     (void **)&ws.d_scan_state, (void **)&ws.d_pstate, (void **)&ws.d_top_rep,
+                     (void **)&ws.d_status};
+    for (void **p : ptrs) *p = nullptr;
+    ws.cap_ops = ws.cap_paths = ws.cap_bands = ws.cap_rblocks = ws.cap_events = ws.cap_stroke = 0;
+    ws.cap_band_slots = ws.cap_pairs = ws.cap_tiles = 0;
+    return arena_carve(ws, ws.d_status, 1);
+}
+
+// Arena mode: the rest of the arena, after the layout's fixed buffers and the
+// scan state, split between the event-sized buffers (88 B per event: no
+// stroke-mode events) and the band lists (4 B per pair), 3 : 1 in bytes.
+static ft_status arena_split(Workspace &ws) {
+    constexpr size_t EVENT_BYTES = 88, SLACK = 32 * 256;  // alignment of the carved buffers
+    const size_t left = ws.arena_bytes > ws.arena_used ? ws.arena_bytes - ws.arena_used : 0;
+    // scan state for the largest event capacity this arena could hold
+    const size_t ev_max = left / EVENT_BYTES;
+    size_t n = 4 * ev_max + 1;
+    if (ws.n_ops > n) n = ws.n_ops;
+    if ((ws.n_bands << ws.band_shift) > n) n = ws.n_bands << ws.band_shift;
+    size_t tiles = (n + SCAN_TILE_MIN - 1) / SCAN_TILE_MIN + 1;
+    if (tiles < SCAN_GRID_MAX) tiles = SCAN_GRID_MAX;
+    const size_t scan_bytes = 2 * tiles * SCAN_SLOTS * sizeof(unsigned long long);
+    if (left < scan_bytes + SLACK + 4096 * (EVENT_BYTES + 4)) return FT_ENOMEM;
+    const size_t room = left - scan_bytes - SLACK;
+    const size_t cap_events = room / 4 * 3 / EVENT_BYTES;
+    ft_status s;
+    if ((s = dev_alloc(ws, ws.d_scan_state, 2 * tiles * SCAN_SLOTS))) return s;
+    ws.cap_tiles = tiles;
+    if ((s = ensure_event_capacity(ws, cap_events))) return s;
+    const size_t rest = ws.arena_bytes - ws.arena_used - 256;
+    if ((s = dev_alloc(ws, ws.d_band_list, rest / 4))) return s;
+    ws.cap_pairs = rest / 4;
     return FT_OK;
 }
 
@@ -1579,6 +1673,10 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     // staging is only rewritten once the previous layout's uploads are done
     if (ws.uploads_pending) FT_HIP(hipStreamSynchronize(ws.stream));
     ws.uploads_pending = false;
+    if (ws.arena) {
+        const ft_status s = arena_rewind(ws);
+        if (s) return s;
+    }
     std::vector<uint32_t> &op_path = ws.h_op_path;
     std::vector<PathDev> &pd = ws.h_paths;
     std::vector<uint32_t> &band_path = ws.h_band_path;
@@ -1630,20 +1728,20 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
         return FT_ENOTSUP;
     ft_status s;
     const size_t old_ops = ws.cap_ops;
-    if ((s = dev_grow(ws.d_ops, ws.cap_ops, n_ops))) return s;
+    if ((s = dev_grow(ws, ws.d_ops, ws.cap_ops, n_ops))) return s;
     if (ws.cap_ops != old_ops || !ws.d_op_cnt) {  // per-op scratch follows the op capacity
-        if ((s = dev_alloc(ws.d_op_path, ws.cap_ops)) || (s = dev_alloc(ws.d_op_order, ws.cap_ops)) ||
-            (s = dev_alloc(ws.d_op_cnt, ws.cap_ops)) ||
-            (s = dev_alloc(ws.d_op_off, ws.cap_ops)) || (s = dev_alloc(ws.d_op_pw, ws.cap_ops)) ||
-            (s = dev_alloc(ws.d_sub_off, ws.cap_ops * OP_LANES)) ||
-            (s = dev_alloc(ws.d_leaf, ws.cap_ops * OP_LANES * LEAF_CACHE)) ||
-            (s = dev_alloc(ws.d_op_lastpw, ws.cap_ops)))
+        if ((s = dev_alloc(ws, ws.d_op_path, ws.cap_ops)) || (s = dev_alloc(ws, ws.d_op_order, ws.cap_ops)) ||
+            (s = dev_alloc(ws, ws.d_op_cnt, ws.cap_ops)) ||
+            (s = dev_alloc(ws, ws.d_op_off, ws.cap_ops)) || (s = dev_alloc(ws, ws.d_op_pw, ws.cap_ops)) ||
+            (s = dev_alloc(ws, ws.d_sub_off, ws.cap_ops * OP_LANES)) ||
+            (s = dev_alloc(ws, ws.d_leaf, ws.cap_ops * OP_LANES * LEAF_CACHE)) ||
+            (s = dev_alloc(ws, ws.d_op_lastpw, ws.cap_ops)))
             return s;
     }
     const size_t old_paths = ws.cap_paths;
-    if ((s = dev_grow(ws.d_paths, ws.cap_paths, n_paths))) return s;
+    if ((s = dev_grow(ws, ws.d_paths, ws.cap_paths, n_paths))) return s;
     if (ws.cap_paths != old_paths || !ws.d_pstate)
-        if ((s = dev_alloc(ws.d_pstate, ws.cap_paths)) || (s = dev_alloc(ws.d_top_rep, ws.cap_paths * TOP_REPS)))
+        if ((s = dev_alloc(ws, ws.d_pstate, ws.cap_paths)) || (s = dev_alloc(ws, ws.d_top_rep, ws.cap_paths * TOP_REPS)))
             return s;
     size_t nb_total = band_path.size();
     // replicas per band counter: enough that the slots spread over about
@@ -1653,34 +1751,38 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     if ((nb_total << shift) >= 0x7FFFFFFFu) return FT_ENOTSUP;
     if (nb_total > ws.cap_bands || !ws.d_band_path) {
         size_t cap = nb_total ? nb_total : 1;
-        if ((s = dev_alloc(ws.d_band_path, cap))) return s;
+        if ((s = dev_alloc(ws, ws.d_band_path, cap))) return s;
         ws.cap_bands = cap;
     }
     const size_t n_slots = nb_total << shift;
     if (n_slots > ws.cap_band_slots || !ws.d_band_cnt) {
         size_t cap = n_slots ? n_slots : 1;
-        if ((s = dev_alloc(ws.d_band_cnt, cap)) || (s = dev_alloc(ws.d_band_off, cap)) ||
-            (s = dev_alloc(ws.d_band_cur, cap)))
+        if ((s = dev_alloc(ws, ws.d_band_cnt, cap)) || (s = dev_alloc(ws, ws.d_band_off, cap)) ||
+            (s = dev_alloc(ws, ws.d_band_cur, cap)))
             return s;
         ws.cap_band_slots = cap;
     }
     ws.band_shift = shift;
-    if ((s = dev_grow(ws.d_rblock, ws.cap_rblocks, rblock.size()))) return s;
+    if ((s = dev_grow(ws, ws.d_rblock, ws.cap_rblocks, rblock.size()))) return s;
     ws.n_rblocks = rblock.size();
     ws.n_ops = n_ops;
     ws.n_paths = n_paths;
     ws.n_bands = nb_total;
     ws.lds_cells = lds;
     ws.h_pstate.resize(n_paths);
-    if (!ws.d_ev_meta) {
-        if ((s = ensure_event_capacity(ws, 16 * n_ops + 1024))) return s;
+    if (ws.arena) {
+        if ((s = arena_split(ws))) return s;
+    } else {
+        if (!ws.d_ev_meta) {
+            if ((s = ensure_event_capacity(ws, 16 * n_ops + 1024))) return s;
+        }
+        if (!ws.d_band_list) {
+            size_t cap = 4 * (16 * n_ops + 1024);
+            if ((s = dev_alloc(ws, ws.d_band_list, cap))) return s;
+            ws.cap_pairs = cap;
+        }
+        if ((s = ensure_scan_capacity(ws))) return s;
     }
-    if (!ws.d_band_list) {
-        size_t cap = 4 * (16 * n_ops + 1024);
-        if ((s = dev_alloc(ws.d_band_list, cap))) return s;
-        ws.cap_pairs = cap;
-    }
-    if ((s = ensure_scan_capacity(ws))) return s;
     if (n_ops) {
         FT_HIP(hipMemcpyAsync(ws.d_ops, ops, n_ops * sizeof(ft_path_op), hipMemcpyHostToDevice, ws.stream));
         FT_HIP(hipMemcpyAsync(ws.d_op_path, op_path.data(), n_ops * 4, hipMemcpyHostToDevice, ws.stream));
@@ -1992,13 +2094,14 @@ ft_status ws_run(Workspace &ws, uint32_t mode, bool read_pstate) {
         if (s != (ft_status)0x100) return s;
         const DevStatus &h = *ws.h_status;
         graph_drop(ws);
+        if (ws.arena) return FT_ENOMEM;  // the arena's capacities are fixed
         if (h.overflow & 1u) {
             if ((s = ensure_event_capacity(ws, (size_t)h.need_events + h.need_events / 4 + 1024))) return s;
             if ((s = ensure_scan_capacity(ws))) return s;
         }
         if (h.overflow & 2u) {
             size_t cap = (size_t)h.need_pairs + h.need_pairs / 4 + 1024;
-            if ((s = dev_alloc(ws.d_band_list, cap))) return s;
+            if ((s = dev_alloc(ws, ws.d_band_list, cap))) return s;
             ws.cap_pairs = cap;
         }
     }

@@ -176,6 +176,27 @@ ft_status ft_batch_stream(ft_batch *b, void **stream);
  * the sums (ms) and the number of timed runs, and resets them. */
 ft_status ft_batch_set_timing(ft_batch *b, int on);
 ft_status ft_batch_timing(ft_batch *b, double *total_ms, double *raster_ms, uint64_t *runs);
+/* Batch fill without allocation.  No footile counterpart: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The same fills as ft_batch_new +
+ * ft_batch_run + ft_batch_counts (offered / stored may be NULL), from host
+ * ops and descriptors into the device rasters the descriptors name, with
+ * every device buffer the kernels need beyond those carved from one fixed
+ * array of FT_NOALLOC_SCRATCH_BYTES per device, allocated when the library's
+ * code loads on that device and held until the process ends.  No hipMalloc,
+ * hipFree or pinned host allocation.  Runs on `stream` (a hipStream_t; NULL:
+ * the null stream) and returns when the batch is done.  More than
+ * FT_NOALLOC_MAX_OPS ops or FT_NOALLOC_MAX_PATHS paths, a layout that does
+ * not fit the scratch, or a run whose events or band pairs outgrow it:
+ * FT_ENOMEM.  Calls on one device share its scratch, which keeps the last
+ * call's contents (each run clears what must start at zero); calls from
+ * threads of this process take turns on a lock, and other users of the same
+ * array on that device would race. */
+#define FT_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define FT_NOALLOC_MAX_OPS (1u << 15)
+#define FT_NOALLOC_MAX_PATHS (1u << 14)
+ft_status ft_batch_run_noalloc(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *paths, size_t n_paths,
+                               void *stream, uint64_t *offered, uint64_t *stored);
 /* Device allocation helpers for callers without a runtime of their own. */
 ft_status ft_device_alloc(size_t bytes, void **dptr);
 ft_status ft_device_free(void *dptr);
