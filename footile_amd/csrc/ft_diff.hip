@@ -32,7 +32,7 @@ namespace diff {
 
 constexpr int DSEG = 16;          // segments per cubic
 constexpr float EPS_W = 1e-5f;    // narrower pieces use the midpoint value
-constexpr int DBLOCK = 128;       // pixels of one row per block
+constexpr int DBLOCK = 256;       // pixels of one row per block (four wavefronts)
 constexpr int KSEG = 8;           // paths per checkpoint segment
 constexpr uint32_t VCACHE = 256;  // vertex gradients a backward block gathers in LDS per path
 constexpr uint64_t COV_MAX_BYTES = 16ull << 30;  // largest kept-coverage buffer
@@ -160,25 +160,6 @@ __device__ inline float wave_sum(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
-}
-
-// Coverage A of pixel X (row Y) by path k.
-__device__ inline float path_area(const DScene &S, uint32_t k, uint32_t off, uint32_t cnt, int X, int Y,
-                                  uint32_t &bad) {
-    const uint32_t vb = S.vtx_off[k], V = S.vtx_off[k + 1] - vb;
-    float A = 0.0f;
-    for (uint32_t q = 0; q < cnt; ++q) {
-        const uint32_t e = S.cell_edge[off + q];
-        bad |= e < V ? 0u : 1u;
-        const uint32_t i = e < V ? e : 0u;
-        const uint32_t i1 = i + 1 < V ? i + 1 : 0;
-        const Piece r = piece_of(S.vtx[vb + i], S.vtx[vb + i1], Y);
-        if (!r.live) continue;
-        float da, db;
-        const float M = mean_clamp((float)X + 1.0f - r.xa, (float)X + 1.0f - r.xb, da, db);
-        A += r.sign * r.h * M;
-    }
-    return A;
 }
 
 __device__ inline float alpha_of(float A, float &dadA) {
@@ -328,31 +309,166 @@ __global__ __launch_bounds__(BLOCK) void k_sort_cells(DScene S, DevStatus *st) {
 // ---------------------------------------------------------------------------
 // Forward, loss, backward
 // ---------------------------------------------------------------------------
+// Row staging.  A block owns DBLOCK pixels of one row of one image.  It
+// stages the (cell count, list offset) of a run of the image's paths for its
+// row (stage_paths) and then their row pieces in windows of ECAP (one
+// thread per piece: edge index, two vertices, piece_of), so that the
+// per-pixel loops read every piece from LDS with one broadcast load instead
+// of a chain of dependent global loads per edge.
+constexpr uint32_t PCHUNK = 4 * DBLOCK;  // paths staged per round (four per thread)
+constexpr uint32_t ECAP = 512;           // pieces staged per window
+constexpr uint32_t LAST_BIT = 0x80000000u, LIVE_BIT = 0x40000000u, J_MASK = 0x3FFFFFFFu;
+
+struct Stage {
+    uint32_t *cnt;  // [PCHUNK] pieces of each staged path in the row
+    uint32_t *off;  // [PCHUNK] its cell's list offset
+    uint32_t *pb;   // [PCHUNK] exclusive scan of cnt
+    float4 *pc;     // [ECAP] piece: x at top, x at bottom, sign * height, 0
+    uint2 *ev;      // [ECAP] staged path index | LAST_BIT | LIVE_BIT, edge's first vertex (local)
+    uint32_t *w;    // [DBLOCK / 64] scan scratch
+};
+
+// Counts and offsets of row Y for the n <= PCHUNK paths from global path
+// kbase; returns their total number of pieces.  Every thread calls it.
+__device__ uint32_t stage_paths(const DScene &S, uint32_t kbase, uint32_t n, int Y, const Stage &sg) {
+    const uint32_t t0 = threadIdx.x * 4u;
+    uint32_t c[4], o[4], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const bool in = t0 + (uint32_t)j < n;
+        const uint64_t cell = (uint64_t)(kbase + (in ? t0 + (uint32_t)j : 0u)) * S.H + (uint32_t)Y;
+        c[j] = in ? S.cell_cnt[cell] : 0u;
+        o[j] = in ? S.cell_off[cell] : 0u;
+        sum += c[j];
+    }
+    const uint32_t incl = wave_incl_dpp<false>(sum);
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) sg.w[wid] = incl;
+    __syncthreads();
+    uint32_t pre = incl - sum, total = 0;
+#pragma unroll
+    for (int w = 0; w < DBLOCK / 64; ++w) {
+        pre += w < wid ? sg.w[w] : 0u;
+        total += sg.w[w];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (t0 + (uint32_t)j < n) {
+            sg.cnt[t0 + j] = c[j];
+            sg.off[t0 + j] = o[j];
+            sg.pb[t0 + j] = pre;
+        }
+        pre += c[j];
+    }
+    __syncthreads();
+    return total;
+}
+
+// Pieces [e0, e0 + n) (n <= ECAP) of the np staged paths into sg.pc / sg.ev
+// at [0, n).  Piece e belongs to the last staged path t with pb[t] <= e.
+// The caller synchronises before and after.
+__device__ void stage_pieces(const DScene &S, uint32_t kbase, uint32_t np, int Y, uint32_t e0, uint32_t n,
+                             const Stage &sg, uint32_t &bad) {
+    for (uint32_t i = threadIdx.x; i < n; i += DBLOCK) {
+        const uint32_t e = e0 + i;
+        uint32_t lo = 0, hi = np;  // pb[lo] <= e < pb[hi] (pb[np] taken as the total)
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sg.pb[mid] <= e)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint32_t q = e - sg.pb[lo];
+        const uint32_t k = kbase + lo;
+        const uint32_t vb = S.vtx_off[k], V = S.vtx_off[k + 1] - vb;
+        const uint32_t edge = S.cell_edge[sg.off[lo] + q];
+        bad |= edge < V ? 0u : 1u;
+        const uint32_t i0 = edge < V ? edge : 0u;
+        const uint32_t i1 = i0 + 1 < V ? i0 + 1 : 0u;
+        const Piece r = piece_of(S.vtx[vb + i0], S.vtx[vb + i1], Y);
+        sg.pc[i] = make_float4(r.xa, r.xb, r.live ? r.sign * r.h : 0.0f, 0.0f);
+        sg.ev[i] = make_uint2(lo | (q + 1 == sg.cnt[lo] ? LAST_BIT : 0u) | (r.live ? LIVE_BIT : 0u), i0);
+    }
+}
+
+// sign * h * M of a staged piece at pixel X, M = mean_clamp(X + 1 - xa,
+// X + 1 - xb).  A wavefront whose pixels [xw0, xw0 + 63] all lie left of the
+// piece (X + 1 <= min x: M = 0) or all right of it (X >= max x: M = 1) takes
+// the answer without evaluating M; these are exactly the values mean_clamp
+// returns there.
+__device__ inline float piece_cover(float4 pc, float Xf, float xw0) {
+    const float lo = fminf(pc.x, pc.y), hi = fmaxf(pc.x, pc.y);
+    if (xw0 + 64.0f <= lo) return 0.0f;
+    if (xw0 >= hi) return pc.z;
+    float da, db;
+    return pc.z * mean_clamp(Xf + 1.0f - pc.x, Xf + 1.0f - pc.y, da, db);
+}
+
+// The image's np paths (from global path k0) composited over a transparent
+// pixel (X, Y).  With ck it also writes the canvas before every KSEG-th path
+// (the backward's checkpoints) and, with cov, each path's coverage (paths
+// with no piece in the row are left out, as their coverage is 0).  Every
+// thread of the block calls it.
+__device__ float4 forward_row(const DScene &S, const float4 *colours, uint32_t k0, uint32_t np, int X, int Y,
+                              uint64_t slot, float4 *ck, uint64_t ck_stride, float *cov, const Stage &sg,
+                              uint32_t &bad) {
+    float4 C = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float A = 0.0f;
+    uint32_t next_ck = 0;
+    const float Xf = (float)X;
+    const float xw0 = (float)(X - (int)(threadIdx.x & 63));
+    for (uint32_t ja = 0; ja < np; ja += PCHUNK) {
+        const uint32_t n = np - ja < PCHUNK ? np - ja : PCHUNK;
+        __syncthreads();  // the previous round's staging is no longer read
+        const uint32_t total = stage_paths(S, k0 + ja, n, Y, sg);
+        for (uint32_t e0 = 0; e0 < total; e0 += ECAP) {
+            const uint32_t ne = total - e0 < ECAP ? total - e0 : ECAP;
+            __syncthreads();
+            stage_pieces(S, k0 + ja, n, Y, e0, ne, sg, bad);
+            __syncthreads();
+            for (uint32_t i = 0; i < ne; ++i) {
+                A += piece_cover(sg.pc[i], Xf, xw0);
+                const uint32_t ev = sg.ev[i].x;
+                if (ev & LAST_BIT) {
+                    const uint32_t j = ja + (ev & J_MASK);
+                    if (ck)
+                        for (; next_ck * KSEG <= j; ++next_ck) ck[(uint64_t)next_ck * ck_stride + slot] = C;
+                    if (ck && cov) cov[(uint64_t)j * ck_stride + slot] = A;  // j < max_paths, slot < ck_stride
+                    float dadA;
+                    C = over(colours[k0 + j], alpha_of(A, dadA), C);
+                    A = 0.0f;
+                }
+            }
+        }
+    }
+    if (ck)
+        for (; next_ck * KSEG < np; ++next_ck) ck[(uint64_t)next_ck * ck_stride + slot] = C;
+    return C;
+}
+
+#define DIFF_STAGE_LDS                                     \
+    __shared__ uint32_t s_cnt[PCHUNK], s_off[PCHUNK];     \
+    __shared__ uint32_t s_pb[PCHUNK];                      \
+    __shared__ float4 s_pc[ECAP];                          \
+    __shared__ uint2 s_ev[ECAP];                           \
+    __shared__ uint32_t s_w[DBLOCK / 64];                  \
+    const Stage sg{s_cnt, s_off, s_pb, s_pc, s_ev, s_w}
+
 // With ck non-null it also leaves what k_backward's replay would write: the
 // canvas before every KSEG-th path, and each path's coverage when cov is
 // non-null (same slots as k_backward).
 __global__ __launch_bounds__(DBLOCK) void k_forward(DScene S, const float4 *colours, float4 *canvas, float4 *ck,
                                                     uint64_t ck_stride, float *cov, DevStatus *st) {
+    DIFF_STAGE_LDS;
     if (run_failed(st)) return;
     const int X = (int)(blockIdx.x * DBLOCK + threadIdx.x), Y = (int)blockIdx.y;
     const uint32_t im = blockIdx.z;
     const uint32_t k0 = S.img_first[im], k1 = S.img_first[im + 1];
     const uint64_t slot = ((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * (gridDim.x * DBLOCK) +
                           (uint64_t)blockIdx.x * DBLOCK + threadIdx.x;
-    float4 C = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t bad = 0;
-    for (uint32_t k = k0; k < k1; ++k) {
-        const uint32_t j = k - k0;
-        if (ck && j % KSEG == 0) ck[(uint64_t)(j / KSEG) * ck_stride + slot] = C;
-        const uint64_t c = (uint64_t)k * S.H + (uint32_t)Y;
-        const uint32_t cnt = S.cell_cnt[c];
-        if (!cnt) continue;
-        float dadA;
-        const float A = path_area(S, k, S.cell_off[c], cnt, X, Y, bad);
-        if (ck && cov) cov[(uint64_t)j * ck_stride + slot] = A;  // j < max_paths, slot < ck_stride
-        const float al = alpha_of(A, dadA);
-        C = over(colours[k], al, C);
-    }
+    const float4 C = forward_row(S, colours, k0, k1 - k0, X, Y, slot, ck, ck_stride, ck ? cov : nullptr, sg, bad);
     if (bad) raise_err(st, FT_EDEVICE);
     if ((uint32_t)X < S.W) canvas[((uint64_t)im * S.H + (uint32_t)Y) * S.W + (uint32_t)X] = C;
 }
@@ -379,63 +495,82 @@ __global__ __launch_bounds__(BLOCK) void k_loss(const float *canvas, const float
     }
 }
 
+// One lane per pixel.  Pass 1 replays the forward into the checkpoints
+// (skipped when k_forward of the same inputs has written them: `replayed`).
+// Pass 2 walks the checkpoint segments last to first: the segment's
+// coverage per path into LDS (recomputed from its staged pieces, or read
+// from the kept coverage buffer), then its paths last to first: the canvas
+// under path j re-composited from the segment's checkpoint, the colour
+// gradient, and per piece of path j the sums S1 = sum dA M, S2 = sum dA
+// dM/dua, S3 = sum dA dM/dub over the block's pixels (a wavefront wholly
+// right of the piece adds its dA total to S1 alone, one wholly left adds
+// nothing), from which one thread per piece forms the vertex gradients.
+// Colour and vertex gradients are gathered per block in LDS (vertices for
+// paths of up to VCACHE of them) and flushed with one global atomic each.
 __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *colours, const float4 *grad_canvas,
                                                      float4 *ck, uint64_t ck_stride, float *cov, uint32_t replayed,
-                                                     float2 *gvtx, float *gcol,
-                                                     DevStatus *st) {
+                                                     float2 *gvtx, float *gcol, DevStatus *st) {
+    DIFF_STAGE_LDS;
     __shared__ float s_A[KSEG][DBLOCK];  // coverage of the segment's paths; each thread uses its own column
+    __shared__ float4 s_S[ECAP];         // per staged piece: S1, S2, S3
     __shared__ float2 s_gv[VCACHE];      // the current path's vertex gradients from this block's pixels
     __shared__ float s_gc[4];            // and its colour gradient
+    __shared__ uint32_t s_any;
     if (run_failed(st)) return;
     const int tid = threadIdx.x, lane = tid & 63;
     for (uint32_t v = tid; v < VCACHE; v += DBLOCK) s_gv[v] = make_float2(0.0f, 0.0f);
     if (tid < 4) s_gc[tid] = 0.0f;
-    __syncthreads();
     const int X = (int)(blockIdx.x * DBLOCK + tid), Y = (int)blockIdx.y;
+    const float Xf = (float)X, xw0 = (float)(X - lane);
     const uint32_t im = blockIdx.z;
     const bool active = (uint32_t)X < S.W;
     const uint64_t slot = ((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * (gridDim.x * DBLOCK) +
                           (uint64_t)blockIdx.x * DBLOCK + tid;
     const uint32_t k0 = S.img_first[im], np = S.img_first[im + 1] - k0;
-    // pass 1: forward, checkpointing the canvas before every KSEG-th path
-    // (skipped when k_forward of the same inputs has written both: `replayed`)
-    float4 C = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t bad = 0;
-    for (uint32_t j = 0; !replayed && j < np; ++j) {
-        if (j % KSEG == 0) ck[(uint64_t)(j / KSEG) * ck_stride + slot] = C;
-        const uint32_t k = k0 + j;
-        const uint64_t c = (uint64_t)k * S.H + (uint32_t)Y;
-        const uint32_t cnt = S.cell_cnt[c];
-        if (!cnt) continue;
-        float dadA;
-        const float A = path_area(S, k, S.cell_off[c], cnt, X, Y, bad);
-        if (cov) cov[(uint64_t)j * ck_stride + slot] = A;  // j < max_paths, slot < ck_stride
-        const float al = alpha_of(A, dadA);
-        C = over(colours[k], al, C);
-    }
+    if (!replayed) (void)forward_row(S, colours, k0, np, X, Y, slot, ck, ck_stride, cov, sg, bad);
     float4 g = active ? grad_canvas[((uint64_t)im * S.H + (uint32_t)Y) * S.W + (uint32_t)X]
                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    // pass 2: segments last to first
     const uint32_t n_seg = (np + KSEG - 1) / KSEG;
     for (uint32_t seg = n_seg; seg-- > 0;) {
         const uint32_t j0 = seg * KSEG, j1 = j0 + KSEG < np ? j0 + KSEG : np;
-        for (uint32_t j = j0; j < j1; ++j) {
-            const uint32_t k = k0 + j;
-            const uint64_t c = (uint64_t)k * S.H + (uint32_t)Y;
-            const uint32_t cnt = S.cell_cnt[c];
-            s_A[j - j0][tid] = !cnt ? 0.0f
-                               : cov  ? cov[(uint64_t)j * ck_stride + slot]
-                                      : path_area(S, k, S.cell_off[c], cnt, X, Y, bad);
+        const uint32_t ns = j1 - j0;
+        __syncthreads();  // the previous segment's staging is no longer read
+        const uint32_t total = stage_paths(S, k0 + j0, ns, Y, sg);
+        // the segment's coverage per path
+        for (uint32_t j = 0; j < ns; ++j) s_A[j][tid] = 0.0f;
+        uint32_t w_e0 = 0, w_n = 0;  // the staged window of pieces
+        if (cov) {
+            for (uint32_t j = 0; j < ns; ++j)
+                if (s_cnt[j]) s_A[j][tid] = cov[(uint64_t)(j0 + j) * ck_stride + slot];
+        } else {
+            float A = 0.0f;
+            for (uint32_t e0 = 0; e0 < total; e0 += ECAP) {
+                const uint32_t ne = total - e0 < ECAP ? total - e0 : ECAP;
+                __syncthreads();
+                stage_pieces(S, k0 + j0, ns, Y, e0, ne, sg, bad);
+                __syncthreads();
+                w_e0 = e0;
+                w_n = ne;
+                for (uint32_t i = 0; i < ne; ++i) {
+                    A += piece_cover(s_pc[i], Xf, xw0);
+                    const uint32_t ev = s_ev[i].x;
+                    if (ev & LAST_BIT) {
+                        s_A[ev & J_MASK][tid] = A;
+                        A = 0.0f;
+                    }
+                }
+            }
         }
         const float4 Cseg = ck[(uint64_t)seg * ck_stride + slot];
-        for (uint32_t j = j1; j-- > j0;) {
-            const uint32_t k = k0 + j;
+        for (uint32_t jl = ns; jl-- > 0;) {
+            const uint32_t j = j0 + jl, k = k0 + j;
             float4 Cb = Cseg;  // canvas before path j
-            for (uint32_t i = j0; i < j; ++i) {
+            for (uint32_t i = 0; i < jl; ++i) {
                 float d;
-                Cb = over(colours[k0 + i], alpha_of(s_A[i - j0][tid], d), Cb);
+                Cb = over(colours[k0 + j0 + i], alpha_of(s_A[i][tid], d), Cb);
             }
-            const float A = s_A[j - j0][tid];
+            const float A = s_A[jl][tid];
             float dadA;
             const float al = alpha_of(A, dadA);
             const float4 cc = colours[k];
@@ -445,7 +580,7 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
             if (__ballot(al != 0.0f)) {
                 const float r0 = wave_sum(g.x * al), r1 = wave_sum(g.y * al), r2 = wave_sum(g.z * al);
                 const float r3 = wave_sum(g.w * al - al * gdotC);
-                if (lane == 0) {  // al != 0 needs edges in this row: flushed below
+                if (lane == 0) {  // al != 0 needs pieces in this row: flushed below
                     atomicAdd(&s_gc[0], r0);
                     atomicAdd(&s_gc[1], r1);
                     atomicAdd(&s_gc[2], r2);
@@ -455,60 +590,91 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
             const float keep = 1.0f - cc.w * al;
             g = make_float4(g.x * keep, g.y * keep, g.z * keep, g.w * keep);
             const float dA = active ? dal * dadA : 0.0f;
-            // k and Y are the same for the whole block, so cnt, V and
-            // `gather` are too: the barriers below are reached by every thread
-            const uint64_t c = (uint64_t)k * S.H + (uint32_t)Y;
-            const uint32_t cnt = S.cell_cnt[c], off = S.cell_off[c];
+            // s_cnt, V and `gather` are the same for the whole block: the
+            // barriers below are reached by every thread
+            const uint32_t cnt = s_cnt[jl];
+            if (!cnt) continue;
             const uint32_t vb = S.vtx_off[k], V = S.vtx_off[k + 1] - vb;
             const bool gather = V <= VCACHE;
-            if (__ballot(dA != 0.0f)) {
-                for (uint32_t q = 0; q < cnt; ++q) {
-                    const uint32_t e = S.cell_edge[off + q];
-                    bad |= e < V ? 0u : 1u;
-                    const uint32_t i = e < V ? e : 0u;
-                    const uint32_t i1 = i + 1 < V ? i + 1 : 0;
-                    const float2 p = S.vtx[vb + i], pq = S.vtx[vb + i1];
-                    const Piece r = piece_of(p, pq, Y);
-                    if (!r.live) continue;
+            if (tid == 0) s_any = 0;
+            __syncthreads();
+            const bool wave_on = __ballot(dA != 0.0f) != 0;
+            if (wave_on && lane == 0) s_any = 1;
+            const float Tw = wave_sum(dA);  // the wavefront's dA total
+            __syncthreads();
+            // pieces of path j: [pb, pb + cnt) of the staged paths, in windows
+            const uint32_t pb = s_pb[jl];
+            for (uint32_t e0 = pb; s_any && e0 < pb + cnt; e0 += ECAP) {
+                const uint32_t ne = pb + cnt - e0 < ECAP ? pb + cnt - e0 : ECAP;
+                uint32_t base = e0 - w_e0;  // where the window's first piece sits
+                if (!(e0 >= w_e0 && e0 + ne <= w_e0 + w_n)) {
+                    __syncthreads();
+                    stage_pieces(S, k0 + j0, ns, Y, e0, ne, sg, bad);
+                    w_e0 = e0;
+                    w_n = ne;
+                    base = 0;
+                }
+                for (uint32_t i = tid; i < ne; i += DBLOCK) s_S[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                __syncthreads();
+                for (uint32_t i = 0; wave_on && i < ne; ++i) {
+                    const float4 pc = s_pc[base + i];
+                    if (!(s_ev[base + i].x & LIVE_BIT)) continue;
+                    const float lo = fminf(pc.x, pc.y), hi = fmaxf(pc.x, pc.y);
+                    if (xw0 + 64.0f <= lo) continue;  // M = 0 on every pixel of the wavefront
+                    if (xw0 >= hi) {                 // M = 1, dM = 0
+                        if (lane == 0) atomicAdd(&s_S[i].x, Tw);
+                        continue;
+                    }
                     float mua, mub;
-                    const float M = mean_clamp((float)X + 1.0f - r.xa, (float)X + 1.0f - r.xb, mua, mub);
+                    const float M = mean_clamp(Xf + 1.0f - pc.x, Xf + 1.0f - pc.y, mua, mub);
                     const float S1 = wave_sum(dA * M), S2 = wave_sum(dA * mua), S3 = wave_sum(dA * mub);
                     if (lane == 0) {
-                        float gr[4];
-                        piece_grad(p, pq, Y, S1, S2, S3, gr);
-                        if (gather) {  // i, i1 < V <= VCACHE
-                            atomicAdd(&s_gv[i].x, gr[0]);
-                            atomicAdd(&s_gv[i].y, gr[1]);
-                            atomicAdd(&s_gv[i1].x, gr[2]);
-                            atomicAdd(&s_gv[i1].y, gr[3]);
-                        } else {
-                            atomicAdd(&gvtx[vb + i].x, gr[0]);
-                            atomicAdd(&gvtx[vb + i].y, gr[1]);
-                            atomicAdd(&gvtx[vb + i1].x, gr[2]);
-                            atomicAdd(&gvtx[vb + i1].y, gr[3]);
-                        }
-                    }
-                }
-            }
-            if (cnt != 0) {  // one global atomic per colour channel and touched vertex, per block
-                __syncthreads();
-                if (tid < 4) {
-                    const float a = s_gc[tid];
-                    if (a != 0.0f) {
-                        atomicAdd(&gcol[4 * (uint64_t)k + tid], a);
-                        s_gc[tid] = 0.0f;
-                    }
-                }
-                for (uint32_t v = tid; gather && v < V; v += DBLOCK) {
-                    const float2 a = s_gv[v];
-                    if ((a.x != 0.0f) | (a.y != 0.0f)) {
-                        atomicAdd(&gvtx[vb + v].x, a.x);
-                        atomicAdd(&gvtx[vb + v].y, a.y);
-                        s_gv[v] = make_float2(0.0f, 0.0f);
+                        atomicAdd(&s_S[i].x, S1);
+                        atomicAdd(&s_S[i].y, S2);
+                        atomicAdd(&s_S[i].z, S3);
                     }
                 }
                 __syncthreads();
+                for (uint32_t i = tid; i < ne; i += DBLOCK) {
+                    const uint2 ev = s_ev[base + i];
+                    const float4 Sm = s_S[i];
+                    if (!(ev.x & LIVE_BIT) || ((Sm.x == 0.0f) & (Sm.y == 0.0f) & (Sm.z == 0.0f))) continue;
+                    const uint32_t i0 = ev.y < V ? ev.y : 0u;  // staged below V
+                    const uint32_t i1 = i0 + 1 < V ? i0 + 1 : 0u;
+                    float gr[4];
+                    piece_grad(S.vtx[vb + i0], S.vtx[vb + i1], Y, Sm.x, Sm.y, Sm.z, gr);
+                    if (gather) {  // i0, i1 < V <= VCACHE
+                        atomicAdd(&s_gv[i0].x, gr[0]);
+                        atomicAdd(&s_gv[i0].y, gr[1]);
+                        atomicAdd(&s_gv[i1].x, gr[2]);
+                        atomicAdd(&s_gv[i1].y, gr[3]);
+                    } else {
+                        atomicAdd(&gvtx[vb + i0].x, gr[0]);
+                        atomicAdd(&gvtx[vb + i0].y, gr[1]);
+                        atomicAdd(&gvtx[vb + i1].x, gr[2]);
+                        atomicAdd(&gvtx[vb + i1].y, gr[3]);
+                    }
+                }
+                __syncthreads();  // s_S and the window are reused
             }
+            // one global atomic per colour channel and touched vertex, per block
+            __syncthreads();
+            if (tid < 4) {
+                const float a = s_gc[tid];
+                if (a != 0.0f) {
+                    atomicAdd(&gcol[4 * (uint64_t)k + tid], a);
+                    s_gc[tid] = 0.0f;
+                }
+            }
+            for (uint32_t v = tid; gather && v < V; v += DBLOCK) {
+                const float2 a = s_gv[v];
+                if ((a.x != 0.0f) | (a.y != 0.0f)) {
+                    atomicAdd(&gvtx[vb + v].x, a.x);
+                    atomicAdd(&gvtx[vb + v].y, a.y);
+                    s_gv[v] = make_float2(0.0f, 0.0f);
+                }
+            }
+            __syncthreads();
         }
     }
     if (bad) raise_err(st, FT_EDEVICE);
