@@ -103,9 +103,10 @@ def bench_atlas(ft, steps, warmup, tuning=None, cpu_seconds=4.0):
                       fresh=True) for b, c, x, y, s in tiles]
     batch = Batch(ops, descs)
     if tuning:
-        b_rows, cells = (int(v) for v in tuning.split(",")[:2])
-        batch.set_tuning(b_rows, cells, True)
-        tuned = {"band_rows_min": b_rows, "lds_cells": cells, "tried": 0}
+        v = [int(x) for x in tuning.split(",")]
+        b_rows, cells, tg = v[0], v[1], (v[2] if len(v) > 2 else 0)
+        batch.set_tuning(b_rows, cells, True, tg)
+        tuned = {"band_rows_min": b_rows, "lds_cells": cells, "tile_groups": tg, "tried": 0}
     else:
         from footile_amd.tune import autotune
         tuned = autotune(batch, descs, len(ops), cache=None)
@@ -133,6 +134,7 @@ def bench_atlas(ft, steps, warmup, tuning=None, cpu_seconds=4.0):
     tile_px = int((tiles[:, 4] ** 2).sum())
     bytes_per = tile_px + op_bytes(ops)
     offered, stored, edges = batch.counts()
+    pairs, cell_adds = batch.raster_stats()  # one more (eager) run that counts the raster's work
     # spot-check tiles against the oracle (fresh s x s Matte8 per tile)
     from oracle import pyoracle as po
     img = atlas.download().reshape(side, side)
@@ -156,17 +158,31 @@ def bench_atlas(ft, steps, warmup, tuning=None, cpu_seconds=4.0):
         "eager_device_ms_per_step": eager_total_ms / max(runs, 1),
         "launch": "hipGraph replay of the whole launch sequence",
         "tuning": {"band_rows_min": tuned["band_rows_min"], "lds_cells": tuned["lds_cells"],
-                   "candidates": tuned["tried"], "tune_s": round(tuned.get("tune_s", 0.0), 3)},
+                   "tile_groups": tuned.get("tile_groups", 0), "candidates": tuned["tried"],
+                   "tune_s": round(tuned.get("tune_s", 0.0), 3)},
         "raster_kernel_ms": raster_avg,
         "raster_kernel_gbs": bytes_per / (raster_avg * 1e-3) / 1e9 if raster_avg > 0 else None,
         "edges_per_pixel": float(edges.sum()) / tile_px,
+        "edge_row_pairs_per_pixel": pairs / tile_px,
+        "lds_atomics_per_pixel": cell_adds / tile_px,
+        "lds_atomics_per_s": cell_adds / (raster_avg * 1e-3) if raster_avg > 0 else None,
         "points_stored": int(stored.sum()),
         "tiles_checked_vs_oracle": checked,
         "cpu_baseline": cpu_atlas(ops, tiles, cpu_seconds, threads=min(16, len(os.sched_getaffinity(0)))),
+        "host_affinity_cores": len(os.sched_getaffinity(0)),
     }
 
 
+def _median_mad(samples):
+    a = np.asarray(samples)
+    med = float(np.median(a))
+    return med, float(np.median(np.abs(a - med)))
+
+
 def bench_plain(ft, reps=50):
+    """C1 through the Plotter API, per call: median and MAD of `reps` calls
+    (GPU: host ops, upload, launch, one host wait; CPU: the oracle on one
+    core, raster allocation inside each sample as fishyb.rs:35 does)."""
     from footile_amd import workloads
     from oracle import pyoracle as po
     fill_ops = workloads.plain_256(0)
@@ -175,21 +191,45 @@ def bench_plain(ft, reps=50):
     for name, call in [("fill", lambda p: p.fill(0, fill_ops)), ("stroke", lambda p: p.stroke(stroke_ops))]:
         p = ft.Plotter(ft.Raster.with_clear(256, 256))
         call(p)
-        t0 = time.perf_counter()
+        ts = []
         for _ in range(reps):
+            t0 = time.perf_counter()
             call(p)
-        out[f"gpu_{name}_us"] = (time.perf_counter() - t0) / reps * 1e6
-        o = po.Plotter(256, 256)
-        t0 = time.perf_counter()
+            ts.append(time.perf_counter() - t0)
+        med, mad = _median_mad(ts)
+        out[f"gpu_{name}_us"], out[f"gpu_{name}_mad_us"] = med * 1e6, mad * 1e6
+        ts = []
         for _ in range(reps):
+            t0 = time.perf_counter()
+            o = po.Plotter(256, 256)
             if name == "fill":
                 o.fill(0, fill_ops)
             else:
                 o.stroke(stroke_ops)
-        out[f"cpu_{name}_us"] = (time.perf_counter() - t0) / reps * 1e6
+            ts.append(time.perf_counter() - t0)
+        med, mad = _median_mad(ts)
+        out[f"cpu_{name}_us"], out[f"cpu_{name}_mad_us"] = med * 1e6, mad * 1e6
+    out["samples"] = reps
     out["workload"] = "plain 256x256 Matte8, 48 line + 16 cubic (seed 0); fill NonZero, stroke w=4 " \
-                      "(Plotter API: host ops, upload + launch + sync per call)"
+                      "(Plotter API: host ops, upload + launch + sync per call); median and MAD per call"
     return out
+
+
+def bench_plotter_random_walk(ft, ops, reps=10):
+    """The random-walk NonZero fill through the drop-in entry (ft_plotter_fill:
+    host ops uploaded per call, one host wait): ms per fill."""
+    from footile_amd.pixel import RGBA8P
+    p = ft.Plotter(ft.Raster.with_clear(4096, 4096, RGBA8P))
+    p.fill(0, ops, (200, 120, 40, 255))
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        p.fill(0, ops, (200, 120, 40, 255))
+        ts.append(time.perf_counter() - t0)
+    med, mad = _median_mad(ts)
+    return {"ms_per_fill": med * 1e3, "mad_ms": mad * 1e3, "mpix_per_s": 4096 * 4096 / med / 1e6,
+            "workload": "random walk 4096x4096 Rgba8p NonZero through ft_plotter_fill (280 KB of ops "
+                        "uploaded per call)", "samples": reps}
 
 
 def _grad_rel(a, b):
@@ -338,7 +378,8 @@ def main():
                     help="band_rows_min,lds_cells,dst_prefetch for the random-walk batch instead of autotuning "
                          "(profiling runs)")
     ap.add_argument("--atlas-tuning", default=None,
-                    help="band_rows_min,lds_cells for the atlas instead of autotuning (profiling runs)")
+                    help="band_rows_min,lds_cells[,tile_groups] for the atlas instead of autotuning "
+                         "(profiling runs)")
     ap.add_argument("--only", choices=["atlas", "diff"], default=None,
                     help="run only the atlas or the differentiable measurement and print its JSON (profiling runs)")
     args = ap.parse_args()
@@ -437,6 +478,8 @@ def main():
         extra["atlas"] = bench_atlas(ft, max(3, args.steps // 4), 1, args.atlas_tuning, min(4.0, args.cpu_seconds))
         print("[bench] plain 256", file=sys.stderr, flush=True)
         extra["plain_256"] = bench_plain(ft)
+        print("[bench] random walk through the Plotter API", file=sys.stderr, flush=True)
+        extra["random_walk_plotter_api"] = bench_plotter_random_walk(ft, ops)
         print("[bench] differentiable C4 / C5", file=sys.stderr, flush=True)
         extra["diff"] = bench_diff(max(4, args.steps // 2), 2, check=not args.no_diff_check)
 

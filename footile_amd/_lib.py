@@ -67,6 +67,7 @@ def _declare(L):
         "ft_batch_set_timing": ([vp, i], i),
         "ft_batch_timing": ([vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), pu64], i),
         "ft_batch_run_noalloc": ([vp, sz, vp, sz, vp, vp, vp], i),
+        "ft_batch_raster_stats": ([vp, pu64, pu64], i),
         "ft_device_alloc": ([sz, pvp], i),
         "ft_device_free": ([vp], i),
         "ft_memcpy_h2d": ([vp, vp, sz], i),

@@ -3,8 +3,8 @@
 // ft_plotter mirrors footile::Plotter<P> (src/plotter.rs:38-381): it owns its
 // raster (here in device memory), its tolerance, transform, join style and
 // the stroke width that persists across calls.  fill and stroke run the whole
-// rasteriser on the GPU; stroke's outline construction (src/stroker.rs) runs
-// on the host between the device flattening pass and the device fill.
+// rasteriser on the GPU, stroke's outline construction (src/stroker.rs)
+// included: it runs between the device flattening and the device fill.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <math.h>
@@ -305,13 +305,14 @@ ft_status ft_batch_new(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *
 
 ft_status ft_batch_set_tuning(ft_batch *b, const ft_tuning *t) {
     if (!b || !t) return FT_EINVAL;
-    if (t->band_rows_min < 1 || t->band_rows_min > 256 || t->lds_cells < 1024 || t->lds_cells > 16384 ||
+    if (t->band_rows_min < 1 || t->band_rows_min > 256 || t->lds_cells < 1024 || t->lds_cells > 32768 ||
         t->lds_cells % ft::CELL_ALIGN)
         return FT_EINVAL;
     ft::Tuning tu;
     tu.band_rows_min = t->band_rows_min;
     tu.lds_cells = t->lds_cells;
     tu.dst_prefetch = t->dst_prefetch ? 1u : 0u;
+    tu.tile_groups = t->tile_groups;
     const ft::Tuning old = b->ws.tuning;
     b->ws.tuning = tu;
     const ft_status s = ft::ws_set_batch(b->ws, b->ops.data(), b->ops.size(), b->descs.data(), b->descs.size());
@@ -327,6 +328,7 @@ ft_status ft_batch_get_tuning(const ft_batch *b, ft_tuning *t) {
     t->band_rows_min = b->ws.tuning.band_rows_min;
     t->lds_cells = b->ws.tuning.lds_cells;
     t->dst_prefetch = b->ws.tuning.dst_prefetch;
+    t->tile_groups = b->ws.tuning.tile_groups;
     return FT_OK;
 }
 
@@ -375,6 +377,11 @@ ft_status ft_batch_counts(ft_batch *b, uint64_t *offered, uint64_t *stored, uint
                                              : ((int32_t)((uint32_t)(s.topkey >> 32) ^ 0x80000000u)) >> 16;
     }
     return FT_OK;
+}
+
+ft_status ft_batch_raster_stats(ft_batch *b, uint64_t *pairs, uint64_t *cell_adds) {
+    if (!b) return FT_EINVAL;
+    return ft::ws_raster_stats(b->ws, pairs, cell_adds);
 }
 
 ft_status ft_batch_stream(ft_batch *b, void **stream) {

@@ -48,6 +48,17 @@ struct Workspace {
     size_t n_rblocks = 0, cap_rblocks = 0;
     uint32_t rb_first[3] = {0, 0, 0}, rb_count[3] = {0, 0, 0};  // raster blocks per pixel format
     uint4 *d_rblock = nullptr;  // raster blocks: (global band, row group, path, 0)
+    // tile scheduler work items (path, row group) of tile-mode paths, per
+    // pixel format, largest first; and the persistent grid per format
+    std::vector<uint2> h_items;
+    uint2 *d_items = nullptr;
+    size_t cap_items = 0;
+    uint32_t it_first[3] = {0, 0, 0}, it_count[3] = {0, 0, 0};
+    uint32_t tile_grid[3] = {0, 0, 0};
+    // raster statistics (ws_raster_stats): per raster block, (edge, row)
+    // pairs walked and nonzero cell deltas added
+    bool stats = false;
+    unsigned long long *d_stats = nullptr;
     // per-op scratch
     unsigned long long *d_op_cnt = nullptr, *d_op_off = nullptr;
     unsigned long long *d_sub_off = nullptr;  // per (op, lane): offset within the op (+ leaf count)
@@ -136,6 +147,9 @@ ft_status ws_launch(Workspace &ws, uint32_t mode);
 ft_status ws_finish(Workspace &ws, bool read_pstate);
 // launch + finish, growing capacities and relaunching on overflow.
 ft_status ws_run(Workspace &ws, uint32_t mode, bool read_pstate);
+// Run the batch once (eagerly) counting the raster's (edge, row) pairs and
+// its nonzero cell deltas (the LDS atomics of the cell pass).
+ft_status ws_raster_stats(Workspace &ws, uint64_t *pairs, uint64_t *adds);
 // Count the last fill run's non-horizontal edges per path into
 // PathState::n_edges (and h_pstate).
 ft_status ws_count_edges(Workspace &ws);

@@ -1187,48 +1187,38 @@ __device__ __forceinline__ void row_multi_any(uint32_t rule, bool fresh, const i
     }
 }
 
-// One block per (band, row group): `rblock[b]` = (global band, group, path);
-// one kernel per pixel format FMT, over the format's stretch of the table
-// from `first`.  The block accumulates the band's listed edges for its G rows
-// in LDS and scans each row into the raster; the band's other groups are
-// other blocks.  Dynamic LDS: front_bytes<FMT>() for the cell pass's edge
+// Edge sources of a raster group: entry k of a band's list, or point k of a
+// path's stored points (every point starts its segment's edge, if any).
+struct BandListSrc {
+    const uint32_t *list;
+    uint32_t off;
+    __device__ uint32_t operator()(uint32_t k) const { return list[off + k]; }
+};
+struct PathEdgeSrc {
+    uint32_t base;
+    __device__ uint32_t operator()(uint32_t k) const { return base + k; }
+};
+
+// Rows [y0, y1) of path P's raster (one LDS group): the cnt listed edges'
+// (edge, row) cell deltas into LDS (i32 atomics), the wave row scan, the rule
+// map and the store / SrcOver.  Every thread of the block calls it with the
+// same arguments.  Dynamic LDS: front_bytes<FMT>() for the cell pass's edge
 // slots and then the composite table (Rgba8p, Graya8p), followed by the
 // rows' i32 cells.
-template <bool PF, int FMT>
-__global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *paths, const uint4 *rblock,
-                                                  uint32_t first_block, uint32_t band_shift,
-                                                  const uint32_t *band_off, const uint32_t *band_list,
-                                                  const EdgeRec *edges, const PathState *ps, DevStatus *st) {
-    extern __shared__ uint4 s_dyn[];
-    __shared__ uint32_t s_segsum[BLOCK / 64];
+// n_pairs / n_adds: the calling thread's running counts of the (edge, row)
+// pairs it walked and of the nonzero cell deltas it added (LDS atomics).
+template <bool PF, int FMT, class Src>
+__device__ __forceinline__ void raster_group(const Limits &L, const PathDev &P, const PathState &S, int fwd,
+                                             uint32_t y0, uint32_t y1, Src src, uint32_t cnt, bool list_ok,
+                                             const EdgeRec *edges, uint4 *s_dyn, uint32_t *s_segsum,
+                                             DevStatus *st, uint32_t &n_pairs_acc, uint32_t &n_adds_acc) {
     constexpr bool HAS_LUT = FMT != FMT_MATTE8;
-    // The front front_bytes<FMT>() hold the cell pass's edge slots, then the
-    // composite table; the rows' cells follow.
     constexpr uint32_t CR = cell_round<FMT>();
     uint4 *const lut = s_dyn;
     EdgeRec *const s_edge = reinterpret_cast<EdgeRec *>(s_dyn);
     uint32_t *const s_pair0 = reinterpret_cast<uint32_t *>(s_edge + CR);
     int32_t *const s_cells = reinterpret_cast<int32_t *>(s_dyn + front_bytes<FMT>() / sizeof(uint4));
-    if (run_failed(st)) return;
-    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
-    const uint4 rb = rblock[first_block + blockIdx.x];
-    const uint32_t gb = rb.x, p = rb.z;
-    if ((gb >= L.n_bands) | (p >= L.n_paths)) {
-        if (threadIdx.x == 0) raise_err(st, FT_EDEVICE);
-        return;
-    }
-    // the band's list: its replica segments, back to back
-    const uint32_t off = band_off[gb << band_shift];
-    const uint32_t end = gb + 1 < L.n_bands ? band_off[(gb + 1) << band_shift] : L.n_pairs;
-    const PathDev &P = paths[p];
-    const uint32_t R = P.band_rows, H = P.height, W = P.width;
-    const uint32_t G = P.group_rows ? P.group_rows : 1u;  // rows whose cells fit in LDS at once
-    const uint32_t b0 = (gb - P.band_base) * R;
-    const uint32_t b1 = b0 + R < H ? b0 + R : H;
-    const uint32_t y0 = b0 + rb.y * G;
-    const uint32_t y1 = y0 + G < b1 ? y0 + G : b1;
-    if (y0 >= y1) return;
-    const PathState S = ps[p];
+    const uint32_t H = P.height, W = P.width;
     const bool has_pts = S.topkey != NO_KEY && S.topidx != NO_IDX;
     const int32_t top_row = has_pts ? top_row_of(S.topkey) : 0;
     const int64_t first = !has_pts ? (int64_t)H : (top_row > 0 ? top_row : 0);
@@ -1293,10 +1283,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
     for (uint32_t c = threadIdx.x; c < rows * Wp / 4; c += BLOCK)
         reinterpret_cast<int4 *>(s_cells)[c] = make_int4(0, 0, 0, 0);
     __syncthreads();
-    const int fwd = (int)S.fwd;
     const int32_t shift = top_row < 0 ? top_row : 0;
-    const bool list_ok = (off <= end) & (end <= L.n_pairs);
-    const uint32_t cnt = list_ok ? end - off : 0u;
     if (!list_ok && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
     // Cell pass, taller groups: the block takes CR list entries at a time; their
     // (edge, row) pairs are laid out by a scan of the rows per edge and spread
@@ -1315,7 +1302,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
         for (int j = 0; j < EB; ++j) {
             const uint32_t k = k0 + (uint32_t)j * BLOCK + threadIdx.x;
             ok[j] = k < cnt;
-            ei[j] = band_list[off + (ok[j] ? k : 0u)];
+            ei[j] = src(ok[j] ? k : 0u);
         }
         EdgeRec E[EB];
 #pragma unroll
@@ -1323,6 +1310,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
             if (ok[j] & (ei[j] >= L.cap_points)) raise_err(st, FT_EDEVICE);
             ok[j] = ok[j] & (ei[j] < L.cap_points);
             E[j] = edges[ok[j] ? ei[j] : 0u];
+            ok[j] = ok[j] & ((E[j].flags & 1u) != 0);  // a point without an edge (path edge lists)
         }
 #pragma unroll
         for (int j = 0; j < EB; ++j) {
@@ -1331,18 +1319,27 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
             int64_t a = (int64_t)fx_int(E[j].y_upper) - shift, b = (int64_t)fx_int(E[j].y_lower) - shift;
             a = a < (int64_t)ry0 ? (int64_t)ry0 : a;
             b = b > (int64_t)y1 - 1 ? (int64_t)y1 - 1 : b;
-            for (int64_t r = a; r <= b; ++r)
-                edge_row_cells<DevAtom>(E[j], (int32_t)(r + shift), ed, (int32_t)W,
-                                        s_cells + (size_t)(r - ry0) * Wp);
+            for (int64_t r = a; r <= b; ++r) {  // edge_row_cells, counted
+                const RowCells c = row_setup(E[j], (int32_t)(r + shift), (int32_t)W);
+                int32_t *const at = s_cells + (size_t)(r - ry0) * Wp + c.x_s;
+                ++n_pairs_acc;
+                row_walk(c, [&](int32_t jj, int32_t d) {
+                    if (d) {
+                        atomicAdd(&at[jj], d * ed);
+                        ++n_adds_acc;
+                    }
+                });
+            }
         }
     }
     for (uint32_t k0 = 0; !direct && list_ok && k0 < cnt; k0 += CR) {
         const uint32_t k = k0 + threadIdx.x;
         bool okk = (threadIdx.x < CR) & (k < cnt);
-        const uint32_t ei = band_list[off + (okk ? k : 0u)];
+        const uint32_t ei = src(okk ? k : 0u);
         if (okk & (ei >= L.cap_points)) raise_err(st, FT_EDEVICE);
         okk = okk & (ei < L.cap_points);
         const EdgeRec E = edges[okk ? ei : 0u];
+        okk = okk & ((E.flags & 1u) != 0);  // a point without an edge (path edge lists)
         int64_t ra = (int64_t)fx_int(E.y_upper) - shift, rb = (int64_t)fx_int(E.y_lower) - shift;
         ra = ra < (int64_t)ry0 ? (int64_t)ry0 : ra;
         rb = rb > (int64_t)y1 - 1 ? (int64_t)y1 - 1 : rb;
@@ -1372,8 +1369,12 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
             const int32_t ed = ((int)((Eo.flags >> 1) & 1u) == fwd) ? 1 : -1;
             const RowCells c = row_setup(Eo, row + shift, (int32_t)W);
             int32_t *const cr = s_cells + (size_t)(row - (int32_t)ry0) * Wp + c.x_s;
+            ++n_pairs_acc;
             row_walk(c, [&](int32_t j, int32_t d) {
-                if (d) atomicAdd(&cr[j], d * ed);
+                if (d) {
+                    atomicAdd(&cr[j], d * ed);
+                    ++n_adds_acc;
+                }
             });
         }
         __syncthreads();  // before the next entries overwrite the slots
@@ -1435,6 +1436,128 @@ __global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *path
         }
         loaded = false;
     }
+}
+
+
+// The block's totals of two per-thread counts into stats[2 slot], [2 slot + 1]
+// (when stats is given).  Every thread calls it.
+__device__ inline void block_stats(uint32_t a, uint32_t b, unsigned long long *stats, uint32_t slot,
+                                   uint32_t *s_w) {
+    if (!stats) return;
+    const uint32_t ia = wave_incl_u32(a), ib = wave_incl_u32(b);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 63) {
+        s_w[2 * (threadIdx.x >> 6)] = ia;
+        s_w[2 * (threadIdx.x >> 6) + 1] = ib;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long ta = 0, tb = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) {
+            ta += s_w[2 * w];
+            tb += s_w[2 * w + 1];
+        }
+        stats[2 * (size_t)slot] = ta;
+        stats[2 * (size_t)slot + 1] = tb;
+    }
+}
+
+// One block per (band, row group): `rblock[b]` = (global band, group, path);
+// one kernel per pixel format FMT, over the format's stretch of the table
+// from `first`.  The block accumulates the band's listed edges for its G rows
+// in LDS and scans each row into the raster; the band's other groups are
+// other blocks.
+template <bool PF, int FMT>
+__global__ __launch_bounds__(BLOCK) void k_raster(Limits L0, const PathDev *paths, const uint4 *rblock,
+                                                  uint32_t first_block, uint32_t band_shift,
+                                                  const uint32_t *band_off, const uint32_t *band_list,
+                                                  const EdgeRec *edges, const PathState *ps, DevStatus *st,
+                                                  unsigned long long *stats) {
+    extern __shared__ uint4 s_dyn[];
+    __shared__ uint32_t s_segsum[BLOCK / 64];
+    __shared__ uint32_t s_stat[2 * BLOCK / 64];
+    if (run_failed(st)) return;
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    const uint4 rb = rblock[first_block + blockIdx.x];
+    const uint32_t gb = rb.x, p = rb.z;
+    if ((gb >= L.n_bands) | (p >= L.n_paths)) {
+        if (threadIdx.x == 0) raise_err(st, FT_EDEVICE);
+        return;
+    }
+    // the band's list: its replica segments, back to back
+    const uint32_t off = band_off[gb << band_shift];
+    const uint32_t end = gb + 1 < L.n_bands ? band_off[(gb + 1) << band_shift] : L.n_pairs;
+    const PathDev &P = paths[p];
+    const uint32_t R = P.band_rows, H = P.height;
+    const uint32_t G = P.group_rows ? P.group_rows : 1u;  // rows whose cells fit in LDS at once
+    const uint32_t b0 = (gb - P.band_base) * R;
+    const uint32_t b1 = b0 + R < H ? b0 + R : H;
+    const uint32_t y0 = b0 + rb.y * G;
+    const uint32_t y1 = y0 + G < b1 ? y0 + G : b1;
+    uint32_t n_pairs = 0, n_adds = 0;
+    if (y0 < y1) {
+        const PathState S = ps[p];
+        const bool list_ok = (off <= end) & (end <= L.n_pairs);
+        raster_group<PF, FMT>(L, P, S, (int)S.fwd, y0, y1, BandListSrc{band_list, off},
+                              list_ok ? end - off : 0u, list_ok, edges, s_dyn, s_segsum, st, n_pairs, n_adds);
+    }
+    block_stats(n_pairs, n_adds, stats, first_block + blockIdx.x, s_stat);
+}
+
+// The atlas tile scheduler: a persistent grid whose blocks take work items
+// (path, row group) of the paths laid out in tile mode from a queue (one
+// counter per pixel format, cleared with the run's status) until it is
+// empty.  Items are ordered largest first.  A tile-mode path has no bands:
+// an item reads the path's own stored points as its edge list (one edge per
+// point, in point order), so nothing is binned.  The winding direction
+// (Fig::get_dir) is evaluated per item.
+template <bool PF, int FMT>
+__global__ __launch_bounds__(BLOCK) void k_tiles(Limits L0, const PathDev *paths, const uint2 *items,
+                                                 uint32_t first_item, uint32_t n_items, const EdgeRec *edges,
+                                                 const I2 *pts, const uint32_t *pt_run, const uint32_t *run_start,
+                                                 const uint32_t *run_end, const PathState *ps, DevStatus *st,
+                                                 unsigned long long *stats, uint32_t stats_slot) {
+    extern __shared__ uint4 s_dyn[];
+    __shared__ uint32_t s_segsum[BLOCK / 64];
+    __shared__ uint32_t s_stat[2 * BLOCK / 64];
+    __shared__ uint32_t s_item, s_fwd;
+    if (run_failed(st)) return;
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t n_pairs = 0, n_adds = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            s_item = atomicAdd(&st->tile_next[FMT], 1u);
+            s_fwd = 0;
+        }
+        __syncthreads();
+        const uint32_t it = s_item;
+        if (it >= n_items) break;  // block-uniform: every thread leaves
+        const uint2 w = items[first_item + it];
+        const uint32_t p = w.x;
+        if (p >= L.n_paths) {
+            if (threadIdx.x == 0) raise_err(st, FT_EDEVICE);
+            break;
+        }
+        const PathDev &P = paths[p];
+        const PathState S = ps[p];
+        const bool has_pts = S.topkey != NO_KEY && S.topidx != NO_IDX;
+        if (threadIdx.x == 0 && has_pts) {
+            uint32_t bad = 0;
+            s_fwd = path_forward(S, L, pts, pt_run, run_start, run_end, bad) ? 1u : 0u;
+            if (bad) raise_err(st, FT_EDEVICE);
+        }
+        const uint32_t G = P.group_rows ? P.group_rows : 1u;
+        const uint32_t y0 = w.y * G;
+        const uint32_t y1 = y0 + G < P.height ? y0 + G : P.height;
+        const bool list_ok = ((uint64_t)S.pt_base + S.stored) <= L.cap_points;
+        __syncthreads();  // s_fwd
+        if (y0 < y1)
+            raster_group<PF, FMT>(L, P, S, (int)s_fwd, y0, y1, PathEdgeSrc{S.pt_base},
+                                  (has_pts && list_ok) ? S.stored : 0u, list_ok, edges, s_dyn, s_segsum, st,
+                                  n_pairs, n_adds);
+        __syncthreads();  // the next item reuses the LDS and s_item
+    }
+    block_stats(n_pairs, n_adds, stats, stats_slot + blockIdx.x, s_stat);
 }
 
 // ---------------------------------------------------------------------------
@@ -1540,7 +1663,7 @@ void ws_free(Workspace &ws) {
                     ws.d_sub_joined, ws.d_it_cnt, ws.d_it_off,
                     ws.d_run_first, ws.d_run_close, ws.d_run_path, ws.d_rblock, ws.d_run_start, ws.d_run_end, ws.d_pts,
                     ws.d_pt_run, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
-                    ws.d_scan_state, ws.d_pstate, ws.d_top_rep, ws.d_status};
+                    ws.d_scan_state, ws.d_pstate, ws.d_top_rep, ws.d_items, ws.d_stats, ws.d_status};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ws.h_status) (void)hipHostFree(ws.h_status);
@@ -1624,10 +1747,10 @@ static ft_status arena_rewind(Workspace &ws) {
                      (void **)&ws.d_pt_run, (void **)&ws.d_edges, (void **)&ws.d_pt_band, (void **)&ws.d_band_cnt,
                      (void **)&ws.d_band_off, (void **)&ws.d_band_cur, (void **)&ws.d_band_list,
                      (void **)&ws.d_scan_state, (void **)&ws.d_pstate, (void **)&ws.d_top_rep,
-                     (void **)&ws.d_status};
+                     (void **)&ws.d_items, (void **)&ws.d_stats, (void **)&ws.d_status};
     for (void **p : ptrs) *p = nullptr;
     ws.cap_ops = ws.cap_paths = ws.cap_bands = ws.cap_rblocks = ws.cap_events = ws.cap_stroke = 0;
-    ws.cap_band_slots = ws.cap_pairs = ws.cap_tiles = 0;
+    ws.cap_band_slots = ws.cap_pairs = ws.cap_tiles = ws.cap_items = 0;
     return arena_carve(ws, ws.d_status, 1);
 }
 
@@ -1664,6 +1787,24 @@ static inline bool flatten_in_one_pass(const Workspace &ws, size_t n_ops) {
     return ws.scan_resident != 0 && blocks <= ws.scan_resident;
 }
 
+// The tile scheduler's persistent grid for pixel format f: as many blocks as
+// the device holds at once with `cells` LDS cells each (at most the items).
+static uint32_t tile_grid_for(const Workspace &ws, int f, uint32_t cells) {
+    const size_t lds = raster_lds_bytes((uint32_t)f, cells);
+    int per_cu = 0;
+    hipError_t e;
+    if (f == FT_MATTE8)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiles<false, FMT_MATTE8>, BLOCK, lds);
+    else if (f == FT_RGBA8P)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiles<true, FMT_RGBA8P>, BLOCK, lds);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiles<true, FMT_GRAYA8P>, BLOCK, lds);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    const uint64_t cus = ws.scan_resident ? ws.scan_resident / 4 : 64;
+    const uint64_t g = cus * (uint64_t)per_cu;
+    return (uint32_t)std::min<uint64_t>(g, ws.it_count[f]);
+}
+
 ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const ft_fill_desc *paths,
                        size_t n_paths, uint32_t extra_flags) {
     if (n_ops >= 0x7FFFFFFFu || n_paths >= 0x7FFFFFFFu) return FT_EINVAL;
@@ -1686,6 +1827,8 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     band_path.clear();
     rblock.clear();
     std::vector<uint4> rblock_f[3];  // per pixel format (ft_format), then concatenated
+    std::vector<uint2> items_f[3];
+    std::vector<uint64_t> cost_f[3];  // estimated work of each item, for the largest-first order
     uint32_t lds = 0;
     for (size_t k = 0; k < n_paths; ++k) {
         const ft_fill_desc &d = paths[k];
@@ -1701,6 +1844,29 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
         }
         PathDev &P = pd[k];
         P = path_from_desc(d, extra_flags, ws.tuning);
+        // Tile mode: a path of at most tuning.tile_groups LDS groups (rows
+        // whose cells fit lds_cells) is rastered by the tile scheduler, one
+        // work item per group, with no bands.
+        const uint32_t wp = cell_stride(d.width);
+        const uint32_t gt = wp ? std::max(1u, ws.tuning.lds_cells / wp) : 1u;
+        const uint64_t tgroups = d.height ? ((uint64_t)d.height + gt - 1) / gt : 0;
+        if (ws.tuning.tile_groups && d.width && d.height && tgroups <= ws.tuning.tile_groups) {
+            P.flags |= PATH_TILE;
+            P.n_bands = 0;
+            P.band_rows = d.height;
+            P.group_rows = gt;
+            P.band_base = (uint32_t)band_path.size();
+            const uint32_t cells = std::min(gt, d.height) * wp;
+            if (cells > lds) lds = cells;
+            for (uint32_t g = 0; g < (uint32_t)tgroups; ++g) {
+                const uint32_t rg = std::min(gt, d.height - g * gt);
+                items_f[d.fmt].push_back(make_uint2((uint32_t)k, g));
+                // edges in the group's rows (about 16 points per op, spread over the
+                // path's rows, a few rows per edge) and the group's pixels
+                cost_f[d.fmt].push_back((uint64_t)d.op_count * 64u * rg / d.height + (uint64_t)d.width * rg);
+            }
+            continue;
+        }
         uint32_t rows, group, nb, cells;
         band_geometry(d.width, d.height, ws.tuning, &rows, &group, &nb, &cells);
         nb = P.n_bands;
@@ -1722,6 +1888,17 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
         ws.rb_count[f] = (uint32_t)rblock_f[f].size();
         rblock.insert(rblock.end(), rblock_f[f].begin(), rblock_f[f].end());
     }
+    ws.h_items.clear();
+    for (int f = 0; f < 3; ++f) {
+        std::vector<uint32_t> order(items_f[f].size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = (uint32_t)i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return cost_f[f][a] > cost_f[f][b]; });
+        ws.it_first[f] = (uint32_t)ws.h_items.size();
+        ws.it_count[f] = (uint32_t)order.size();
+        for (uint32_t i : order) ws.h_items.push_back(items_f[f][i]);
+    }
+    if (ws.h_items.size() >= 0x7FFFFFFFu) return FT_ENOTSUP;
     if (rblock.size() >= 0x7FFFFFFFu) return FT_ENOTSUP;
     // the tuning asks for more LDS than a block has
     if (raster_lds_bytes(FMT_MATTE8, lds) > ws.lds_max || raster_lds_bytes(FMT_RGBA8P, lds) > ws.lds_max)
@@ -1765,6 +1942,8 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     ws.band_shift = shift;
     if ((s = dev_grow(ws, ws.d_rblock, ws.cap_rblocks, rblock.size()))) return s;
     ws.n_rblocks = rblock.size();
+    if ((s = dev_grow(ws, ws.d_items, ws.cap_items, ws.h_items.size()))) return s;
+    for (int f = 0; f < 3; ++f) ws.tile_grid[f] = ws.it_count[f] ? tile_grid_for(ws, f, lds) : 0u;
     ws.n_ops = n_ops;
     ws.n_paths = n_paths;
     ws.n_bands = nb_total;
@@ -1811,6 +1990,9 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     if (!rblock.empty())
         FT_HIP(hipMemcpyAsync(ws.d_rblock, rblock.data(), rblock.size() * sizeof(uint4), hipMemcpyHostToDevice,
                               ws.stream));
+    if (!ws.h_items.empty())
+        FT_HIP(hipMemcpyAsync(ws.d_items, ws.h_items.data(), ws.h_items.size() * sizeof(uint2),
+                              hipMemcpyHostToDevice, ws.stream));
     ws.uploads_pending = true;
     return FT_OK;
 }
@@ -1829,7 +2011,7 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev);
 ft_status ws_launch(Workspace &ws, uint32_t mode) {
     ws.edges_counted = false;
     hipEvent_t *ev = nullptr;
-    if (ws.timing && mode == MODE_FILL && ws.n_bands && ws.n_pending < Workspace::TIMING_RING) {
+    if (ws.timing && mode == MODE_FILL && ws.n_paths && ws.n_pending < Workspace::TIMING_RING) {
         if (!ws.ev) {
             ws.ev = new hipEvent_t[3 * Workspace::TIMING_RING]();
             for (int i = 0; i < 3 * Workspace::TIMING_RING; ++i) FT_HIP(hipEventCreate(&ws.ev[i]));
@@ -2005,40 +2187,66 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
             (wide ? k_bin<4> : k_bin<1>)<<<ge, BLOCK, 0, s>>>(
                 L0, ws.band_shift, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list, ws.cap_pairs, ws.d_pts,
                 ws.d_pt_run, ws.d_run_start, ws.d_run_end, ws.d_pstate, st);
-            if (ws.n_rblocks) {
-                // one raster kernel per pixel format present; the events
-                // bracket the first and the last of them
-                int last = -1, first = -1;
-                for (int f = 0; f < 3; ++f)
-                    if (ws.rb_count[f]) {
-                        if (first < 0) first = f;
-                        last = f;
-                    }
-                const bool pf = ws.tuning.dst_prefetch != 0;
-                for (int f = 0; f < 3; ++f) {
-                    if (!ws.rb_count[f]) continue;
-                    hipEvent_t e0 = (ev && f == first) ? ev[1] : nullptr, e1 = (ev && f == last) ? ev[2] : nullptr;
-                    const dim3 grid(ws.rb_count[f]);
-                    const uint32_t lds = (uint32_t)raster_lds_bytes((uint32_t)f, ws.lds_cells);
-                    if (f == FT_MATTE8)
-                        hipExtLaunchKernelGGL(k_raster<false, FMT_MATTE8>, grid, dim3(BLOCK), lds, s, e0, e1, 0, L0,
-                                              ws.d_paths, ws.d_rblock, ws.rb_first[f], ws.band_shift, ws.d_band_off,
-                                              ws.d_band_list, ws.d_edges, ws.d_pstate, st);
-                    else if (f == FT_RGBA8P)
-                        hipExtLaunchKernelGGL(pf ? k_raster<true, FMT_RGBA8P> : k_raster<false, FMT_RGBA8P>, grid,
-                                              dim3(BLOCK), lds, s, e0, e1, 0, L0, ws.d_paths, ws.d_rblock,
-                                              ws.rb_first[f], ws.band_shift, ws.d_band_off, ws.d_band_list,
-                                              ws.d_edges, ws.d_pstate, st);
-                    else
-                        hipExtLaunchKernelGGL(pf ? k_raster<true, FMT_GRAYA8P> : k_raster<false, FMT_GRAYA8P>, grid,
-                                              dim3(BLOCK), lds, s, e0, e1, 0, L0, ws.d_paths, ws.d_rblock,
-                                              ws.rb_first[f], ws.band_shift, ws.d_band_off, ws.d_band_list,
-                                              ws.d_edges, ws.d_pstate, st);
-                }
-            } else if (ev) {
-                FT_HIP(hipEventRecord(ev[1], s));
-                FT_HIP(hipEventRecord(ev[2], s));
+        }
+        // The raster kernels: one per pixel format with band blocks, then the
+        // tile scheduler per format with tile-mode items; the timing events
+        // bracket the first and the last of them.
+        struct RasterLaunch {
+            int f;
+            bool tile;
+        };
+        RasterLaunch rl[6];
+        int n_rl = 0;
+        for (int f = 0; f < 3; ++f)
+            if (ws.n_bands && ws.rb_count[f]) rl[n_rl++] = {f, false};
+        for (int f = 0; f < 3; ++f)
+            if (ws.it_count[f] && ws.tile_grid[f]) rl[n_rl++] = {f, true};
+        const bool pf = ws.tuning.dst_prefetch != 0;
+        unsigned long long *const stats = ws.stats ? ws.d_stats : nullptr;
+        uint32_t tile_slot = (uint32_t)ws.n_rblocks;  // per-block stats: band blocks, then tile blocks
+        for (int k = 0; k < n_rl; ++k) {
+            const int f = rl[k].f;
+            hipEvent_t e0 = (ev && k == 0) ? ev[1] : nullptr, e1 = (ev && k == n_rl - 1) ? ev[2] : nullptr;
+            const uint32_t lds = (uint32_t)raster_lds_bytes((uint32_t)f, ws.lds_cells);
+            if (rl[k].tile) {
+                const dim3 grid(ws.tile_grid[f]);
+                const uint32_t nf = ws.it_first[f], ni = ws.it_count[f];
+                if (f == FT_MATTE8)
+                    hipExtLaunchKernelGGL(k_tiles<false, FMT_MATTE8>, grid, dim3(BLOCK), lds, s, e0, e1, 0, L0,
+                                          ws.d_paths, ws.d_items, nf, ni, ws.d_edges, ws.d_pts, ws.d_pt_run,
+                                          ws.d_run_start, ws.d_run_end, ws.d_pstate, st, stats, tile_slot);
+                else if (f == FT_RGBA8P)
+                    hipExtLaunchKernelGGL(pf ? k_tiles<true, FMT_RGBA8P> : k_tiles<false, FMT_RGBA8P>, grid,
+                                          dim3(BLOCK), lds, s, e0, e1, 0, L0, ws.d_paths, ws.d_items, nf, ni,
+                                          ws.d_edges, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end,
+                                          ws.d_pstate, st, stats, tile_slot);
+                else
+                    hipExtLaunchKernelGGL(pf ? k_tiles<true, FMT_GRAYA8P> : k_tiles<false, FMT_GRAYA8P>, grid,
+                                          dim3(BLOCK), lds, s, e0, e1, 0, L0, ws.d_paths, ws.d_items, nf, ni,
+                                          ws.d_edges, ws.d_pts, ws.d_pt_run, ws.d_run_start, ws.d_run_end,
+                                          ws.d_pstate, st, stats, tile_slot);
+                tile_slot += ws.tile_grid[f];
+                continue;
             }
+            const dim3 grid(ws.rb_count[f]);
+            if (f == FT_MATTE8)
+                hipExtLaunchKernelGGL(k_raster<false, FMT_MATTE8>, grid, dim3(BLOCK), lds, s, e0, e1, 0, L0,
+                                      ws.d_paths, ws.d_rblock, ws.rb_first[f], ws.band_shift, ws.d_band_off,
+                                      ws.d_band_list, ws.d_edges, ws.d_pstate, st, stats);
+            else if (f == FT_RGBA8P)
+                hipExtLaunchKernelGGL(pf ? k_raster<true, FMT_RGBA8P> : k_raster<false, FMT_RGBA8P>, grid,
+                                      dim3(BLOCK), lds, s, e0, e1, 0, L0, ws.d_paths, ws.d_rblock, ws.rb_first[f],
+                                      ws.band_shift, ws.d_band_off, ws.d_band_list, ws.d_edges, ws.d_pstate, st,
+                                      stats);
+            else
+                hipExtLaunchKernelGGL(pf ? k_raster<true, FMT_GRAYA8P> : k_raster<false, FMT_GRAYA8P>, grid,
+                                      dim3(BLOCK), lds, s, e0, e1, 0, L0, ws.d_paths, ws.d_rblock, ws.rb_first[f],
+                                      ws.band_shift, ws.d_band_off, ws.d_band_list, ws.d_edges, ws.d_pstate, st,
+                                      stats);
+        }
+        if (ev && n_rl == 0) {
+            FT_HIP(hipEventRecord(ev[1], s));
+            FT_HIP(hipEventRecord(ev[2], s));
         }
     }
     FT_HIP(hipGetLastError());
@@ -2083,6 +2291,32 @@ ft_status ws_count_edges(Workspace &ws) {
         FT_HIP(hipStreamSynchronize(ws.stream));
         ws.edges_counted = true;
     }
+    return FT_OK;
+}
+
+ft_status ws_raster_stats(Workspace &ws, uint64_t *pairs, uint64_t *adds) {
+    size_t slots = ws.n_rblocks;
+    for (int f = 0; f < 3; ++f) slots += ws.tile_grid[f];
+    std::vector<unsigned long long> h(2 * (slots ? slots : 1), 0ull);
+    ft_status s;
+    if ((s = dev_alloc(ws, ws.d_stats, h.size()))) return s;
+    FT_HIP(hipMemcpyAsync(ws.d_stats, h.data(), h.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                          ws.stream));
+    const bool graph = ws.graph;
+    ws.graph = false;  // an eager run with the counting kernels' outputs
+    ws.stats = true;
+    s = ws_run(ws, MODE_FILL, false);
+    ws.stats = false;
+    ws.graph = graph;
+    if (s) return s;
+    FT_HIP(hipMemcpy(h.data(), ws.d_stats, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    uint64_t a = 0, b = 0;
+    for (size_t i = 0; i < slots; ++i) {
+        a += h[2 * i];
+        b += h[2 * i + 1];
+    }
+    if (pairs) *pairs = a;
+    if (adds) *adds = b;
     return FT_OK;
 }
 

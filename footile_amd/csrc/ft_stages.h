@@ -715,7 +715,8 @@ FT_HD inline EdgeRec make_edge(I2 q, I2 q1) {
 FT_HD inline bool edge_bands(const EdgeRec &E, const PathDev &P, unsigned long long topkey, const Limits &L,
                              uint32_t &g0, uint32_t &g1) {
     int32_t r0, r1;
-    const bool rows = edge_rows(E, top_row_of(topkey), P.height, r0, r1) & ((E.flags & 1u) != 0);
+    const bool rows = edge_rows(E, top_row_of(topkey), P.height, r0, r1) & ((E.flags & 1u) != 0) &
+                      (P.n_bands != 0);  // tile-mode paths have no bands
     const uint32_t br = P.band_rows ? P.band_rows : 1;
     g0 = P.band_base + (rows ? (uint32_t)r0 / br : 0u);
     g1 = P.band_base + (rows ? (uint32_t)r1 / br : 0u);
@@ -970,8 +971,10 @@ constexpr uint32_t CELL_ALIGN = 16;   // row stride of cells in LDS
 
 struct Tuning {
     uint32_t band_rows_min = BAND_ROWS_MIN;  // 1 .. 256
-    uint32_t lds_cells = LDS_CELLS;          // 1024 .. 16384 (64 KiB), a multiple of CELL_ALIGN
+    uint32_t lds_cells = LDS_CELLS;          // 1024 .. 32768, a multiple of CELL_ALIGN (and fitting LDS)
     uint32_t dst_prefetch = 1;               // raster blocks load their destination chunks first
+    uint32_t tile_groups = 0;                // paths of at most this many LDS groups go to the tile
+                                             // scheduler (0: none)
 };
 
 FT_HD inline uint32_t cell_stride(uint32_t width) { return (width + CELL_ALIGN - 1) / CELL_ALIGN * CELL_ALIGN; }

@@ -50,7 +50,8 @@ struct PathDev {
 enum : uint32_t {
     PATH_FRESH = 1u,  // region starts cleared: rows above the figure are zeroed
     PATH_RAW = 2u,    // points go to Fig::add_point untransformed (Fig-level entry)
-    PATH_WHOLE = 4u   // the path owns every event of the run (a stroke outline)
+    PATH_WHOLE = 4u,  // the path owns every event of the run (a stroke outline)
+    PATH_TILE = 8u    // rastered by the tile scheduler from its own points (no bands)
 };
 
 // Per-path results of a run (cleared at the start of each run).
@@ -78,7 +79,7 @@ struct DevStatus {
     unsigned long long stroke_total;  // stroker outline: events << 32 | closes
     uint32_t st_subs, st_items;       // sub-strokes kept; outline items
     uint32_t spin_fail;
-    uint32_t pad[2];
+    uint32_t tile_next[3];  // tile scheduler queue heads, one per pixel format
 };
 
 struct EdgeRec {

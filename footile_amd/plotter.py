@@ -220,7 +220,7 @@ class FillDesc:
 
 class _Tuning(ctypes.Structure):
     _fields_ = [("band_rows_min", ctypes.c_uint32), ("lds_cells", ctypes.c_uint32),
-                ("dst_prefetch", ctypes.c_uint32)]
+                ("dst_prefetch", ctypes.c_uint32), ("tile_groups", ctypes.c_uint32)]
 
 
 class Batch:
@@ -259,15 +259,25 @@ class Batch:
               "ft_batch_timing")
         return t.value, r.value, n.value
 
-    def set_tuning(self, band_rows_min, lds_cells, dst_prefetch=True):
-        """Raster geometry (ft_batch_set_tuning); re-lays the batch out."""
-        t = _Tuning(int(band_rows_min), int(lds_cells), 1 if dst_prefetch else 0)
+    def set_tuning(self, band_rows_min, lds_cells, dst_prefetch=True, tile_groups=0):
+        """Raster geometry (ft_batch_set_tuning); re-lays the batch out.
+        tile_groups > 0 sends paths of at most that many LDS row groups to the
+        tile scheduler."""
+        t = _Tuning(int(band_rows_min), int(lds_cells), 1 if dst_prefetch else 0, int(tile_groups))
         check(lib().ft_batch_set_tuning(self._h, ctypes.byref(t)), "ft_batch_set_tuning")
 
     def tuning(self):
+        """(band_rows_min, lds_cells, dst_prefetch, tile_groups)"""
         t = _Tuning()
         check(lib().ft_batch_get_tuning(self._h, ctypes.byref(t)), "ft_batch_get_tuning")
-        return t.band_rows_min, t.lds_cells, bool(t.dst_prefetch)
+        return t.band_rows_min, t.lds_cells, bool(t.dst_prefetch), t.tile_groups
+
+    def raster_stats(self):
+        """(edge, row) pairs walked and nonzero cell deltas added by one run
+        (ft_batch_raster_stats; runs the batch once)."""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().ft_batch_raster_stats(self._h, ctypes.byref(a), ctypes.byref(b)), "ft_batch_raster_stats")
+        return a.value, b.value
 
     def set_graph(self, on=True):
         """Replay launches from a captured HIP graph (ft_batch_set_graph)."""

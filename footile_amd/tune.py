@@ -1,9 +1,10 @@
 """Autotuner of a batch's raster geometry (ft_batch_set_tuning).
 
-Candidates are (band_rows_min, lds_cells, dst_prefetch): band height for
-edge binning, the LDS cell budget that fixes the rows per raster block, and
-(for read-modify-write rasters) whether raster blocks load their destination
-before the cell pass.  Each
+Candidates are (band_rows_min, lds_cells, dst_prefetch, tile_groups): band
+height for edge binning, the LDS cell budget that fixes the rows per raster
+block, (for read-modify-write rasters) whether raster blocks load their
+destination before the cell pass, and the largest path (in LDS row groups)
+the tile scheduler takes instead of the bands.  Each
 candidate is timed on the batch itself with device events (whole launch
 sequence, median of a few launches after one warm-up run); the fastest wins.
 Results are cached in a JSON file keyed by a signature of the batch layout
@@ -21,6 +22,11 @@ from ._lib import FootileError
 
 BAND_ROWS = (4, 8, 16, 32, 64)
 LDS_CELLS = (2048, 3072, 4096, 8192, 16384)
+# tile scheduler candidates: paths of at most TILE_GROUPS LDS row groups of
+# TILE_CELLS cells leave the bands
+TILE_GROUPS = (1, 2, 4, 8)
+TILE_CELLS = (4096, 8192, 16384, 32768)
+TILE_BAND_ROWS = (8, 16)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "footile_amd", "tuning.json")
 
 
@@ -35,6 +41,22 @@ def signature(descs, n_ops):
     key = json.dumps({"paths": len(descs), "w": hist(w), "h": hist(h), "fmt": fmt,
                       "ops_log2": int(np.log2(max(n_ops, 1)))}, sort_keys=True)
     return hashlib.sha1(key.encode()).hexdigest()[:16]
+
+
+def _tile_candidates(descs, prefetch):
+    """Tile scheduler settings that move at least a tenth of the paths out of
+    the bands (a path of height h and cell row stride wp takes
+    ceil(h / (lds_cells // wp)) groups)."""
+    w = np.array([d.c.width for d in descs], np.int64)
+    h = np.array([d.c.height for d in descs], np.int64)
+    wp = (w + 15) // 16 * 16
+    out = []
+    for c in TILE_CELLS:
+        groups = -(-h // np.maximum(1, c // np.maximum(wp, 1)))
+        for tg in TILE_GROUPS:
+            if np.count_nonzero(groups <= tg) * 10 >= max(len(descs), 1):
+                out += [(b, c, p, tg) for b in TILE_BAND_ROWS for p in prefetch]
+    return out
 
 
 def _time(batch, reps):
@@ -60,17 +82,18 @@ def autotune(batch, descs, n_ops, cache=DEFAULT_CACHE, reps=3, candidates=None, 
             table = {}
     if key in table:
         e = table[key]
-        batch.set_tuning(e["band_rows_min"], e["lds_cells"], e.get("dst_prefetch", True))
+        batch.set_tuning(e["band_rows_min"], e["lds_cells"], e.get("dst_prefetch", True), e.get("tile_groups", 0))
         return dict(e, cached=True, tried=0)
     # the destination prefetch only matters for read-modify-write rasters
     rmw = any(int(d.c.fmt) != 0 and not int(d.c.fresh) for d in descs)
     prefetch = (True, False) if rmw else (True,)
-    cands = candidates or [(b, c, p) for b in BAND_ROWS for c in LDS_CELLS for p in prefetch]
+    cands = candidates or ([(b, c, p, 0) for b in BAND_ROWS for c in LDS_CELLS for p in prefetch] +
+                           _tile_candidates(descs, prefetch))
     best = None
     t0 = time.perf_counter()
-    for b, c, p in cands:
+    for b, c, p, tg in cands:
         try:
-            batch.set_tuning(b, c, p)
+            batch.set_tuning(b, c, p, tg)
         except FootileError as e:
             if e.code != 5:  # FT_ENOTSUP
                 raise
@@ -78,12 +101,12 @@ def autotune(batch, descs, n_ops, cache=DEFAULT_CACHE, reps=3, candidates=None, 
         batch.run()
         ms = _time(batch, reps)
         if verbose:
-            print(f"[tune] band_rows_min={b} lds_cells={c} dst_prefetch={p}: {ms:.3f} ms")
-        if best is None or ms < best[3]:
-            best = (b, c, p, ms)
-    batch.set_tuning(best[0], best[1], best[2])
-    entry = {"band_rows_min": best[0], "lds_cells": best[1], "dst_prefetch": best[2], "ms": best[3],
-             "tune_s": time.perf_counter() - t0}
+            print(f"[tune] band_rows_min={b} lds_cells={c} dst_prefetch={p} tile_groups={tg}: {ms:.3f} ms")
+        if best is None or ms < best[4]:
+            best = (b, c, p, tg, ms)
+    batch.set_tuning(best[0], best[1], best[2], best[3])
+    entry = {"band_rows_min": best[0], "lds_cells": best[1], "dst_prefetch": best[2], "tile_groups": best[3],
+             "ms": best[4], "tune_s": time.perf_counter() - t0}
     if cache:
         try:
             os.makedirs(os.path.dirname(cache), exist_ok=True)

@@ -154,14 +154,19 @@ ft_status ft_batch_counts(ft_batch *b, uint64_t *offered, uint64_t *stored, uint
                           int32_t *top_rows);
 /* Raster geometry of a batch: row bands of at least band_rows_min rows
  * (1..256) are the unit of edge binning, and each band is rasterised in
- * groups of rows whose i32 cells fit lds_cells (1024..16384, a multiple of
- * 16); one block per group.  Setting it re-lays the batch out.  No reference
- * counterpart: the autotuner (footile_amd.tune) picks it per workload. */
+ * groups of rows whose i32 cells fit lds_cells (1024..32768, a multiple of
+ * 16, within the device's LDS per block); one block per group.  A path of at
+ * most tile_groups such groups (0: none) skips the bands: the atlas tile
+ * scheduler, one persistent launch per pixel format, rasters its groups from
+ * the path's own edges, largest work items first.  Setting it re-lays the
+ * batch out.  No reference counterpart: the autotuner (footile_amd.tune)
+ * picks it per workload. */
 typedef struct ft_tuning {
     uint32_t band_rows_min;
     uint32_t lds_cells;
     uint32_t dst_prefetch; /* 1: raster blocks of Rgba8p/Graya8p read their
                               destination before the cell pass */
+    uint32_t tile_groups;
 } ft_tuning;
 ft_status ft_batch_set_tuning(ft_batch *b, const ft_tuning *t);
 ft_status ft_batch_get_tuning(const ft_batch *b, ft_tuning *t);
@@ -169,6 +174,10 @@ ft_status ft_batch_get_tuning(const ft_batch *b, ft_tuning *t);
  * once and replays it (re-captured after any re-layout or capacity growth);
  * the raster kernel is then not timed separately. */
 ft_status ft_batch_set_graph(ft_batch *b, int on);
+/* Work statistics of the raster: runs the batch once and returns the
+ * (edge, row) pairs its cell pass walked and the nonzero signed-area cell
+ * deltas it added (one LDS atomic each). */
+ft_status ft_batch_raster_stats(ft_batch *b, uint64_t *pairs, uint64_t *cell_adds);
 /* The batch's HIP stream (hipStream_t) for event timing by the caller. */
 ft_status ft_batch_stream(ft_batch *b, void **stream);
 /* Event timing on the batch stream: when on, every launch records HIP events

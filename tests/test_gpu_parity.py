@@ -322,13 +322,19 @@ def test_batch_tuning_keeps_rasters(ft, oracle):
         o = oracle.Plotter(int(s), int(s))
         assert o.fill(0, ops[b:b + c]) == 0
         assert np.array_equal(want.reshape(1024, 1024)[y:y + s, x:x + s].reshape(-1), o.pixels()), f"tile {t}"
-    for band_rows, cells, pf in [(1, 1024, True), (4, 2048, False), (64, 16384, True), (256, 8192, False),
-                                 (16, 4096, True)]:
-        batch.set_tuning(band_rows, cells, pf)
-        assert batch.tuning() == (band_rows, cells, pf)
+    for band_rows, cells, pf, tg in [(1, 1024, True, 0), (4, 2048, False, 0), (64, 16384, True, 0),
+                                     (256, 8192, False, 0), (16, 4096, True, 0), (16, 4096, True, 1),
+                                     (8, 2048, False, 2), (16, 8192, True, 8), (4, 1024, True, 64),
+                                     (16, 32768, True, 4)]:
+        try:
+            batch.set_tuning(band_rows, cells, pf, tg)
+        except ft.FootileError as e:
+            assert e.code == 5 and cells > 16384  # more LDS per block than the device has
+            continue
+        assert batch.tuning() == (band_rows, cells, pf, tg)
         atlas.upload(np.full(1024 * 1024, 77, dtype=np.uint8))
         batch.run()
-        assert np.array_equal(atlas.download(), want), (band_rows, cells)
+        assert np.array_equal(atlas.download(), want), (band_rows, cells, tg)
     with pytest.raises(ft.FootileError):
         batch.set_tuning(0, 4096)
     with pytest.raises(ft.FootileError):
@@ -363,7 +369,8 @@ def test_batch_graph_replays_match_eager(ft, oracle):
 
 def test_batch_rmw_formats_with_and_without_prefetch(ft, oracle):
     """Rgba8p / Graya8p batches over existing content: the destination
-    prefetch on and off give the oracle's bytes."""
+    prefetch on and off, through the bands and through the tile scheduler,
+    give the oracle's bytes."""
     from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
     rng = np.random.default_rng(31)
     for fmt in (RGBA8P, GRAYA8P):
@@ -386,9 +393,9 @@ def test_batch_rmw_formats_with_and_without_prefetch(ft, oracle):
             ops_all.append(ops)
             at += len(ops)
         batch = Batch(np.concatenate(ops_all), descs)
-        for pf in (True, False):
+        for pf, tg in ((True, 0), (False, 0), (True, 4), (False, 4)):  # bands, then the tile scheduler
             dev.upload(init)
-            batch.set_tuning(16, 4096, pf)
+            batch.set_tuning(16, 4096, pf, tg)
             batch.run()
             got = dev.download().reshape(side, side * bpp)
             want = init.reshape(side, side * bpp).copy()
@@ -397,7 +404,7 @@ def test_batch_rmw_formats_with_and_without_prefetch(ft, oracle):
                 o.set_pixels(want[y:y + h, x * bpp:(x + w) * bpp].reshape(-1))
                 assert o.fill(rule, ops, clr) == 0
                 want[y:y + h, x * bpp:(x + w) * bpp] = o.pixels().reshape(h, w * bpp)
-            assert np.array_equal(got, want), (fmt, pf)
+            assert np.array_equal(got, want), (fmt, pf, tg)
 
 
 def test_batch_over_a_million_events(ft, oracle):
