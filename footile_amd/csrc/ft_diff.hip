@@ -34,7 +34,6 @@ constexpr int DSEG = 16;          // segments per cubic
 constexpr float EPS_W = 1e-5f;    // narrower pieces use the midpoint value
 constexpr int DBLOCK = 256;       // pixels of one row per block (four wavefronts)
 constexpr int KSEG = 8;           // paths per checkpoint segment
-constexpr uint32_t VCACHE = 256;  // vertex gradients a backward block gathers in LDS per path
 constexpr uint64_t COV_MAX_BYTES = 16ull << 30;  // largest kept-coverage buffer
 
 struct DScene {
@@ -156,10 +155,40 @@ __device__ inline void piece_grad(float2 p, float2 q, int Y, float S1, float S2,
     g[3] = down ? gd[3] : gd[1];
 }
 
+// One DPP step of a wavefront float scan: v + (v of the source lane, or 0).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float v) {
+    const int src = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xf, false);
+    return v + __builtin_bit_cast(float, src);
+}
+
+// Sum over the 64 lanes (all active), returned to every lane: an inclusive
+// scan with DPP (row_shr 1, 2, 4, 8 within rows of 16 lanes, then
+// row_bcast:15 and row_bcast:31), read from lane 63.  The same lanes add in
+// the same order on every call.
 __device__ inline float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v = dpp_add<0x111, 0xf>(v);
+    v = dpp_add<0x112, 0xf>(v);
+    v = dpp_add<0x114, 0xf>(v);
+    v = dpp_add<0x118, 0xf>(v);
+    v = dpp_add<0x142, 0xa>(v);
+    v = dpp_add<0x143, 0xc>(v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+
+// Three sums at once (their DPP steps interleave).
+__device__ inline void wave_sum3(float &a, float &b, float &c) {
+#define FT_DPP3(CTRL, RM) a = dpp_add<CTRL, RM>(a); b = dpp_add<CTRL, RM>(b); c = dpp_add<CTRL, RM>(c);
+    FT_DPP3(0x111, 0xf)
+    FT_DPP3(0x112, 0xf)
+    FT_DPP3(0x114, 0xf)
+    FT_DPP3(0x118, 0xf)
+    FT_DPP3(0x142, 0xa)
+    FT_DPP3(0x143, 0xc)
+#undef FT_DPP3
+    a = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a), 63));
+    b = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, b), 63));
+    c = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c), 63));
 }
 
 __device__ inline float alpha_of(float A, float &dadA) {
@@ -310,63 +339,60 @@ __global__ __launch_bounds__(BLOCK) void k_sort_cells(DScene S, DevStatus *st) {
 // Forward, loss, backward
 // ---------------------------------------------------------------------------
 // Row staging.  A block owns DBLOCK pixels of one row of one image.  It
-// stages the (cell count, list offset) of a run of the image's paths for its
-// row (stage_paths) and then their row pieces in windows of ECAP (one
-// thread per piece: edge index, two vertices, piece_of), so that the
-// per-pixel loops read every piece from LDS with one broadcast load instead
-// of a chain of dependent global loads per edge.
-constexpr uint32_t PCHUNK = 4 * DBLOCK;  // paths staged per round (four per thread)
-constexpr uint32_t ECAP = 512;           // pieces staged per window
+// stages the (cell count, list offset, colour) of a run of the image's paths
+// for its row (stage_paths) and then their row pieces in windows of ECAP
+// (one thread per piece: edge index, two vertices, piece_of), so that the
+// per-pixel loops read every piece and colour from LDS with one broadcast
+// load instead of a chain of dependent global loads per edge and path.
+constexpr uint32_t PCHUNK = DBLOCK;  // paths staged per round (one per thread)
+constexpr uint32_t ECAP = 512;       // pieces staged per window
 constexpr uint32_t LAST_BIT = 0x80000000u, LIVE_BIT = 0x40000000u, J_MASK = 0x3FFFFFFFu;
 
 struct Stage {
     uint32_t *cnt;  // [PCHUNK] pieces of each staged path in the row
     uint32_t *off;  // [PCHUNK] its cell's list offset
     uint32_t *pb;   // [PCHUNK] exclusive scan of cnt
+    float4 *col;    // [PCHUNK] its colour
     float4 *pc;     // [ECAP] piece: x at top, x at bottom, sign * height, 0
-    uint2 *ev;      // [ECAP] staged path index | LAST_BIT | LIVE_BIT, edge's first vertex (local)
+    uint32_t *ev;   // [ECAP] staged path index | LAST_BIT | LIVE_BIT
+    uint2 *vi;      // [ECAP] global indices of the edge's two vertices (or null: not staged)
+    float4 *pq;     // [ECAP] their coordinates (or null)
     uint32_t *w;    // [DBLOCK / 64] scan scratch
 };
 
-// Counts and offsets of row Y for the n <= PCHUNK paths from global path
-// kbase; returns their total number of pieces.  Every thread calls it.
-__device__ uint32_t stage_paths(const DScene &S, uint32_t kbase, uint32_t n, int Y, const Stage &sg) {
-    const uint32_t t0 = threadIdx.x * 4u;
-    uint32_t c[4], o[4], sum = 0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const bool in = t0 + (uint32_t)j < n;
-        const uint64_t cell = (uint64_t)(kbase + (in ? t0 + (uint32_t)j : 0u)) * S.H + (uint32_t)Y;
-        c[j] = in ? S.cell_cnt[cell] : 0u;
-        o[j] = in ? S.cell_off[cell] : 0u;
-        sum += c[j];
-    }
-    const uint32_t incl = wave_incl_dpp<false>(sum);
+// Counts, offsets and colours of row Y for the n <= PCHUNK paths from global
+// path kbase; returns their total number of pieces.  Every thread calls it.
+__device__ uint32_t stage_paths(const DScene &S, const float4 *colours, uint32_t kbase, uint32_t n, int Y,
+                                const Stage &sg) {
+    const uint32_t t = threadIdx.x;
+    const bool in = t < n;
+    const uint64_t cell = (uint64_t)(kbase + (in ? t : 0u)) * S.H + (uint32_t)Y;
+    const uint32_t c = in ? S.cell_cnt[cell] : 0u;
+    const uint32_t o = in ? S.cell_off[cell] : 0u;
+    const float4 col = colours[kbase + (in ? t : 0u)];
+    const uint32_t incl = wave_incl_dpp<false>(c);
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 63) sg.w[wid] = incl;
     __syncthreads();
-    uint32_t pre = incl - sum, total = 0;
+    uint32_t pre = incl - c, total = 0;
 #pragma unroll
     for (int w = 0; w < DBLOCK / 64; ++w) {
         pre += w < wid ? sg.w[w] : 0u;
         total += sg.w[w];
     }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (t0 + (uint32_t)j < n) {
-            sg.cnt[t0 + j] = c[j];
-            sg.off[t0 + j] = o[j];
-            sg.pb[t0 + j] = pre;
-        }
-        pre += c[j];
+    if (in) {
+        sg.cnt[t] = c;
+        sg.off[t] = o;
+        sg.pb[t] = pre;
+        sg.col[t] = col;
     }
     __syncthreads();
     return total;
 }
 
-// Pieces [e0, e0 + n) (n <= ECAP) of the np staged paths into sg.pc / sg.ev
-// at [0, n).  Piece e belongs to the last staged path t with pb[t] <= e.
-// The caller synchronises before and after.
+// Pieces [e0, e0 + n) (n <= ECAP) of the np staged paths into the piece
+// arrays at [0, n).  Piece e belongs to the last staged path t with
+// pb[t] <= e.  The caller synchronises before and after.
 __device__ void stage_pieces(const DScene &S, uint32_t kbase, uint32_t np, int Y, uint32_t e0, uint32_t n,
                              const Stage &sg, uint32_t &bad) {
     for (uint32_t i = threadIdx.x; i < n; i += DBLOCK) {
@@ -386,9 +412,14 @@ __device__ void stage_pieces(const DScene &S, uint32_t kbase, uint32_t np, int Y
         bad |= edge < V ? 0u : 1u;
         const uint32_t i0 = edge < V ? edge : 0u;
         const uint32_t i1 = i0 + 1 < V ? i0 + 1 : 0u;
-        const Piece r = piece_of(S.vtx[vb + i0], S.vtx[vb + i1], Y);
+        const float2 a = S.vtx[vb + i0], b = S.vtx[vb + i1];
+        const Piece r = piece_of(a, b, Y);
         sg.pc[i] = make_float4(r.xa, r.xb, r.live ? r.sign * r.h : 0.0f, 0.0f);
-        sg.ev[i] = make_uint2(lo | (q + 1 == sg.cnt[lo] ? LAST_BIT : 0u) | (r.live ? LIVE_BIT : 0u), i0);
+        sg.ev[i] = lo | (q + 1 == sg.cnt[lo] ? LAST_BIT : 0u) | (r.live ? LIVE_BIT : 0u);
+        if (sg.vi) {
+            sg.vi[i] = make_uint2(vb + i0, vb + i1);
+            sg.pq[i] = make_float4(a.x, a.y, b.x, b.y);
+        }
     }
 }
 
@@ -421,7 +452,7 @@ __device__ float4 forward_row(const DScene &S, const float4 *colours, uint32_t k
     for (uint32_t ja = 0; ja < np; ja += PCHUNK) {
         const uint32_t n = np - ja < PCHUNK ? np - ja : PCHUNK;
         __syncthreads();  // the previous round's staging is no longer read
-        const uint32_t total = stage_paths(S, k0 + ja, n, Y, sg);
+        const uint32_t total = stage_paths(S, colours, k0 + ja, n, Y, sg);
         for (uint32_t e0 = 0; e0 < total; e0 += ECAP) {
             const uint32_t ne = total - e0 < ECAP ? total - e0 : ECAP;
             __syncthreads();
@@ -429,14 +460,14 @@ __device__ float4 forward_row(const DScene &S, const float4 *colours, uint32_t k
             __syncthreads();
             for (uint32_t i = 0; i < ne; ++i) {
                 A += piece_cover(sg.pc[i], Xf, xw0);
-                const uint32_t ev = sg.ev[i].x;
+                const uint32_t ev = sg.ev[i];
                 if (ev & LAST_BIT) {
-                    const uint32_t j = ja + (ev & J_MASK);
+                    const uint32_t t = ev & J_MASK, j = ja + t;
                     if (ck)
                         for (; next_ck * KSEG <= j; ++next_ck) ck[(uint64_t)next_ck * ck_stride + slot] = C;
                     if (ck && cov) cov[(uint64_t)j * ck_stride + slot] = A;  // j < max_paths, slot < ck_stride
                     float dadA;
-                    C = over(colours[k0 + j], alpha_of(A, dadA), C);
+                    C = over(sg.col[t], alpha_of(A, dadA), C);
                     A = 0.0f;
                 }
             }
@@ -447,20 +478,20 @@ __device__ float4 forward_row(const DScene &S, const float4 *colours, uint32_t k
     return C;
 }
 
-#define DIFF_STAGE_LDS                                     \
-    __shared__ uint32_t s_cnt[PCHUNK], s_off[PCHUNK];     \
-    __shared__ uint32_t s_pb[PCHUNK];                      \
-    __shared__ float4 s_pc[ECAP];                          \
-    __shared__ uint2 s_ev[ECAP];                           \
-    __shared__ uint32_t s_w[DBLOCK / 64];                  \
-    const Stage sg{s_cnt, s_off, s_pb, s_pc, s_ev, s_w}
+#define DIFF_STAGE_LDS(VI, PQ)                                          \
+    __shared__ uint32_t s_cnt[PCHUNK], s_off[PCHUNK], s_pb[PCHUNK];    \
+    __shared__ float4 s_col[PCHUNK];                                    \
+    __shared__ float4 s_pc[ECAP];                                       \
+    __shared__ uint32_t s_ev[ECAP];                                     \
+    __shared__ uint32_t s_w[DBLOCK / 64];                               \
+    const Stage sg{s_cnt, s_off, s_pb, s_col, s_pc, s_ev, VI, PQ, s_w}
 
 // With ck non-null it also leaves what k_backward's replay would write: the
 // canvas before every KSEG-th path, and each path's coverage when cov is
 // non-null (same slots as k_backward).
 __global__ __launch_bounds__(DBLOCK) void k_forward(DScene S, const float4 *colours, float4 *canvas, float4 *ck,
                                                     uint64_t ck_stride, float *cov, DevStatus *st) {
-    DIFF_STAGE_LDS;
+    DIFF_STAGE_LDS(nullptr, nullptr);
     if (run_failed(st)) return;
     const int X = (int)(blockIdx.x * DBLOCK + threadIdx.x), Y = (int)blockIdx.y;
     const uint32_t im = blockIdx.z;
@@ -497,29 +528,29 @@ __global__ __launch_bounds__(BLOCK) void k_loss(const float *canvas, const float
 
 // One lane per pixel.  Pass 1 replays the forward into the checkpoints
 // (skipped when k_forward of the same inputs has written them: `replayed`).
-// Pass 2 walks the checkpoint segments last to first: the segment's
-// coverage per path into LDS (recomputed from its staged pieces, or read
-// from the kept coverage buffer), then its paths last to first: the canvas
-// under path j re-composited from the segment's checkpoint, the colour
-// gradient, and per piece of path j the sums S1 = sum dA M, S2 = sum dA
-// dM/dua, S3 = sum dA dM/dub over the block's pixels (a wavefront wholly
-// right of the piece adds its dA total to S1 alone, one wholly left adds
-// nothing), from which one thread per piece forms the vertex gradients.
-// Colour and vertex gradients are gathered per block in LDS (vertices for
-// paths of up to VCACHE of them) and flushed with one global atomic each.
+// Pass 2 walks the checkpoint segments last to first.  Per segment: the
+// segment's paths (counts, offsets, colours) and their coverage per pixel
+// into LDS (recomputed from the staged pieces, or read from the kept
+// coverage buffer); then its paths last to first, with no block barrier: the
+// canvas under path j re-composited from the segment's checkpoint, the colour
+// gradient (wave sums into LDS), and per piece of path j the sums S1 = sum
+// dA M, S2 = sum dA dM/dua, S3 = sum dA dM/dub over the block's pixels (a
+// wavefront wholly right of the piece adds its dA total to S1 alone, one
+// wholly left adds nothing), accumulated in LDS.  When the walk leaves the
+// staged window of pieces (and at the segment's end) one thread per piece
+// forms its vertex gradients from S1..S3 with global atomics; the colour
+// gradients are flushed once per segment.
 __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *colours, const float4 *grad_canvas,
                                                      float4 *ck, uint64_t ck_stride, float *cov, uint32_t replayed,
                                                      float2 *gvtx, float *gcol, DevStatus *st) {
-    DIFF_STAGE_LDS;
+    __shared__ uint2 s_vi[ECAP];
+    __shared__ float4 s_pq[ECAP];
+    DIFF_STAGE_LDS(s_vi, s_pq);
     __shared__ float s_A[KSEG][DBLOCK];  // coverage of the segment's paths; each thread uses its own column
     __shared__ float4 s_S[ECAP];         // per staged piece: S1, S2, S3
-    __shared__ float2 s_gv[VCACHE];      // the current path's vertex gradients from this block's pixels
-    __shared__ float s_gc[4];            // and its colour gradient
-    __shared__ uint32_t s_any;
+    __shared__ float s_gc[KSEG][4];      // colour gradients of the segment's paths
     if (run_failed(st)) return;
     const int tid = threadIdx.x, lane = tid & 63;
-    for (uint32_t v = tid; v < VCACHE; v += DBLOCK) s_gv[v] = make_float2(0.0f, 0.0f);
-    if (tid < 4) s_gc[tid] = 0.0f;
     const int X = (int)(blockIdx.x * DBLOCK + tid), Y = (int)blockIdx.y;
     const float Xf = (float)X, xw0 = (float)(X - lane);
     const uint32_t im = blockIdx.z;
@@ -531,15 +562,37 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
     if (!replayed) (void)forward_row(S, colours, k0, np, X, Y, slot, ck, ck_stride, cov, sg, bad);
     float4 g = active ? grad_canvas[((uint64_t)im * S.H + (uint32_t)Y) * S.W + (uint32_t)X]
                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // the vertex gradients of the staged window's pieces [0, n) from their
+    // sums (then cleared); every thread calls it
+    auto flush_pieces = [&](uint32_t n) {
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += DBLOCK) {
+            const float4 Sm = s_S[i];
+            if (((Sm.x != 0.0f) | (Sm.y != 0.0f) | (Sm.z != 0.0f)) && (s_ev[i] & LIVE_BIT)) {
+                const float4 pq = s_pq[i];
+                const uint2 vi = s_vi[i];
+                float gr[4];
+                piece_grad(make_float2(pq.x, pq.y), make_float2(pq.z, pq.w), Y, Sm.x, Sm.y, Sm.z, gr);
+                atomicAdd(&gvtx[vi.x].x, gr[0]);
+                atomicAdd(&gvtx[vi.x].y, gr[1]);
+                atomicAdd(&gvtx[vi.y].x, gr[2]);
+                atomicAdd(&gvtx[vi.y].y, gr[3]);
+            }
+            s_S[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        __syncthreads();
+    };
+    for (uint32_t i = tid; i < ECAP; i += DBLOCK) s_S[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const uint32_t n_seg = (np + KSEG - 1) / KSEG;
     for (uint32_t seg = n_seg; seg-- > 0;) {
         const uint32_t j0 = seg * KSEG, j1 = j0 + KSEG < np ? j0 + KSEG : np;
         const uint32_t ns = j1 - j0;
         __syncthreads();  // the previous segment's staging is no longer read
-        const uint32_t total = stage_paths(S, k0 + j0, ns, Y, sg);
+        const uint32_t total = stage_paths(S, colours, k0 + j0, ns, Y, sg);
+        if (tid < KSEG * 4) s_gc[tid >> 2][tid & 3] = 0.0f;
         // the segment's coverage per path
         for (uint32_t j = 0; j < ns; ++j) s_A[j][tid] = 0.0f;
-        uint32_t w_e0 = 0, w_n = 0;  // the staged window of pieces
+        uint32_t w_lo = 0, w_n = 0;  // staged window: pieces [w_lo, w_lo + w_n) of the segment
         if (cov) {
             for (uint32_t j = 0; j < ns; ++j)
                 if (s_cnt[j]) s_A[j][tid] = cov[(uint64_t)(j0 + j) * ck_stride + slot];
@@ -550,11 +603,11 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                 __syncthreads();
                 stage_pieces(S, k0 + j0, ns, Y, e0, ne, sg, bad);
                 __syncthreads();
-                w_e0 = e0;
+                w_lo = e0;
                 w_n = ne;
                 for (uint32_t i = 0; i < ne; ++i) {
                     A += piece_cover(s_pc[i], Xf, xw0);
-                    const uint32_t ev = s_ev[i].x;
+                    const uint32_t ev = s_ev[i];
                     if (ev & LAST_BIT) {
                         s_A[ev & J_MASK][tid] = A;
                         A = 0.0f;
@@ -562,63 +615,59 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                 }
             }
         }
+        __syncthreads();  // s_gc cleared before the walk's atomics
         const float4 Cseg = ck[(uint64_t)seg * ck_stride + slot];
         for (uint32_t jl = ns; jl-- > 0;) {
-            const uint32_t j = j0 + jl, k = k0 + j;
             float4 Cb = Cseg;  // canvas before path j
             for (uint32_t i = 0; i < jl; ++i) {
                 float d;
-                Cb = over(colours[k0 + j0 + i], alpha_of(s_A[i][tid], d), Cb);
+                Cb = over(s_col[i], alpha_of(s_A[i][tid], d), Cb);
             }
             const float A = s_A[jl][tid];
             float dadA;
             const float al = alpha_of(A, dadA);
-            const float4 cc = colours[k];
+            const float4 cc = s_col[jl];
             const float dal = g.x * (cc.x - cc.w * Cb.x) + g.y * (cc.y - cc.w * Cb.y) + g.z * (cc.z - cc.w * Cb.z) +
                               g.w * (cc.w - cc.w * Cb.w);
             const float gdotC = g.x * Cb.x + g.y * Cb.y + g.z * Cb.z + g.w * Cb.w;
             if (__ballot(al != 0.0f)) {
-                const float r0 = wave_sum(g.x * al), r1 = wave_sum(g.y * al), r2 = wave_sum(g.z * al);
-                const float r3 = wave_sum(g.w * al - al * gdotC);
-                if (lane == 0) {  // al != 0 needs pieces in this row: flushed below
-                    atomicAdd(&s_gc[0], r0);
-                    atomicAdd(&s_gc[1], r1);
-                    atomicAdd(&s_gc[2], r2);
-                    atomicAdd(&s_gc[3], r3);
+                float r0 = g.x * al, r1 = g.y * al, r2 = g.z * al;
+                float r3 = g.w * al - al * gdotC;
+                wave_sum3(r0, r1, r2);
+                r3 = wave_sum(r3);
+                if (lane == 0) {
+                    atomicAdd(&s_gc[jl][0], r0);
+                    atomicAdd(&s_gc[jl][1], r1);
+                    atomicAdd(&s_gc[jl][2], r2);
+                    atomicAdd(&s_gc[jl][3], r3);
                 }
             }
             const float keep = 1.0f - cc.w * al;
             g = make_float4(g.x * keep, g.y * keep, g.z * keep, g.w * keep);
             const float dA = active ? dal * dadA : 0.0f;
-            // s_cnt, V and `gather` are the same for the whole block: the
-            // barriers below are reached by every thread
-            const uint32_t cnt = s_cnt[jl];
+            // s_cnt and s_pb are the same for the whole block, so the window
+            // decisions (and the barriers in them) are too
+            const uint32_t cnt = s_cnt[jl], pb = s_pb[jl];
             if (!cnt) continue;
-            const uint32_t vb = S.vtx_off[k], V = S.vtx_off[k + 1] - vb;
-            const bool gather = V <= VCACHE;
-            if (tid == 0) s_any = 0;
-            __syncthreads();
             const bool wave_on = __ballot(dA != 0.0f) != 0;
-            if (wave_on && lane == 0) s_any = 1;
             const float Tw = wave_sum(dA);  // the wavefront's dA total
-            __syncthreads();
-            // pieces of path j: [pb, pb + cnt) of the staged paths, in windows
-            const uint32_t pb = s_pb[jl];
-            for (uint32_t e0 = pb; s_any && e0 < pb + cnt; e0 += ECAP) {
-                const uint32_t ne = pb + cnt - e0 < ECAP ? pb + cnt - e0 : ECAP;
-                uint32_t base = e0 - w_e0;  // where the window's first piece sits
-                if (!(e0 >= w_e0 && e0 + ne <= w_e0 + w_n)) {
+            // the path's pieces [pb, pb + cnt), in windows from the top
+            for (uint32_t e_hi = pb + cnt; e_hi > pb;) {
+                const uint32_t e_lo = e_hi - pb > ECAP ? e_hi - ECAP : pb;
+                if (!(e_lo >= w_lo && e_hi <= w_lo + w_n)) {
+                    // the walk leaves the staged window: its pieces' gradients, then
+                    // a window ending at e_hi (pieces of this path and the ones before)
+                    flush_pieces(w_n);
+                    const uint32_t lo = e_hi > ECAP ? e_hi - ECAP : 0u;
+                    stage_pieces(S, k0 + j0, ns, Y, lo, e_hi - lo, sg, bad);
                     __syncthreads();
-                    stage_pieces(S, k0 + j0, ns, Y, e0, ne, sg, bad);
-                    w_e0 = e0;
-                    w_n = ne;
-                    base = 0;
+                    w_lo = lo;
+                    w_n = e_hi - lo;
                 }
-                for (uint32_t i = tid; i < ne; i += DBLOCK) s_S[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                __syncthreads();
-                for (uint32_t i = 0; wave_on && i < ne; ++i) {
-                    const float4 pc = s_pc[base + i];
-                    if (!(s_ev[base + i].x & LIVE_BIT)) continue;
+                for (uint32_t e = e_lo; wave_on && e < e_hi; ++e) {
+                    const uint32_t i = e - w_lo;
+                    const float4 pc = s_pc[i];
+                    if (!(s_ev[i] & LIVE_BIT)) continue;
                     const float lo = fminf(pc.x, pc.y), hi = fmaxf(pc.x, pc.y);
                     if (xw0 + 64.0f <= lo) continue;  // M = 0 on every pixel of the wavefront
                     if (xw0 >= hi) {                 // M = 1, dM = 0
@@ -627,54 +676,21 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                     }
                     float mua, mub;
                     const float M = mean_clamp(Xf + 1.0f - pc.x, Xf + 1.0f - pc.y, mua, mub);
-                    const float S1 = wave_sum(dA * M), S2 = wave_sum(dA * mua), S3 = wave_sum(dA * mub);
+                    float S1 = dA * M, S2 = dA * mua, S3 = dA * mub;
+                    wave_sum3(S1, S2, S3);
                     if (lane == 0) {
                         atomicAdd(&s_S[i].x, S1);
                         atomicAdd(&s_S[i].y, S2);
                         atomicAdd(&s_S[i].z, S3);
                     }
                 }
-                __syncthreads();
-                for (uint32_t i = tid; i < ne; i += DBLOCK) {
-                    const uint2 ev = s_ev[base + i];
-                    const float4 Sm = s_S[i];
-                    if (!(ev.x & LIVE_BIT) || ((Sm.x == 0.0f) & (Sm.y == 0.0f) & (Sm.z == 0.0f))) continue;
-                    const uint32_t i0 = ev.y < V ? ev.y : 0u;  // staged below V
-                    const uint32_t i1 = i0 + 1 < V ? i0 + 1 : 0u;
-                    float gr[4];
-                    piece_grad(S.vtx[vb + i0], S.vtx[vb + i1], Y, Sm.x, Sm.y, Sm.z, gr);
-                    if (gather) {  // i0, i1 < V <= VCACHE
-                        atomicAdd(&s_gv[i0].x, gr[0]);
-                        atomicAdd(&s_gv[i0].y, gr[1]);
-                        atomicAdd(&s_gv[i1].x, gr[2]);
-                        atomicAdd(&s_gv[i1].y, gr[3]);
-                    } else {
-                        atomicAdd(&gvtx[vb + i0].x, gr[0]);
-                        atomicAdd(&gvtx[vb + i0].y, gr[1]);
-                        atomicAdd(&gvtx[vb + i1].x, gr[2]);
-                        atomicAdd(&gvtx[vb + i1].y, gr[3]);
-                    }
-                }
-                __syncthreads();  // s_S and the window are reused
+                e_hi = e_lo;
             }
-            // one global atomic per colour channel and touched vertex, per block
-            __syncthreads();
-            if (tid < 4) {
-                const float a = s_gc[tid];
-                if (a != 0.0f) {
-                    atomicAdd(&gcol[4 * (uint64_t)k + tid], a);
-                    s_gc[tid] = 0.0f;
-                }
-            }
-            for (uint32_t v = tid; gather && v < V; v += DBLOCK) {
-                const float2 a = s_gv[v];
-                if ((a.x != 0.0f) | (a.y != 0.0f)) {
-                    atomicAdd(&gvtx[vb + v].x, a.x);
-                    atomicAdd(&gvtx[vb + v].y, a.y);
-                    s_gv[v] = make_float2(0.0f, 0.0f);
-                }
-            }
-            __syncthreads();
+        }
+        flush_pieces(w_n);  // the segment's last window (and its barrier before the colours)
+        if (tid < (int)ns * 4) {
+            const float a = s_gc[tid >> 2][tid & 3];
+            if (a != 0.0f) atomicAdd(&gcol[4 * (uint64_t)(k0 + j0 + (tid >> 2)) + (tid & 3)], a);
         }
     }
     if (bad) raise_err(st, FT_EDEVICE);
