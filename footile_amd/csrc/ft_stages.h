@@ -851,24 +851,22 @@ FT_HD inline bool path_forward(const PathState &s, const Limits &L, const I2 *pt
     return widdershins(wsub(qr.x, q.x), wsub(qr.y, q.y), wsub(qf.x, q.x), wsub(qf.y, q.y));
 }
 
-// Signed-area cell deltas of one edge on one scan row: the closed form of
+// Signed-area cell deltas of one edge on one scan row: the per-row form of
 // Edge::{starting_cov, continuing_cov, calculate_x_limits_*, scan_area,
 // first_cov, step_cov} (fig.rs:238-321).  Cells left of x = 0 land in cell 0;
 // cells at x >= W are dropped (fig.rs:292-295).
 //
 // The reference walks the row pixel by pixel: at step k the covered
 // fraction is x_pix(k) = min(pixel_cov(min(xc0 + k * st, ONE)), cov), and
-// pixel x = x_s + j receives x_pix(k_s + j) - x_pix(k_s + j - 1) until
-// x_pix reaches cov.  pixel_cov(u) = (u + 128) >> 8 for 0 <= u <= ONE, so
-// x_pix(k) >= cov exactly when xc0 + k * st >= 256 * cov - 128 (ONE is past
-// that threshold), which gives the number of cells in closed form.
-// row_setup describes the row; row_walk produces its cells in order.
+// pixel x = x_s + j receives x_pix(k_s + j) - x_pix(k_s + j - 1), up to and
+// including the first pixel where x_pix reaches cov, or the raster's right
+// edge.  row_setup describes the row; row_walk produces its cells in order.
 struct RowCells {
     int64_t xc0, st;  // x_cov at step k: min(xc0 + k * st, ONE)
     int32_t cov;      // the row's coverage, 1..256
     int32_t x_s;      // first cell
     int32_t k_s;      // step of the first cell
-    int32_t n;        // cells (0: none)
+    int32_t n;        // cells before the right edge (0: none); the walk ends earlier once x_pix reaches cov
 };
 
 FT_HD inline RowCells row_setup(const EdgeRec &E, int32_t sr, int32_t W) {
@@ -899,20 +897,16 @@ FT_HD inline RowCells row_setup(const EdgeRec &E, int32_t sr, int32_t W) {
     const int64_t k0 = min_pix < 0 ? -(int64_t)min_pix : 0;
     c.k_s = (int32_t)(k0 < 65537 ? k0 : 65537);
     c.x_s = min_pix < 0 ? 0 : min_pix;
-    // first j >= 0 with xc0 + (k_s + j) * st >= 256 * cov - 128
-    // (left <= 256 * 256 and st <= ONE, so the division fits 32 bits)
-    const int64_t left = ((int64_t)cov * 256 - 128) - (c.xc0 + (int64_t)c.k_s * c.st);
-    const int64_t j_stop = left <= 0 ? 0 : (int64_t)(((uint32_t)left + (uint32_t)c.st - 1u) / (uint32_t)c.st);
-    const int64_t room = (int64_t)W - c.x_s;
-    c.n = (int32_t)(j_stop + 1 < room ? j_stop + 1 : room);
+    c.n = W - c.x_s;
     return c;
 }
 
-// The row's cells in order: f(j, delta of cell x_s + j) for j in [0, n).
+// The row's cells in order: f(j, delta of cell x_s + j) for j = 0, 1, ...
+// up to the first cell whose covered fraction reaches cov, or j = n - 1.
 // x_cov = min(xc0 + k * st, ONE) is walked from k = k_s in 32 bits: it
 // starts at or below ONE (xc0 >= 0), stays at ONE once capped, and below the
 // cap one more step (0 < st <= ONE) fits; on [0, ONE], pixel_cov(u) is
-// (u + 128) >> 8.
+// (u + 128) >> 8, which reaches 256 >= cov at ONE, so the walk ends.
 template <class F>
 FT_HD inline void row_walk(const RowCells &c, F &&f) {
     const int64_t xk = c.xc0 + (int64_t)c.k_s * c.st;
@@ -924,6 +918,7 @@ FT_HD inline void row_walk(const RowCells &c, F &&f) {
         const int32_t cur = xp < c.cov ? xp : c.cov;
         xc = xc < FX_ONE - st ? xc + st : FX_ONE;
         f(j, cur - prev);
+        if (cur >= c.cov) break;
         prev = cur;
     }
 }
@@ -977,7 +972,14 @@ struct Tuning {
                                              // scheduler (0: none)
 };
 
-FT_HD inline uint32_t cell_stride(uint32_t width) { return (width + CELL_ALIGN - 1) / CELL_ALIGN * CELL_ALIGN; }
+// Row stride of the LDS cells: the width rounded up to CELL_ALIGN (the row
+// pass reads whole 16-cell chunks, and cells past the width read as 0), plus
+// 4 so that the same column of consecutive rows falls in different LDS banks
+// (a 32-bank LDS sees a multiple of 32 words as the same bank; the (edge,
+// row) pairs of one edge sit in consecutive rows at nearby columns).
+FT_HD inline uint32_t cell_stride(uint32_t width) {
+    return (width + CELL_ALIGN - 1) / CELL_ALIGN * CELL_ALIGN + 4u;
+}
 
 // rows: band height; group: rows per LDS group; cells: LDS cells per block.
 FT_HD inline void band_geometry(uint32_t width, uint32_t height, const Tuning &t, uint32_t *rows, uint32_t *group,

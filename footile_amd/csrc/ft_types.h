@@ -79,7 +79,7 @@ struct DevStatus {
     unsigned long long stroke_total;  // stroker outline: events << 32 | closes
     uint32_t st_subs, st_items;       // sub-strokes kept; outline items
     uint32_t spin_fail;
-    uint32_t tile_next[3];  // tile scheduler queue heads, one per pixel format
+    uint32_t pad[3];
 };
 
 struct EdgeRec {

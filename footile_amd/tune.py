@@ -104,7 +104,7 @@ def autotune(batch, descs, n_ops, cache=DEFAULT_CACHE, reps=3, candidates=None, 
             print(f"[tune] band_rows_min={b} lds_cells={c} dst_prefetch={p} tile_groups={tg}: {ms:.3f} ms")
         if best is None or ms < best[4]:
             best = (b, c, p, tg, ms)
-    batch.set_tuning(best[0], best[1], best[2], best[3])
+    batch.set_tuning(*best[:4])
     entry = {"band_rows_min": best[0], "lds_cells": best[1], "dst_prefetch": best[2], "tile_groups": best[3],
              "ms": best[4], "tune_s": time.perf_counter() - t0}
     if cache:

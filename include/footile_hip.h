@@ -63,7 +63,10 @@ typedef struct {
 #define FT_FLATTEN_MAX_DEPTH 32
 #define FT_FLATTEN_MAX_LEAVES (1u << 20)
 #define FT_MAX_POINTS 65535u /* Vid::MAX, src/vid.rs:17 */
-#define FT_MAX_WIDTH 16384u
+#define FT_MAX_WIDTH 32768u /* a raster row's i32 cells (128 KiB at this width) are
+                              held in LDS at once; a device whose LDS per block is
+                              smaller rejects wide rows with FT_ENOTSUP when the
+                              batch is laid out */
 
 /* ---------------------------------------------------------------------------
  * Plotter -- replaces footile::Plotter<P> (src/plotter.rs:38-381).

@@ -142,7 +142,7 @@ int stroke_outline(const F4 *ev, uint32_t n_ev, const PathDev &P, uint32_t join,
 }  // namespace
 
 // The reference's pixel walk along one scan row (fig.rs:285-321), step by
-// step, as a check of the closed form in row_setup and the walk in row_walk.
+// step, as a check of the set-up in row_setup and the walk in row_walk.
 static void walk_row(const EdgeRec &E, int32_t sr, int32_t ed, int32_t W, int32_t *cells) {
     const int32_t r0 = fx_int(E.y_upper), r1 = fx_int(E.y_lower);
     const bool starting = sr == r0, ending = sr == r1;
@@ -173,7 +173,7 @@ static void walk_row(const EdgeRec &E, int32_t sr, int32_t ed, int32_t W, int32_
 
 extern "C" {
 
-// Random edges and rows: cells from edge_row_cells (closed form) against
+// Random edges and rows: cells from edge_row_cells (row_setup + row_walk) against
 // walk_row (step by step).  Returns the number of mismatching rows.
 int ftm_row_cells_check(uint64_t seed, int iters) {
     uint64_t z = seed;

@@ -54,7 +54,7 @@ def test_mirror_random_walk_4096(oracle):
 
 
 def test_row_cells_closed_form_matches_pixel_walk():
-    """The per-row cell count in closed form and the 32-bit cell walk
+    """The per-row set-up in closed form and the 32-bit cell walk
     (row_setup / row_walk, shared with the gfx950 raster) against the
     pixel-by-pixel walk of fig.rs:285-321 on random edges, rows and raster
     widths, including edges left of x = 0 and past the right edge."""
