@@ -59,9 +59,6 @@ struct Workspace {
     // pairs walked and nonzero cell deltas added
     bool stats = false;
     unsigned long long *d_stats = nullptr;
-    // the small-figure path's ops (one workgroup, ws_small_fill)
-    ft_path_op *d_small_ops = nullptr;
-    size_t cap_small_ops = 0;
     // per-op scratch
     unsigned long long *d_op_cnt = nullptr, *d_op_off = nullptr;
     unsigned long long *d_sub_off = nullptr;  // per (op, lane): offset within the op (+ leaf count)
@@ -150,11 +147,6 @@ ft_status ws_launch(Workspace &ws, uint32_t mode);
 ft_status ws_finish(Workspace &ws, bool read_pstate);
 // launch + finish, growing capacities and relaunching on overflow.
 ft_status ws_run(Workspace &ws, uint32_t mode, bool read_pstate);
-// One fill of host ops in one workgroup when it is small enough (ops, events
-// and sub-figures in LDS): sets *handled and the offered / stored point
-// counts; otherwise *handled is false and nothing was drawn.
-ft_status ws_small_fill(Workspace &ws, const ft_path_op *ops, size_t n_ops, const ft_fill_desc &d, bool *handled,
-                        uint64_t *offered, uint64_t *stored);
 // Run the batch once (eagerly) counting the raster's (edge, row) pairs and
 // its nonzero cell deltas (the LDS atomics of the cell pass).
 ft_status ws_raster_stats(Workspace &ws, uint64_t *pairs, uint64_t *adds);
