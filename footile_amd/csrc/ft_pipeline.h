@@ -40,6 +40,8 @@ struct Workspace {
     std::vector<PathDev> h_paths;
     std::vector<uint4> h_rblock;
     bool uploads_pending = false;  // a layout's uploads may not have run yet (cleared by a stream sync)
+    uint8_t *h_pin = nullptr;      // pinned staging of the layout's uploads
+    size_t cap_pin = 0;
     bool op_ordered = false;
     PathDev *d_paths = nullptr;
     uint32_t *d_band_path = nullptr;

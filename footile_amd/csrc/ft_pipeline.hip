@@ -1662,6 +1662,7 @@ void ws_free(Workspace &ws) {
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ws.h_status) (void)hipHostFree(ws.h_status);
+    if (ws.h_pin) (void)hipHostFree(ws.h_pin);
     if (ws.ev) {
         for (int i = 0; i < 3 * Workspace::TIMING_RING; ++i)
             if (ws.ev[i]) (void)hipEventDestroy(ws.ev[i]);
@@ -1882,13 +1883,17 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
             }
         }
     }
-    for (int f = 0; f < 3; ++f) {  // largest first within each format
+    for (int f = 0; f < 3; ++f) {  // largest first within each format (a single path's blocks are alike)
+        ws.rb_first[f] = (uint32_t)rblock.size();
+        ws.rb_count[f] = (uint32_t)rblock_f[f].size();
+        if (n_paths < 2) {
+            rblock.insert(rblock.end(), rblock_f[f].begin(), rblock_f[f].end());
+            continue;
+        }
         std::vector<uint32_t> order(rblock_f[f].size());
         for (size_t i = 0; i < order.size(); ++i) order[i] = (uint32_t)i;
         std::stable_sort(order.begin(), order.end(),
                          [&](uint32_t a, uint32_t b) { return rcost_f[f][a] > rcost_f[f][b]; });
-        ws.rb_first[f] = (uint32_t)rblock.size();
-        ws.rb_count[f] = (uint32_t)order.size();
         for (uint32_t i : order) rblock.push_back(rblock_f[f][i]);
     }
     ws.h_items.clear();
@@ -1965,10 +1970,6 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
         }
         if ((s = ensure_scan_capacity(ws))) return s;
     }
-    if (n_ops) {
-        FT_HIP(hipMemcpyAsync(ws.d_ops, ops, n_ops * sizeof(ft_path_op), hipMemcpyHostToDevice, ws.stream));
-        FT_HIP(hipMemcpyAsync(ws.d_op_path, op_path.data(), n_ops * 4, hipMemcpyHostToDevice, ws.stream));
-    }
     // Fill runs whose (op, lane) items fit one resident grid flatten in one
     // pass (k_flatten), which takes them in op order.  Larger batches flatten
     // their ops cubics first, then quads, then the rest (stable within each),
@@ -1984,18 +1985,48 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
                 const uint32_t cls = tag == FT_OP_CUBIC ? 0u : (tag == FT_OP_QUAD ? 1u : 2u);
                 if (cls == pass) ws.h_op_order[at++] = (uint32_t)i;
             }
-        FT_HIP(hipMemcpyAsync(ws.d_op_order, ws.h_op_order.data(), n_ops * 4, hipMemcpyHostToDevice, ws.stream));
     }
-    if (n_paths)
-        FT_HIP(hipMemcpyAsync(ws.d_paths, pd.data(), n_paths * sizeof(PathDev), hipMemcpyHostToDevice, ws.stream));
-    if (nb_total)
-        FT_HIP(hipMemcpyAsync(ws.d_band_path, band_path.data(), nb_total * 4, hipMemcpyHostToDevice, ws.stream));
-    if (!rblock.empty())
-        FT_HIP(hipMemcpyAsync(ws.d_rblock, rblock.data(), rblock.size() * sizeof(uint4), hipMemcpyHostToDevice,
-                              ws.stream));
-    if (!ws.h_items.empty())
-        FT_HIP(hipMemcpyAsync(ws.d_items, ws.h_items.data(), ws.h_items.size() * sizeof(uint2),
-                              hipMemcpyHostToDevice, ws.stream));
+    // The layout's uploads, staged through one pinned host buffer so that each
+    // copy is a plain DMA (copies from pageable memory go through the
+    // runtime's own staging, one host round trip each).  Arena workspaces,
+    // which may not allocate, copy from the sources directly.
+    struct Upload {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    };
+    const Upload up[] = {
+        {ws.d_ops, ops, n_ops * sizeof(ft_path_op)},
+        {ws.d_op_path, op_path.data(), n_ops * 4},
+        {ws.d_op_order, ws.h_op_order.data(), ws.op_ordered ? n_ops * 4 : 0},
+        {ws.d_paths, pd.data(), n_paths * sizeof(PathDev)},
+        {ws.d_band_path, band_path.data(), nb_total * 4},
+        {ws.d_rblock, rblock.data(), rblock.size() * sizeof(uint4)},
+        {ws.d_items, ws.h_items.data(), ws.h_items.size() * sizeof(uint2)},
+    };
+    size_t staged = 0;
+    for (const Upload &u : up) staged += (u.bytes + 255) / 256 * 256;
+    if (!ws.arena && staged > ws.cap_pin) {
+        if (ws.h_pin) (void)hipHostFree(ws.h_pin);
+        ws.h_pin = nullptr;
+        ws.cap_pin = 0;
+        if (hipHostMalloc(reinterpret_cast<void **>(&ws.h_pin), staged, hipHostMallocDefault) != hipSuccess) {
+            ws.h_pin = nullptr;
+            return FT_ENOMEM;
+        }
+        ws.cap_pin = staged;
+    }
+    size_t at = 0;
+    for (const Upload &u : up) {
+        if (!u.bytes) continue;
+        const void *src = u.src;
+        if (!ws.arena) {
+            memcpy(ws.h_pin + at, u.src, u.bytes);
+            src = ws.h_pin + at;
+            at += (u.bytes + 255) / 256 * 256;
+        }
+        FT_HIP(hipMemcpyAsync(u.dst, src, u.bytes, hipMemcpyHostToDevice, ws.stream));
+    }
     ws.uploads_pending = true;
     return FT_OK;
 }
