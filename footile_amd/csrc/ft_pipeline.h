@@ -27,6 +27,7 @@ struct Workspace {
     int device = 0;
     size_t lds_max = 65536;  // LDS bytes per block on the device
     uint32_t scan_resident = 0;  // scan blocks resident at once (0: unknown)
+    uint32_t flat_resident = 0;  // k_flatten blocks resident at once (0: k_flatten unused)
     // batch layout
     size_t n_ops = 0, n_paths = 0, n_bands = 0;
     size_t cap_ops = 0, cap_paths = 0, cap_bands = 0;

@@ -213,23 +213,25 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uin
 
 // Flattening in one pass, for fill runs whose (op, lane) items fit one
 // resident grid at one item per thread and are taken in op order: a block
-// counts its items (keeping up to LEAF_CACHE points per lane), takes its
-// prefix by look-back over the blocks before it, and emits them.  Writes each
-// op's offset (op_off, as the op scan does) and the run's event and close
-// totals (as k_emit does).
+// counts its items (keeping each curve lane's first LEAF_CACHE points in LDS),
+// takes its prefix by look-back over the blocks before it, and emits them (a
+// copy from LDS unless a lane has more leaves).  Writes each op's offset
+// (op_off, as the op scan does) and the run's event and close totals (as
+// k_emit does).
 __global__ __launch_bounds__(BLOCK) void k_flatten(const ft_path_op *ops, const uint32_t *op_path,
                                                    const PathDev *paths, uint32_t n_ops,
-                                                   unsigned long long *op_off, I2 *leaf, size_t stride,
-                                                   unsigned long long *lb, EventSink sink, DevStatus *st) {
+                                                   unsigned long long *op_off, unsigned long long *lb,
+                                                   EventSink sink, DevStatus *st) {
     __shared__ unsigned long long s_wave[BLOCK / 64];
     __shared__ unsigned long long s_prefix;
+    __shared__ I2 s_leaf[LEAF_CACHE * BLOCK];  // point k of thread j at s_leaf[k * BLOCK + j]
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint64_t n_items = (uint64_t)n_ops * OP_LANES;
     const uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     const bool live = t < n_items;
     const uint32_t i = live ? (uint32_t)(t >> FLAT_SPLIT) : 0u, s = (uint32_t)t & (OP_LANES - 1);
     OpCount c = {0, 0, 0, 0};
-    if (live) c = count_op(ops, op_path, paths, i, s, MODE_FILL, nullptr, leaf + t, stride);
+    if (live) c = count_op(ops, op_path, paths, i, s, MODE_FILL, nullptr, s_leaf + threadIdx.x, BLOCK);
     const unsigned long long v = live ? pack_count(c) : 0ull;
     uint32_t leaves = c.leaves;
 #pragma unroll
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(BLOCK) void k_flatten(const ft_path_op *ops, const 
     // a lane whose events would pass the capacity writes nothing: the host
     // grows the buffers and runs again
     if (!live || (off >> 32) + c.ne > sink.cap_events || (uint32_t)off + c.nc > sink.cap_runs) return;
-    if (emit_op(ops, op_path, paths, i, s, MODE_FILL, nullptr, off, sink, leaf + t, c.leaves, stride))
+    if (emit_op(ops, op_path, paths, i, s, MODE_FILL, nullptr, off, sink, s_leaf + threadIdx.x, c.leaves, BLOCK))
         raise_err(st, FT_EDEVICE);
 }
 
@@ -1616,10 +1618,23 @@ static ft_status dev_alloc(Workspace &ws, T *&ptr, size_t n) {
     return FT_OK;
 }
 
+// Blocks of k_flatten the device holds at once (every block of its grid
+// takes a look-back prefix, so all must be resident): its occupancy per
+// compute unit, at most the four per CU that the scans assume, times the CUs.
+static ft_status flatten_resident(Workspace &ws) {
+    int per_cu = 0, cus = 0;
+    FT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_flatten, BLOCK, 0));
+    FT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws.device));
+    const uint32_t g = (uint32_t)((cus > 0 ? cus : 0) * (per_cu < 4 ? (per_cu > 0 ? per_cu : 0) : 4));
+    ws.flat_resident = g < SCAN_GRID_MAX ? g : SCAN_GRID_MAX;
+    return FT_OK;
+}
+
 ft_status ws_init_arena(Workspace &ws, void *arena, size_t bytes, hipStream_t stream) {
     if (!arena || (reinterpret_cast<uintptr_t>(arena) & 255u)) return FT_EINVAL;
     FT_HIP(hipGetDevice(&ws.device));
     ws.scan_resident = scan_resident_blocks();
+    if (ft_status s = flatten_resident(ws)) return s;
     int lds_max = 0;
     FT_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ws.device));
     ws.lds_max = (size_t)lds_max;
@@ -1634,6 +1649,7 @@ ft_status ws_init_arena(Workspace &ws, void *arena, size_t bytes, hipStream_t st
 ft_status ws_init(Workspace &ws) {
     FT_HIP(hipGetDevice(&ws.device));
     ws.scan_resident = scan_resident_blocks();
+    if (ft_status s = flatten_resident(ws)) return s;
     int lds_max = 0;
     FT_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ws.device));
     ws.lds_max = (size_t)lds_max;
@@ -1780,7 +1796,7 @@ static ft_status arena_split(Workspace &ws) {
 // One-pass flattening (k_flatten): one item per thread, every block resident.
 static inline bool flatten_in_one_pass(const Workspace &ws, size_t n_ops) {
     const size_t blocks = (n_ops * OP_LANES + BLOCK - 1) / BLOCK;
-    return ws.scan_resident != 0 && blocks <= ws.scan_resident;
+    return blocks <= ws.flat_resident;
 }
 
 // The tile scheduler's persistent grid for pixel format f: as many blocks as
@@ -2124,8 +2140,8 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         em.run_end = ws.d_run_end;
         const size_t n_items = (size_t)n_ops * OP_LANES;
         if (mode == MODE_FILL && !order && flatten_in_one_pass(ws, n_ops)) {
-            k_flatten<<<(uint32_t)((n_items + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
-                ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, ws.d_op_off, leaf, ws.cap_ops * OP_LANES, lb, em, st);
+            k_flatten<<<(uint32_t)((n_items + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths,
+                                                                                 n_ops, ws.d_op_off, lb, em, st);
         } else {
             k_count<<<grid_for(n_items), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode, lastpw, order,
                                                         ws.d_op_cnt, ws.d_sub_off, leaf, ws.cap_ops * OP_LANES, st);

@@ -483,6 +483,76 @@ def test_degenerate_inputs(ft, oracle):
                 assert g.last_counts() == o.last_counts(), name
 
 
+def _fig_cases():
+    """Paths that exercise every Fig rule: the 65,535-point cap with and
+    without its closing pop, duplicate and coincident points, several
+    subfigures with closes, a close right after a move, a lone move, no ops,
+    a flat figure, saturating coordinates, and plain random figures."""
+    rng = np.random.default_rng(44)
+    n = 66000
+    pts = np.stack([rng.uniform(0, 250, n), rng.uniform(0, 250, n)], axis=1).astype(np.float32)
+    pts[65534] = pts[0]
+    base = [(MOVE, tuple(pts[0]))] + [(LINE, tuple(p)) for p in pts[1:]] + [(CLOSE, ())]
+    tail = [(MOVE, (5.0, 5.0)), (LINE, (60.0, 9.0)), (LINE, (30.0, 70.0)), (CLOSE, ())]
+    cases = [base, base + tail, base[:65536] + [(CLOSE, ())] + tail,
+             base[:65535] + [(LINE, tuple(pts[0])), (CLOSE, ())] + tail,
+             [], [(MOVE, (3.0, 4.0)), (CLOSE, ())],
+             [(MOVE, (1.0, 5.0)), (LINE, (12.0, 5.0)), (LINE, (7.0, 5.0)), (CLOSE, ())],
+             [(MOVE, (-1.0e6, -1.0e6)), (LINE, (1.0e6, 3.0e4)), (LINE, (16.0, 1.0e7)),
+              (QUAD, (-5.0e5, 10.0, 8.0, 8.0)), (CLOSE, ())],
+             [(MOVE, (10.0, 10.0)), (CLOSE, ()), (MOVE, (20.0, 20.0)), (LINE, (20.0, 20.0)), (LINE, (90.0, 30.0)),
+              (LINE, (90.0, 30.0)), (LINE, (40.0, 100.0)), (LINE, (20.0, 20.0)), (CLOSE, ()), (CLOSE, ()),
+              (MOVE, (150.0, 10.0)), (LINE, (200.0, 60.0)), (LINE, (120.0, 90.0))]]
+    while len(cases) < 80:
+        k = len(cases)
+        ops = []
+        for _ in range(int(rng.integers(1, 4))):  # subfigures, some left open
+            ops.append((MOVE, tuple(rng.uniform(0, 256, 2).astype(np.float32))))
+            for _ in range(int(rng.integers(1, 12))):
+                r = rng.random()
+                if r < 0.15 and ops[-1][0] == LINE:
+                    ops.append(ops[-1])  # a duplicate point
+                elif r < 0.5:
+                    ops.append((LINE, tuple(rng.uniform(0, 256, 2).astype(np.float32))))
+                elif r < 0.75:
+                    ops.append((QUAD, tuple(rng.uniform(0, 256, 4).astype(np.float32))))
+                else:
+                    ops.append((CUBIC, tuple(rng.uniform(0, 256, 6).astype(np.float32))))
+            if k % 3:
+                ops.append((CLOSE, ()))
+        cases.append(ops)
+    return cases
+
+
+def test_batch_of_many_paths_fig_rules(ft, oracle):
+    """A batch of 80 paths holding the Fig's edge cases side by side (so the
+    ingest scans run across path boundaries): every raster and every path's
+    offered / stored counts against a Plotter of its own, both rules."""
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    cases = _fig_cases()
+    arrays = [ops_array(c) for c in cases]
+    ops = np.concatenate([a for a in arrays if len(a)])
+    w = h = 256
+    for rule in (0, 1):
+        bufs, descs, at = [], [], 0
+        for k, a in enumerate(arrays):
+            b = DeviceBuffer(w * h)
+            b.upload(np.full(w * h, k, dtype=np.uint8))
+            bufs.append(b)
+            descs.append(FillDesc(at, len(a), b.ptr, w, h, w, MATTE8, rule=rule, clr=(255,)))
+            at += len(a)
+        batch = Batch(ops, descs)
+        batch.run()
+        offered, stored, _ = batch.counts()
+        for k, a in enumerate(arrays):
+            o = oracle.Plotter(w, h)
+            o.set_pixels(np.full(w * h, k, dtype=np.uint8))
+            assert o.fill(rule, a, (255,)) == 0
+            got = bufs[k].download()
+            assert np.array_equal(got, o.pixels()), f"rule {rule} path {k}: {int((got != o.pixels()).sum())} bytes"
+            assert (offered[k], stored[k]) == o.last_counts(), f"rule {rule} path {k}"
+
+
 def test_batch_band_pair_overflow_grows_and_reruns(ft, oracle):
     """Long near-vertical edges over one-row bands give far more (edge, band)
     pairs than the first run's list capacity: the run reports the overflow,
