@@ -380,8 +380,9 @@ def main():
     ap.add_argument("--atlas-tuning", default=None,
                     help="band_rows_min,lds_cells[,tile_groups] for the atlas instead of autotuning "
                          "(profiling runs)")
-    ap.add_argument("--only", choices=["atlas", "diff"], default=None,
-                    help="run only the atlas or the differentiable measurement and print its JSON (profiling runs)")
+    ap.add_argument("--only", choices=["atlas", "diff", "plain"], default=None,
+                    help="run only the atlas, the differentiable or the plain 256x256 Plotter measurement and "
+                         "print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -397,6 +398,9 @@ def main():
     if args.only == "atlas":
         print(json.dumps(bench_atlas(ft, args.steps, args.warmup, args.atlas_tuning, min(4.0, args.cpu_seconds))),
               flush=True)
+        return
+    if args.only == "plain":
+        print(json.dumps(bench_plain(ft)), flush=True)
         return
     if args.only == "diff":
         print(json.dumps(bench_diff(args.steps, args.warmup, check=not args.no_diff_check)), flush=True)

@@ -30,7 +30,12 @@ struct Workspace {
     uint32_t flat_resident = 0;  // k_flatten blocks resident at once (0: k_flatten unused)
     // batch layout
     size_t n_ops = 0, n_paths = 0, n_bands = 0;
-    size_t cap_ops = 0, cap_paths = 0, cap_bands = 0;
+    size_t cap_ops = 0, cap_paths = 0;
+    // The layout's uploads (ops, op -> path, op order, paths, band -> path,
+    // raster blocks, tile items) are sections of one device block, filled
+    // from the pinned staging by one copy; the pointers below point into it.
+    uint8_t *d_lay = nullptr;
+    size_t cap_lay = 0;
     ft_path_op *d_ops = nullptr;
     uint32_t *d_op_path = nullptr;
     uint32_t *d_op_order = nullptr;  // flattening order of the ops (a permutation)
@@ -48,14 +53,13 @@ struct Workspace {
     uint32_t *d_band_path = nullptr;
     uint32_t lds_cells = 0;  // dynamic LDS cells needed by the raster kernel
     // raster blocks: (global band, row group) per block
-    size_t n_rblocks = 0, cap_rblocks = 0;
+    size_t n_rblocks = 0;
     uint32_t rb_first[3] = {0, 0, 0}, rb_count[3] = {0, 0, 0};  // raster blocks per pixel format
     uint4 *d_rblock = nullptr;  // raster blocks: (global band, row group, path, 0)
     // tile scheduler work items (path, row group) of tile-mode paths, per
     // pixel format, largest first; and the persistent grid per format
     std::vector<uint2> h_items;
     uint2 *d_items = nullptr;
-    size_t cap_items = 0;
     uint32_t it_first[3] = {0, 0, 0}, it_count[3] = {0, 0, 0};
     uint32_t tile_grid[3] = {0, 0, 0};
     // raster statistics (ws_raster_stats): per raster block, (edge, row)
@@ -103,9 +107,16 @@ struct Workspace {
     // results
     PathState *d_pstate = nullptr;
     unsigned long long *d_top_rep = nullptr;  // per path: TOP_REPS partial top keys
+    // Outside arena mode the status and the path states are one device block
+    // and one pinned host block (status first, STATUS_BYTES long), so that a
+    // run's final copy can bring both back at once.
     DevStatus *d_status = nullptr;
     DevStatus *h_status = nullptr;  // pinned
-    std::vector<PathState> h_pstate;
+    PathState *h_pstate = nullptr;  // host copy of d_pstate (pinned, or h_pstate_store in arena mode)
+    uint8_t *d_sp = nullptr, *h_sp = nullptr;
+    size_t cap_sp_paths = 0;
+    std::vector<PathState> h_pstate_store;
+    bool pstate_copied = false;  // the last eager run's final copy included the path states
     bool edges_counted = false;  // PathState::n_edges holds the last run's counts
     // band / LDS geometry of the next ws_set_batch
     Tuning tuning;
@@ -145,7 +156,8 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
 ft_status ws_upload_ops(Workspace &ws, const ft_path_op *ops, size_t n_ops);
 // Enqueue a run.  mode = MODE_FILL renders; MODE_STROKE_FLATTEN stops after
 // emitting (x, y, width, kind) events for the host stroker.
-ft_status ws_launch(Workspace &ws, uint32_t mode);
+// read_pstate: an eager run also copies the path states back with its status.
+ft_status ws_launch(Workspace &ws, uint32_t mode, bool read_pstate = false);
 // Wait and read the status block; returns the run's status.
 ft_status ws_finish(Workspace &ws, bool read_pstate);
 // launch + finish, growing capacities and relaunching on overflow.

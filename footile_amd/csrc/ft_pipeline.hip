@@ -1646,6 +1646,32 @@ ft_status ws_init_arena(Workspace &ws, void *arena, size_t bytes, hipStream_t st
     return FT_OK;
 }
 
+// The status + path-state blocks (device and pinned host) for n_paths paths;
+// the previous blocks are freed.
+constexpr size_t STATUS_BYTES = 256;
+static_assert(sizeof(DevStatus) <= STATUS_BYTES, "the status fits in front of the path states");
+
+static ft_status sp_grow(Workspace &ws, size_t n_paths) {
+    const size_t bytes = STATUS_BYTES + n_paths * sizeof(PathState);
+    uint8_t *d = nullptr, *h = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&d), bytes) != hipSuccess) return FT_ENOMEM;
+    if (hipHostMalloc(reinterpret_cast<void **>(&h), bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipFree(d);
+        return FT_ENOMEM;
+    }
+    memset(h, 0, bytes);
+    if (ws.d_sp) (void)hipFree(ws.d_sp);
+    if (ws.h_sp) (void)hipHostFree(ws.h_sp);
+    ws.d_sp = d;
+    ws.h_sp = h;
+    ws.cap_sp_paths = n_paths;
+    ws.d_status = reinterpret_cast<DevStatus *>(d);
+    ws.h_status = reinterpret_cast<DevStatus *>(h);
+    ws.d_pstate = reinterpret_cast<PathState *>(d + STATUS_BYTES);
+    ws.h_pstate = reinterpret_cast<PathState *>(h + STATUS_BYTES);
+    return FT_OK;
+}
+
 ft_status ws_init(Workspace &ws) {
     FT_HIP(hipGetDevice(&ws.device));
     ws.scan_resident = scan_resident_blocks();
@@ -1654,10 +1680,7 @@ ft_status ws_init(Workspace &ws) {
     FT_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ws.device));
     ws.lds_max = (size_t)lds_max;
     FT_HIP(hipStreamCreateWithFlags(&ws.stream, hipStreamNonBlocking));
-    FT_HIP(hipMalloc(reinterpret_cast<void **>(&ws.d_status), sizeof(DevStatus)));
-    FT_HIP(hipHostMalloc(reinterpret_cast<void **>(&ws.h_status), sizeof(DevStatus), hipHostMallocDefault));
-    memset(ws.h_status, 0, sizeof(DevStatus));
-    return FT_OK;
+    return sp_grow(ws, 16);
 }
 
 void ws_free(Workspace &ws) {
@@ -1666,18 +1689,18 @@ void ws_free(Workspace &ws) {
         ws = Workspace();
         return;
     }
-    void *bufs[] = {ws.d_ops, ws.d_op_path, ws.d_op_order, ws.d_paths, ws.d_band_path, ws.d_op_cnt, ws.d_op_off,
-                    ws.d_sub_off,
-                    ws.d_leaf, ws.d_op_pw,
-                    ws.d_op_lastpw, ws.d_ev_xy, ws.d_ev_f, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_ev_last,
+    // (the layout's upload pointers point into d_lay, the status and path
+    // states into d_sp)
+    void *bufs[] = {ws.d_lay, ws.d_op_cnt, ws.d_op_off, ws.d_sub_off, ws.d_leaf, ws.d_op_pw, ws.d_op_lastpw,
+                    ws.d_ev_xy, ws.d_ev_f, ws.d_ev_meta, ws.d_ev_kept, ws.d_ev_idx, ws.d_ev_last,
                     ws.d_st_flags, ws.d_st_off, ws.d_st_last, ws.d_spts, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
                     ws.d_sub_joined, ws.d_it_cnt, ws.d_it_off,
-                    ws.d_run_first, ws.d_run_close, ws.d_run_path, ws.d_rblock, ws.d_run_start, ws.d_run_end, ws.d_pts,
+                    ws.d_run_first, ws.d_run_close, ws.d_run_path, ws.d_run_start, ws.d_run_end, ws.d_pts,
                     ws.d_pt_run, ws.d_edges, ws.d_pt_band, ws.d_band_cnt, ws.d_band_off, ws.d_band_cur, ws.d_band_list,
-                    ws.d_scan_state, ws.d_pstate, ws.d_top_rep, ws.d_items, ws.d_stats, ws.d_status};
+                    ws.d_scan_state, ws.d_top_rep, ws.d_stats, ws.d_sp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    if (ws.h_status) (void)hipHostFree(ws.h_status);
+    if (ws.h_sp) (void)hipHostFree(ws.h_sp);
     if (ws.h_pin) (void)hipHostFree(ws.h_pin);
     if (ws.ev) {
         for (int i = 0; i < 3 * Workspace::TIMING_RING; ++i)
@@ -1747,7 +1770,8 @@ ft_status ws_upload_ops(Workspace &ws, const ft_path_op *ops, size_t n_ops) {
 // carve the status block first.
 static ft_status arena_rewind(Workspace &ws) {
     ws.arena_used = 0;
-    void **ptrs[] = {(void **)&ws.d_ops, (void **)&ws.d_op_path, (void **)&ws.d_op_order, (void **)&ws.d_paths,
+    void **ptrs[] = {(void **)&ws.d_lay, (void **)&ws.d_ops, (void **)&ws.d_op_path, (void **)&ws.d_op_order,
+                     (void **)&ws.d_paths,
                      (void **)&ws.d_band_path, (void **)&ws.d_op_cnt, (void **)&ws.d_op_off, (void **)&ws.d_sub_off,
                      (void **)&ws.d_leaf, (void **)&ws.d_op_pw, (void **)&ws.d_op_lastpw, (void **)&ws.d_ev_xy,
                      (void **)&ws.d_ev_f, (void **)&ws.d_ev_meta, (void **)&ws.d_ev_kept, (void **)&ws.d_ev_idx,
@@ -1761,8 +1785,8 @@ static ft_status arena_rewind(Workspace &ws) {
                      (void **)&ws.d_scan_state, (void **)&ws.d_pstate, (void **)&ws.d_top_rep,
                      (void **)&ws.d_items, (void **)&ws.d_stats, (void **)&ws.d_status};
     for (void **p : ptrs) *p = nullptr;
-    ws.cap_ops = ws.cap_paths = ws.cap_bands = ws.cap_rblocks = ws.cap_events = ws.cap_stroke = 0;
-    ws.cap_band_slots = ws.cap_pairs = ws.cap_tiles = ws.cap_items = 0;
+    ws.cap_ops = ws.cap_paths = ws.cap_events = ws.cap_stroke = ws.cap_lay = 0;
+    ws.cap_band_slots = ws.cap_pairs = ws.cap_tiles = 0;
     return arena_carve(ws, ws.d_status, 1);
 }
 
@@ -1792,6 +1816,8 @@ static ft_status arena_split(Workspace &ws) {
     ws.cap_pairs = rest / 4;
     return FT_OK;
 }
+
+constexpr int LAY_SECTIONS = 7;  // ops, op -> path, op order, paths, band -> path, raster blocks, tile items
 
 // One-pass flattening (k_flatten): one item per thread, every block resident.
 static inline bool flatten_in_one_pass(const Workspace &ws, size_t n_ops) {
@@ -1927,34 +1953,36 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
     // the tuning asks for more LDS than a block has
     if (raster_lds_bytes(FMT_MATTE8, lds) > ws.lds_max || raster_lds_bytes(FMT_RGBA8P, lds) > ws.lds_max)
         return FT_ENOTSUP;
-    ft_status s;
-    const size_t old_ops = ws.cap_ops;
-    if ((s = dev_grow(ws, ws.d_ops, ws.cap_ops, n_ops))) return s;
-    if (ws.cap_ops != old_ops || !ws.d_op_cnt) {  // per-op scratch follows the op capacity
-        if ((s = dev_alloc(ws, ws.d_op_path, ws.cap_ops)) || (s = dev_alloc(ws, ws.d_op_order, ws.cap_ops)) ||
-            (s = dev_alloc(ws, ws.d_op_cnt, ws.cap_ops)) ||
+    ft_status s = FT_OK;
+    if (n_ops > ws.cap_ops || !ws.d_op_cnt) {  // per-op scratch
+        ws.cap_ops = n_ops ? n_ops : 1;
+        if ((s = dev_alloc(ws, ws.d_op_cnt, ws.cap_ops)) ||
             (s = dev_alloc(ws, ws.d_op_off, ws.cap_ops)) || (s = dev_alloc(ws, ws.d_op_pw, ws.cap_ops)) ||
             (s = dev_alloc(ws, ws.d_sub_off, ws.cap_ops * OP_LANES)) ||
             (s = dev_alloc(ws, ws.d_leaf, ws.cap_ops * OP_LANES * LEAF_CACHE)) ||
-            (s = dev_alloc(ws, ws.d_op_lastpw, ws.cap_ops)))
+            (s = dev_alloc(ws, ws.d_op_lastpw, ws.cap_ops))) {
+            ws.cap_ops = 0;
             return s;
+        }
     }
-    const size_t old_paths = ws.cap_paths;
-    if ((s = dev_grow(ws, ws.d_paths, ws.cap_paths, n_paths))) return s;
-    if (ws.cap_paths != old_paths || !ws.d_pstate)
-        if ((s = dev_alloc(ws, ws.d_pstate, ws.cap_paths)) || (s = dev_alloc(ws, ws.d_top_rep, ws.cap_paths * TOP_REPS)))
+    if (n_paths > ws.cap_paths || !ws.d_pstate) {  // per-path results
+        const size_t cap = n_paths ? n_paths : 1;
+        if (ws.arena)
+            s = dev_alloc(ws, ws.d_pstate, cap);
+        else if (cap > ws.cap_sp_paths)
+            s = sp_grow(ws, cap);
+        if (s || (s = dev_alloc(ws, ws.d_top_rep, cap * TOP_REPS))) {
+            ws.cap_paths = 0;
             return s;
+        }
+        ws.cap_paths = cap;
+    }
     size_t nb_total = band_path.size();
     // replicas per band counter: enough that the slots spread over about
     // BAND_SLOTS_MIN counters, at most 2^BAND_SHIFT_MAX
     uint32_t shift = 0;
     while (shift < BAND_SHIFT_MAX && (nb_total << shift) < BAND_SLOTS_MIN) ++shift;
     if ((nb_total << shift) >= 0x7FFFFFFFu) return FT_ENOTSUP;
-    if (nb_total > ws.cap_bands || !ws.d_band_path) {
-        size_t cap = nb_total ? nb_total : 1;
-        if ((s = dev_alloc(ws, ws.d_band_path, cap))) return s;
-        ws.cap_bands = cap;
-    }
     const size_t n_slots = nb_total << shift;
     if (n_slots > ws.cap_band_slots || !ws.d_band_cnt) {
         size_t cap = n_slots ? n_slots : 1;
@@ -1964,15 +1992,41 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
         ws.cap_band_slots = cap;
     }
     ws.band_shift = shift;
-    if ((s = dev_grow(ws, ws.d_rblock, ws.cap_rblocks, rblock.size()))) return s;
     ws.n_rblocks = rblock.size();
-    if ((s = dev_grow(ws, ws.d_items, ws.cap_items, ws.h_items.size()))) return s;
+    // the upload block: its sections at 256-byte aligned offsets (the
+    // pinned staging below uses the same offsets)
+    const size_t sec_bytes[LAY_SECTIONS] = {n_ops * sizeof(ft_path_op), n_ops * 4, n_ops * 4,
+                                            n_paths * sizeof(PathDev), nb_total * 4, rblock.size() * sizeof(uint4),
+                                            ws.h_items.size() * sizeof(uint2)};
+    size_t sec_off[LAY_SECTIONS], lay_bytes = 0;
+    for (int k = 0; k < LAY_SECTIONS; ++k) {
+        sec_off[k] = lay_bytes;
+        lay_bytes += (sec_bytes[k] + 255) / 256 * 256;
+    }
+    if (lay_bytes > ws.cap_lay || !ws.d_lay) {
+        const size_t cap = ws.arena ? lay_bytes : lay_bytes + lay_bytes / 4 + 256;
+        if ((s = dev_alloc(ws, ws.d_lay, cap))) {
+            ws.cap_lay = 0;
+            return s;
+        }
+        ws.cap_lay = cap;
+    }
+    ws.d_ops = reinterpret_cast<ft_path_op *>(ws.d_lay + sec_off[0]);
+    ws.d_op_path = reinterpret_cast<uint32_t *>(ws.d_lay + sec_off[1]);
+    ws.d_op_order = reinterpret_cast<uint32_t *>(ws.d_lay + sec_off[2]);
+    ws.d_paths = reinterpret_cast<PathDev *>(ws.d_lay + sec_off[3]);
+    ws.d_band_path = reinterpret_cast<uint32_t *>(ws.d_lay + sec_off[4]);
+    ws.d_rblock = reinterpret_cast<uint4 *>(ws.d_lay + sec_off[5]);
+    ws.d_items = reinterpret_cast<uint2 *>(ws.d_lay + sec_off[6]);
     for (int f = 0; f < 3; ++f) ws.tile_grid[f] = ws.it_count[f] ? tile_grid_for(ws, f, lds) : 0u;
     ws.n_ops = n_ops;
     ws.n_paths = n_paths;
     ws.n_bands = nb_total;
     ws.lds_cells = lds;
-    ws.h_pstate.resize(n_paths);
+    if (ws.arena) {
+        ws.h_pstate_store.resize(n_paths);
+        ws.h_pstate = ws.h_pstate_store.data();
+    }
     if (ws.arena) {
         if ((s = arena_split(ws))) return s;
     } else {
@@ -2002,46 +2056,34 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
                 if (cls == pass) ws.h_op_order[at++] = (uint32_t)i;
             }
     }
-    // The layout's uploads, staged through one pinned host buffer so that each
-    // copy is a plain DMA (copies from pageable memory go through the
-    // runtime's own staging, one host round trip each).  Arena workspaces,
-    // which may not allocate, copy from the sources directly.
-    struct Upload {
-        void *dst;
-        const void *src;
-        size_t bytes;
-    };
-    const Upload up[] = {
-        {ws.d_ops, ops, n_ops * sizeof(ft_path_op)},
-        {ws.d_op_path, op_path.data(), n_ops * 4},
-        {ws.d_op_order, ws.h_op_order.data(), ws.op_ordered ? n_ops * 4 : 0},
-        {ws.d_paths, pd.data(), n_paths * sizeof(PathDev)},
-        {ws.d_band_path, band_path.data(), nb_total * 4},
-        {ws.d_rblock, rblock.data(), rblock.size() * sizeof(uint4)},
-        {ws.d_items, ws.h_items.data(), ws.h_items.size() * sizeof(uint2)},
-    };
-    size_t staged = 0;
-    for (const Upload &u : up) staged += (u.bytes + 255) / 256 * 256;
-    if (!ws.arena && staged > ws.cap_pin) {
-        if (ws.h_pin) (void)hipHostFree(ws.h_pin);
-        ws.h_pin = nullptr;
-        ws.cap_pin = 0;
-        if (hipHostMalloc(reinterpret_cast<void **>(&ws.h_pin), staged, hipHostMallocDefault) != hipSuccess) {
+    // The layout's uploads: staged into one pinned host buffer at the upload
+    // block's offsets and sent with one copy (a copy from pageable memory goes
+    // through the runtime's own staging, a host round trip each).  Arena
+    // workspaces, which may not allocate, copy each section from its source.
+    const void *sec_src[LAY_SECTIONS] = {ops, op_path.data(), ws.h_op_order.data(), pd.data(),
+                                         band_path.data(), rblock.data(), ws.h_items.data()};
+    size_t sec_len[LAY_SECTIONS];
+    for (int k = 0; k < LAY_SECTIONS; ++k) sec_len[k] = sec_bytes[k];
+    if (!ws.op_ordered) sec_len[2] = 0;
+    if (ws.arena) {
+        for (int k = 0; k < LAY_SECTIONS; ++k)
+            if (sec_len[k])
+                FT_HIP(hipMemcpyAsync(ws.d_lay + sec_off[k], sec_src[k], sec_len[k], hipMemcpyHostToDevice,
+                                      ws.stream));
+    } else if (lay_bytes) {
+        if (lay_bytes > ws.cap_pin) {
+            if (ws.h_pin) (void)hipHostFree(ws.h_pin);
             ws.h_pin = nullptr;
-            return FT_ENOMEM;
+            ws.cap_pin = 0;
+            if (hipHostMalloc(reinterpret_cast<void **>(&ws.h_pin), lay_bytes, hipHostMallocDefault) != hipSuccess) {
+                ws.h_pin = nullptr;
+                return FT_ENOMEM;
+            }
+            ws.cap_pin = lay_bytes;
         }
-        ws.cap_pin = staged;
-    }
-    size_t at = 0;
-    for (const Upload &u : up) {
-        if (!u.bytes) continue;
-        const void *src = u.src;
-        if (!ws.arena) {
-            memcpy(ws.h_pin + at, u.src, u.bytes);
-            src = ws.h_pin + at;
-            at += (u.bytes + 255) / 256 * 256;
-        }
-        FT_HIP(hipMemcpyAsync(u.dst, src, u.bytes, hipMemcpyHostToDevice, ws.stream));
+        for (int k = 0; k < LAY_SECTIONS; ++k)
+            if (sec_len[k]) memcpy(ws.h_pin + sec_off[k], sec_src[k], sec_len[k]);
+        FT_HIP(hipMemcpyAsync(ws.d_lay, ws.h_pin, lay_bytes, hipMemcpyHostToDevice, ws.stream));
     }
     ws.uploads_pending = true;
     return FT_OK;
@@ -2053,13 +2095,14 @@ static inline uint32_t grid_for(size_t n) {
     return (uint32_t)(g > GRID_MAX ? GRID_MAX : g);
 }
 
-static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev);
+static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool with_pstate);
 
 // Enqueue one run.  In graph mode a fill run is captured once into a HIP
 // graph (the launch sequence has no memset nodes and no host waits) and then
 // replayed; any change of buffers or layout drops the graph.
-ft_status ws_launch(Workspace &ws, uint32_t mode) {
+ft_status ws_launch(Workspace &ws, uint32_t mode, bool read_pstate) {
     ws.edges_counted = false;
+    ws.pstate_copied = false;
     hipEvent_t *ev = nullptr;
     if (ws.timing && mode == MODE_FILL && ws.n_paths && ws.n_pending < Workspace::TIMING_RING) {
         if (!ws.ev) {
@@ -2068,11 +2111,18 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
         }
         ev = ws.ev + 3 * ws.n_pending++;
     }
-    if (!(ws.graph && mode == MODE_FILL)) return ws_enqueue(ws, mode, ev);
+    if (!(ws.graph && mode == MODE_FILL)) {
+        // an eager run brings the path states back with its status when they
+        // share the status block (not inside a graph: every replay would copy)
+        const bool with_ps = read_pstate && ws.d_sp && ws.n_paths;
+        const ft_status st = ws_enqueue(ws, mode, ev, with_ps);
+        ws.pstate_copied = with_ps && st == FT_OK;
+        return st;
+    }
     if (!ws.gexec) {
         hipGraph_t g = nullptr;
         FT_HIP(hipStreamBeginCapture(ws.stream, hipStreamCaptureModeThreadLocal));
-        const ft_status st = ws_enqueue(ws, mode, nullptr);
+        const ft_status st = ws_enqueue(ws, mode, nullptr, false);
         const hipError_t e = hipStreamEndCapture(ws.stream, &g);
         if (st) {
             if (g) (void)hipGraphDestroy(g);
@@ -2095,7 +2145,7 @@ ft_status ws_launch(Workspace &ws, uint32_t mode) {
     return FT_OK;
 }
 
-static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
+static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool with_pstate) {
     if (mode == MODE_STROKE) {
         const ft_status e = ensure_stroke_capacity(ws);
         if (e) return e;
@@ -2300,14 +2350,16 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev) {
         }
     }
     FT_HIP(hipGetLastError());
-    FT_HIP(hipMemcpyAsync(ws.h_status, ws.d_status, sizeof(DevStatus), hipMemcpyDeviceToHost, s));
+    const size_t back = with_pstate ? STATUS_BYTES + ws.n_paths * sizeof(PathState) : sizeof(DevStatus);
+    FT_HIP(hipMemcpyAsync(ws.h_status, ws.d_status, back, hipMemcpyDeviceToHost, s));
     return FT_OK;
 }
 
 ft_status ws_finish(Workspace &ws, bool read_pstate) {
-    if (read_pstate && ws.n_paths)
-        FT_HIP(hipMemcpyAsync(ws.h_pstate.data(), ws.d_pstate, ws.n_paths * sizeof(PathState),
-                              hipMemcpyDeviceToHost, ws.stream));
+    if (read_pstate && ws.n_paths && !ws.pstate_copied)
+        FT_HIP(hipMemcpyAsync(ws.h_pstate, ws.d_pstate, ws.n_paths * sizeof(PathState), hipMemcpyDeviceToHost,
+                              ws.stream));
+    ws.pstate_copied = false;
     FT_HIP(hipStreamSynchronize(ws.stream));
     ws.uploads_pending = false;
     for (int k = 0; k < ws.n_pending; ++k) {
@@ -2336,8 +2388,8 @@ ft_status ws_count_edges(Workspace &ws) {
         k_edge_count<<<grid_for(ws.cap_events), BLOCK, 0, ws.stream>>>(L0, ws.d_edges, ws.d_pt_run, ws.d_run_path,
                                                                         ws.d_pstate, ws.d_status);
         FT_HIP(hipGetLastError());
-        FT_HIP(hipMemcpyAsync(ws.h_pstate.data(), ws.d_pstate, ws.n_paths * sizeof(PathState),
-                              hipMemcpyDeviceToHost, ws.stream));
+        FT_HIP(hipMemcpyAsync(ws.h_pstate, ws.d_pstate, ws.n_paths * sizeof(PathState), hipMemcpyDeviceToHost,
+                              ws.stream));
         FT_HIP(hipStreamSynchronize(ws.stream));
         ws.edges_counted = true;
     }
@@ -2372,7 +2424,7 @@ ft_status ws_raster_stats(Workspace &ws, uint64_t *pairs, uint64_t *adds) {
 
 ft_status ws_run(Workspace &ws, uint32_t mode, bool read_pstate) {
     for (int attempt = 0; attempt < 4; ++attempt) {
-        ft_status s = ws_launch(ws, mode);
+        ft_status s = ws_launch(ws, mode, read_pstate);
         if (s) return s;
         s = ws_finish(ws, read_pstate);
         if (s != (ft_status)0x100) return s;
