@@ -81,7 +81,17 @@ ft_status plotter_fill_ops(ft_plotter *p, ft_fill_rule rule, const ft_path_op *o
                            const uint8_t clr[4]) {
     if (n > 0x7FFFFFF0u) return FT_EINVAL;
     ft_fill_desc d = plotter_desc(p, rule, n, clr);
-    ft_status s = ft::ws_set_batch(p->ws, ops, n, &d, 1);
+    // small figures: two launches (the full pipeline otherwise)
+    bool handled = false;
+    uint64_t offered = 0, stored = 0;
+    ft_status s = ft::ws_small_fill(p->ws, ops, n, d, &handled, &offered, &stored);
+    if (s) return s;
+    if (handled) {
+        p->last_offered = offered;
+        p->last_stored = stored;
+        return FT_OK;
+    }
+    s = ft::ws_set_batch(p->ws, ops, n, &d, 1);
     if (s) return s;
     s = ft::ws_run(p->ws, ft::MODE_FILL, true);
     if (s) return s;

@@ -1558,6 +1558,298 @@ __global__ __launch_bounds__(BLOCK) void k_tiles(Limits L0, const PathDev *paths
 }
 
 // ---------------------------------------------------------------------------
+// Small figures: the Plotter's fast path.
+// ---------------------------------------------------------------------------
+// A fill of at most SMALL_OPS ops whose events fit SMALL_EVENTS runs every
+// stage before the raster in one block with its data in LDS: the (op, lane)
+// counts and their block scan, the emit, the Fig ingest scans (pushed marks,
+// kept points), compaction with the top point, Edge::new per point and the
+// winding direction.  It writes the edge list and the path's state; the
+// raster then runs as one block per LDS row group (k_small_raster).  Two
+// launches, one upload and one readback instead of the pipeline's dozen
+// launches.  Anything the block does not cover (more events or sub-figures,
+// a flattening error, an index check) makes it write nothing and report "not
+// handled" (the raster blocks then do nothing); the caller runs the full
+// pipeline, which gives the same result or the same error.
+constexpr int SBLOCK = 1024;                     // threads of k_small_front
+constexpr uint32_t SMALL_OPS = 256;
+constexpr uint32_t SMALL_ARG_OPS = 128;          // ops passed inside the launch itself (no upload)
+constexpr uint32_t SMALL_ITEMS = SMALL_OPS * OP_LANES;
+constexpr uint32_t SMALL_EVENTS = 2048;
+constexpr uint32_t SMALL_RUNS = SMALL_OPS + 1;    // every op closes at most one sub-figure, plus the last
+constexpr uint32_t SMALL_PER_THREAD = SMALL_EVENTS / SBLOCK;
+constexpr uint32_t SMALL_NOT_HANDLED = 0x10u;     // DevStatus::overflow bit
+constexpr uint32_t SMALL_RASTER_BLOCKS = 64;      // raster blocks a small figure aims for
+static_assert(SMALL_ITEMS == SMALL_EVENTS, "op lanes and events share the scan shape");
+
+struct SmallLds {
+    union {
+        struct {  // events: flatten to compaction
+            I2 xy[SMALL_EVENTS];
+            uint32_t meta[SMALL_EVENTS], last[SMALL_EVENTS], kept[SMALL_EVENTS], idx[SMALL_EVENTS];
+        } ev;
+        EdgeRec edges[SMALL_EVENTS];  // after compaction
+    } a;
+    union {
+        uint32_t cnt_e[SMALL_ITEMS];   // (op, lane) event counts, then their exclusive scan
+        I2 pts[SMALL_EVENTS];          // after the emit
+    } b;
+    union {
+        uint32_t cnt_c[SMALL_ITEMS];   // (op, lane) close counts, then their exclusive scan
+        uint32_t pt_run[SMALL_EVENTS];
+    } c;
+    int32_t run_first[SMALL_RUNS];
+    uint32_t run_close[SMALL_RUNS], run_path[SMALL_RUNS], run_start[SMALL_RUNS], run_end[SMALL_RUNS];
+    uint32_t op_path[SMALL_OPS];
+    ft_path_op ops[SMALL_OPS];
+    PathDev path;
+    PathState ps;
+    uint32_t w[SBLOCK / 64], w2[SBLOCK / 64];
+    unsigned long long wk[SBLOCK / 64];
+    uint32_t flag;
+};
+
+// Block-wide exclusive scan (sum, or inclusive max when MAX) of a[0, n) in
+// place, n <= SBLOCK * SMALL_PER_THREAD, each thread taking consecutive
+// items; returns the total (or the maximum).  Every thread calls it.
+template <bool MAX>
+__device__ uint32_t small_scan(uint32_t *a, uint32_t n, uint32_t *s_w) {
+    const uint32_t t0 = threadIdx.x * SMALL_PER_THREAD;
+    uint32_t v[SMALL_PER_THREAD], run = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < SMALL_PER_THREAD; ++j) {
+        v[j] = t0 + j < n ? a[t0 + j] : 0u;
+        run = MAX ? (v[j] > run ? v[j] : run) : run + v[j];
+    }
+    const uint32_t incl = wave_incl_dpp<MAX>(run);
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) s_w[wid] = incl;
+    __syncthreads();
+    uint32_t pre = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < SBLOCK / 64; ++w) {
+        const uint32_t x = s_w[w];
+        if (MAX) {
+            pre = (w < wid && x > pre) ? x : pre;
+            total = x > total ? x : total;
+        } else {
+            pre += w < wid ? x : 0u;
+            total += x;
+        }
+    }
+    // the thread's own prefix: the block's part, then the wave's before it
+    uint32_t acc;
+    if (MAX) {
+        const uint32_t prev = (uint32_t)__shfl_up((int)incl, 1, 64);
+        acc = (threadIdx.x & 63) ? (prev > pre ? prev : pre) : pre;
+    } else {
+        acc = pre + incl - run;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < SMALL_PER_THREAD; ++j) {
+        if (MAX) {
+            acc = v[j] > acc ? v[j] : acc;
+            if (t0 + j < n) a[t0 + j] = acc;  // inclusive max
+        } else {
+            if (t0 + j < n) a[t0 + j] = acc;  // exclusive sum
+            acc += v[j];
+        }
+    }
+    __syncthreads();
+    return total;
+}
+
+// Ops of a small figure carried inside the launch (n_ops <= SMALL_ARG_OPS).
+struct SmallOps {
+    ft_path_op op[SMALL_ARG_OPS];
+};
+
+// ops: the figure's ops in device memory, or null when `inl` carries them.
+__global__ __launch_bounds__(SBLOCK) void k_small_front(const ft_path_op *ops, SmallOps inl, uint32_t n_ops,
+                                                        PathDev P, EdgeRec *edges, PathState *ps, DevStatus *st) {
+    __shared__ SmallLds S;
+    const uint32_t tid = threadIdx.x;
+    if (tid < sizeof(DevStatus) / 4) reinterpret_cast<uint32_t *>(st)[tid] = 0;
+    if (tid == 0) {
+        S.path = P;
+        S.flag = 0;
+    }
+    const uint32_t n_in = n_ops < SMALL_OPS ? n_ops : SMALL_OPS;
+    for (uint32_t i = tid; i < SMALL_OPS; i += SBLOCK) S.op_path[i] = 0;
+    for (uint32_t i = tid; i < n_in; i += SBLOCK) S.ops[i] = ops ? ops[i] : inl.op[i < SMALL_ARG_OPS ? i : 0];
+    __syncthreads();
+    // 1. (op, lane) counts
+    const uint32_t n_items = n_ops * OP_LANES;
+    uint32_t err = 0;
+    for (uint32_t t = tid; t < SMALL_ITEMS; t += SBLOCK) {
+        OpCount c = {0, 0, 0, 0, 0};
+        if (t < n_items) c = count_op(S.ops, S.op_path, &S.path, t >> FLAT_SPLIT, t & (OP_LANES - 1), MODE_FILL, nullptr);
+        err |= c.err;
+        S.b.cnt_e[t] = c.ne;
+        S.c.cnt_c[t] = c.nc;
+    }
+    if (err) S.flag = 1;
+    __syncthreads();
+    const uint32_t n_events = small_scan<false>(S.b.cnt_e, n_items, S.w);
+    const uint32_t n_runs = small_scan<false>(S.c.cnt_c, n_items, S.w2);
+    if (S.flag | (n_events > SMALL_EVENTS) | (n_runs > SMALL_RUNS)) {
+        if (tid == 0) atomicOr(&st->overflow, SMALL_NOT_HANDLED);
+        return;
+    }
+    // 2. emit the events into LDS
+    EventSink em{};
+    em.mode = MODE_FILL;
+    em.cap_events = SMALL_EVENTS;
+    em.cap_runs = SMALL_RUNS;
+    em.ev_xy = S.a.ev.xy;
+    em.ev_meta = S.a.ev.meta;
+    em.run_first = S.run_first;
+    em.run_close = S.run_close;
+    em.run_path = S.run_path;
+    em.run_start = S.run_start;
+    em.run_end = S.run_end;
+    uint32_t bad = 0;
+    for (uint32_t t = tid; t < n_items; t += SBLOCK) {
+        const unsigned long long off = ((unsigned long long)S.b.cnt_e[t] << 32) | S.c.cnt_c[t];
+        bad |= emit_op(S.ops, S.op_path, &S.path, t >> FLAT_SPLIT, t & (OP_LANES - 1), MODE_FILL, nullptr, off, em);
+    }
+    __syncthreads();
+    Limits L{};
+    L.n_ops = n_ops;
+    L.n_paths = 1;
+    L.n_bands = 0;
+    L.n_events = n_events;
+    L.n_runs = n_runs;
+    L.cap_points = SMALL_EVENTS;
+    L.n_pairs = 0;
+    // 3. Fig ingest: pushed marks and their running maximum, then kept points
+    for (uint32_t e = tid; e < n_events; e += SBLOCK)
+        ingest_event(e, L, S.a.ev.meta, S.a.ev.xy, S.run_first, S.run_path, S.a.ev.last, bad);
+    __syncthreads();
+    (void)small_scan<true>(S.a.ev.last, n_events, S.w);
+    for (uint32_t e = tid; e < n_events; e += SBLOCK) {
+        const uint32_t k = kept_event(e, L, S.a.ev.meta, S.a.ev.xy, S.run_first, S.run_close, S.a.ev.last, bad);
+        S.a.ev.kept[e] = k;
+        S.a.ev.idx[e] = k;
+    }
+    __syncthreads();
+    const uint32_t n_kept = small_scan<false>(S.a.ev.idx, n_events, S.w);
+    // 4. the path's counts (no cap below 65,535 points) and compaction with
+    // the top (y, x) key
+    if (tid == 0) {
+        PathState ps;
+        ps.topkey = NO_KEY;
+        ps.topidx = NO_IDX;
+        ps.offered = n_events - n_runs;
+        ps.repop = 0;
+        ps.pt_base = 0;
+        ps.n_edges = 0;
+        ps.stored = n_kept;
+        ps.fwd = 0;
+        S.ps = ps;
+    }
+    __syncthreads();
+    unsigned long long kmin = NO_KEY;
+    for (uint32_t e = tid; e < n_events; e += SBLOCK) {
+        TopCand tc;
+        compact_event(e, L, S.a.ev.meta, S.a.ev.xy, S.a.ev.kept, S.a.ev.idx, S.run_first, S.run_path, S.run_start,
+                      S.run_end, S.b.pts, S.c.pt_run, &S.ps, bad, &tc);
+        if (tc.path != NO_IDX && tc.key < kmin) kmin = tc.key;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long u = __shfl_xor(kmin, o, 64);
+        kmin = u < kmin ? u : kmin;
+    }
+    if ((tid & 63) == 0) S.wk[tid >> 6] = kmin;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long m = S.wk[0];
+        for (int w = 1; w < SBLOCK / 64; ++w) m = S.wk[w] < m ? S.wk[w] : m;
+        S.ps.topkey = m;
+    }
+    __syncthreads();
+    // 5. Edge::new per stored point (the events' LDS now holds the edges)
+    EdgeRec E[SMALL_PER_THREAD];
+    uint32_t tmin = NO_IDX;
+#pragma unroll
+    for (uint32_t j = 0; j < SMALL_PER_THREAD; ++j) {
+        const uint32_t i = tid + j * SBLOCK;
+        E[j].flags = 0;
+        if (i < n_kept) {
+            const EdgeEval r = edge_eval(i, L, S.c.pt_run, S.run_path, S.run_start, S.run_end, &S.path, S.b.pts,
+                                         &S.ps, bad);
+            E[j] = r.E;
+            if (r.top && r.top_local < tmin) tmin = r.top_local;
+        }
+    }
+    __syncthreads();  // every event read is done: the edges take their LDS
+#pragma unroll
+    for (uint32_t j = 0; j < SMALL_PER_THREAD; ++j) {
+        const uint32_t i = tid + j * SBLOCK;
+        if (i < n_kept) S.a.edges[i] = E[j];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t u = (uint32_t)__shfl_xor((int)tmin, o, 64);
+        tmin = u < tmin ? u : tmin;
+    }
+    if ((tid & 63) == 0) S.w[tid >> 6] = tmin;
+    if (bad) S.flag = 1;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t m = S.w[0];
+        for (int w = 1; w < SBLOCK / 64; ++w) m = S.w[w] < m ? S.w[w] : m;
+        S.ps.topidx = m;
+        uint32_t b2 = 0;
+        const bool has = S.ps.topkey != NO_KEY && m != NO_IDX;
+        S.ps.fwd = has ? (path_forward(S.ps, L, S.b.pts, S.c.pt_run, S.run_start, S.run_end, b2) ? 1u : 0u) : 0u;
+        if (b2) S.flag = 1;
+    }
+    __syncthreads();
+    if (S.flag) {  // an index check failed: let the full pipeline run (and report)
+        if (tid == 0) atomicOr(&st->overflow, SMALL_NOT_HANDLED);
+        return;
+    }
+    // 6. the edge list and the path's state for the raster blocks
+    for (uint32_t i = tid; i < n_kept; i += SBLOCK) edges[i] = S.a.edges[i];
+    if (tid == 0) {
+        ps[0] = S.ps;
+        st->n_events = n_events;
+        st->n_closes = n_runs;
+        st->n_kept = n_kept;
+    }
+}
+
+// The raster of a small figure: block b rasters row group b (P.group_rows
+// rows) from the edge list k_small_front wrote.
+template <int FMT>
+__global__ __launch_bounds__(BLOCK) void k_small_raster(PathDev P, const EdgeRec *edges, const PathState *ps,
+                                                        DevStatus *st) {
+    extern __shared__ uint4 s_dyn[];
+    __shared__ uint32_t s_segsum[BLOCK / 64];
+    if (run_failed(st)) return;  // not handled, or an error: nothing is drawn
+    Limits L{};
+    L.n_ops = 0;
+    L.n_paths = 1;
+    L.n_bands = 0;
+    L.n_events = st->n_events;
+    L.n_runs = st->n_closes;
+    L.cap_points = SMALL_EVENTS;
+    L.n_pairs = 0;
+    const uint32_t n_kept = st->n_kept;
+    const PathState S = ps[0];
+    const uint32_t G = P.group_rows ? P.group_rows : 1u;
+    const uint32_t y0 = blockIdx.x * G;
+    const uint32_t y1 = y0 + G < P.height ? y0 + G : P.height;
+    const bool has_pts = S.topkey != NO_KEY && S.topidx != NO_IDX && n_kept <= SMALL_EVENTS;
+    uint32_t n_pairs = 0, n_adds = 0;
+    if (y0 < y1)
+        raster_group<false, FMT>(L, P, S, (int)S.fwd, y0, y1, PathEdgeSrc{0u}, has_pts ? n_kept : 0u, true, edges,
+                                 s_dyn, s_segsum, st, n_pairs, n_adds);
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 #define FT_HIP(call)                                   \
@@ -2419,6 +2711,86 @@ ft_status ws_raster_stats(Workspace &ws, uint64_t *pairs, uint64_t *adds) {
     }
     if (pairs) *pairs = a;
     if (adds) *adds = b;
+    return FT_OK;
+}
+
+ft_status ws_small_fill(Workspace &ws, const ft_path_op *ops, size_t n_ops, const ft_fill_desc &d, bool *handled,
+                        uint64_t *offered, uint64_t *stored) {
+    *handled = false;
+    if (ws.arena || !n_ops || n_ops > SMALL_OPS || d.fmt > FT_GRAYA8P || !d.width || !d.height ||
+        d.width > FT_MAX_WIDTH)
+        return FT_OK;
+    // the figure's raster geometry: row groups as in tile mode, no bands,
+    // short enough that the raster has SMALL_RASTER_BLOCKS blocks or more
+    PathDev P = path_from_desc(d, 0u, ws.tuning);
+    const uint32_t wp = cell_stride(d.width);
+    const uint32_t g_lds = std::max(1u, ws.tuning.lds_cells / wp);
+    const uint32_t g_par = std::max(1u, (d.height + SMALL_RASTER_BLOCKS - 1) / SMALL_RASTER_BLOCKS);
+    const uint32_t G = std::min(d.height, std::min(g_lds, g_par));
+    const size_t lds = raster_lds_bytes(d.fmt, (size_t)G * wp);
+    if (lds > ws.lds_max) return FT_OK;
+    P.flags |= PATH_TILE;
+    P.n_bands = 0;
+    P.band_base = 0;
+    P.band_rows = d.height;
+    P.group_rows = G;
+    const uint32_t groups = (d.height + G - 1) / G;
+    graph_drop(ws);
+    if (ws.uploads_pending) FT_HIP(hipStreamSynchronize(ws.stream));
+    ws.uploads_pending = false;
+    ft_status s;
+    if (!ws.d_edges || ws.cap_events < SMALL_EVENTS)
+        if ((s = ensure_event_capacity(ws, SMALL_EVENTS))) return s;
+    // the ops: inside the launch when they fit, else through the pinned
+    // staging into the upload block
+    SmallOps inl;  // the launch copies its arguments
+    const ft_path_op *d_ops = nullptr;
+    const size_t bytes = n_ops * sizeof(ft_path_op);
+    if (n_ops <= SMALL_ARG_OPS) {
+        memcpy(inl.op, ops, bytes);
+    } else {
+        if (bytes > ws.cap_lay || !ws.d_lay) {
+            if ((s = dev_alloc(ws, ws.d_lay, bytes + 256))) {
+                ws.cap_lay = 0;
+                return s;
+            }
+            ws.cap_lay = bytes + 256;
+        }
+        if (bytes > ws.cap_pin) {
+            if (ws.h_pin) (void)hipHostFree(ws.h_pin);
+            ws.h_pin = nullptr;
+            ws.cap_pin = 0;
+            if (hipHostMalloc(reinterpret_cast<void **>(&ws.h_pin), bytes, hipHostMallocDefault) != hipSuccess) {
+                ws.h_pin = nullptr;
+                return FT_ENOMEM;
+            }
+            ws.cap_pin = bytes;
+        }
+        memcpy(ws.h_pin, ops, bytes);
+        d_ops = reinterpret_cast<const ft_path_op *>(ws.d_lay);
+        FT_HIP(hipMemcpyAsync(ws.d_lay, ws.h_pin, bytes, hipMemcpyHostToDevice, ws.stream));
+    }
+    // the layout's pointers into the upload block are stale from here on: the
+    // next full-pipeline call lays the batch out again
+    ws.n_ops = ws.n_paths = ws.n_bands = 0;
+    k_small_front<<<1, SBLOCK, 0, ws.stream>>>(d_ops, inl, (uint32_t)n_ops, P, ws.d_edges, ws.d_pstate,
+                                               ws.d_status);
+    if (d.fmt == FT_MATTE8)
+        k_small_raster<FMT_MATTE8><<<groups, BLOCK, lds, ws.stream>>>(P, ws.d_edges, ws.d_pstate, ws.d_status);
+    else if (d.fmt == FT_RGBA8P)
+        k_small_raster<FMT_RGBA8P><<<groups, BLOCK, lds, ws.stream>>>(P, ws.d_edges, ws.d_pstate, ws.d_status);
+    else
+        k_small_raster<FMT_GRAYA8P><<<groups, BLOCK, lds, ws.stream>>>(P, ws.d_edges, ws.d_pstate, ws.d_status);
+    FT_HIP(hipGetLastError());
+    FT_HIP(hipMemcpyAsync(ws.h_status, ws.d_status, STATUS_BYTES + sizeof(PathState), hipMemcpyDeviceToHost,
+                          ws.stream));
+    FT_HIP(hipStreamSynchronize(ws.stream));
+    const DevStatus &h = *ws.h_status;
+    if (h.overflow & SMALL_NOT_HANDLED) return FT_OK;
+    if (h.err) return (ft_status)h.err;
+    *handled = true;
+    if (offered) *offered = ws.h_pstate[0].offered;
+    if (stored) *stored = ws.h_pstate[0].stored;
     return FT_OK;
 }
 

@@ -627,3 +627,31 @@ def test_atlas_full_size_every_tile(ft, oracle):
             assert (int(offered[t]), int(stored[t])) == counts, f"tile {t}"
             covered[y:y + s, x:x + s] = True
     assert np.all(img[~covered] == 77)
+
+
+def test_small_figure_path_and_its_limits(ft, oracle):
+    """Plotter fills of up to 256 ops whose events fit 2,048 run as small
+    figures (one block before the raster, one raster block per row group);
+    past either limit (257 ops; fine curves with more events) the full
+    pipeline runs, in the same plotter, between small fills.  Every case
+    against the oracle, in every format, with rasters taller than one LDS row
+    group."""
+    rng = np.random.default_rng(77)
+    cases = []
+    for n_sub, kinds in ((20, (LINE,)), (40, (LINE, QUAD, CUBIC))):
+        ops = random_path(rng, 200, 300, n_sub=n_sub, kinds=kinds)
+        cases.append(ops[:256] if len(ops) > 256 else ops)
+    cases.append(random_path(rng, 200, 300, n_sub=1, kinds=(QUAD, CUBIC)))  # curves within the event limit
+    many = random_path(rng, 200, 300, n_sub=80, kinds=(LINE,))
+    assert len(many) > 257
+    cases.append(many[:257])
+    fine = ops_array([(MOVE, (5.0, 5.0))] + [(CUBIC, (250.0 * (k % 2), 150.0, 5.0 + k, 290.0, 195.0, 5.0 + 2 * k))
+                                            for k in range(40)] + [(CLOSE, ())])
+    cases.append(fine)
+    for fmt in (MATTE8, RGBA8P, GRAYA8P):
+        clr = {MATTE8: (255,), RGBA8P: (90, 160, 30, 200), GRAYA8P: (70, 140)}[fmt]
+        init = rng.integers(0, 256, size=200 * 300 * BPP[fmt]).astype(np.uint8) if fmt != MATTE8 else None
+        calls = []
+        for k, ops in enumerate(cases):
+            calls.append(("fill", k % 2, ops, clr))
+        compare_fill(ft, oracle, 200, 300, fmt, calls, init=init, tol=0.05, tag=f"small path fmt {fmt}")
