@@ -1088,6 +1088,18 @@ __device__ __forceinline__ void chunk_out(const uint32_t *v, uint32_t base, uint
     }
     if (full) {
         uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        if constexpr (!FRESH) {
+            // Outside the figure over a transparent destination SrcOver gives
+            // 0 (S(0) = 0 and 0 * invw = 0): a wavefront whose every lane has
+            // such a chunk stores zeros without the table or the blend.
+            uint32_t any = q.x | q.y | q.z | q.w;
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) any |= cov_of<RULE>(base + v[j]);
+            if (__all(any == 0u)) {
+                *reinterpret_cast<gu32x4 *>(dst) = (u32x4){0u, 0u, 0u, 0u};
+                return;
+            }
+        }
         if constexpr (FMT == FMT_RGBA8P) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
