@@ -287,6 +287,12 @@ __global__ __launch_bounds__(BLOCK) void k_rows(DScene S, DevStatus *st) {
     }
 }
 
+// FILL_LANES threads per edge, each taking every FILL_LANES-th run of
+// FILL_ROWS rows, with the run's FILL_ROWS slot reservations (returning
+// atomics on the cells' cursors) in flight together: a tall edge does not
+// wait on one atomic per row in turn.
+constexpr uint32_t FILL_LANES = 4, FILL_ROWS = 8;
+
 __global__ __launch_bounds__(BLOCK) void k_fill_cells(DScene S, DevStatus *st) {
     if (run_failed(st)) return;
     if (st->n_pairs > S.cap_pairs) {
@@ -296,24 +302,38 @@ __global__ __launch_bounds__(BLOCK) void k_fill_cells(DScene S, DevStatus *st) {
         }
         return;
     }
-    for (uint32_t v = blockIdx.x * BLOCK + threadIdx.x; v < S.n_vtx; v += gridDim.x * BLOCK) {
+    const uint64_t n_items = (uint64_t)S.n_vtx * FILL_LANES;
+    for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_items; t += (uint64_t)gridDim.x * BLOCK) {
+        const uint32_t v = (uint32_t)(t / FILL_LANES), q = (uint32_t)(t % FILL_LANES);
         uint32_t k, r0, r1;
         float2 a, b;
         edge_at(S, v, k, a, b);
         if (!edge_row_range(a, b, S.H, r0, r1)) continue;
         const uint32_t local = v - S.vtx_off[k];
-        for (uint32_t r = r0; r <= r1; ++r) {
-            const uint64_t c = (uint64_t)k * S.H + r;
-            const uint32_t at = S.cell_off[c] + atomicAdd(&S.cell_cur[c], 1u);
-            if (at < S.cap_pairs)
-                S.cell_edge[at] = local;
-            else
-                raise_err(st, FT_EDEVICE);
+        const uint64_t c0 = (uint64_t)k * S.H;
+        for (uint32_t rb = r0 + q * FILL_ROWS; rb <= r1; rb += FILL_LANES * FILL_ROWS) {
+            uint32_t at[FILL_ROWS];
+#pragma unroll
+            for (uint32_t u = 0; u < FILL_ROWS; ++u)
+                at[u] = rb + u <= r1 ? S.cell_off[c0 + rb + u] + atomicAdd(&S.cell_cur[c0 + rb + u], 1u) : 0u;
+#pragma unroll
+            for (uint32_t u = 0; u < FILL_ROWS; ++u) {
+                if (rb + u > r1) break;
+                if (at[u] < S.cap_pairs)
+                    S.cell_edge[at[u]] = local;
+                else
+                    raise_err(st, FT_EDEVICE);
+            }
         }
     }
 }
 
-// Each cell's edges in ascending order, so that sums run in edge order.
+// Each cell's edges in ascending order, so that sums run in edge order.  A
+// cell of at most SORT_REGS edges is loaded into registers (all loads in
+// flight), sorted by an odd-even transposition network and stored back;
+// longer cells are sorted in place.
+constexpr uint32_t SORT_REGS = 16;
+
 __global__ __launch_bounds__(BLOCK) void k_sort_cells(DScene S, DevStatus *st) {
     if (run_failed(st)) return;
     for (uint32_t c = blockIdx.x * BLOCK + threadIdx.x; c < S.n_cells; c += gridDim.x * BLOCK) {
@@ -323,6 +343,25 @@ __global__ __launch_bounds__(BLOCK) void k_sort_cells(DScene S, DevStatus *st) {
             continue;
         }
         uint32_t *e = S.cell_edge + off;
+        if (n <= SORT_REGS) {
+            uint32_t x[SORT_REGS];
+#pragma unroll
+            for (uint32_t i = 0; i < SORT_REGS; ++i) x[i] = i < n ? e[i] : 0xFFFFFFFFu;
+#pragma unroll
+            for (uint32_t pass = 0; pass < SORT_REGS; ++pass) {
+#pragma unroll
+                for (uint32_t i = pass & 1u; i + 1 < SORT_REGS; i += 2) {
+                    const uint32_t lo = x[i] < x[i + 1] ? x[i] : x[i + 1];
+                    const uint32_t hi = x[i] < x[i + 1] ? x[i + 1] : x[i];
+                    x[i] = lo;
+                    x[i + 1] = hi;
+                }
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < SORT_REGS; ++i)
+                if (i < n) e[i] = x[i];
+            continue;
+        }
         for (uint32_t i = 1; i < n; ++i) {
             const uint32_t x = e[i];
             uint32_t j = i;
@@ -835,7 +874,7 @@ ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, flo
                                  nullptr, d->d_status, false, true);
     scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_off, nullptr, nc, nc,
                                  d->d_lb + 2 * d->cap_tiles, &d->d_status->n_pairs, d->d_status, false, false);
-    k_fill_cells<<<grid_of(d->n_vtx), BLOCK, 0, s>>>(S, d->d_status);
+    k_fill_cells<<<grid_of((size_t)d->n_vtx * FILL_LANES), BLOCK, 0, s>>>(S, d->d_status);
     k_sort_cells<<<grid_of(d->n_cells), BLOCK, 0, s>>>(S, d->d_status);
     FD_HIP(hipGetLastError());
     return FT_OK;
