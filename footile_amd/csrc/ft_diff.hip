@@ -287,6 +287,94 @@ __global__ __launch_bounds__(BLOCK) void k_rows(DScene S, DevStatus *st) {
     }
 }
 
+// Edge lists per (path, row) in edge order, one block per path, without
+// atomics or a sort: the block stages its edges' row ranges in LDS; each
+// wavefront takes every fourth row of the path's span and walks the edges in
+// chunks of 64, giving each crossing edge its slot by a ballot count of the
+// crossing edges before it.  Paths of more than CELL_EDGES edges take the
+// atomic fill and the sort below instead.
+constexpr uint32_t CELL_EDGES = 2048;
+
+__global__ __launch_bounds__(BLOCK) void k_cells(DScene S, DevStatus *st) {
+    __shared__ uint2 s_rows[CELL_EDGES];
+    __shared__ uint32_t s_lo[BLOCK / 64], s_hi[BLOCK / 64];
+    if (run_failed(st)) return;
+    if (st->n_pairs > S.cap_pairs) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            atomicOr(&st->overflow, 2u);
+            st->need_pairs = st->n_pairs;
+        }
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (uint32_t k = blockIdx.x; k < S.n_paths; k += gridDim.x) {
+        const uint32_t vb = S.vtx_off[k], V = S.vtx_off[k + 1] - vb;
+        if (V > CELL_EDGES) continue;  // uniform: left to the atomic fill
+        uint32_t lo = 0xFFFFFFFFu, hi = 0;
+        for (uint32_t e = threadIdx.x; e < V; e += BLOCK) {
+            uint32_t r0 = 1, r1 = 0;
+            const float2 a = S.vtx[vb + e], b = S.vtx[vb + (e + 1 < V ? e + 1 : 0u)];
+            if (!edge_row_range(a, b, S.H, r0, r1)) {
+                r0 = 1;
+                r1 = 0;
+            }
+            s_rows[e] = make_uint2(r0, r1);
+            if (r0 <= r1) {
+                lo = r0 < lo ? r0 : lo;
+                hi = r1 > hi ? r1 : hi;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
+            lo = l2 < lo ? l2 : lo;
+            hi = h2 > hi ? h2 : hi;
+        }
+        if (lane == 0) {
+            s_lo[wid] = lo;
+            s_hi[wid] = hi;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w) {
+            lo = s_lo[w] < lo ? s_lo[w] : lo;
+            hi = s_hi[w] > hi ? s_hi[w] : hi;
+        }
+        const uint64_t c0 = (uint64_t)k * S.H;
+        uint32_t bad = 0;
+        for (uint32_t rb = lo + wid * 64; lo <= hi && rb <= hi; rb += BLOCK) {
+            // the list offsets of this wavefront's next 64 rows rb, rb + 1, ...
+            // (rows interleave over the wavefronts in runs of 64)
+            const uint32_t my_row = rb + lane;
+            const uint32_t my_off = my_row <= hi ? S.cell_off[c0 + my_row] : 0u;
+            const uint32_t n_r = hi - rb + 1 < 64u ? hi - rb + 1 : 64u;
+            for (uint32_t i = 0; i < n_r; ++i) {
+                const uint32_t r = rb + i;
+                const uint32_t base = (uint32_t)__shfl((int)my_off, (int)i, 64);
+                uint32_t run = 0;
+                for (uint32_t e0 = 0; e0 < V; e0 += 64) {
+                    const uint32_t e = e0 + lane;
+                    const uint2 rr = e < V ? s_rows[e] : make_uint2(1u, 0u);
+                    const bool x = (rr.x <= r) & (r <= rr.y);
+                    const unsigned long long m = __ballot(x);
+                    if (x) {
+                        const uint32_t at = base + run +
+                                            __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        if (at < S.cap_pairs)
+                            S.cell_edge[at] = e;
+                        else
+                            bad = 1;
+                    }
+                    run += (uint32_t)__popcll(m);
+                }
+            }
+        }
+        if (bad) raise_err(st, FT_EDEVICE);
+        __syncthreads();  // s_rows and s_lo / s_hi are the next path's
+    }
+}
+
 // FILL_LANES threads per edge, each taking every FILL_LANES-th run of
 // FILL_ROWS rows, with the run's FILL_ROWS slot reservations (returning
 // atomics on the cells' cursors) in flight together: a tall edge does not
@@ -308,6 +396,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill_cells(DScene S, DevStatus *st) {
         uint32_t k, r0, r1;
         float2 a, b;
         edge_at(S, v, k, a, b);
+        if (S.vtx_off[k + 1] - S.vtx_off[k] <= CELL_EDGES) continue;  // k_cells has the path
         if (!edge_row_range(a, b, S.H, r0, r1)) continue;
         const uint32_t local = v - S.vtx_off[k];
         const uint64_t c0 = (uint64_t)k * S.H;
@@ -329,10 +418,28 @@ __global__ __launch_bounds__(BLOCK) void k_fill_cells(DScene S, DevStatus *st) {
 }
 
 // Each cell's edges in ascending order, so that sums run in edge order.  A
-// cell of at most SORT_REGS edges is loaded into registers (all loads in
-// flight), sorted by an odd-even transposition network and stored back;
-// longer cells are sorted in place.
-constexpr uint32_t SORT_REGS = 16;
+// cell of 2 to 16 edges is loaded into registers (all loads in flight),
+// sorted by an odd-even transposition network of 8 or 16 places and stored
+// back; longer cells are sorted in place.
+template <uint32_t N>
+__device__ inline void sort_cell_regs(uint32_t *e, uint32_t n) {
+    uint32_t x[N];
+#pragma unroll
+    for (uint32_t i = 0; i < N; ++i) x[i] = i < n ? e[i] : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t pass = 0; pass < N; ++pass) {
+#pragma unroll
+        for (uint32_t i = pass & 1u; i + 1 < N; i += 2) {
+            const uint32_t lo = x[i] < x[i + 1] ? x[i] : x[i + 1];
+            const uint32_t hi = x[i] < x[i + 1] ? x[i + 1] : x[i];
+            x[i] = lo;
+            x[i + 1] = hi;
+        }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < N; ++i)
+        if (i < n) e[i] = x[i];
+}
 
 __global__ __launch_bounds__(BLOCK) void k_sort_cells(DScene S, DevStatus *st) {
     if (run_failed(st)) return;
@@ -342,34 +449,22 @@ __global__ __launch_bounds__(BLOCK) void k_sort_cells(DScene S, DevStatus *st) {
             raise_err(st, FT_EDEVICE);
             continue;
         }
+        if (n < 2 || S.vtx_off[c / S.H + 1] - S.vtx_off[c / S.H] <= CELL_EDGES) continue;  // k_cells: in order
         uint32_t *e = S.cell_edge + off;
-        if (n <= SORT_REGS) {
-            uint32_t x[SORT_REGS];
-#pragma unroll
-            for (uint32_t i = 0; i < SORT_REGS; ++i) x[i] = i < n ? e[i] : 0xFFFFFFFFu;
-#pragma unroll
-            for (uint32_t pass = 0; pass < SORT_REGS; ++pass) {
-#pragma unroll
-                for (uint32_t i = pass & 1u; i + 1 < SORT_REGS; i += 2) {
-                    const uint32_t lo = x[i] < x[i + 1] ? x[i] : x[i + 1];
-                    const uint32_t hi = x[i] < x[i + 1] ? x[i + 1] : x[i];
-                    x[i] = lo;
-                    x[i + 1] = hi;
+        if (n <= 8) {
+            sort_cell_regs<8>(e, n);
+        } else if (n <= 16) {
+            sort_cell_regs<16>(e, n);
+        } else {
+            for (uint32_t i = 1; i < n; ++i) {
+                const uint32_t x = e[i];
+                uint32_t j = i;
+                while (j > 0 && e[j - 1] > x) {
+                    e[j] = e[j - 1];
+                    --j;
                 }
+                e[j] = x;
             }
-#pragma unroll
-            for (uint32_t i = 0; i < SORT_REGS; ++i)
-                if (i < n) e[i] = x[i];
-            continue;
-        }
-        for (uint32_t i = 1; i < n; ++i) {
-            const uint32_t x = e[i];
-            uint32_t j = i;
-            while (j > 0 && e[j - 1] > x) {
-                e[j] = e[j - 1];
-                --j;
-            }
-            e[j] = x;
         }
     }
 }
@@ -800,6 +895,7 @@ struct ft_diff {
     hipStream_t stream = nullptr;
     uint32_t n_images = 0, W = 0, H = 0, n_paths = 0, max_paths = 0;
     uint32_t n_pts = 0, n_vtx = 0;
+    uint32_t max_path_vtx = 0;  // most vertices (edges) of one path
     uint64_t n_cells = 0, canvas_floats = 0;
     uint32_t *d_pt_off = nullptr, *d_vtx_off = nullptr, *d_vtx_path = nullptr, *d_img_first = nullptr,
              *d_pt_path = nullptr;
@@ -874,8 +970,11 @@ ft_status launch_geometry(ft_diff *d, const float *points, bool clear_grads, flo
                                  nullptr, d->d_status, false, true);
     scan_launch<uint32_t, false>(s, scan_resident_blocks(), d->d_cell_cnt, d->d_cell_off, nullptr, nc, nc,
                                  d->d_lb + 2 * d->cap_tiles, &d->d_status->n_pairs, d->d_status, false, false);
-    k_fill_cells<<<grid_of((size_t)d->n_vtx * FILL_LANES), BLOCK, 0, s>>>(S, d->d_status);
-    k_sort_cells<<<grid_of(d->n_cells), BLOCK, 0, s>>>(S, d->d_status);
+    k_cells<<<(uint32_t)std::min<size_t>(std::max<size_t>(d->n_paths, 1), 65535), BLOCK, 0, s>>>(S, d->d_status);
+    if (d->max_path_vtx > CELL_EDGES) {
+        k_fill_cells<<<grid_of((size_t)d->n_vtx * FILL_LANES), BLOCK, 0, s>>>(S, d->d_status);
+        k_sort_cells<<<grid_of(d->n_cells), BLOCK, 0, s>>>(S, d->d_status);
+    }
     FD_HIP(hipGetLastError());
     return FT_OK;
 }
@@ -1032,6 +1131,7 @@ ft_status ft_diff_new(const ft_diff_layout *L, ft_diff **out) {
     d->max_paths = max_paths;
     d->n_pts = (uint32_t)np;
     d->n_vtx = (uint32_t)nv;
+    for (uint32_t k = 0; k < L->n_paths; ++k) d->max_path_vtx = std::max(d->max_path_vtx, vtx_off[k + 1] - vtx_off[k]);
     d->n_cells = cells;
     d->canvas_floats = 4ull * L->n_images * L->width * L->height;
     const dim3 g = raster_grid(d);
