@@ -187,20 +187,29 @@ ft_status ft_plotter_fill(ft_plotter *p, ft_fill_rule rule, const ft_path_op *op
 
 // Plotter::stroke (plotter.rs:356-365): the ops are flattened with widths,
 // the stroker stages build the outline, and the outline is filled NonZero --
-// all in one device run.
+// all in one device run (two launches for a small figure).
 ft_status ft_plotter_stroke(ft_plotter *p, const ft_path_op *ops, size_t n, const uint8_t clr[4]) {
     if (!p || !valid_ops(ops, n)) return FT_EINVAL;
     if (n > 0x7FFFFFF0u) return FT_EINVAL;
     ft_fill_desc d = plotter_desc(p, FT_NON_ZERO, n, clr);
-    ft_status s = ft::ws_set_batch(p->ws, ops, n, &d, 1, ft::PATH_WHOLE);
-    if (s) return s;
     p->ws.join = (uint32_t)p->join;
     p->ws.miter = p->miter;
     p->ws.stroke_tol_sq = p->tolerance * p->tolerance;
-    s = ft::ws_run(p->ws, ft::MODE_STROKE, true);
+    bool handled = false;
+    uint64_t offered = 0, stored = 0;
+    ft_status s = ft::ws_small_fill(p->ws, ops, n, d, &handled, &offered, &stored, ft::MODE_STROKE);
     if (s) return s;
-    p->last_offered = p->ws.h_pstate[0].offered;
-    p->last_stored = p->ws.h_pstate[0].stored;
+    if (handled) {
+        p->last_offered = offered;
+        p->last_stored = stored;
+    } else {
+        s = ft::ws_set_batch(p->ws, ops, n, &d, 1, ft::PATH_WHOLE);
+        if (s) return s;
+        s = ft::ws_run(p->ws, ft::MODE_STROKE, true);
+        if (s) return s;
+        p->last_offered = p->ws.h_pstate[0].offered;
+        p->last_stored = p->ws.h_pstate[0].stored;
+    }
     p->s_width = final_pen_width(ops, n, p->s_width);
     return FT_OK;
 }

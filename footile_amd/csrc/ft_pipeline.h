@@ -162,12 +162,14 @@ ft_status ws_launch(Workspace &ws, uint32_t mode, bool read_pstate = false);
 ft_status ws_finish(Workspace &ws, bool read_pstate);
 // launch + finish, growing capacities and relaunching on overflow.
 ft_status ws_run(Workspace &ws, uint32_t mode, bool read_pstate);
-// One fill of host ops as a small figure when it is small enough (ops,
-// events and sub-figures in one block's LDS): sets *handled and the offered /
-// stored point counts; otherwise *handled is false and nothing was drawn.
-// The workspace's batch layout is invalid afterwards.
+// One fill (mode MODE_FILL) or stroke (MODE_STROKE, with the workspace's
+// join, miter and stroke tolerance) of host ops as a small figure when it is
+// small enough (ops, events, sub-figures and, for a stroke, the outline in
+// one block's LDS): sets *handled and the offered / stored point counts;
+// otherwise *handled is false and nothing was drawn.  The workspace's batch
+// layout is invalid afterwards.
 ft_status ws_small_fill(Workspace &ws, const ft_path_op *ops, size_t n_ops, const ft_fill_desc &d, bool *handled,
-                        uint64_t *offered, uint64_t *stored);
+                        uint64_t *offered, uint64_t *stored, uint32_t mode = MODE_FILL);
 // Run the batch once (eagerly) counting the raster's (edge, row) pairs and
 // its nonzero cell deltas (the LDS atomics of the cell pass).
 ft_status ws_raster_stats(Workspace &ws, uint64_t *pairs, uint64_t *adds);

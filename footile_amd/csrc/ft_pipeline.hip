@@ -1579,8 +1579,11 @@ __global__ __launch_bounds__(BLOCK) void k_tiles(Limits L0, const PathDev *paths
 // winding direction.  It writes the edge list and the path's state; the
 // raster then runs as one block per LDS row group (k_small_raster).  Two
 // launches, one upload and one readback instead of the pipeline's dozen
-// launches.  Anything the block does not cover (more events or sub-figures,
-// a flattening error, an index check) makes it write nothing and report "not
+// launches.  A stroke does the same with the stroker's stages in between
+// (flattened events with widths, stored points, sub-strokes, the outline's
+// items), as long as the outline's events and closes fit the same limits.
+// Anything the block does not cover (more events or sub-figures, a
+// flattening error, an index check) makes it write nothing and report "not
 // handled" (the raster blocks then do nothing); the caller runs the full
 // pipeline, which gives the same result or the same error.
 constexpr int SBLOCK = 1024;                     // threads of k_small_front
@@ -1600,6 +1603,10 @@ struct SmallLds {
             I2 xy[SMALL_EVENTS];
             uint32_t meta[SMALL_EVENTS], last[SMALL_EVENTS], kept[SMALL_EVENTS], idx[SMALL_EVENTS];
         } ev;
+        struct {  // a stroke's flattened (x, y, width, kind) events and its last close so far
+            F4 f[SMALL_EVENTS];
+            uint32_t last[SMALL_EVENTS];
+        } st;
         EdgeRec edges[SMALL_EVENTS];  // after compaction
     } a;
     union {
@@ -1676,39 +1683,16 @@ struct SmallOps {
     ft_path_op op[SMALL_ARG_OPS];
 };
 
-// ops: the figure's ops in device memory, or null when `inl` carries them.
-__global__ __launch_bounds__(SBLOCK) void k_small_front(const ft_path_op *ops, SmallOps inl, uint32_t n_ops,
-                                                        PathDev P, EdgeRec *edges, PathState *ps, DevStatus *st) {
-    __shared__ SmallLds S;
-    const uint32_t tid = threadIdx.x;
-    if (tid < sizeof(DevStatus) / 4) reinterpret_cast<uint32_t *>(st)[tid] = 0;
-    if (tid == 0) {
-        S.path = P;
-        S.flag = 0;
-    }
-    const uint32_t n_in = n_ops < SMALL_OPS ? n_ops : SMALL_OPS;
-    for (uint32_t i = tid; i < SMALL_OPS; i += SBLOCK) S.op_path[i] = 0;
-    for (uint32_t i = tid; i < n_in; i += SBLOCK) S.ops[i] = ops ? ops[i] : inl.op[i < SMALL_ARG_OPS ? i : 0];
-    __syncthreads();
-    // 1. (op, lane) counts
-    const uint32_t n_items = n_ops * OP_LANES;
-    uint32_t err = 0;
-    for (uint32_t t = tid; t < SMALL_ITEMS; t += SBLOCK) {
-        OpCount c = {0, 0, 0, 0, 0};
-        if (t < n_items) c = count_op(S.ops, S.op_path, &S.path, t >> FLAT_SPLIT, t & (OP_LANES - 1), MODE_FILL, nullptr);
-        err |= c.err;
-        S.b.cnt_e[t] = c.ne;
-        S.c.cnt_c[t] = c.nc;
-    }
-    if (err) S.flag = 1;
-    __syncthreads();
-    const uint32_t n_events = small_scan<false>(S.b.cnt_e, n_items, S.w);
-    const uint32_t n_runs = small_scan<false>(S.c.cnt_c, n_items, S.w2);
-    if (S.flag | (n_events > SMALL_EVENTS) | (n_runs > SMALL_RUNS)) {
-        if (tid == 0) atomicOr(&st->overflow, SMALL_NOT_HANDLED);
-        return;
-    }
-    // 2. emit the events into LDS
+// A small stroke's stroker data beside SmallLds: the pen-width marks of the
+// ops and their running maximum, the stored points and the sub-strokes.
+struct SmallStrokeLds {
+    F4 spts[SMALL_EVENTS];
+    uint32_t lastpw[SMALL_OPS];
+    uint32_t sub_start[SMALL_RUNS], sub_event[SMALL_RUNS], sub_n[SMALL_RUNS], sub_joined[SMALL_RUNS];
+};
+
+// Fill events of a small figure go to the event and run arrays in LDS.
+__device__ inline EventSink small_fill_sink(SmallLds &S) {
     EventSink em{};
     em.mode = MODE_FILL;
     em.cap_events = SMALL_EVENTS;
@@ -1720,12 +1704,189 @@ __global__ __launch_bounds__(SBLOCK) void k_small_front(const ft_path_op *ops, S
     em.run_path = S.run_path;
     em.run_start = S.run_start;
     em.run_end = S.run_end;
-    uint32_t bad = 0;
+    return em;
+}
+
+// Stages 1-2 in one block: the (op, lane) event and close counts, their block
+// scans, and the emit through `em`.  False (in every thread, nothing emitted)
+// when flattening fails or the events or closes exceed their limits.
+__device__ bool small_flatten(SmallLds &S, uint32_t n_ops, uint32_t mode, const uint32_t *lastpw, EventSink em,
+                              uint32_t max_runs, uint32_t &n_events, uint32_t &n_runs, uint32_t &bad) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n_items = n_ops * OP_LANES;
+    uint32_t err = 0;
+    for (uint32_t t = tid; t < SMALL_ITEMS; t += SBLOCK) {
+        OpCount c = {0, 0, 0, 0, 0};
+        if (t < n_items) c = count_op(S.ops, S.op_path, &S.path, t >> FLAT_SPLIT, t & (OP_LANES - 1), mode, lastpw);
+        err |= c.err;
+        S.b.cnt_e[t] = c.ne;
+        S.c.cnt_c[t] = c.nc;
+    }
+    if (err) S.flag = 1;
+    __syncthreads();
+    n_events = small_scan<false>(S.b.cnt_e, n_items, S.w);
+    n_runs = small_scan<false>(S.c.cnt_c, n_items, S.w2);
+    if (S.flag | (n_events > SMALL_EVENTS) | (n_runs > max_runs)) return false;
     for (uint32_t t = tid; t < n_items; t += SBLOCK) {
         const unsigned long long off = ((unsigned long long)S.b.cnt_e[t] << 32) | S.c.cnt_c[t];
-        bad |= emit_op(S.ops, S.op_path, &S.path, t >> FLAT_SPLIT, t & (OP_LANES - 1), MODE_FILL, nullptr, off, em);
+        bad |= emit_op(S.ops, S.op_path, &S.path, t >> FLAT_SPLIT, t & (OP_LANES - 1), mode, lastpw, off, em);
     }
     __syncthreads();
+    return true;
+}
+
+// The stroker (the k_st_* stages) in one block, from the flattened events in
+// S.a.st.f to the outline's fill events and runs.  False (in every thread)
+// when the sub-strokes, outline items, events or closes exceed their limits
+// or a join fails to flatten; nothing usable is written then.
+__device__ bool small_stroker(SmallLds &S, SmallStrokeLds &T, uint32_t n_sev, StrokeParams sp, uint32_t &n_events,
+                              uint32_t &n_runs, uint32_t &bad) {
+    const uint32_t tid = threadIdx.x;
+    // flags (k_st_flags): pushed -> stored point index, first point of a run
+    // -> sub-stroke index, closes -> the last close so far
+    for (uint32_t e = tid; e < n_sev; e += SBLOCK) {
+        const StrokeFlags f = stroke_flags(e, S.a.st.f);
+        S.b.cnt_e[e] = f.pushed ? 1u : 0u;
+        S.c.cnt_c[e] = f.first ? 1u : 0u;
+        S.a.st.last[e] = f.is_pt ? 0u : e + 1;
+    }
+    __syncthreads();
+    const uint32_t n_pts = small_scan<false>(S.b.cnt_e, n_sev, S.w);  // below the 65,535 cap
+    const uint32_t n_subs = small_scan<false>(S.c.cnt_c, n_sev, S.w2);
+    (void)small_scan<true>(S.a.st.last, n_sev, S.w);
+    if (n_subs > SMALL_RUNS) return false;
+    // compaction (k_st_compact)
+    for (uint32_t e = tid; e < n_sev; e += SBLOCK) {
+        const StrokeFlags f = stroke_flags(e, S.a.st.f);
+        const uint32_t sidx = S.b.cnt_e[e], sub = S.c.cnt_c[e];
+        if (f.pushed) T.spts[sidx] = S.a.st.f[e];
+        if (f.first) {
+            T.sub_start[sub] = sidx;
+            T.sub_event[sub] = e;
+        }
+    }
+    __syncthreads();
+    // sub-strokes (k_st_subs; every point is kept)
+    for (uint32_t s = tid; s < n_subs; s += SBLOCK) {
+        const uint32_t a = T.sub_start[s];
+        const bool more = s + 1 < n_subs;
+        const uint32_t end = more ? T.sub_start[s + 1] : n_pts;
+        const uint32_t range_end = more ? T.sub_event[s + 1] : n_sev;
+        const uint32_t lc = S.a.st.last[range_end ? range_end - 1 : 0];
+        const bool has_close = (range_end > 0) & (lc > 0) & (lc - 1 >= T.sub_event[s]) & (lc - 1 < n_sev);
+        const F4 ce = S.a.st.f[has_close ? lc - 1 : 0];
+        T.sub_n[s] = end > a ? end - a : 0u;
+        T.sub_joined[s] = (has_close & (ce.w == 2.0f)) ? 1u : 0u;
+    }
+    const uint32_t n_items = n_subs ? 2u * (n_pts + n_subs) + 1u : 1u;
+    if (n_items > SMALL_EVENTS) return false;
+    __syncthreads();
+    // outline items: line and close counts (k_st_count)
+    auto sub_of = [&](uint32_t t, StrokeSub &sub, bool &fwd, uint32_t &k) {
+        const uint32_t si = st_item_sub(t, T.sub_start, n_subs);
+        sub.start = T.sub_start[si];
+        sub.n = T.sub_n[si];
+        sub.joined = T.sub_joined[si] != 0;
+        const uint32_t local = t - 2u * (sub.start + si);
+        fwd = local <= sub.n;
+        k = fwd ? local : local - (sub.n + 1);
+    };
+    uint32_t err = 0;
+    for (uint32_t t = tid; t < n_items; t += SBLOCK) {
+        uint32_t ne = 0, nc = 0;
+        if (t + 1 == n_items) {
+            ne = nc = 1;  // Plotter::fill's final close
+        } else if (n_subs) {
+            StrokeSub sub;
+            bool fwd;
+            uint32_t k;
+            sub_of(t, sub, fwd, k);
+            if (sub.n)
+                err |= stroke_item<DevMath>(
+                    sub, fwd, k, T.spts, sp.join, sp.miter, sp.tol_sq, [&](SV2) { ++ne; },
+                    [&]() {
+                        ++ne;
+                        ++nc;
+                    });
+        }
+        S.b.cnt_e[t] = ne;
+        S.c.cnt_c[t] = nc;
+    }
+    if (err) S.flag = 1;
+    __syncthreads();
+    n_events = small_scan<false>(S.b.cnt_e, n_items, S.w);
+    n_runs = small_scan<false>(S.c.cnt_c, n_items, S.w2);
+    if (S.flag | (n_events > SMALL_EVENTS) | (n_runs > SMALL_RUNS)) return false;
+    // the outline's events (k_st_emit), transformed again as Plotter::fill does
+    const EventSink sink = small_fill_sink(S);
+    for (uint32_t t = tid; t < n_items; t += SBLOCK) {
+        EventSink em = sink;
+        em.e = S.b.cnt_e[t];
+        em.run = S.c.cnt_c[t];
+        em.p = 0;
+        if (t + 1 == n_items) {
+            em.close(true);
+        } else if (n_subs) {
+            StrokeSub sub;
+            bool fwd;
+            uint32_t k;
+            sub_of(t, sub, fwd, k);
+            if (sub.n)
+                stroke_item<DevMath>(
+                    sub, fwd, k, T.spts, sp.join, sp.miter, sp.tol_sq,
+                    [&](SV2 q) { em.point(xform(S.path, q.x, q.y, 0.0f)); }, [&]() { em.close(true); });
+        }
+        bad |= em.bad;
+    }
+    __syncthreads();
+    return true;
+}
+
+// ops: the figure's ops in device memory, or null when `inl` carries them.
+// STROKE: Plotter::stroke (flatten with widths, the stroker, then the
+// outline's fill); else Plotter::fill.
+template <bool STROKE>
+__global__ __launch_bounds__(SBLOCK) void k_small_front(const ft_path_op *ops, SmallOps inl, uint32_t n_ops,
+                                                        PathDev P, StrokeParams sp, EdgeRec *edges, PathState *ps,
+                                                        DevStatus *st) {
+    __shared__ SmallLds S;
+    const uint32_t tid = threadIdx.x;
+    if (tid < sizeof(DevStatus) / 4) reinterpret_cast<uint32_t *>(st)[tid] = 0;
+    if (tid == 0) {
+        S.path = P;
+        S.flag = 0;
+    }
+    const uint32_t n_in = n_ops < SMALL_OPS ? n_ops : SMALL_OPS;
+    for (uint32_t i = tid; i < SMALL_OPS; i += SBLOCK) S.op_path[i] = 0;
+    for (uint32_t i = tid; i < n_in; i += SBLOCK) S.ops[i] = ops ? ops[i] : inl.op[i < SMALL_ARG_OPS ? i : 0];
+    __syncthreads();
+    // 1-2. flatten (and for a stroke, the stroker) into the fill events in LDS
+    uint32_t n_events = 0, n_runs = 0, bad = 0;
+    bool ok;
+    if constexpr (STROKE) {
+        __shared__ SmallStrokeLds T;
+        // the last PenWidth op up to each op (a running maximum of i + 1); the
+        // pipeline's exclusive form differs only at PenWidth ops, whose width
+        // is never asked for
+        for (uint32_t i = tid; i < SMALL_OPS; i += SBLOCK)
+            T.lastpw[i] = (i < n_in && S.ops[i].tag == TAG_PEN_WIDTH) ? i + 1 : 0u;
+        __syncthreads();
+        (void)small_scan<true>(T.lastpw, n_in, S.w);
+        EventSink em{};
+        em.mode = MODE_STROKE;
+        em.cap_events = SMALL_EVENTS;
+        em.cap_runs = SMALL_EVENTS;
+        em.ev_f = S.a.st.f;
+        uint32_t n_sev = 0, n_scl = 0;
+        ok = small_flatten(S, n_ops, MODE_STROKE, T.lastpw, em, SMALL_EVENTS, n_sev, n_scl, bad) &&
+             small_stroker(S, T, n_sev, sp, n_events, n_runs, bad);
+    } else {
+        ok = small_flatten(S, n_ops, MODE_FILL, nullptr, small_fill_sink(S), SMALL_RUNS, n_events, n_runs, bad);
+    }
+    if (!ok) {
+        if (tid == 0) atomicOr(&st->overflow, SMALL_NOT_HANDLED);
+        return;
+    }
     Limits L{};
     L.n_ops = n_ops;
     L.n_paths = 1;
@@ -2727,10 +2888,10 @@ ft_status ws_raster_stats(Workspace &ws, uint64_t *pairs, uint64_t *adds) {
 }
 
 ft_status ws_small_fill(Workspace &ws, const ft_path_op *ops, size_t n_ops, const ft_fill_desc &d, bool *handled,
-                        uint64_t *offered, uint64_t *stored) {
+                        uint64_t *offered, uint64_t *stored, uint32_t mode) {
     *handled = false;
     if (ws.arena || !n_ops || n_ops > SMALL_OPS || d.fmt > FT_GRAYA8P || !d.width || !d.height ||
-        d.width > FT_MAX_WIDTH)
+        d.width > FT_MAX_WIDTH || (mode != MODE_FILL && mode != MODE_STROKE))
         return FT_OK;
     // the figure's raster geometry: row groups as in tile mode, no bands,
     // short enough that the raster has SMALL_RASTER_BLOCKS blocks or more
@@ -2785,8 +2946,16 @@ ft_status ws_small_fill(Workspace &ws, const ft_path_op *ops, size_t n_ops, cons
     // the layout's pointers into the upload block are stale from here on: the
     // next full-pipeline call lays the batch out again
     ws.n_ops = ws.n_paths = ws.n_bands = 0;
-    k_small_front<<<1, SBLOCK, 0, ws.stream>>>(d_ops, inl, (uint32_t)n_ops, P, ws.d_edges, ws.d_pstate,
-                                               ws.d_status);
+    StrokeParams sp;
+    sp.join = ws.join;
+    sp.miter = ws.miter;
+    sp.tol_sq = ws.stroke_tol_sq;
+    if (mode == MODE_STROKE)
+        k_small_front<true><<<1, SBLOCK, 0, ws.stream>>>(d_ops, inl, (uint32_t)n_ops, P, sp, ws.d_edges,
+                                                         ws.d_pstate, ws.d_status);
+    else
+        k_small_front<false><<<1, SBLOCK, 0, ws.stream>>>(d_ops, inl, (uint32_t)n_ops, P, sp, ws.d_edges,
+                                                          ws.d_pstate, ws.d_status);
     if (d.fmt == FT_MATTE8)
         k_small_raster<FMT_MATTE8><<<groups, BLOCK, lds, ws.stream>>>(P, ws.d_edges, ws.d_pstate, ws.d_status);
     else if (d.fmt == FT_RGBA8P)

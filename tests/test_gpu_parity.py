@@ -655,3 +655,41 @@ def test_small_figure_path_and_its_limits(ft, oracle):
         for k, ops in enumerate(cases):
             calls.append(("fill", k % 2, ops, clr))
         compare_fill(ft, oracle, 200, 300, fmt, calls, init=init, tol=0.05, tag=f"small path fmt {fmt}")
+
+
+def test_small_stroke_path_and_its_limits(ft, oracle):
+    """Plotter strokes whose flattened events and outline fit 2,048 events
+    (and 257 closes) run the stroker inside the small-figure block; bigger
+    outlines (round joins on a long polyline, random curves) fall back to the
+    full pipeline in the same plotter.  The oracle's outline point counts show
+    which side of the limit each case is on; every stroke is compared byte
+    for byte, with pen widths changed inside paths, degenerate sub-strokes and
+    every join."""
+    rng = np.random.default_rng(17)
+    poly = ops_array([(PEN_WIDTH, (12.0,)), (MOVE, (100.0, 100.0))] +
+                     [(LINE, tuple(rng.uniform(10, [190, 290]).astype(np.float32))) for _ in range(200)])
+    widths = []
+    for k in range(12):
+        widths += [(PEN_WIDTH, (float(1 + 2 * k),)), (LINE, tuple(rng.uniform(10, [190, 290]).astype(np.float32)))]
+    widths = ops_array([(MOVE, (20.0, 20.0))] + widths + [(CLOSE, ())])
+    degenerate = ops_array([(MOVE, (5.0, 5.0)), (MOVE, (30.0, 40.0)), (LINE, (30.0, 40.0)), (LINE, (90.0, 40.0)),
+                            (LINE, (90.0, 40.0)), (CLOSE, ()), (CLOSE, ()), (LINE, (150.0, 150.0)),
+                            (MOVE, (60.0, 200.0)), (MOVE, (70.0, 210.0)), (LINE, (70.0, 210.0)), (CLOSE, ()),
+                            (MOVE, (120.0, 250.0)), (QUAD, (180.0, 280.0, 120.0, 290.0)), (LINE, (120.0, 250.0))])
+    lines = random_path(rng, 200, 300, n_sub=6, kinds=(LINE,), close_prob=0.5)
+    many = random_path(rng, 200, 300, n_sub=60, kinds=(LINE,), close_prob=0.5)[:256]
+    curves = random_path(rng, 200, 300, n_sub=5, kinds=(LINE, QUAD, CUBIC), close_prob=0.5)
+    cases = [(poly, (0, 4.0), True), (poly, (1, 0.0), True), (poly, (2, 0.0), False), (widths, (0, 4.0), True),
+             (widths, (2, 0.0), True), (degenerate, (0, 1.5), True), (lines, (0, 4.0), True),
+             (many, (1, 0.0), True), (curves, (0, 4.0), False)]
+    for fmt in (MATTE8, RGBA8P):
+        clr = (255,) if fmt == MATTE8 else (30, 160, 90, 220)
+        for k, (ops, join, small) in enumerate(cases):
+            g, o = compare_fill(ft, oracle, 200, 300, fmt, [("stroke", ops, clr), ("fill", 1, ops, clr),
+                                                           ("stroke", ops, clr)],
+                                join=join, tag=f"small stroke {k} join {join} fmt {fmt}")
+            assert g.last_counts() == o.last_counts()
+            o2 = oracle_plotter(oracle, 200, 300, fmt, join=join)
+            assert o2.stroke(ops, clr) == 0
+            offered = o2.last_counts()[0]
+            assert (offered < 1900) if small else (offered > 2100), (k, offered)
