@@ -1213,6 +1213,19 @@ struct PathEdgeSrc {
     __device__ uint32_t operator()(uint32_t k) const { return base + k; }
 };
 
+// A long (edge, row) cell walk that the direct cell pass hands to a whole
+// wavefront (SPLIT): the row's set-up, its first cell's index in the
+// group's LDS cells and the edge's winding sign.  The queue and its count
+// take the front of the block's LDS, which the direct pass does not use and
+// the composite table only fills afterwards.
+struct LongWalk {
+    RowCells c;
+    int32_t at, ed;
+};
+constexpr int32_t LONG_CELLS = 16;  // walks of more cells than this are split over a wavefront
+constexpr uint32_t LONG_QUEUE = (LUT_BYTES - sizeof(uint4)) / sizeof(LongWalk);
+static_assert(sizeof(uint4) + LONG_QUEUE * sizeof(LongWalk) <= LUT_BYTES, "the queue fits every format's front");
+
 // Rows [y0, y1) of path P's raster (one LDS group): the cnt listed edges'
 // (edge, row) cell deltas into LDS (i32 atomics), the wave row scan, the rule
 // map and the store / SrcOver.  Every thread of the block calls it with the
@@ -1221,7 +1234,11 @@ struct PathEdgeSrc {
 // rows' i32 cells.
 // n_pairs / n_adds: the calling thread's running counts of the (edge, row)
 // pairs it walked and of the nonzero cell deltas it added (LDS atomics).
-template <bool PF, int FMT, class Src>
+// SPLIT: the direct cell pass queues walks longer than LONG_CELLS cells (up
+// to LONG_QUEUE of them; a lane walks the rest itself) and every wavefront
+// then takes queued walks, one cell per lane (row_cell_delta), so that one
+// shallow edge does not hold up its block.
+template <bool PF, int FMT, class Src, bool SPLIT = false>
 __device__ __forceinline__ void raster_group(const Limits &L, const PathDev &P, const PathState &S, int fwd,
                                              uint32_t y0, uint32_t y1, Src src, uint32_t cnt, bool list_ok,
                                              const EdgeRec *edges, uint4 *s_dyn, uint32_t *s_segsum,
@@ -1296,6 +1313,9 @@ __device__ __forceinline__ void raster_group(const Limits &L, const PathDev &P, 
     }
     for (uint32_t c = threadIdx.x; c < rows * Wp / 4; c += BLOCK)
         reinterpret_cast<int4 *>(s_cells)[c] = make_int4(0, 0, 0, 0);
+    uint32_t *const q_n = reinterpret_cast<uint32_t *>(s_dyn);
+    LongWalk *const q = reinterpret_cast<LongWalk *>(s_dyn + 1);
+    if (SPLIT && threadIdx.x == 0) *q_n = 0;
     __syncthreads();
     const int32_t shift = top_row < 0 ? top_row : 0;
     if (!list_ok && threadIdx.x == 0) raise_err(st, FT_EDEVICE);
@@ -1337,6 +1357,19 @@ __device__ __forceinline__ void raster_group(const Limits &L, const PathDev &P, 
                 const RowCells c = row_setup(E[j], (int32_t)(r + shift), (int32_t)W);
                 int32_t *const at = s_cells + (size_t)(r - ry0) * Wp + c.x_s;
                 ++n_pairs_acc;
+                if constexpr (SPLIT) {
+                    if (row_cells_exceed(c, LONG_CELLS)) {
+                        const uint32_t qi = atomicAdd(q_n, 1u);
+                        if (qi < LONG_QUEUE) {
+                            LongWalk w;
+                            w.c = c;
+                            w.at = (int32_t)((uint32_t)(r - ry0) * Wp + (uint32_t)c.x_s);
+                            w.ed = ed;
+                            q[qi] = w;
+                            continue;
+                        }
+                    }
+                }
                 row_walk(c, [&](int32_t jj, int32_t d) {
                     if (d) {
                         atomicAdd(&at[jj], d * ed);
@@ -1392,6 +1425,24 @@ __device__ __forceinline__ void raster_group(const Limits &L, const PathDev &P, 
             });
         }
         __syncthreads();  // before the next entries overwrite the slots
+    }
+    if constexpr (SPLIT) {
+        if (direct) {
+            __syncthreads();
+            const uint32_t nq = *q_n < LONG_QUEUE ? *q_n : LONG_QUEUE;
+            for (uint32_t qi = (uint32_t)wid; qi < nq; qi += NW) {
+                const LongWalk w = q[qi];
+                const int32_t m = row_cell_count(w.c);
+                for (int32_t j = lane; j < m; j += 64) {
+                    const int32_t d = row_cell_delta(w.c, j);
+                    if (d) {
+                        atomicAdd(&s_cells[w.at + j], d * w.ed);
+                        ++n_adds_acc;
+                    }
+                }
+            }
+            __syncthreads();  // the composite table overwrites the queue
+        }
     }
     if constexpr (HAS_LUT)
         for (uint32_t a = threadIdx.x; a < LUT_ENTRIES; a += BLOCK) lut[a] = lut_entry(fmt, P.clr, a);
@@ -2018,8 +2069,9 @@ __global__ __launch_bounds__(BLOCK) void k_small_raster(PathDev P, const EdgeRec
     const bool has_pts = S.topkey != NO_KEY && S.topidx != NO_IDX && n_kept <= SMALL_EVENTS;
     uint32_t n_pairs = 0, n_adds = 0;
     if (y0 < y1)
-        raster_group<false, FMT>(L, P, S, (int)S.fwd, y0, y1, PathEdgeSrc{0u}, has_pts ? n_kept : 0u, true, edges,
-                                 s_dyn, s_segsum, st, n_pairs, n_adds);
+        raster_group<false, FMT, PathEdgeSrc, true>(L, P, S, (int)S.fwd, y0, y1, PathEdgeSrc{0u},
+                                                    has_pts ? n_kept : 0u, true, edges, s_dyn, s_segsum, st, n_pairs,
+                                                    n_adds);
 }
 
 // ---------------------------------------------------------------------------
@@ -2894,12 +2946,14 @@ ft_status ws_small_fill(Workspace &ws, const ft_path_op *ops, size_t n_ops, cons
         d.width > FT_MAX_WIDTH || (mode != MODE_FILL && mode != MODE_STROKE))
         return FT_OK;
     // the figure's raster geometry: row groups as in tile mode, no bands,
-    // short enough that the raster has SMALL_RASTER_BLOCKS blocks or more
+    // short enough that the raster has SMALL_RASTER_BLOCKS blocks or more,
+    // and at most DIRECT_ROWS rows (the direct cell pass, which splits long
+    // cell walks over wavefronts)
     PathDev P = path_from_desc(d, 0u, ws.tuning);
     const uint32_t wp = cell_stride(d.width);
     const uint32_t g_lds = std::max(1u, ws.tuning.lds_cells / wp);
     const uint32_t g_par = std::max(1u, (d.height + SMALL_RASTER_BLOCKS - 1) / SMALL_RASTER_BLOCKS);
-    const uint32_t G = std::min(d.height, std::min(g_lds, g_par));
+    const uint32_t G = std::min(std::min(d.height, DIRECT_ROWS), std::min(g_lds, g_par));
     const size_t lds = raster_lds_bytes(d.fmt, (size_t)G * wp);
     if (lds > ws.lds_max) return FT_OK;
     P.flags |= PATH_TILE;

@@ -923,6 +923,38 @@ FT_HD inline void row_walk(const RowCells &c, F &&f) {
     }
 }
 
+// row_walk in closed form, so that one long walk can be split over many
+// lanes.  The walk's x_cov at cell j is min(xk + j * st, ONE) with
+// xk = xc0 + k_s * st, and it ends at the first cell whose (x_cov + 128) >> 8
+// reaches cov, i.e. whose xk + j * st >= cov * 256 - 128 (below ONE, as
+// cov <= 256), or at cell n - 1.
+// The number of cells row_walk visits.
+FT_HD inline int32_t row_cell_count(const RowCells &c) {
+    if (c.n <= 0) return 0;
+    const int64_t xk = c.xc0 + (int64_t)c.k_s * c.st;
+    const int64_t t = (int64_t)c.cov * 256 - 128;
+    const int64_t last = xk >= t ? 0 : (t - xk + c.st - 1) / c.st;
+    return last + 1 < (int64_t)c.n ? (int32_t)(last + 1) : c.n;
+}
+
+// row_cell_count(c) > m, without the division: cell m - 1 does not end the
+// walk.
+FT_HD inline bool row_cells_exceed(const RowCells &c, int32_t m) {
+    return (c.n > m) & ((m <= 0) | (c.xc0 + ((int64_t)c.k_s + m - 1) * c.st < (int64_t)c.cov * 256 - 128));
+}
+
+// The covered fraction after cell j (row_walk's `cur`), and cell j's delta.
+FT_HD inline int32_t row_cell_cov(const RowCells &c, int32_t j) {
+    const int64_t x = c.xc0 + ((int64_t)c.k_s + j) * c.st;
+    const int32_t xc = x < FX_ONE ? (int32_t)x : FX_ONE;
+    const int32_t xp = (xc + 128) >> 8;
+    return xp < c.cov ? xp : c.cov;
+}
+
+FT_HD inline int32_t row_cell_delta(const RowCells &c, int32_t j) {
+    return row_cell_cov(c, j) - (j > 0 ? row_cell_cov(c, j - 1) : 0);
+}
+
 template <class A>
 FT_HD inline void edge_row_cells(const EdgeRec &E, int32_t sr, int32_t ed, int32_t W, int32_t *cells) {
     const RowCells c = row_setup(E, sr, W);

@@ -185,7 +185,7 @@ int ftm_row_cells_check(uint64_t seed, int iters) {
         return t ^ (t >> 31);
     };
     int bad = 0;
-    std::vector<int32_t> a(64), b(64);
+    std::vector<int32_t> a(64), b(64), c3(64);
     for (int it = 0; it < iters; ++it) {
         const int32_t W = 1 + (int32_t)(next() % 48);
         // a segment between two Fixed points within [-24, 72) px, mostly short
@@ -209,6 +209,22 @@ int ftm_row_cells_check(uint64_t seed, int iters) {
             edge_row_cells<HostAtom>(E, sr, ed, W, a.data());
             walk_row(E, sr, ed, W, b.data());
             if (a != b) ++bad;
+            // the closed form (row_cell_count / row_cell_delta) cell by cell
+            // against the serial walk, and the division-free length test
+            std::fill(c3.begin(), c3.end(), 0);
+            const RowCells rc = row_setup(E, sr, W);
+            const int32_t m = row_cell_count(rc);
+            int32_t walked = 0;
+            bool same = true;
+            row_walk(rc, [&](int32_t j, int32_t d) {
+                same = same && j == walked && j < m && d == row_cell_delta(rc, j);
+                ++walked;
+            });
+            if (!same || walked != m) ++bad;
+            for (int32_t k : {0, 1, 2, 5, 16, 40})
+                if (row_cells_exceed(rc, k) != (m > k)) ++bad;
+            for (int32_t j = 0; j < m; ++j) c3[rc.x_s + j] += row_cell_delta(rc, j) * ed;
+            if (c3 != a) ++bad;
         }
     }
     return bad;
