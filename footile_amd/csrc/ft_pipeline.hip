@@ -1225,6 +1225,9 @@ struct LongWalk {
 constexpr int32_t LONG_CELLS = 16;  // walks of more cells than this are split over a wavefront
 constexpr uint32_t LONG_QUEUE = (LUT_BYTES - sizeof(uint4)) / sizeof(LongWalk);
 static_assert(sizeof(uint4) + LONG_QUEUE * sizeof(LongWalk) <= LUT_BYTES, "the queue fits every format's front");
+constexpr uint32_t SPLIT_LIST = 2048;  // list entries a SPLIT cell pass filters at a time
+// LDS a SPLIT raster group needs beyond raster_lds_bytes: the filtered list
+constexpr size_t SPLIT_LIST_BYTES = SPLIT_LIST * sizeof(uint32_t);
 
 // Rows [y0, y1) of path P's raster (one LDS group): the cnt listed edges'
 // (edge, row) cell deltas into LDS (i32 atomics), the wave row scan, the rule
@@ -1234,10 +1237,14 @@ static_assert(sizeof(uint4) + LONG_QUEUE * sizeof(LongWalk) <= LUT_BYTES, "the q
 // rows' i32 cells.
 // n_pairs / n_adds: the calling thread's running counts of the (edge, row)
 // pairs it walked and of the nonzero cell deltas it added (LDS atomics).
-// SPLIT: the direct cell pass queues walks longer than LONG_CELLS cells (up
-// to LONG_QUEUE of them; a lane walks the rest itself) and every wavefront
-// then takes queued walks, one cell per lane (row_cell_delta), so that one
-// shallow edge does not hold up its block.
+// SPLIT (small figures: few edges, rows of any width): the cell pass first
+// lists the entries whose rows meet the group's (SPLIT_LIST at a time, in
+// SPLIT_LIST_BYTES of LDS after the P.group_rows rows of cells), then each
+// thread takes (listed edge, group row) pairs, so the work spreads over the
+// block whatever the edges' sizes; walks longer than LONG_CELLS cells are
+// queued (up to LONG_QUEUE of them; a lane walks the rest itself) and every
+// wavefront then takes queued walks, one cell per lane (row_cell_delta).
+// The listed order does not matter: the cell adds are integer adds.
 template <bool PF, int FMT, class Src, bool SPLIT = false>
 __device__ __forceinline__ void raster_group(const Limits &L, const PathDev &P, const PathState &S, int fwd,
                                              uint32_t y0, uint32_t y1, Src src, uint32_t cnt, bool list_ok,
@@ -1329,7 +1336,54 @@ __device__ __forceinline__ void raster_group(const Limits &L, const PathDev &P, 
     // most that many of the group's rows.
     constexpr int EB = 4;
     const bool direct = rows <= DIRECT_ROWS;
-    for (uint32_t k0 = 0; direct && list_ok && k0 < cnt; k0 += (uint32_t)BLOCK * EB) {
+    if constexpr (SPLIT) {
+        uint32_t *const s_list = reinterpret_cast<uint32_t *>(s_cells + (size_t)P.group_rows * Wp);
+        uint32_t *const s_nl = q_n + 1;
+        for (uint32_t k0 = 0; list_ok && k0 < cnt; k0 += SPLIT_LIST) {
+            if (threadIdx.x == 0) *s_nl = 0;
+            __syncthreads();
+            const uint32_t kn = cnt - k0 < SPLIT_LIST ? cnt - k0 : SPLIT_LIST;
+            for (uint32_t k = threadIdx.x; k < kn; k += BLOCK) {
+                const uint32_t ei = src(k0 + k);
+                const bool in = ei < L.cap_points;
+                if (!in) raise_err(st, FT_EDEVICE);
+                const EdgeRec E = edges[in ? ei : 0u];
+                const int64_t a = (int64_t)fx_int(E.y_upper) - shift, b = (int64_t)fx_int(E.y_lower) - shift;
+                if (in & ((E.flags & 1u) != 0) & (a < (int64_t)y1) & (b >= (int64_t)ry0))
+                    s_list[atomicAdd(s_nl, 1u)] = ei;
+            }
+            __syncthreads();
+            const uint32_t n_pairs = *s_nl * rows;
+            for (uint32_t p = threadIdx.x; p < n_pairs; p += BLOCK) {
+                const EdgeRec E = edges[s_list[p / rows]];
+                const int64_t r = (int64_t)ry0 + p % rows;
+                if ((r < (int64_t)fx_int(E.y_upper) - shift) | (r > (int64_t)fx_int(E.y_lower) - shift)) continue;
+                const int32_t ed = ((int)((E.flags >> 1) & 1u) == fwd) ? 1 : -1;
+                const RowCells c = row_setup(E, (int32_t)(r + shift), (int32_t)W);
+                int32_t *const at = s_cells + (size_t)(r - ry0) * Wp + c.x_s;
+                ++n_pairs_acc;
+                if (row_cells_exceed(c, LONG_CELLS)) {
+                    const uint32_t qi = atomicAdd(q_n, 1u);
+                    if (qi < LONG_QUEUE) {
+                        LongWalk w;
+                        w.c = c;
+                        w.at = (int32_t)((uint32_t)(r - ry0) * Wp + (uint32_t)c.x_s);
+                        w.ed = ed;
+                        q[qi] = w;
+                        continue;
+                    }
+                }
+                row_walk(c, [&](int32_t jj, int32_t d) {
+                    if (d) {
+                        atomicAdd(&at[jj], d * ed);
+                        ++n_adds_acc;
+                    }
+                });
+            }
+            __syncthreads();  // before the next entries rewrite the list
+        }
+    }
+    for (uint32_t k0 = 0; !SPLIT && direct && list_ok && k0 < cnt; k0 += (uint32_t)BLOCK * EB) {
         uint32_t ei[EB];
         bool ok[EB];
 #pragma unroll
@@ -1357,19 +1411,6 @@ __device__ __forceinline__ void raster_group(const Limits &L, const PathDev &P, 
                 const RowCells c = row_setup(E[j], (int32_t)(r + shift), (int32_t)W);
                 int32_t *const at = s_cells + (size_t)(r - ry0) * Wp + c.x_s;
                 ++n_pairs_acc;
-                if constexpr (SPLIT) {
-                    if (row_cells_exceed(c, LONG_CELLS)) {
-                        const uint32_t qi = atomicAdd(q_n, 1u);
-                        if (qi < LONG_QUEUE) {
-                            LongWalk w;
-                            w.c = c;
-                            w.at = (int32_t)((uint32_t)(r - ry0) * Wp + (uint32_t)c.x_s);
-                            w.ed = ed;
-                            q[qi] = w;
-                            continue;
-                        }
-                    }
-                }
                 row_walk(c, [&](int32_t jj, int32_t d) {
                     if (d) {
                         atomicAdd(&at[jj], d * ed);
@@ -1379,7 +1420,7 @@ __device__ __forceinline__ void raster_group(const Limits &L, const PathDev &P, 
             }
         }
     }
-    for (uint32_t k0 = 0; !direct && list_ok && k0 < cnt; k0 += CR) {
+    for (uint32_t k0 = 0; !SPLIT && !direct && list_ok && k0 < cnt; k0 += CR) {
         const uint32_t k = k0 + threadIdx.x;
         bool okk = (threadIdx.x < CR) & (k < cnt);
         const uint32_t ei = src(okk ? k : 0u);
@@ -1426,23 +1467,21 @@ __device__ __forceinline__ void raster_group(const Limits &L, const PathDev &P, 
         }
         __syncthreads();  // before the next entries overwrite the slots
     }
-    if constexpr (SPLIT) {
-        if (direct) {
-            __syncthreads();
-            const uint32_t nq = *q_n < LONG_QUEUE ? *q_n : LONG_QUEUE;
-            for (uint32_t qi = (uint32_t)wid; qi < nq; qi += NW) {
-                const LongWalk w = q[qi];
-                const int32_t m = row_cell_count(w.c);
-                for (int32_t j = lane; j < m; j += 64) {
-                    const int32_t d = row_cell_delta(w.c, j);
-                    if (d) {
-                        atomicAdd(&s_cells[w.at + j], d * w.ed);
-                        ++n_adds_acc;
-                    }
+    if constexpr (SPLIT) {  // the queued long walks, one per wavefront at a time
+        __syncthreads();
+        const uint32_t nq = *q_n < LONG_QUEUE ? *q_n : LONG_QUEUE;
+        for (uint32_t qi = (uint32_t)wid; qi < nq; qi += NW) {
+            const LongWalk w = q[qi];
+            const int32_t m = row_cell_count(w.c);
+            for (int32_t j = lane; j < m; j += 64) {
+                const int32_t d = row_cell_delta(w.c, j);
+                if (d) {
+                    atomicAdd(&s_cells[w.at + j], d * w.ed);
+                    ++n_adds_acc;
                 }
             }
-            __syncthreads();  // the composite table overwrites the queue
         }
+        __syncthreads();  // the composite table overwrites the queue
     }
     if constexpr (HAS_LUT)
         for (uint32_t a = threadIdx.x; a < LUT_ENTRIES; a += BLOCK) lut[a] = lut_entry(fmt, P.clr, a);
@@ -1647,6 +1686,7 @@ constexpr uint32_t SMALL_PER_THREAD = SMALL_EVENTS / SBLOCK;
 constexpr uint32_t SMALL_NOT_HANDLED = 0x10u;     // DevStatus::overflow bit
 constexpr uint32_t SMALL_RASTER_BLOCKS = 64;      // raster blocks a small figure aims for
 static_assert(SMALL_ITEMS == SMALL_EVENTS, "op lanes and events share the scan shape");
+static_assert(SMALL_EVENTS <= SPLIT_LIST, "a small figure's edges are listed in one pass");
 
 struct SmallLds {
     union {
@@ -1671,6 +1711,7 @@ struct SmallLds {
     int32_t run_first[SMALL_RUNS];
     uint32_t run_close[SMALL_RUNS], run_path[SMALL_RUNS], run_start[SMALL_RUNS], run_end[SMALL_RUNS];
     uint32_t op_path[SMALL_OPS];
+    uint32_t op_item[SMALL_OPS];  // the op's first (op, lane) item
     ft_path_op ops[SMALL_OPS];
     PathDev path;
     PathState ps;
@@ -1735,11 +1776,15 @@ struct SmallOps {
 };
 
 // A small stroke's stroker data beside SmallLds: the pen-width marks of the
-// ops and their running maximum, the stored points and the sub-strokes.
+// ops and their running maximum, the stored points, the sub-strokes, and
+// the first ITEM_PTS (transformed) outline points of every outline item,
+// which its count computes and its emit then replays.
+constexpr uint32_t ITEM_PTS = 2;
 struct SmallStrokeLds {
     F4 spts[SMALL_EVENTS];
     uint32_t lastpw[SMALL_OPS];
     uint32_t sub_start[SMALL_RUNS], sub_event[SMALL_RUNS], sub_n[SMALL_RUNS], sub_joined[SMALL_RUNS];
+    SV2 item_pt[ITEM_PTS][SMALL_EVENTS];
 };
 
 // Fill events of a small figure go to the event and run arrays in LDS.
@@ -1758,17 +1803,56 @@ __device__ inline EventSink small_fill_sink(SmallLds &S) {
     return em;
 }
 
+// The op of (op, lane) item t: the last op whose first item is at or before
+// t (every op has at least one item).
+__device__ inline uint32_t small_item_op(const uint32_t *op_item, uint32_t n_ops, uint32_t t) {
+    uint32_t lo = 0, hi = n_ops;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (op_item[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+constexpr uint32_t SMALL_SPLIT = 5;  // a small figure's curves take 32 lanes when the items fit
+
 // Stages 1-2 in one block: the (op, lane) event and close counts, their block
-// scans, and the emit through `em`.  False (in every thread, nothing emitted)
-// when flattening fails or the events or closes exceed their limits.
+// scans, and the emit through `em`.  A curve's subdivision tree is split
+// SMALL_SPLIT levels down over 2^SMALL_SPLIT lanes (FLAT_SPLIT if those
+// items would not fit SMALL_ITEMS), every other op takes one lane: the
+// points and their order are the same for any split.  False (in every
+// thread, nothing emitted) when flattening fails or the events or closes
+// exceed their limits.
 __device__ bool small_flatten(SmallLds &S, uint32_t n_ops, uint32_t mode, const uint32_t *lastpw, EventSink em,
                               uint32_t max_runs, uint32_t &n_events, uint32_t &n_runs, uint32_t &bad) {
     const uint32_t tid = threadIdx.x;
-    const uint32_t n_items = n_ops * OP_LANES;
+    static_assert(SMALL_OPS <= SBLOCK, "one op per thread");
+    const uint32_t tag = tid < n_ops ? S.ops[tid].tag : TAG_CLOSE;
+    const bool curve = (tag == TAG_QUAD) | (tag == TAG_CUBIC);
+    const uint32_t n_curves = (uint32_t)__syncthreads_count(curve ? 1 : 0);
+    const uint32_t sp = n_curves * ((1u << SMALL_SPLIT) - 1u) + n_ops <= SMALL_ITEMS ? SMALL_SPLIT : FLAT_SPLIT;
+    if (tid < SMALL_OPS) S.op_item[tid] = tid < n_ops ? (curve ? 1u << sp : 1u) : 0u;
+    __syncthreads();
+    const uint32_t n_items = small_scan<false>(S.op_item, n_ops, S.w);
+    // item t's op, lane and the op's last lane
+    auto lane_of = [&](uint32_t t, uint32_t &i, uint32_t &s, uint32_t &last) {
+        i = small_item_op(S.op_item, n_ops, t);
+        const uint32_t first = S.op_item[i];
+        const uint32_t end = i + 1 < n_ops ? S.op_item[i + 1] : n_items;
+        s = t - first;
+        last = end - first - 1;
+    };
     uint32_t err = 0;
     for (uint32_t t = tid; t < SMALL_ITEMS; t += SBLOCK) {
         OpCount c = {0, 0, 0, 0, 0};
-        if (t < n_items) c = count_op(S.ops, S.op_path, &S.path, t >> FLAT_SPLIT, t & (OP_LANES - 1), mode, lastpw);
+        if (t < n_items) {
+            uint32_t i, s, last;
+            lane_of(t, i, s, last);
+            c = count_op(S.ops, S.op_path, &S.path, i, s, mode, lastpw, nullptr, 0, sp, last);
+        }
         err |= c.err;
         S.b.cnt_e[t] = c.ne;
         S.c.cnt_c[t] = c.nc;
@@ -1780,7 +1864,9 @@ __device__ bool small_flatten(SmallLds &S, uint32_t n_ops, uint32_t mode, const 
     if (S.flag | (n_events > SMALL_EVENTS) | (n_runs > max_runs)) return false;
     for (uint32_t t = tid; t < n_items; t += SBLOCK) {
         const unsigned long long off = ((unsigned long long)S.b.cnt_e[t] << 32) | S.c.cnt_c[t];
-        bad |= emit_op(S.ops, S.op_path, &S.path, t >> FLAT_SPLIT, t & (OP_LANES - 1), mode, lastpw, off, em);
+        uint32_t i, s, last;
+        lane_of(t, i, s, last);
+        bad |= emit_op(S.ops, S.op_path, &S.path, i, s, mode, lastpw, off, em, nullptr, 0, 0, sp, last);
     }
     __syncthreads();
     return true;
@@ -1852,9 +1938,18 @@ __device__ bool small_stroker(SmallLds &S, SmallStrokeLds &T, uint32_t n_sev, St
             bool fwd;
             uint32_t k;
             sub_of(t, sub, fwd, k);
+            // an item's points all come before its close (at most one)
             if (sub.n)
                 err |= stroke_item<DevMath>(
-                    sub, fwd, k, T.spts, sp.join, sp.miter, sp.tol_sq, [&](SV2) { ++ne; },
+                    sub, fwd, k, T.spts, sp.join, sp.miter, sp.tol_sq,
+                    [&](SV2 q) {
+                        if (ne < ITEM_PTS) {
+                            const WP w = xform(S.path, q.x, q.y, 0.0f);
+                            T.item_pt[ne][t].x = w.x;
+                            T.item_pt[ne][t].y = w.y;
+                        }
+                        ++ne;
+                    },
                     [&]() {
                         ++ne;
                         ++nc;
@@ -1868,15 +1963,28 @@ __device__ bool small_stroker(SmallLds &S, SmallStrokeLds &T, uint32_t n_sev, St
     n_events = small_scan<false>(S.b.cnt_e, n_items, S.w);
     n_runs = small_scan<false>(S.c.cnt_c, n_items, S.w2);
     if (S.flag | (n_events > SMALL_EVENTS) | (n_runs > SMALL_RUNS)) return false;
-    // the outline's events (k_st_emit), transformed again as Plotter::fill does
+    // the outline's events (k_st_emit), transformed again as Plotter::fill
+    // does: an item of at most ITEM_PTS points replays what its count kept
     const EventSink sink = small_fill_sink(S);
     for (uint32_t t = tid; t < n_items; t += SBLOCK) {
         EventSink em = sink;
         em.e = S.b.cnt_e[t];
         em.run = S.c.cnt_c[t];
         em.p = 0;
+        const uint32_t e1 = t + 1 < n_items ? S.b.cnt_e[t + 1] : n_events;
+        const uint32_t r1 = t + 1 < n_items ? S.c.cnt_c[t + 1] : n_runs;
+        const uint32_t n_cl = r1 - (uint32_t)em.run, n_pt = (e1 - (uint32_t)em.e) - n_cl;
         if (t + 1 == n_items) {
             em.close(true);
+        } else if (n_pt <= ITEM_PTS) {
+            for (uint32_t j = 0; j < n_pt; ++j) {
+                WP w;
+                w.x = T.item_pt[j][t].x;
+                w.y = T.item_pt[j][t].y;
+                w.w = 0.0f;
+                em.point(w);
+            }
+            if (n_cl) em.close(true);
         } else if (n_subs) {
             StrokeSub sub;
             bool fwd;
@@ -2954,7 +3062,7 @@ ft_status ws_small_fill(Workspace &ws, const ft_path_op *ops, size_t n_ops, cons
     const uint32_t g_lds = std::max(1u, ws.tuning.lds_cells / wp);
     const uint32_t g_par = std::max(1u, (d.height + SMALL_RASTER_BLOCKS - 1) / SMALL_RASTER_BLOCKS);
     const uint32_t G = std::min(std::min(d.height, DIRECT_ROWS), std::min(g_lds, g_par));
-    const size_t lds = raster_lds_bytes(d.fmt, (size_t)G * wp);
+    const size_t lds = raster_lds_bytes(d.fmt, (size_t)G * wp) + SPLIT_LIST_BYTES;
     if (lds > ws.lds_max) return FT_OK;
     P.flags |= PATH_TILE;
     P.n_bands = 0;

@@ -123,30 +123,32 @@ FT_HD uint32_t flatten_walk(const Curve &root, float tol_sq, uint32_t max_depth,
     }
 }
 
-// A curve is flattened by OP_LANES lanes.  Lane s takes the subtree
-// FLAT_SPLIT levels below the root whose left/right choices are the bits of
-// s, most significant first; a node that is already within tolerance above
-// that depth is a leaf of the lowest lane beneath it, and the other lanes
-// beneath it emit nothing.  Concatenated in lane order the lanes' points are
-// the recursion's points in order (plotter.rs:250-262,314-330).
+// A curve is flattened by 2^sp lanes (OP_LANES in the pipeline, sp =
+// FLAT_SPLIT).  Lane s takes the subtree sp levels below the root whose
+// left/right choices are the bits of s, most significant first; a node that
+// is already within tolerance above that depth is a leaf of the lowest lane
+// beneath it, and the other lanes beneath it emit nothing.  Concatenated in
+// lane order the lanes' points are the recursion's points in order
+// (plotter.rs:250-262,314-330), whatever sp (< FT_FLATTEN_MAX_DEPTH) is.
 constexpr uint32_t FLAT_SPLIT = 3;
 constexpr uint32_t OP_LANES = 1u << FLAT_SPLIT;
 
 template <class Curve, class Emit>
-FT_HD uint32_t flatten_share(const Curve &root, float tol_sq, uint32_t s, uint32_t &leaves, Emit &&emit) {
+FT_HD uint32_t flatten_share(const Curve &root, float tol_sq, uint32_t s, uint32_t &leaves, Emit &&emit,
+                             uint32_t sp = FLAT_SPLIT) {
     Curve cur = root;
-    for (uint32_t lv = 0; lv < FLAT_SPLIT; ++lv) {
+    for (uint32_t lv = 0; lv < sp; ++lv) {
         Curve L, R;
         if (split(cur, tol_sq, L, R)) {
-            if ((s & ((1u << (FLAT_SPLIT - lv)) - 1u)) == 0) {
+            if ((s & ((1u << (sp - lv)) - 1u)) == 0) {
                 ++leaves;
                 emit(end_of(cur));
             }
             return 0;
         }
-        cur = ((s >> (FLAT_SPLIT - 1 - lv)) & 1u) ? R : L;
+        cur = ((s >> (sp - 1 - lv)) & 1u) ? R : L;
     }
-    return flatten_walk(cur, tol_sq, FT_FLATTEN_MAX_DEPTH - FLAT_SPLIT, leaves, emit);
+    return flatten_walk(cur, tol_sq, FT_FLATTEN_MAX_DEPTH - sp, leaves, emit);
 }
 
 // Index of the op that last set the pen before op i (plotter.rs:128-130,
@@ -228,9 +230,12 @@ enum : uint32_t { OPC_CURVE = 1u, OPC_LAST_CLOSE = 2u };
 // so that emitting the lane is a copy unless it has more leaves.
 constexpr uint32_t LEAF_CACHE = 8;
 
+// sp / last_lane: a curve's lanes split its tree sp levels down, and the
+// op's last lane (which emits the path's final close) is last_lane; the
+// pipeline gives every op OP_LANES lanes.
 FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, const PathDev *paths, uint32_t i,
                               uint32_t s, uint32_t mode, const uint32_t *lastpw, I2 *cache = nullptr,
-                              size_t stride = 0) {
+                              size_t stride = 0, uint32_t sp = FLAT_SPLIT, uint32_t last_lane = OP_LANES - 1) {
     OpCount c = {0, 0, 0, 0, 0};
     const uint32_t p = op_path[i];
     if (p == NO_IDX) return c;
@@ -255,8 +260,8 @@ FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, co
                 keep[k * stride] = v;
             }
         };
-        const uint32_t rc = o.tag == TAG_QUAD ? flatten_share(quad_of(o, pen, sw, P), P.tol_sq, s, c.leaves, cnt)
-                                              : flatten_share(cubic_of(o, pen, sw, P), P.tol_sq, s, c.leaves, cnt);
+        const uint32_t rc = o.tag == TAG_QUAD ? flatten_share(quad_of(o, pen, sw, P), P.tol_sq, s, c.leaves, cnt, sp)
+                                              : flatten_share(cubic_of(o, pen, sw, P), P.tol_sq, s, c.leaves, cnt, sp);
         if (rc)
             c.err = rc;
         else
@@ -267,7 +272,7 @@ FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, co
     case TAG_PEN_WIDTH: break;
     default: c.err = lead ? FT_EINVAL : 0u; break;
     }
-    if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count && s == OP_LANES - 1) {
+    if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count && s == last_lane) {
         ++c.ne;
         ++c.nc;
         c.flags |= OPC_LAST_CLOSE;
@@ -356,7 +361,8 @@ struct EventSink {
 // LEAF_CACHE).
 FT_HD inline uint32_t emit_op(const ft_path_op *ops, const uint32_t *op_path, const PathDev *paths, uint32_t i,
                               uint32_t s, uint32_t mode, const uint32_t *lastpw, unsigned long long off,
-                              EventSink em, const I2 *cache = nullptr, uint32_t cached = 0, size_t stride = 0) {
+                              EventSink em, const I2 *cache = nullptr, uint32_t cached = 0, size_t stride = 0,
+                              uint32_t sp = FLAT_SPLIT, uint32_t last_lane = OP_LANES - 1) {
     const uint32_t p = op_path[i];
     if (p == NO_IDX) return 0;
     const PathDev &P = paths[p];
@@ -389,14 +395,14 @@ FT_HD inline uint32_t emit_op(const ft_path_op *ops, const uint32_t *op_path, co
         uint32_t leaves = 0;
         auto put = [&](WP q) { em.point(q); };
         if (o.tag == TAG_QUAD)
-            flatten_share(quad_of(o, pen, sw, P), P.tol_sq, s, leaves, put);
+            flatten_share(quad_of(o, pen, sw, P), P.tol_sq, s, leaves, put, sp);
         else
-            flatten_share(cubic_of(o, pen, sw, P), P.tol_sq, s, leaves, put);
+            flatten_share(cubic_of(o, pen, sw, P), P.tol_sq, s, leaves, put, sp);
         break;
     }
     default: break;
     }
-    if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count && s == OP_LANES - 1) em.close(true);
+    if (mode == MODE_FILL && i + 1 == P.op_begin + P.op_count && s == last_lane) em.close(true);
     return em.bad;
 }
 

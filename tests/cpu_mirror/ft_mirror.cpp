@@ -173,6 +173,64 @@ static void walk_row(const EdgeRec &E, int32_t sr, int32_t ed, int32_t W, int32_
 
 extern "C" {
 
+// Random quads and cubics flattened by 2^sp lanes (flatten_share, sp = 0..6)
+// against the whole tree walked from the root in one lane: the lanes' points
+// in lane order must be the same points in the same order, whatever sp.
+// Returns the number of mismatching (curve, sp) cases.
+int ftm_split_check(uint64_t seed, int iters) {
+    uint64_t z = seed;
+    auto next = [&]() {
+        z += 0x9E3779B97F4A7C15ull;
+        uint64_t t = z;
+        t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ull;
+        t = (t ^ (t >> 27)) * 0x94D049BB133111EBull;
+        return t ^ (t >> 31);
+    };
+    auto unif = [&](float lo, float hi) { return lo + (hi - lo) * (float)((next() >> 11) * (1.0 / 9007199254740992.0)); };
+    int bad = 0;
+    for (int it = 0; it < iters; ++it) {
+        const float ext = (next() & 3) ? 256.0f : 4096.0f;
+        const float tol = (next() & 1) ? 0.3f : 0.01f;
+        auto wp = [&]() {
+            WP p;
+            p.x = unif(-16.0f, ext);
+            p.y = unif(-16.0f, ext);
+            p.w = unif(0.5f, 8.0f);
+            return p;
+        };
+        const bool cubic = (next() & 1) != 0;
+        Cubic c;
+        c.a = wp();
+        c.b = wp();
+        c.c = wp();
+        c.d = wp();
+        Quad q;
+        q.a = c.a;
+        q.b = c.b;
+        q.c = c.c;
+        std::vector<WP> want, got;
+        uint32_t leaves = 0, rc = 0;
+        auto put_w = [&](WP p) { want.push_back(p); };
+        rc = cubic ? flatten_walk(c, tol * tol, FT_FLATTEN_MAX_DEPTH, leaves, put_w)
+                   : flatten_walk(q, tol * tol, FT_FLATTEN_MAX_DEPTH, leaves, put_w);
+        for (uint32_t sp = 0; sp <= 6; ++sp) {
+            got.clear();
+            uint32_t rs = 0;
+            for (uint32_t lane = 0; lane < (1u << sp); ++lane) {
+                uint32_t lv = 0;
+                auto put_g = [&](WP p) { got.push_back(p); };
+                rs |= cubic ? flatten_share(c, tol * tol, lane, lv, put_g, sp)
+                            : flatten_share(q, tol * tol, lane, lv, put_g, sp);
+            }
+            bool same = (rs != 0) == (rc != 0) && (rc != 0 || got.size() == want.size());
+            for (size_t k = 0; same && rc == 0 && k < got.size(); ++k)
+                same = got[k].x == want[k].x && got[k].y == want[k].y && got[k].w == want[k].w;
+            if (!same) ++bad;
+        }
+    }
+    return bad;
+}
+
 // Random edges and rows: cells from edge_row_cells (row_setup + row_walk) against
 // walk_row (step by step).  Returns the number of mismatching rows.
 int ftm_row_cells_check(uint64_t seed, int iters) {

@@ -32,6 +32,8 @@ def lib():
         L.ftm_run.restype = ctypes.c_int
         L.ftm_row_cells_check.argtypes = [ctypes.c_uint64, ctypes.c_int]
         L.ftm_row_cells_check.restype = ctypes.c_int
+        L.ftm_split_check.argtypes = [ctypes.c_uint64, ctypes.c_int]
+        L.ftm_split_check.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -90,3 +92,9 @@ def fill(width, height, fmt, rule, ops, clr=(255, 255, 255, 255), init=None, tra
 def row_cells_check(seed, iters):
     """Rows whose closed-form cells differ from the step-by-step walk."""
     return lib().ftm_row_cells_check(seed, iters)
+
+
+def split_check(seed, iters):
+    """Curves whose points, flattened by 2^sp lanes (sp = 0..6), differ from
+    the walk from the root."""
+    return lib().ftm_split_check(seed, iters)

@@ -60,3 +60,12 @@ def test_row_cells_closed_form_matches_pixel_walk():
     widths, including edges left of x = 0 and past the right edge."""
     from tests.cpu_mirror import mirror
     assert mirror.row_cells_check(12345, 200000) == 0
+
+
+def test_curve_lanes_any_split_depth():
+    """A curve flattened by 2^sp lanes (flatten_share, shared with the gfx950
+    kernels; the pipeline splits 3 levels down, small figures 5) gives the
+    same points in the same order as the whole subdivision tree walked from
+    the root, for sp = 0..6, coarse and fine tolerances and large curves."""
+    from tests.cpu_mirror import mirror
+    assert mirror.split_check(5, 3000) == 0
