@@ -371,6 +371,9 @@ ft_status ft_batch_launch(ft_batch *b) {
 ft_status ft_batch_sync(ft_batch *b) {
     if (!b) return FT_EINVAL;
     ft_status s = ft::ws_finish(b->ws, true);
+    // a run whose kept flags needed the ingest pass runs again with it
+    if (s == (ft_status)0x100 && !(b->ws.h_status->overflow & ~ft::OV_LONG_WALK))
+        return ft::ws_run(b->ws, ft::MODE_FILL, true);
     return s == (ft_status)0x100 ? FT_ENOMEM : s;
 }
 

@@ -49,6 +49,10 @@ struct Workspace {
     uint8_t *h_pin = nullptr;      // pinned staging of the layout's uploads
     size_t cap_pin = 0;
     bool op_ordered = false;
+    // the kept flags take the ingest pass (pushed marks and their running
+    // maximum) instead of their local form: set for the rest of a layout
+    // once a run's closing walk was too long
+    bool ingest_pass = false;
     PathDev *d_paths = nullptr;
     uint32_t *d_band_path = nullptr;
     uint32_t lds_cells = 0;  // dynamic LDS cells needed by the raster kernel

@@ -334,7 +334,38 @@ struct KeptSrc {
     }
 };
 
+// KeptLocalSrc: kept_event_local's flags, with no ingest pass before them;
+// pass 1 also stores the flags and each run's first point, and raises
+// OV_LONG_WALK when a closing walk is too long (the host then launches the
+// run again with IngestSrc and KeptSrc).
+struct KeptLocalSrc {
+    static constexpr uint32_t ITEMS = 4;
+    Limits L;
+    const uint32_t *meta;
+    const I2 *xy;
+    int32_t *run_first;
+    const uint32_t *run_close;
+    uint32_t *kept;
+    uint32_t bad;
+    DevStatus *st;
+    __device__ void bind(DevStatus *s) {
+        st = s;
+        L = run_limits(s, L.n_ops, L.n_paths, L.n_bands, L.cap_points);
+    }
+    __device__ uint32_t value(uint64_t i, bool first) {
+        uint32_t lw = 0, fr;
+        const uint32_t v = kept_event_local((uint32_t)i, L, meta, xy, run_close, KEPT_WALK, bad, lw, fr);
+        if (first) {
+            kept[i] = v;
+            if (fr != NO_IDX) run_first[fr] = (int32_t)i;
+            if (lw) atomicOr(&st->overflow, OV_LONG_WALK);
+        }
+        return v;
+    }
+};
+
 static_assert(IngestSrc::ITEMS == KeptSrc::ITEMS, "one tile size decides both ingest scans");
+static_assert(KeptLocalSrc::ITEMS == KeptSrc::ITEMS, "one tile size decides the kept scans");
 constexpr size_t TOP_FEW_PATHS = 64;  // batches below this many paths keep TOP_REPS partial top keys each
 static_assert((TOP_REPS & (TOP_REPS - 1)) == 0, "block residues by mask");
 
@@ -360,6 +391,23 @@ __global__ __launch_bounds__(BLOCK) void k_kept(Limits L0, const uint32_t *ev_me
     uint32_t bad = 0;
     for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK)
         ev_kept[e] = kept_event(e, L, ev_meta, ev_xy, run_first, run_close, ev_last, bad);
+    if (bad) raise_err(st, FT_EDEVICE);
+}
+
+// The kept flags' local form as a pass of its own (runs whose scans have
+// several tiles per block).
+__global__ __launch_bounds__(BLOCK) void k_kept_local(Limits L0, const uint32_t *ev_meta, const I2 *ev_xy,
+                                                      int32_t *run_first, const uint32_t *run_close,
+                                                      uint32_t *ev_kept, DevStatus *st) {
+    if (run_failed(st)) return;
+    const Limits L = run_limits(st, L0.n_ops, L0.n_paths, L0.n_bands, L0.cap_points);
+    uint32_t bad = 0, lw = 0;
+    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < L.n_events; e += gridDim.x * BLOCK) {
+        uint32_t fr;
+        ev_kept[e] = kept_event_local(e, L, ev_meta, ev_xy, run_close, KEPT_WALK, bad, lw, fr);
+        if (fr != NO_IDX) run_first[fr] = (int32_t)e;
+    }
+    if (lw) atomicOr(&st->overflow, OV_LONG_WALK);
     if (bad) raise_err(st, FT_EDEVICE);
 }
 
@@ -2472,6 +2520,7 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
                        size_t n_paths, uint32_t extra_flags) {
     if (n_ops >= 0x7FFFFFFFu || n_paths >= 0x7FFFFFFFu) return FT_EINVAL;
     graph_drop(ws);
+    ws.ingest_pass = false;
     // host staging kept in the workspace: the uploads below need not complete
     // before this returns (the run that follows is ordered after them); the
     // staging is only rewritten once the previous layout's uploads are done
@@ -2873,27 +2922,47 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
         // while every scan block holds one tile, else by a pass of their own.
         const bool in_scan =
             scan_grid_for<uint32_t, KeptSrc::ITEMS>(ws.cap_events, ws.scan_resident).tiles_per_block <= 1;
-        // pushed-point marks -> their running maximum (the last pushed point so far)
-        if (in_scan) {
-            const IngestSrc ing{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, 0u};
-            scan_launch_src<uint32_t, true>(s, ws.scan_resident, ing, ws.d_ev_last, &st->n_events, 0,
-                                            (uint32_t)ws.cap_events, lb + 8 * tiles, nullptr, st, true, true);
-        } else {
-            k_ingest<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, ws.d_ev_last,
-                                          st);
-            scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_ev_last, ws.d_ev_last, &st->n_events, 0,
-                                        (uint32_t)ws.cap_events, lb + 8 * tiles, nullptr, st, true, true);
-        }
-        // kept flags -> point indices
-        const KeptSrc kp{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_last, ws.d_ev_kept, 0u};
-        if (in_scan) {
-            scan_launch_src<uint32_t, false>(s, ws.scan_resident, kp, ws.d_ev_idx, &st->n_events, 0,
+        if (!ws.ingest_pass) {
+            // kept flags (each run's first point and closing walk found
+            // locally) -> point indices
+            const KeptLocalSrc kl{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_kept, 0u,
+                                  nullptr};
+            if (in_scan) {
+                scan_launch_src<uint32_t, false>(s, ws.scan_resident, kl, ws.d_ev_idx, &st->n_events, 0,
+                                                 (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true,
+                                                 false);
+            } else {
+                k_kept_local<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close,
+                                                  ws.d_ev_kept, st);
+                scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_ev_kept, ws.d_ev_idx, &st->n_events, 0,
                                              (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true, false);
+            }
         } else {
-            k_kept<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_last,
-                                        ws.d_ev_kept, st);
-            scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_ev_kept, ws.d_ev_idx, &st->n_events, 0,
-                                         (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true, false);
+            // pushed-point marks -> their running maximum (the last pushed
+            // point so far)
+            if (in_scan) {
+                const IngestSrc ing{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, 0u};
+                scan_launch_src<uint32_t, true>(s, ws.scan_resident, ing, ws.d_ev_last, &st->n_events, 0,
+                                                (uint32_t)ws.cap_events, lb + 8 * tiles, nullptr, st, true, true);
+            } else {
+                k_ingest<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path,
+                                              ws.d_ev_last, st);
+                scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_ev_last, ws.d_ev_last, &st->n_events, 0,
+                                            (uint32_t)ws.cap_events, lb + 8 * tiles, nullptr, st, true, true);
+            }
+            // kept flags -> point indices
+            const KeptSrc kp{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_last,
+                             ws.d_ev_kept, 0u};
+            if (in_scan) {
+                scan_launch_src<uint32_t, false>(s, ws.scan_resident, kp, ws.d_ev_idx, &st->n_events, 0,
+                                                 (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true,
+                                                 false);
+            } else {
+                k_kept<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close,
+                                            ws.d_ev_last, ws.d_ev_kept, st);
+                scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_ev_kept, ws.d_ev_idx, &st->n_events, 0,
+                                             (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true, false);
+            }
         }
         k_cap<<<grid_for(ws.n_paths * 64), BLOCK, 0, s>>>(L0, ws.d_paths, ws.d_op_off, ws.d_ev_meta, ws.d_ev_xy,
                                                      ws.d_ev_kept, ws.d_ev_idx, ws.d_run_first, ws.d_pstate, st);
@@ -3145,6 +3214,8 @@ ft_status ws_run(Workspace &ws, uint32_t mode, bool read_pstate) {
         if (s != (ft_status)0x100) return s;
         const DevStatus &h = *ws.h_status;
         graph_drop(ws);
+        if (h.overflow & OV_LONG_WALK) ws.ingest_pass = true;  // again, with the ingest pass
+        if (!(h.overflow & ~OV_LONG_WALK)) continue;
         if (ws.arena) return FT_ENOMEM;  // the arena's capacities are fixed
         if (h.overflow & 1u) {
             if ((s = ensure_event_capacity(ws, (size_t)h.need_events + h.need_events / 4 + 1024))) return s;

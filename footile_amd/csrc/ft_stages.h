@@ -487,6 +487,55 @@ FT_HD inline uint32_t kept_event(uint32_t e, const Limits &L, const uint32_t *me
     return (q.is_pt & q.pushed & rok & !pop) ? 1u : 0u;
 }
 
+// Points a run's closing walk visits at most in kept_event_local.
+constexpr uint32_t KEPT_WALK = 32;
+
+// kept_event without the ingest pass and its running maximum.  The runs are
+// consecutive event ranges, each ending in its close, so run r's first event
+// follows run r - 1's close; the run's first point is that event unless the
+// run is only its close.  The run's last pushed point is found by walking
+// back from the close over unpushed points (repeats of the point before),
+// visiting at most `walk` points: a longer walk sets `long_walk` and keeps
+// nothing (the caller then runs the ingest pass instead).  `first_of`
+// receives e's run when e is the run's first point (else NO_IDX), for
+// run_first.
+FT_HD inline uint32_t kept_event_local(uint32_t e, const Limits &L, const uint32_t *meta, const I2 *xy,
+                                       const uint32_t *run_close, uint32_t walk, uint32_t &bad, uint32_t &long_walk,
+                                       uint32_t &first_of) {
+    const Pushed q = pushed_event(e, L, meta, xy);
+    const uint32_t run = q.ok ? q.run : 0;
+    const uint32_t c = run_close[run];
+    const uint32_t pc = run > 0 ? run_close[run - 1] : NO_IDX;
+    const uint32_t fe = pc + 1;  // 0 for run 0
+    const bool cok = (c < L.n_events) & ((run == 0) | (pc < c));
+    const bool has = cok & (fe < c);  // the run has points
+    // the last pushed point: the highest k in [fe, c) with k == fe or a point
+    // unlike the one before
+    uint32_t rl = NO_IDX, lw = 0;
+    if (has) {
+        uint32_t k = c - 1;
+        for (uint32_t s = 0;; ++s) {
+            const bool pushed = (k == fe) || !i2_eq(xy[k], xy[k - 1]);
+            if (pushed) {
+                rl = k;
+                break;
+            }
+            if (s + 1 >= walk) {
+                lw = 1;
+                break;
+            }
+            --k;
+        }
+    }
+    const bool rok = has & (rl != NO_IDX);
+    const I2 f = xy[rok ? fe : e], l = xy[rok ? rl : e];
+    const bool pop = i2_eq(f, l) & (e == rl);
+    long_walk |= lw;
+    first_of = (q.is_pt & has & (e == fe)) ? run : NO_IDX;
+    bad |= ((q.is_pt & q.pushed & !rok & !lw) ? 1u : 0u) | (q.ok ? 0u : 1u) | ((q.ok & !cok) ? 1u : 0u);
+    return (q.is_pt & q.pushed & rok & !pop) ? 1u : 0u;
+}
+
 // Local point index of a kept point with global index idx, or NO_IDX when the
 // 65,535 cap drops it; `repop` tells whether the cap's re-pop removed 65534.
 FT_HD inline uint32_t capped_local(uint32_t idx, uint32_t base, uint32_t repop) {

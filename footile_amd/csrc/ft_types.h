@@ -68,9 +68,13 @@ struct PathState {
 
 constexpr uint32_t N_SCAN_SLOTS = 9;
 
+// DevStatus::overflow bit: a run's closing walk was too long for the kept
+// flags' local form; the run is launched again with the ingest pass.
+constexpr uint32_t OV_LONG_WALK = 0x20u;
+
 struct DevStatus {
     uint32_t err;        // first ft_status error raised by a kernel
-    uint32_t overflow;   // bit 0: events, bit 1: band pairs
+    uint32_t overflow;   // bit 0: events, bit 1: band pairs, OV_LONG_WALK
     uint32_t n_events, n_closes;
     uint32_t n_kept, n_pairs;
     uint32_t need_events, need_pairs;

@@ -409,6 +409,15 @@ int ftm_run(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *descs, size
         idx[e] = n_kept;
         n_kept += kept[e];
     }
+    // the device's usual form (no ingest pass: each run's first point and
+    // closing walk found locally) must agree wherever its walk is short
+    for (uint32_t e = 0; e < n_events; ++e) {
+        uint32_t lw = 0, first_of = NO_IDX, b2 = 0;
+        const uint32_t k = kept_event_local(e, L, ev_meta.data(), ev_xy.data(), run_close.data(), KEPT_WALK, b2, lw,
+                                            first_of);
+        if (!lw && (k != kept[e] || b2 != 0)) bad = 1;
+        if (first_of != NO_IDX && run_first[first_of] != (int32_t)e) bad = 1;
+    }
     L.cap_points = n_kept;
     // stage 4: cap, compaction, top point
     for (uint32_t p = 0; p < n_paths; ++p)

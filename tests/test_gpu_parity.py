@@ -553,6 +553,61 @@ def test_batch_of_many_paths_fig_rules(ft, oracle):
             assert (offered[k], stored[k]) == o.last_counts(), f"rule {rule} path {k}"
 
 
+def _repeat_tail_path(rng, n, repeats, to_first, w=250.0):
+    """One closed run of n random points whose last point repeats `repeats`
+    times before the close; with to_first the repeated point is the run's
+    first point, so the close pops it."""
+    pts = rng.uniform(0, w, size=(n, 2)).astype(np.float32)
+    last = tuple(pts[0]) if to_first else tuple(pts[-1])
+    ops = [(MOVE, tuple(pts[0]))] + [(LINE, tuple(p)) for p in pts[1:]]
+    ops += [(LINE, last)] * repeats + [(CLOSE, ())]
+    return ops
+
+
+def test_long_closing_walks_fall_back_to_the_ingest_pass(ft, oracle):
+    """The kept flags find each run's last pushed point by walking back from
+    its close over repeated points, at most 32 of them; a longer walk makes
+    the run start again with the ingest pass and its running maximum.  Runs
+    ending in 31, 32, 33, 40 and 50 repeats (of the last point, or of the
+    first, which the close then pops), beside ordinary runs: through the
+    Plotter (one scan pass for the kept flags) and in a batch of more than
+    2^20 events (a kept pass of its own), every raster and count against the
+    oracle."""
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    rng = np.random.default_rng(321)
+    cases = []
+    for repeats in (31, 32, 33, 40, 50):
+        for to_first in (False, True):
+            ops = _repeat_tail_path(rng, 2500, repeats, to_first)
+            ops += _repeat_tail_path(rng, 60, 3, False)  # an ordinary run after it
+            cases.append(ops_array(ops))
+    for k, a in enumerate(cases):
+        for rule in (0, 1):
+            g, o = compare_fill(ft, oracle, 256, 256, MATTE8, [("fill", rule, a, (255,))], tag=f"walk {k} rule {rule}")
+            assert g.last_counts() == o.last_counts(), k
+    # a batch past 2^20 events
+    paths = [ops_array(_repeat_tail_path(rng, 60000, (0, 40, 5, 33)[k % 4], k % 3 == 0, 64.0)) for k in range(18)]
+    ops = np.concatenate(paths)
+    w = h = 64
+    bufs, descs, at = [], [], 0
+    for k, a in enumerate(paths):
+        b = DeviceBuffer(w * h)
+        b.upload(np.full(w * h, k, dtype=np.uint8))
+        bufs.append(b)
+        descs.append(FillDesc(at, len(a), b.ptr, w, h, w, MATTE8, rule=k % 2, clr=(255,)))
+        at += len(a)
+    batch = Batch(ops, descs)
+    batch.run()
+    offered, stored, _ = batch.counts()
+    for k, a in enumerate(paths):
+        o = oracle.Plotter(w, h)
+        o.set_pixels(np.full(w * h, k, dtype=np.uint8))
+        assert o.fill(k % 2, a, (255,)) == 0
+        got = bufs[k].download()
+        assert np.array_equal(got, o.pixels()), f"path {k}: {int((got != o.pixels()).sum())} bytes"
+        assert (offered[k], stored[k]) == o.last_counts(), f"path {k}"
+
+
 def test_batch_band_pair_overflow_grows_and_reruns(ft, oracle):
     """Long near-vertical edges over one-row bands give far more (edge, band)
     pairs than the first run's list capacity: the run reports the overflow,
