@@ -750,13 +750,26 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
             }
         }
         __syncthreads();  // s_gc cleared before the walk's atomics
-        const float4 Cseg = ck[(uint64_t)seg * ck_stride + slot];
-        for (uint32_t jl = ns; jl-- > 0;) {
-            float4 Cb = Cseg;  // canvas before path j
-            for (uint32_t i = 0; i < jl; ++i) {
-                float d;
-                Cb = over(s_col[i], alpha_of(s_A[i][tid], d), Cb);
+        // the canvas under each of the segment's paths, composited once from
+        // the checkpoint (the same operations, in the same order, as a
+        // composite from the checkpoint for each path)
+        float4 Cbs[KSEG];
+        {
+            float4 C = ck[(uint64_t)seg * ck_stride + slot];
+#pragma unroll
+            for (uint32_t i = 0; i < (uint32_t)KSEG; ++i) {
+                Cbs[i] = C;
+                if (i + 1 < ns) {
+                    float d;
+                    C = over(s_col[i], alpha_of(s_A[i][tid], d), C);
+                }
             }
+        }
+        for (uint32_t jl = ns; jl-- > 0;) {
+            float4 Cb = Cbs[0];  // canvas before path j
+#pragma unroll
+            for (uint32_t k = 1; k < (uint32_t)KSEG; ++k)
+                if (k == jl) Cb = Cbs[k];
             const float A = s_A[jl][tid];
             float dadA;
             const float al = alpha_of(A, dadA);
