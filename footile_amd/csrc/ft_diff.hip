@@ -592,18 +592,35 @@ __device__ float4 forward_row(const DScene &S, const float4 *colours, uint32_t k
             __syncthreads();
             stage_pieces(S, k0 + ja, n, Y, e0, ne, sg, bad);
             __syncthreads();
-            for (uint32_t i = 0; i < ne; ++i) {
+            auto close_path = [&](uint32_t ev) {  // the path's last piece: composite it
+                const uint32_t t = ev & J_MASK, j = ja + t;
+                if (ck)
+                    for (; next_ck * KSEG <= j; ++next_ck) ck[(uint64_t)next_ck * ck_stride + slot] = C;
+                if (ck && cov) cov[(uint64_t)j * ck_stride + slot] = A;  // j < max_paths, slot < ck_stride
+                float dadA;
+                C = over(sg.col[t], alpha_of(A, dadA), C);
+                A = 0.0f;
+            };
+            // four pieces' loads at a time, their covers added in order
+            uint32_t i = 0;
+            for (; i + 4 <= ne; i += 4) {
+                const float4 p0 = sg.pc[i], p1 = sg.pc[i + 1], p2 = sg.pc[i + 2], p3 = sg.pc[i + 3];
+                const uint4 ev = *reinterpret_cast<const uint4 *>(sg.ev + i);
+                const float c0 = piece_cover(p0, Xf, xw0), c1 = piece_cover(p1, Xf, xw0);
+                const float c2 = piece_cover(p2, Xf, xw0), c3 = piece_cover(p3, Xf, xw0);
+                A += c0;
+                if (ev.x & LAST_BIT) close_path(ev.x);
+                A += c1;
+                if (ev.y & LAST_BIT) close_path(ev.y);
+                A += c2;
+                if (ev.z & LAST_BIT) close_path(ev.z);
+                A += c3;
+                if (ev.w & LAST_BIT) close_path(ev.w);
+            }
+            for (; i < ne; ++i) {
                 A += piece_cover(sg.pc[i], Xf, xw0);
                 const uint32_t ev = sg.ev[i];
-                if (ev & LAST_BIT) {
-                    const uint32_t t = ev & J_MASK, j = ja + t;
-                    if (ck)
-                        for (; next_ck * KSEG <= j; ++next_ck) ck[(uint64_t)next_ck * ck_stride + slot] = C;
-                    if (ck && cov) cov[(uint64_t)j * ck_stride + slot] = A;  // j < max_paths, slot < ck_stride
-                    float dadA;
-                    C = over(sg.col[t], alpha_of(A, dadA), C);
-                    A = 0.0f;
-                }
+                if (ev & LAST_BIT) close_path(ev);
             }
         }
     }
@@ -616,7 +633,7 @@ __device__ float4 forward_row(const DScene &S, const float4 *colours, uint32_t k
     __shared__ uint32_t s_cnt[PCHUNK], s_off[PCHUNK], s_pb[PCHUNK];    \
     __shared__ float4 s_col[PCHUNK];                                    \
     __shared__ float4 s_pc[ECAP];                                       \
-    __shared__ uint32_t s_ev[ECAP];                                     \
+    __shared__ __align__(16) uint32_t s_ev[ECAP];                       \
     __shared__ uint32_t s_w[DBLOCK / 64];                               \
     const Stage sg{s_cnt, s_off, s_pb, s_col, s_pc, s_ev, VI, PQ, s_w}
 
@@ -739,7 +756,24 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                 __syncthreads();
                 w_lo = e0;
                 w_n = ne;
-                for (uint32_t i = 0; i < ne; ++i) {
+                // four pieces' loads at a time, their covers added in order
+                uint32_t i = 0;
+                for (; i + 4 <= ne; i += 4) {
+                    const float4 p0 = s_pc[i], p1 = s_pc[i + 1], p2 = s_pc[i + 2], p3 = s_pc[i + 3];
+                    const uint4 ev = *reinterpret_cast<const uint4 *>(s_ev + i);
+                    const float c[4] = {piece_cover(p0, Xf, xw0), piece_cover(p1, Xf, xw0), piece_cover(p2, Xf, xw0),
+                                        piece_cover(p3, Xf, xw0)};
+                    const uint32_t e[4] = {ev.x, ev.y, ev.z, ev.w};
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        A += c[k];
+                        if (e[k] & LAST_BIT) {
+                            s_A[e[k] & J_MASK][tid] = A;
+                            A = 0.0f;
+                        }
+                    }
+                }
+                for (; i < ne; ++i) {
                     A += piece_cover(s_pc[i], Xf, xw0);
                     const uint32_t ev = s_ev[i];
                     if (ev & LAST_BIT) {
