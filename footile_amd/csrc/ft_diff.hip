@@ -715,13 +715,19 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     // the vertex gradients of the staged window's pieces [0, n) from their
     // sums (then cleared); every thread calls it
+    // (every thread owns staged pieces tid and tid + DBLOCK; both are loaded
+    // before either's gradient is formed)
+    static_assert(ECAP == 2 * DBLOCK, "two staged pieces per thread");
     auto flush_pieces = [&](uint32_t n) {
         __syncthreads();
-        for (uint32_t i = tid; i < n; i += DBLOCK) {
-            const float4 Sm = s_S[i];
-            if (((Sm.x != 0.0f) | (Sm.y != 0.0f) | (Sm.z != 0.0f)) && (s_ev[i] & LIVE_BIT)) {
-                const float4 pq = s_pq[i];
-                const uint2 vi = s_vi[i];
+        const uint32_t ia = tid, ib = tid + DBLOCK;
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 Sa = ia < n ? s_S[ia] : zero, Sb = ib < n ? s_S[ib] : zero;
+        const uint32_t ea = s_ev[ia], eb = s_ev[ib];
+        const float4 pqa = s_pq[ia], pqb = s_pq[ib];
+        const uint2 via = s_vi[ia], vib = s_vi[ib];
+        auto grad_of = [&](float4 Sm, uint32_t ev, float4 pq, uint2 vi) {
+            if (((Sm.x != 0.0f) | (Sm.y != 0.0f) | (Sm.z != 0.0f)) && (ev & LIVE_BIT)) {
                 float gr[4];
                 piece_grad(make_float2(pq.x, pq.y), make_float2(pq.z, pq.w), Y, Sm.x, Sm.y, Sm.z, gr);
                 atomicAdd(&gvtx[vi.x].x, gr[0]);
@@ -729,8 +735,11 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                 atomicAdd(&gvtx[vi.y].x, gr[2]);
                 atomicAdd(&gvtx[vi.y].y, gr[3]);
             }
-            s_S[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
+        };
+        grad_of(Sa, ea, pqa, via);
+        grad_of(Sb, eb, pqb, vib);
+        if (ia < n) s_S[ia] = zero;
+        if (ib < n) s_S[ib] = zero;
         __syncthreads();
     };
     for (uint32_t i = tid; i < ECAP; i += DBLOCK) s_S[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -845,15 +854,15 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                     w_lo = lo;
                     w_n = e_hi - lo;
                 }
-                for (uint32_t e = e_lo; wave_on && e < e_hi; ++e) {
-                    const uint32_t i = e - w_lo;
-                    const float4 pc = s_pc[i];
-                    if (!(s_ev[i] & LIVE_BIT)) continue;
+                // a piece's sums: S1 = Σ dA·M, S2 = Σ dA·∂M/∂u_a, S3 = Σ dA·∂M/∂u_b
+                // over the wavefront
+                auto piece_sums = [&](uint32_t i, float4 pc, uint32_t ev) {
+                    if (!(ev & LIVE_BIT)) return;
                     const float lo = fminf(pc.x, pc.y), hi = fmaxf(pc.x, pc.y);
-                    if (xw0 + 64.0f <= lo) continue;  // M = 0 on every pixel of the wavefront
-                    if (xw0 >= hi) {                 // M = 1, dM = 0
+                    if (xw0 + 64.0f <= lo) return;  // M = 0 on every pixel of the wavefront
+                    if (xw0 >= hi) {               // M = 1, dM = 0
                         if (lane == 0) atomicAdd(&s_S[i].x, Tw);
-                        continue;
+                        return;
                     }
                     float mua, mub;
                     const float M = mean_clamp(Xf + 1.0f - pc.x, Xf + 1.0f - pc.y, mua, mub);
@@ -864,7 +873,19 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
                         atomicAdd(&s_S[i].y, S2);
                         atomicAdd(&s_S[i].z, S3);
                     }
+                };
+                // four pieces' loads at a time
+                uint32_t e = e_lo;
+                for (; wave_on && e + 4 <= e_hi; e += 4) {
+                    const uint32_t i = e - w_lo;
+                    const float4 p0 = s_pc[i], p1 = s_pc[i + 1], p2 = s_pc[i + 2], p3 = s_pc[i + 3];
+                    const uint32_t v0 = s_ev[i], v1 = s_ev[i + 1], v2 = s_ev[i + 2], v3 = s_ev[i + 3];
+                    piece_sums(i, p0, v0);
+                    piece_sums(i + 1, p1, v1);
+                    piece_sums(i + 2, p2, v2);
+                    piece_sums(i + 3, p3, v3);
                 }
+                for (; wave_on && e < e_hi; ++e) piece_sums(e - w_lo, s_pc[e - w_lo], s_ev[e - w_lo]);
                 e_hi = e_lo;
             }
         }
