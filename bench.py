@@ -180,7 +180,7 @@ def _median_mad(samples):
 
 
 def bench_plain(ft, reps=50):
-    """C1 through the Plotter API, per call: median and MAD of `reps` calls
+    """The plain 256x256 fill and stroke through the Plotter API, per call: median and MAD of `reps` calls
     (GPU: host ops, upload, launch, one host wait; CPU: the oracle on one
     core, raster allocation inside each sample as fishyb.rs:35 does)."""
     from footile_amd import workloads
@@ -269,27 +269,27 @@ def _grad_errors(got, want):
 
 
 def bench_diff(steps, warmup, check=True):
-    """Differentiable configs: C4 (one 512^2 image, 256 paths x 8 cubics) and
-    C5 (64 images of 256^2, U[32,512] paths x U[4,16] cubics), fwd + loss +
+    """Differentiable workloads: one 512^2 image (256 paths x 8 cubics) and
+    64 images of 256^2 (U[32,512] paths x U[4,16] cubics), fwd + loss +
     bwd per step (one ft_diff_step call, coverage recomputed in the backward);
     gradients against the FP64 oracle.  The opt-in kept-coverage handle is
     timed beside each as its own row."""
     from footile_amd import diffscene as ds
     from footile_amd.diff import DiffRenderer
     out = {}
-    # C4
-    sc = ds.scene_c4()
+    # one 512x512 image
+    sc = ds.scene_one_image()
     r = DiffRenderer.for_scene(sc)
     b = _diff_buffers(r, sc)
     dt, loss = _diff_time(r, b, steps, warmup)
-    c4 = {"workload": "C4: 512x512 fp32 RGBA, 256 closed paths x 8 cubics (seed 3), L2 to a target rendered "
+    one = {"workload": "512x512 fp32 RGBA, 256 closed paths x 8 cubics (seed 3), L2 to a target rendered "
                       "from params + N(0,2px) / colours + N(0,0.05); one ft_diff_step, coverage recomputed in "
                       "the backward",
           "grad_steps_per_s": 1.0 / dt, "ms_per_step": dt * 1e3, "loss": loss}
     rk = DiffRenderer.for_scene(sc, keep_coverage=True)
     bk = _diff_buffers(rk, sc)
     dtk, _ = _diff_time(rk, bk, steps, warmup)
-    c4["kept_coverage_variant"] = {"grad_steps_per_s": 1.0 / dtk, "ms_per_step": dtk * 1e3,
+    one["kept_coverage_variant"] = {"grad_steps_per_s": 1.0 / dtk, "ms_per_step": dtk * 1e3,
                                    "extra_bytes": 4 * sc.n_paths * sc.width * sc.height}
     del rk, bk
     if check:
@@ -298,25 +298,25 @@ def bench_diff(steps, warmup, check=True):
         t0 = time.perf_counter()
         want_loss, want_gp, want_gc = pd.Scene(sc.n_images, sc.width, sc.height, sc.path_image,
                                                sc.path_cubics).backward(sc.points, sc.colours, target)
-        c4["fp64_oracle_s"] = time.perf_counter() - t0
+        one["fp64_oracle_s"] = time.perf_counter() - t0
         gp = b["gp"].download(dtype=np.float32).reshape(-1, 2)
         gc = b["gc"].download(dtype=np.float32).reshape(-1, 4)
-        c4["grad_err_points"] = _grad_errors(gp, want_gp)
-        c4["grad_err_colours"] = _grad_errors(gc, want_gc)
-        c4["loss_rel_err"] = abs(loss - want_loss) / abs(want_loss)
-    out["c4"] = c4
-    # C5: the batched step vs one step per image
-    sc = ds.scene_c5()
+        one["grad_err_points"] = _grad_errors(gp, want_gp)
+        one["grad_err_colours"] = _grad_errors(gc, want_gc)
+        one["loss_rel_err"] = abs(loss - want_loss) / abs(want_loss)
+    out["image_512"] = one
+    # 64 images of 256x256: the batched step vs one step per image
+    sc = ds.scene_many_images()
     r = DiffRenderer.for_scene(sc)
     b = _diff_buffers(r, sc)
     dt, loss = _diff_time(r, b, max(2, steps // 2), 1)
-    c5 = {"workload": "C5: 64 images 256x256 fp32 RGBA, U[32,512] paths x U[4,16] cubics (seed 4); all images "
+    many = {"workload": "64 images 256x256 fp32 RGBA, U[32,512] paths x U[4,16] cubics (seed 4); all images "
                       "in one ft_diff_step (one host wait), coverage recomputed in the backward",
           "grad_steps_per_s": 1.0 / dt, "ms_per_step": dt * 1e3, "paths": int(sc.n_paths)}
     rk = DiffRenderer.for_scene(sc, keep_coverage=True)
     bk = _diff_buffers(rk, sc)
     dtk, _ = _diff_time(rk, bk, max(2, steps // 2), 1)
-    c5["kept_coverage_variant"] = {"grad_steps_per_s": 1.0 / dtk, "ms_per_step": dtk * 1e3}
+    many["kept_coverage_variant"] = {"grad_steps_per_s": 1.0 / dtk, "ms_per_step": dtk * 1e3}
     del rk, bk
     # per-image launches of the same scenes
     per_image = 0.0
@@ -339,8 +339,8 @@ def bench_diff(steps, warmup, check=True):
         for rs, bs, *_ in subs:
             rs.step(bs["P"], bs["C"], bs["T"], bs["canvas"], bs["gcan"], bs["gp"], bs["gc"])
     per_image = (time.perf_counter() - t0) / reps
-    c5["per_image_step_ms"] = per_image * 1e3
-    c5["speedup_vs_per_image_step"] = per_image / dt
+    many["per_image_step_ms"] = per_image * 1e3
+    many["speedup_vs_per_image_step"] = per_image / dt
     if check:
         from concurrent.futures import ThreadPoolExecutor
         from oracle import pydiff as pd
@@ -358,11 +358,11 @@ def bench_diff(steps, warmup, check=True):
 
         with ThreadPoolExecutor(min(8, len(os.sched_getaffinity(0)))) as ex:
             res = list(ex.map(grads, checked))
-        c5["grad_err_points"] = _grad_errors(np.concatenate([g[0] for g in res]), np.concatenate([g[1] for g in res]))
-        c5["grad_err_colours"] = _grad_errors(np.concatenate([g[2] for g in res]),
+        many["grad_err_points"] = _grad_errors(np.concatenate([g[0] for g in res]), np.concatenate([g[1] for g in res]))
+        many["grad_err_colours"] = _grad_errors(np.concatenate([g[2] for g in res]),
                                               np.concatenate([g[3] for g in res]))
-        c5["grad_checked_images"] = checked
-    out["c5"] = c5
+        many["grad_checked_images"] = checked
+    out["images_64x256"] = many
     return out
 
 
@@ -484,7 +484,7 @@ def main():
         extra["plain_256"] = bench_plain(ft)
         print("[bench] random walk through the Plotter API", file=sys.stderr, flush=True)
         extra["random_walk_plotter_api"] = bench_plotter_random_walk(ft, ops)
-        print("[bench] differentiable C4 / C5", file=sys.stderr, flush=True)
+        print("[bench] differentiable fills", file=sys.stderr, flush=True)
         extra["diff"] = bench_diff(max(4, args.steps // 2), 2, check=not args.no_diff_check)
 
     cpu_step_px = 2 * W * (H - max(int(tops[0]), 0))

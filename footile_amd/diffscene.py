@@ -1,11 +1,11 @@
 """Seeded scenes for the differentiable fill (same SplitMix64 streams as the
 rasteriser workloads).
 
-  scene_c4   one 512x512 image, 256 closed paths of 8 cubics, control points
-             uniform in the canvas, premultiplied RGBA colours with alpha in
-             [0.3, 0.9].                                               seed 3
-  scene_c5   64 images of 256x256, each with U[32, 512] paths of U[4, 16]
-             cubics, same colour law.                                 seed 4
+  scene_one_image     one 512x512 image, 256 closed paths of 8 cubics, control
+                      points uniform in the canvas, premultiplied RGBA colours
+                      with alpha in [0.3, 0.9].                         seed 3
+  scene_many_images   64 images of 256x256, each with U[32, 512] paths of
+                      U[4, 16] cubics, same colour law.                 seed 4
 
 `perturbed` returns the parameters a target is rendered from: control points
 + N(0, 2 px), colours + N(0, 0.05).
@@ -38,7 +38,7 @@ def _colours(rng, n):
     return np.stack(rgb + [a], axis=1).astype(np.float32)
 
 
-def scene_c4(seed=3, width=512, height=512, n_paths=256, cubics=8):
+def scene_one_image(seed=3, width=512, height=512, n_paths=256, cubics=8):
     rng = SplitMix64(seed)
     path_image = np.zeros(n_paths, np.uint32)
     path_cubics = np.full(n_paths, cubics, np.uint32)
@@ -49,7 +49,7 @@ def scene_c4(seed=3, width=512, height=512, n_paths=256, cubics=8):
     return DiffScene(1, width, height, path_image, path_cubics, points, _colours(rng, n_paths))
 
 
-def scene_c5(seed=4, n_images=64, size=256, min_paths=32, max_paths=512, min_cubics=4, max_cubics=16):
+def scene_many_images(seed=4, n_images=64, size=256, min_paths=32, max_paths=512, min_cubics=4, max_cubics=16):
     rng = SplitMix64(seed)
     image, cubics = [], []
     for i in range(n_images):

@@ -162,9 +162,9 @@ def test_vjp_matches_central_differences(oracle):
 
 
 def test_scene_generators_are_deterministic():
-    a = ds.scene_c4(seed=3, width=64, height=48, n_paths=8)
-    b = ds.scene_c4(seed=3, width=64, height=48, n_paths=8)
+    a = ds.scene_one_image(seed=3, width=64, height=48, n_paths=8)
+    b = ds.scene_one_image(seed=3, width=64, height=48, n_paths=8)
     assert np.array_equal(a.points, b.points) and np.array_equal(a.colours, b.colours)
-    c = ds.scene_c5(seed=4, n_images=3, size=32, min_paths=4, max_paths=20)
+    c = ds.scene_many_images(seed=4, n_images=3, size=32, min_paths=4, max_paths=20)
     assert c.n_images == 3 and c.points.shape == (int(np.sum(1 + 3 * c.path_cubics)), 2)
     assert np.all(np.diff(c.path_image.astype(np.int64)) >= 0)

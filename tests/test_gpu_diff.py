@@ -86,11 +86,11 @@ def test_vjp_matches_fp64(oracle):
 
 
 def test_c4_shape_reduced(oracle):
-    """The C4 scene law (8-cubic paths across the canvas) at 128x128, 32 paths."""
+    """The one-image scene law (8-cubic paths across the canvas) at 128x128, 32 paths."""
     from footile_amd import diffscene as ds
     from footile_amd.diff import device_array
     from footile_amd.plotter import DeviceBuffer
-    sc = ds.scene_c4(seed=3, width=128, height=128, n_paths=32)
+    sc = ds.scene_one_image(seed=3, width=128, height=128, n_paths=32)
     tp, tc = ds.perturbed(sc)
     _, target = _oracle_scene(sc).forward(tp, tc)
     target = target.astype(np.float32)
@@ -203,7 +203,7 @@ def test_c4_full_size_vs_fp64(oracle):
     from footile_amd import diffscene as ds
     from footile_amd.diff import device_array
     from footile_amd.plotter import DeviceBuffer
-    sc = ds.scene_c4()
+    sc = ds.scene_one_image()
     r, P, C = _gpu(sc)
     tp, tc = ds.perturbed(sc)
     T = r.new_canvas()
@@ -235,7 +235,7 @@ def test_c5_full_layout_vs_fp64(oracle):
     from footile_amd.diff import device_array
     from footile_amd.plotter import DeviceBuffer
     from oracle import pydiff as pd
-    sc = ds.scene_c5()
+    sc = ds.scene_many_images()
     r, P, C = _gpu(sc)
     tp, tc = ds.perturbed(sc)
     T = r.new_canvas()
