@@ -700,6 +700,13 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
     __shared__ float s_A[KSEG][DBLOCK];  // coverage of the segment's paths; each thread uses its own column
     __shared__ float4 s_S[ECAP];         // per staged piece: S1, S2, S3
     __shared__ float s_gc[KSEG][4];      // colour gradients of the segment's paths
+    // coverage recompute, per staged window: each wavefront's sum of the
+    // pieces wholly left of it per path, and its list of the pieces that
+    // touch its columns
+    constexpr int NW = DBLOCK / 64;
+    __shared__ float s_Lw[KSEG][NW];
+    __shared__ uint16_t s_tl[NW][ECAP];
+    __shared__ uint32_t s_tn[NW];
     if (run_failed(st)) return;
     const int tid = threadIdx.x, lane = tid & 63;
     const int X = (int)(blockIdx.x * DBLOCK + tid), Y = (int)blockIdx.y;
@@ -757,38 +764,58 @@ __global__ __launch_bounds__(DBLOCK) void k_backward(DScene S, const float4 *col
             for (uint32_t j = 0; j < ns; ++j)
                 if (s_cnt[j]) s_A[j][tid] = cov[(uint64_t)(j0 + j) * ck_stride + slot];
         } else {
-            float A = 0.0f;
+            const int wid = tid >> 6;
+            const float xb0 = (float)(blockIdx.x * DBLOCK);
             for (uint32_t e0 = 0; e0 < total; e0 += ECAP) {
                 const uint32_t ne = total - e0 < ECAP ? total - e0 : ECAP;
                 __syncthreads();
                 stage_pieces(S, k0 + j0, ns, Y, e0, ne, sg, bad);
+                if (tid < KSEG * NW) s_Lw[tid / NW][tid % NW] = 0.0f;
+                if (tid < NW) s_tn[tid] = 0;
                 __syncthreads();
                 w_lo = e0;
                 w_n = ne;
-                // four pieces' loads at a time, their covers added in order
-                uint32_t i = 0;
-                for (; i + 4 <= ne; i += 4) {
-                    const float4 p0 = s_pc[i], p1 = s_pc[i + 1], p2 = s_pc[i + 2], p3 = s_pc[i + 3];
-                    const uint4 ev = *reinterpret_cast<const uint4 *>(s_ev + i);
-                    const float c[4] = {piece_cover(p0, Xf, xw0), piece_cover(p1, Xf, xw0), piece_cover(p2, Xf, xw0),
-                                        piece_cover(p3, Xf, xw0)};
-                    const uint32_t e[4] = {ev.x, ev.y, ev.z, ev.w};
+                // each staged piece against each wavefront's 64 columns (the
+                // tests of piece_cover): wholly right adds nothing, wholly
+                // left its full sign * height, else it goes on the list
+                for (uint32_t i = tid; i < ne; i += DBLOCK) {
+                    const float4 pc = s_pc[i];
+                    const uint32_t ev = s_ev[i];
+                    if (!(ev & LIVE_BIT)) continue;  // no coverage
+                    const float lo = fminf(pc.x, pc.y), hi = fmaxf(pc.x, pc.y);
+                    const uint32_t j = ev & J_MASK;
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        A += c[k];
-                        if (e[k] & LAST_BIT) {
-                            s_A[e[k] & J_MASK][tid] = A;
-                            A = 0.0f;
+                    for (int w = 0; w < NW; ++w) {
+                        const float x0 = xb0 + 64.0f * (float)w;
+                        if (x0 + 64.0f <= lo) continue;
+                        if (x0 >= hi) {
+                            atomicAdd(&s_Lw[j][w], pc.z);
+                        } else {
+                            const uint32_t at = atomicAdd(&s_tn[w], 1u);
+                            s_tl[w][at] = (uint16_t)i;
                         }
                     }
                 }
-                for (; i < ne; ++i) {
-                    A += piece_cover(s_pc[i], Xf, xw0);
-                    const uint32_t ev = s_ev[i];
-                    if (ev & LAST_BIT) {
-                        s_A[ev & J_MASK][tid] = A;
-                        A = 0.0f;
-                    }
+                __syncthreads();
+                // the window's coverage of each path at this lane's pixel
+                for (uint32_t j = 0; j < ns; ++j) s_A[j][tid] += s_Lw[j][wid];
+                const uint32_t nt = s_tn[wid];
+                uint32_t k = 0;
+                for (; k + 2 <= nt; k += 2) {
+                    const uint32_t ia = s_tl[wid][k], ib = s_tl[wid][k + 1];
+                    const float4 pa = s_pc[ia], pb = s_pc[ib];
+                    const uint32_t ja = s_ev[ia] & J_MASK, jb = s_ev[ib] & J_MASK;
+                    float da, db;
+                    const float ca = pa.z * mean_clamp(Xf + 1.0f - pa.x, Xf + 1.0f - pa.y, da, db);
+                    const float cb = pb.z * mean_clamp(Xf + 1.0f - pb.x, Xf + 1.0f - pb.y, da, db);
+                    s_A[ja][tid] += ca;
+                    s_A[jb][tid] += cb;
+                }
+                if (k < nt) {
+                    const uint32_t i = s_tl[wid][k];
+                    const float4 pc = s_pc[i];
+                    float da, db;
+                    s_A[s_ev[i] & J_MASK][tid] += pc.z * mean_clamp(Xf + 1.0f - pc.x, Xf + 1.0f - pc.y, da, db);
                 }
             }
         }
