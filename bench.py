@@ -8,8 +8,13 @@ device-resident ops of two fills):
     4096x4096 Rgba8p raster A and EvenOdd into a 4096x4096 Rgba8p raster B,
     colour Rgba8p(200, 120, 40, 255).
 value = filled Mpixels/s = sum over the two fills of W * (H - max(top_row, 0))
-per step / wall time per step.  The first step's rasters are compared byte for
-byte with the CPU oracle before timing ("parity": true).
+per step / wall time per step, with the launches pipelined (the batch's two
+workspaces take turns, so that a launch's flattening-to-binning front end
+overlaps the previous launch's raster; the rasters keep launch order).  The
+first step's rasters are compared byte for byte with the CPU oracle before
+timing, and three pipelined launches with three launched one at a time
+("parity": true when both match).  The same steps launched one at a time are
+reported as extra.ms_per_step_one_at_a_time.
 
 Also reported under "extra": the batched atlas (16,384 tiles in a 16384^2
 Matte8 atlas, one launch) and the plain 256x256 fill/stroke latency through
@@ -26,6 +31,10 @@ import time
 import numpy as np
 
 TIMING_EVERY = 4
+# Workspaces taking turns in pipelined launches: each has its own front-end
+# stream beside the batch's raster stream, and three plus that one are the
+# four hardware queues HIP gives a process by default (depth 4 measured slower).
+PIPE_DEPTH = 3
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -118,7 +127,7 @@ def bench_atlas(ft, steps, warmup, tuning=None, cpu_seconds=4.0):
         batch.launch()
     batch.sync()
     eager_total_ms, raster_ms, runs = batch.timing()
-    # the step: one replay of the captured launch graph
+    # one replay of the captured launch graph per launch
     batch.set_graph(True)
     for _ in range(warmup + 1):
         batch.launch()
@@ -128,9 +137,21 @@ def bench_atlas(ft, steps, warmup, tuning=None, cpu_seconds=4.0):
     for _ in range(steps):
         batch.launch()
     batch.sync()
-    dt = (time.perf_counter() - t0) / steps
+    dt_graph = (time.perf_counter() - t0) / steps
     total_ms, _, graph_runs = batch.timing()
     batch.set_graph(False)
+    # the step: pipelined eager launches (later launches' front ends overlap
+    # an earlier launch's raster)
+    batch.set_pipeline(PIPE_DEPTH)
+    for _ in range(warmup + 1):
+        batch.launch()
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.launch()
+    batch.sync()
+    dt = (time.perf_counter() - t0) / steps
+    batch.set_pipeline(1)
     tile_px = int((tiles[:, 4] ** 2).sum())
     bytes_per = tile_px + op_bytes(ops)
     offered, stored, edges = batch.counts()
@@ -150,13 +171,14 @@ def bench_atlas(ft, steps, warmup, tuning=None, cpu_seconds=4.0):
     raster_avg = raster_ms / max(runs, 1)
     return {
         "workload": "atlas: 16,384 paths, lognormal(3.5,1) segments, 60/30/10 cubic/quad/line, "
-                    "tiles 32..256 in a 16384^2 Matte8 atlas, one graph launch, autotuned raster geometry",
+                    "tiles 32..256 in a 16384^2 Matte8 atlas, one launch, autotuned raster geometry",
         "paths_per_s": len(tiles) / dt,
         "mpix_per_s": tile_px / dt / 1e6,
         "ms_per_step": dt * 1e3,
-        "device_ms_per_step": total_ms / max(graph_runs, 1),
+        "launch": f"pipelined: {PIPE_DEPTH} workspaces take turns",
+        "graph_ms_per_step": dt_graph * 1e3,
+        "graph_device_ms_per_step": total_ms / max(graph_runs, 1),
         "eager_device_ms_per_step": eager_total_ms / max(runs, 1),
-        "launch": "hipGraph replay of the whole launch sequence",
         "tuning": {"band_rows_min": tuned["band_rows_min"], "lds_cells": tuned["lds_cells"],
                    "tile_groups": tuned.get("tile_groups", 0), "candidates": tuned["tried"],
                    "tune_s": round(tuned.get("tune_s", 0.0), 3)},
@@ -440,13 +462,14 @@ def main():
     if not parity:
         print("[bench] PARITY FAILURE: GPU rasters differ from the CPU oracle", file=sys.stderr)
 
-    print(f"[bench] warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
+    print(f"[bench] warmup {args.warmup}, timed {args.steps} one launch at a time", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         batch.launch()
         batch.sync()
-    # Event timing of the raster kernel on every TIMING_EVERY-th step of the
-    # timed loop: timed launches carry events in their first and raster
-    # dispatches, which costs a few microseconds of queue time per launch.
+    # Launches one at a time (each launch's front end waits for the previous
+    # raster), with event timing of the raster kernel on every
+    # TIMING_EVERY-th step: timed launches carry events in their first and
+    # raster dispatches, which costs a few microseconds of queue time.
     batch.timing()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -456,9 +479,41 @@ def main():
     t1 = time.perf_counter()
     batch.set_timing(False)
     total_ms, raster_ms, runs = batch.timing()
-    sec_per_step = (t1 - t0) / args.steps
+    sec_per_step_serial = (t1 - t0) / args.steps
     raster_avg_ms = raster_ms / max(runs, 1)
     achieved = bytes_per_step / (raster_avg_ms * 1e-3) / 1e9
+
+    # The step: pipelined launches (two workspaces take turns; a launch's
+    # front end overlaps the previous launch's raster, rasters keep launch
+    # order).  First their bytes against the same launches one at a time.
+    def three_launches(pipelined):
+        A.zero()
+        B.zero()
+        for _ in range(3):
+            batch.launch()
+            if not pipelined:
+                batch.sync()
+        batch.sync()
+        return A.download(), B.download()
+
+    serial3 = three_launches(False)
+    batch.set_pipeline(PIPE_DEPTH)
+    piped3 = three_launches(True)
+    pipe_ok = bool(np.array_equal(serial3[0], piped3[0]) and np.array_equal(serial3[1], piped3[1]))
+    del serial3, piped3
+    if not pipe_ok:
+        print("[bench] PARITY FAILURE: pipelined launches differ from serial ones", file=sys.stderr)
+    print(f"[bench] warmup {args.warmup}, timed {args.steps} pipelined", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        batch.launch()
+    batch.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        batch.launch()
+    batch.sync()
+    t1 = time.perf_counter()
+    batch.set_pipeline(False)
+    sec_per_step = (t1 - t0) / args.steps
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_random_walk.json")
@@ -468,6 +523,10 @@ def main():
 
     extra = {
         "paths_per_s": 2 / sec_per_step,
+        "launch": f"pipelined: {PIPE_DEPTH} workspaces take turns, later launches' front ends overlap an "
+                  "earlier launch's raster",
+        "pipelined_matches_serial": pipe_ok,
+        "ms_per_step_one_at_a_time": sec_per_step_serial * 1e3,
         "device_ms_per_step": total_ms / max(runs, 1),
         "raster_kernel_ms": raster_avg_ms,
         "points_offered_per_fill": int(offered[0]),
@@ -505,7 +564,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "random_walk 4096x4096 Rgba8p, 10,000 cubics (seed 1), NonZero + EvenOdd",
                    "width": W, "height": H, "n_cubics": 10000, "fills_per_step": 2},
-        "parity": parity,
+        "parity": parity and pipe_ok,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_raster", "algorithmic_bytes_per_launch": bytes_per_step},

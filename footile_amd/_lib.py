@@ -64,6 +64,7 @@ def _declare(L):
         "ft_batch_set_tuning": ([vp, vp], i),
         "ft_batch_get_tuning": ([vp, vp], i),
         "ft_batch_set_graph": ([vp, i], i),
+        "ft_batch_set_pipeline": ([vp, i], i),
         "ft_batch_set_timing": ([vp, i], i),
         "ft_batch_timing": ([vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), pu64], i),
         "ft_batch_run_noalloc": ([vp, sz, vp, sz, vp, vp, vp], i),

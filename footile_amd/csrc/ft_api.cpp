@@ -35,10 +35,34 @@ struct ft_plotter {
 
 struct ft_batch {
     Workspace ws;
+    // Pipelined launches (ft_batch_set_pipeline): ws and depth - 1 twins
+    // hold the same layout and take turns; every raster runs on ws.stream,
+    // in launch order.
+    static constexpr int PIPE_MAX = 4;
+    Workspace twin[PIPE_MAX - 1];
+    int depth = 1;
+    int turn = 0;                  // the slot of the next launch (0: ws)
+    bool live[PIPE_MAX] = {};      // launched since the last sync
+    Workspace &slot(int k) { return k ? twin[k - 1] : ws; }
     size_t n_paths = 0;
     std::vector<ft_path_op> ops;  // host copies, for a re-layout after a tuning change
     std::vector<ft_fill_desc> descs;
 };
+
+// The twins of a pipelined batch take over the main workspace's capacities
+// and kept-flag form after the main one grew or changed them.
+static ft_status twin_follow(ft_batch *b) {
+    for (int k = 1; k < b->depth; ++k)
+        if (ft_status s = ft::ws_reserve_like(b->slot(k), b->ws)) return s;
+    return FT_OK;
+}
+
+// Drop the twins (the caller has drained the batch's stream).
+static void twins_free(ft_batch *b) {
+    for (int k = 1; k < b->depth; ++k) ft::ws_free(b->slot(k));
+    ft::ws_pipe_detach(b->ws);
+    b->depth = 1;
+}
 
 namespace {
 
@@ -334,12 +358,17 @@ ft_status ft_batch_set_tuning(ft_batch *b, const ft_tuning *t) {
     tu.tile_groups = t->tile_groups;
     const ft::Tuning old = b->ws.tuning;
     b->ws.tuning = tu;
-    const ft_status s = ft::ws_set_batch(b->ws, b->ops.data(), b->ops.size(), b->descs.data(), b->descs.size());
+    ft_status s = ft::ws_set_batch(b->ws, b->ops.data(), b->ops.size(), b->descs.data(), b->descs.size());
     if (s) {
         b->ws.tuning = old;
         (void)ft::ws_set_batch(b->ws, b->ops.data(), b->ops.size(), b->descs.data(), b->descs.size());
+        return s;
     }
-    return s;
+    for (int k = 1; k < b->depth && !s; ++k) {
+        b->slot(k).tuning = tu;
+        s = ft::ws_set_batch(b->slot(k), b->ops.data(), b->ops.size(), b->descs.data(), b->descs.size());
+    }
+    return s ? s : twin_follow(b);
 }
 
 ft_status ft_batch_get_tuning(const ft_batch *b, ft_tuning *t) {
@@ -352,34 +381,90 @@ ft_status ft_batch_get_tuning(const ft_batch *b, ft_tuning *t) {
 }
 
 ft_status ft_batch_set_graph(ft_batch *b, int on) {
-    if (!b) return FT_EINVAL;
+    if (!b || (on && b->depth > 1)) return FT_EINVAL;
     b->ws.graph = on != 0;
     return FT_OK;
 }
 
 void ft_batch_free(ft_batch *b) {
     if (!b) return;
+    if (b->depth > 1) {
+        (void)hipStreamSynchronize(b->ws.stream);
+        twins_free(b);  // (their raster stream is ws's)
+    }
     ft::ws_free(b->ws);
     delete b;
 }
 
+ft_status ft_batch_set_pipeline(ft_batch *b, int depth) {
+    if (!b || depth < 0 || depth > ft_batch::PIPE_MAX || (depth > 1 && b->ws.graph)) return FT_EINVAL;
+    if (depth < 1) depth = 1;
+    if (depth == b->depth) return FT_OK;
+    // every launch so far ends on ws.stream (its front end precedes its raster)
+    if (hipStreamSynchronize(b->ws.stream) != hipSuccess) return FT_EDEVICE;
+    for (bool &l : b->live) l = false;
+    b->turn = 0;
+    twins_free(b);
+    if (depth == 1) return FT_OK;
+    ft_status s = ft::ws_pipe_attach(b->ws, b->ws.stream);
+    for (int k = 1; k < depth && !s; ++k) {
+        ft::Workspace &w = b->slot(k);
+        b->depth = k + 1;  // (twins_free frees this one on failure)
+        s = ft::ws_init(w);
+        if (!s) s = ft::ws_pipe_attach(w, b->ws.stream);
+        if (!s) {
+            w.tuning = b->ws.tuning;
+            w.timing = b->ws.timing;
+            s = ft::ws_set_batch(w, b->ops.data(), b->ops.size(), b->descs.data(), b->descs.size());
+        }
+        if (!s) s = ft::ws_reserve_like(w, b->ws);
+    }
+    if (s) {
+        (void)hipStreamSynchronize(b->ws.stream);
+        twins_free(b);
+    }
+    return s;
+}
+
 ft_status ft_batch_launch(ft_batch *b) {
     if (!b) return FT_EINVAL;
-    return ft::ws_launch(b->ws, ft::MODE_FILL);
+    const int k = b->turn;
+    b->turn = (k + 1) % b->depth;
+    const ft_status s = ft::ws_launch(b->slot(k), ft::MODE_FILL);
+    if (!s) b->live[k] = true;
+    return s;
 }
 
 ft_status ft_batch_sync(ft_batch *b) {
     if (!b) return FT_EINVAL;
-    ft_status s = ft::ws_finish(b->ws, true);
+    // (a pipelined batch: every workspace's raster and status copy ends on
+    // ws.stream, so the first finish waits for all of them)
+    ft_status s = FT_OK;
+    uint32_t ov = 0;
+    for (int k = 0; k < b->depth; ++k) {
+        if (!(k == 0 || b->live[k])) continue;
+        ft::Workspace &w = b->slot(k);
+        const ft_status sk = ft::ws_finish(w, true);
+        if (sk == (ft_status)0x100)
+            ov |= w.h_status->overflow;
+        else if (sk && !s)
+            s = sk;
+    }
+    for (bool &l : b->live) l = false;
+    b->turn = 0;
+    if (s) return s;
     // a run whose kept flags needed the ingest pass runs again with it
-    if (s == (ft_status)0x100 && !(b->ws.h_status->overflow & ~ft::OV_LONG_WALK))
-        return ft::ws_run(b->ws, ft::MODE_FILL, true);
-    return s == (ft_status)0x100 ? FT_ENOMEM : s;
+    if (ov && !(ov & ~ft::OV_LONG_WALK)) {
+        s = ft::ws_run(b->ws, ft::MODE_FILL, true);
+        return s ? s : twin_follow(b);
+    }
+    return ov ? FT_ENOMEM : FT_OK;
 }
 
 ft_status ft_batch_run(ft_batch *b) {
     if (!b) return FT_EINVAL;
-    return ft::ws_run(b->ws, ft::MODE_FILL, true);
+    const ft_status s = ft::ws_run(b->ws, ft::MODE_FILL, true);
+    return s ? s : twin_follow(b);
 }
 
 ft_status ft_batch_counts(ft_batch *b, uint64_t *offered, uint64_t *stored, uint64_t *edges,
@@ -414,17 +499,25 @@ ft_status ft_batch_stream(ft_batch *b, void **stream) {
 
 ft_status ft_batch_set_timing(ft_batch *b, int on) {
     if (!b) return FT_EINVAL;
-    b->ws.timing = on != 0;
+    for (int k = 0; k < ft_batch::PIPE_MAX; ++k) b->slot(k).timing = on != 0;
     return FT_OK;
 }
 
 ft_status ft_batch_timing(ft_batch *b, double *total_ms, double *raster_ms, uint64_t *runs) {
     if (!b) return FT_EINVAL;
-    if (total_ms) *total_ms = b->ws.t_total_ms;
-    if (raster_ms) *raster_ms = b->ws.t_raster_ms;
-    if (runs) *runs = b->ws.n_timed;
-    b->ws.t_total_ms = b->ws.t_raster_ms = 0.0;
-    b->ws.n_timed = 0;
+    double total = 0.0, raster = 0.0;
+    uint64_t n = 0;
+    for (int k = 0; k < ft_batch::PIPE_MAX; ++k) {
+        ft::Workspace &w = b->slot(k);
+        total += w.t_total_ms;
+        raster += w.t_raster_ms;
+        n += w.n_timed;
+        w.t_total_ms = w.t_raster_ms = 0.0;
+        w.n_timed = 0;
+    }
+    if (total_ms) *total_ms = total;
+    if (raster_ms) *raster_ms = raster;
+    if (runs) *runs = n;
     return FT_OK;
 }
 

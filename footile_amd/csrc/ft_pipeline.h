@@ -24,6 +24,15 @@ namespace ft {
 
 struct Workspace {
     hipStream_t stream = nullptr;
+    bool stream_owned = true;  // ws_free destroys `stream`
+    // Pipelined runs (ws_pipe_attach): the front end (every kernel before the
+    // raster) goes to `fstream` once this workspace's previous raster and
+    // status copy are done (`ev_free`, recorded on `stream`), and the raster
+    // follows on `stream` after the front end (`ev_front`).  Two workspaces
+    // sharing `stream` then overlap one run's front end with the other's
+    // raster, while the rasters stay in launch order.  Null: one stream.
+    hipStream_t fstream = nullptr;
+    hipEvent_t ev_front = nullptr, ev_free = nullptr;
     int device = 0;
     size_t lds_max = 65536;  // LDS bytes per block on the device
     uint32_t scan_resident = 0;  // scan blocks resident at once (0: unknown)
@@ -148,6 +157,15 @@ struct Workspace {
 
 // Allocate the stream and status buffers.
 ft_status ws_init(Workspace &ws);
+// Pipelined runs: give `ws` a front-end stream and its two events, and make
+// `main` its raster stream (a stream of its own is destroyed; `main` then
+// belongs to the workspace that created it).  Detach returns to one stream
+// (the caller has drained both).
+ft_status ws_pipe_attach(Workspace &ws, hipStream_t main);
+void ws_pipe_detach(Workspace &ws);
+// Grow dst's event, scan and pair capacities to src's, and take over its
+// kept-flag form (for a twin workspace of the same layout).
+ft_status ws_reserve_like(Workspace &dst, const Workspace &src);
 // Arena mode: no allocation; device buffers come from `arena` (256-byte
 // aligned, `bytes` long), work goes to `stream`.
 ft_status ws_init_arena(Workspace &ws, void *arena, size_t bytes, hipStream_t stream);

@@ -2381,8 +2381,39 @@ void ws_free(Workspace &ws) {
         delete[] ws.ev;
     }
     graph_drop(ws);
-    if (ws.stream) (void)hipStreamDestroy(ws.stream);
+    ws_pipe_detach(ws);
+    if (ws.stream && ws.stream_owned) (void)hipStreamDestroy(ws.stream);
     ws = Workspace();
+}
+
+ft_status ws_pipe_attach(Workspace &ws, hipStream_t main) {
+    if (ws.arena || !main) return FT_EINVAL;
+    graph_drop(ws);
+    if (!ws.fstream) {
+        FT_HIP(hipStreamCreateWithFlags(&ws.fstream, hipStreamNonBlocking));
+        FT_HIP(hipEventCreateWithFlags(&ws.ev_front, hipEventDisableTiming));
+        FT_HIP(hipEventCreateWithFlags(&ws.ev_free, hipEventDisableTiming));
+    }
+    if (ws.stream != main) {
+        // the old stream's work (a layout upload) completes before it goes
+        if (ws.stream) {
+            FT_HIP(hipStreamSynchronize(ws.stream));
+            if (ws.stream_owned) FT_HIP(hipStreamDestroy(ws.stream));
+        }
+        ws.stream = main;
+        ws.stream_owned = false;
+    }
+    // the first front end waits for everything on the raster stream so far
+    FT_HIP(hipEventRecord(ws.ev_free, ws.stream));
+    return FT_OK;
+}
+
+void ws_pipe_detach(Workspace &ws) {
+    if (ws.fstream) (void)hipStreamDestroy(ws.fstream);
+    if (ws.ev_front) (void)hipEventDestroy(ws.ev_front);
+    if (ws.ev_free) (void)hipEventDestroy(ws.ev_free);
+    ws.fstream = nullptr;
+    ws.ev_front = ws.ev_free = nullptr;
 }
 
 static ft_status ensure_event_capacity(Workspace &ws, size_t need) {
@@ -2430,6 +2461,20 @@ static ft_status ensure_scan_capacity(Workspace &ws) {
     ft_status s;
     if ((s = dev_alloc(ws, ws.d_scan_state, 2 * tiles * SCAN_SLOTS))) return s;
     ws.cap_tiles = tiles;
+    return FT_OK;
+}
+
+ft_status ws_reserve_like(Workspace &dst, const Workspace &src) {
+    if (dst.arena) return FT_EINVAL;
+    ft_status s;
+    if (src.cap_events > dst.cap_events && (s = ensure_event_capacity(dst, src.cap_events))) return s;
+    if ((s = ensure_scan_capacity(dst))) return s;
+    if (src.cap_pairs > dst.cap_pairs) {
+        if ((s = dev_alloc(dst, dst.d_band_list, src.cap_pairs))) return s;
+        dst.cap_pairs = src.cap_pairs;
+    }
+    dst.ingest_pass = src.ingest_pass;
+    graph_drop(dst);
     return FT_OK;
 }
 
@@ -2785,7 +2830,7 @@ ft_status ws_launch(Workspace &ws, uint32_t mode, bool read_pstate) {
         }
         ev = ws.ev + 3 * ws.n_pending++;
     }
-    if (!(ws.graph && mode == MODE_FILL)) {
+    if (!(ws.graph && mode == MODE_FILL) || ws.fstream) {
         // an eager run brings the path states back with its status when they
         // share the status block (not inside a graph: every replay would copy)
         const bool with_ps = read_pstate && ws.d_sp && ws.n_paths;
@@ -2824,7 +2869,21 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
         const ft_status e = ensure_stroke_capacity(ws);
         if (e) return e;
     }
-    hipStream_t s = ws.stream;
+    // `s` is the front end's stream until hand_over() moves to the raster
+    // stream (one stream unless the workspace is pipelined)
+    const hipStream_t rs = ws.stream;
+    hipStream_t s = rs;
+    if (ws.fstream) {
+        if (ws.uploads_pending) FT_HIP(hipEventRecord(ws.ev_free, rs));  // a new layout's uploads
+        FT_HIP(hipStreamWaitEvent(ws.fstream, ws.ev_free, 0));
+        s = ws.fstream;
+    }
+    auto hand_over = [&]() -> hipError_t {
+        if (s == rs) return hipSuccess;
+        s = rs;
+        const hipError_t e = hipEventRecord(ws.ev_front, ws.fstream);
+        return e != hipSuccess ? e : hipStreamWaitEvent(rs, ws.ev_front, 0);
+    };
     const uint32_t n_ops = (uint32_t)ws.n_ops;
     const size_t tiles = ws.cap_tiles;
     unsigned long long *lb = ws.d_scan_state;  // look-back words: 2 * tiles per scan slot
@@ -2985,6 +3044,7 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
         // The raster kernels: one per pixel format with band blocks, then the
         // tile scheduler per format with tile-mode items; the timing events
         // bracket the first and the last of them.
+        FT_HIP(hand_over());
         struct RasterLaunch {
             int f;
             bool tile;
@@ -3044,8 +3104,10 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
         }
     }
     FT_HIP(hipGetLastError());
+    FT_HIP(hand_over());
     const size_t back = with_pstate ? STATUS_BYTES + ws.n_paths * sizeof(PathState) : sizeof(DevStatus);
     FT_HIP(hipMemcpyAsync(ws.h_status, ws.d_status, back, hipMemcpyDeviceToHost, s));
+    if (ws.fstream) FT_HIP(hipEventRecord(ws.ev_free, rs));  // this workspace's buffers are free again
     return FT_OK;
 }
 

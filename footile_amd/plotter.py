@@ -283,6 +283,13 @@ class Batch:
         """Replay launches from a captured HIP graph (ft_batch_set_graph)."""
         check(lib().ft_batch_set_graph(self._h, 1 if on else 0), "ft_batch_set_graph")
 
+    def set_pipeline(self, depth=2):
+        """Take `depth` workspaces in turn (True: 2, False: 1) so that later
+        launches' front ends overlap an earlier launch's raster
+        (ft_batch_set_pipeline); rasters keep launch order."""
+        depth = 2 if depth is True else (1 if depth is False else int(depth))
+        check(lib().ft_batch_set_pipeline(self._h, depth), "ft_batch_set_pipeline")
+
     def counts(self):
         """(offered, stored, edges) per path of the last run."""
         return self.results()[:3]

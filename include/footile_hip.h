@@ -177,6 +177,16 @@ ft_status ft_batch_get_tuning(const ft_batch *b, ft_tuning *t);
  * once and replays it (re-captured after any re-layout or capacity growth);
  * the raster kernel is then not timed separately. */
 ft_status ft_batch_set_graph(ft_batch *b, int on);
+/* Pipelined launches (no counterpart in the reference, which renders one fill
+ * at a time): with depth 2 to 4 the batch keeps depth workspaces of the same
+ * layout, and consecutive ft_batch_launch calls take them in turn, so that
+ * the front ends of later launches (flattening through binning, each on a
+ * stream of its own) overlap the raster of an earlier one.  The rasters run
+ * on the batch's stream in launch order, so every launch blends onto the
+ * result of the one before, as without pipelining.  Depth 0 or 1: one
+ * workspace, no overlap.  Each extra workspace costs a set of device buffers;
+ * not combinable with graph mode (FT_EINVAL). */
+ft_status ft_batch_set_pipeline(ft_batch *b, int depth);
 /* Work statistics of the raster: runs the batch once and returns the
  * (edge, row) pairs its cell pass walked and the nonzero signed-area cell
  * deltas it added (one LDS atomic each). */

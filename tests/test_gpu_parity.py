@@ -369,6 +369,81 @@ def test_batch_graph_replays_match_eager(ft, oracle):
     batch.set_graph(False)
 
 
+def test_batch_pipelined_launches_match_serial(ft, oracle):
+    """Pipelined launches (two workspaces taking turns, one launch's front end
+    overlapping the previous raster) give the bytes of the same launches run
+    one at a time.  A translucent colour blends again at every launch, so a
+    raster that ran out of order or beside the previous one, or a front end
+    that overwrote buffers a raster still read, changes the result."""
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    W = H = 4096
+    ops = workloads.random_walk(1)
+    clr = (90, 50, 20, 96)
+    A, B = DeviceBuffer(W * H * 4), DeviceBuffer(W * H * 4)
+    both = np.concatenate([ops, ops])
+    batch = Batch(both, [FillDesc(0, len(ops), A.ptr, W, H, W * 4, RGBA8P, rule=0, clr=clr),
+                         FillDesc(len(ops), len(ops), B.ptr, W, H, W * 4, RGBA8P, rule=1, clr=clr)])
+    batch.run()
+    results = batch.results()
+
+    def serial(n):
+        A.zero()
+        B.zero()
+        for _ in range(n):
+            batch.launch()
+            batch.sync()
+        return A.download(), B.download()
+
+    want = {n: serial(n) for n in (1, 3, 4, 5)}
+    assert not np.array_equal(want[4][0], want[5][0])  # every launch shows
+    g = oracle.Plotter(W, H, RGBA8P)
+    assert g.fill(0, ops, clr) == 0
+    assert np.array_equal(want[1][0], g.pixels())
+
+    def pipelined(n):
+        A.zero()
+        B.zero()
+        for _ in range(n):
+            batch.launch()
+        batch.sync()
+        return A.download(), B.download()
+
+    for depth in (2, 3, 4):
+        batch.set_pipeline(depth)
+        for n in (5, 4, 1):
+            got = pipelined(n)
+            assert np.array_equal(got[0], want[n][0]) and np.array_equal(got[1], want[n][1]), (depth, n)
+            r = batch.results()
+            assert all(np.array_equal(x, y) for x, y in zip(r, results))
+    with pytest.raises(ft.FootileError):
+        batch.set_graph(True)
+    with pytest.raises(ft.FootileError):
+        batch.set_pipeline(5)
+    # a re-layout reaches every workspace
+    batch.set_tuning(8, 8192, True)
+    got = pipelined(5)
+    assert np.array_equal(got[0], want[5][0]) and np.array_equal(got[1], want[5][1])
+    batch.set_pipeline(False)
+    got = serial(3)
+    assert np.array_equal(got[0], want[3][0]) and np.array_equal(got[1], want[3][1])
+
+    # fresh atlas tiles, with the run that sizes the twin workspace in between
+    batch, atlas, descs, ops, tiles = _atlas_batch()
+    batch.set_pipeline(True)
+    batch.run()
+    want_atlas = atlas.download()
+    for rep in range(3):
+        atlas.upload(np.full(1024 * 1024, 77, dtype=np.uint8))
+        for _ in range(3 + rep):
+            batch.launch()
+        batch.sync()
+        assert np.array_equal(atlas.download(), want_atlas), f"atlas rep {rep}"
+    for t, (b, c, x, y, s) in enumerate(tiles[:8]):
+        o = oracle.Plotter(int(s), int(s))
+        assert o.fill(0, ops[b:b + c]) == 0
+        assert np.array_equal(want_atlas.reshape(1024, 1024)[y:y + s, x:x + s].reshape(-1), o.pixels()), t
+
+
 def test_batch_rmw_formats_with_and_without_prefetch(ft, oracle):
     """Rgba8p / Graya8p batches over existing content: the destination
     prefetch on and off, through the bands and through the tile scheduler,
