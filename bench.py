@@ -480,8 +480,7 @@ def main():
     batch.set_timing(False)
     total_ms, raster_ms, runs = batch.timing()
     sec_per_step_serial = (t1 - t0) / args.steps
-    raster_avg_ms = raster_ms / max(runs, 1)
-    achieved = bytes_per_step / (raster_avg_ms * 1e-3) / 1e9
+    raster_serial_ms = raster_ms / max(runs, 1)
 
     # The step: pipelined launches (two workspaces take turns; a launch's
     # front end overlaps the previous launch's raster, rasters keep launch
@@ -507,13 +506,21 @@ def main():
     for _ in range(args.warmup):
         batch.launch()
     batch.sync()
+    # (the raster kernel's events, every TIMING_EVERY-th launch, time it
+    # beside the front ends it overlaps)
+    batch.timing()
     t0 = time.perf_counter()
     for k in range(args.steps):
+        batch.set_timing(k % TIMING_EVERY == 0)
         batch.launch()
     batch.sync()
     t1 = time.perf_counter()
+    batch.set_timing(False)
+    total_p_ms, raster_p_ms, runs_p = batch.timing()
     batch.set_pipeline(False)
     sec_per_step = (t1 - t0) / args.steps
+    raster_avg_ms = raster_p_ms / max(runs_p, 1)
+    achieved = bytes_per_step / (raster_avg_ms * 1e-3) / 1e9
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_random_walk.json")
@@ -528,7 +535,9 @@ def main():
         "pipelined_matches_serial": pipe_ok,
         "ms_per_step_one_at_a_time": sec_per_step_serial * 1e3,
         "device_ms_per_step": total_ms / max(runs, 1),
+        "device_ms_per_launch_pipelined": total_p_ms / max(runs_p, 1),
         "raster_kernel_ms": raster_avg_ms,
+        "raster_kernel_ms_one_at_a_time": raster_serial_ms,
         "points_offered_per_fill": int(offered[0]),
         "points_stored_per_fill": int(stored[0]),
         "edges_per_fill": int(edges[0]),

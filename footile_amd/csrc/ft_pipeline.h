@@ -28,10 +28,15 @@ struct Workspace {
     // Pipelined runs (ws_pipe_attach): the front end (every kernel before the
     // raster) goes to `fstream` once this workspace's previous raster and
     // status copy are done (`ev_free`, recorded on `stream`), and the raster
-    // follows on `stream` after the front end (`ev_front`).  Two workspaces
-    // sharing `stream` then overlap one run's front end with the other's
-    // raster, while the rasters stay in launch order.  Null: one stream.
+    // follows on `stream` after the front end (`ev_front`).  Workspaces
+    // sharing `stream`, each with a front-end stream of its own, then overlap
+    // their front ends with each other and with an earlier raster, while the
+    // rasters stay in launch order.  Null: one stream.  (Front ends may run
+    // side by side because their scans and the one-pass flattening hand out
+    // look-back tickets on arrival: no block waits on a block that has not
+    // started.)
     hipStream_t fstream = nullptr;
+    bool fstream_owned = false;
     hipEvent_t ev_front = nullptr, ev_free = nullptr;
     int device = 0;
     size_t lds_max = 65536;  // LDS bytes per block on the device
@@ -157,11 +162,11 @@ struct Workspace {
 
 // Allocate the stream and status buffers.
 ft_status ws_init(Workspace &ws);
-// Pipelined runs: give `ws` a front-end stream and its two events, and make
-// `main` its raster stream (a stream of its own is destroyed; `main` then
-// belongs to the workspace that created it).  Detach returns to one stream
-// (the caller has drained both).
-ft_status ws_pipe_attach(Workspace &ws, hipStream_t main);
+// Pipelined runs: give `ws` the front-end stream `front` (null: a new one that
+// `ws` owns) and its two events, and make `main` its raster stream (a stream
+// of its own is destroyed; `main` then belongs to the workspace that created
+// it).  Detach returns to one stream (the caller has drained both).
+ft_status ws_pipe_attach(Workspace &ws, hipStream_t main, hipStream_t front = nullptr);
 void ws_pipe_detach(Workspace &ws);
 // Grow dst's event, scan and pair capacities to src's, and take over its
 // kept-flag form (for a twin workspace of the same layout).

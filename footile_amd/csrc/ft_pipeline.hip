@@ -213,9 +213,10 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uin
 
 // Flattening in one pass, for fill runs whose (op, lane) items fit one
 // resident grid at one item per thread and are taken in op order: a block
-// counts its items (keeping each curve lane's first LEAF_CACHE points in LDS),
-// takes its prefix by look-back over the blocks before it, and emits them (a
-// copy from LDS unless a lane has more leaves).  Writes each op's offset
+// takes the items of its look-back ticket, counts them (keeping each curve
+// lane's first LEAF_CACHE points in LDS), takes its prefix by look-back over
+// the tickets before it, and emits them (a copy from LDS unless a lane has
+// more leaves).  Writes each op's offset
 // (op_off, as the op scan does) and the run's event and close totals (as
 // k_emit does).
 __global__ __launch_bounds__(BLOCK) void k_flatten(const ft_path_op *ops, const uint32_t *op_path,
@@ -224,10 +225,15 @@ __global__ __launch_bounds__(BLOCK) void k_flatten(const ft_path_op *ops, const 
                                                    EventSink sink, DevStatus *st) {
     __shared__ unsigned long long s_wave[BLOCK / 64];
     __shared__ unsigned long long s_prefix;
+    __shared__ uint32_t s_tk;
     __shared__ I2 s_leaf[LEAF_CACHE * BLOCK];  // point k of thread j at s_leaf[k * BLOCK + j]
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint64_t n_items = (uint64_t)n_ops * OP_LANES;
-    const uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    // the block's items are those of its look-back ticket
+    if (threadIdx.x == 0) s_tk = lookback_ticket(lb);
+    __syncthreads();
+    const uint32_t b = s_tk;
+    const uint64_t t = (uint64_t)b * BLOCK + threadIdx.x;
     const bool live = t < n_items;
     const uint32_t i = live ? (uint32_t)(t >> FLAT_SPLIT) : 0u, s = (uint32_t)t & (OP_LANES - 1);
     OpCount c = {0, 0, 0, 0};
@@ -249,14 +255,14 @@ __global__ __launch_bounds__(BLOCK) void k_flatten(const ft_path_op *ops, const 
         btot += s_wave[w];
     }
     if (wid == 0) {
-        const unsigned long long prefix = lookback_prefix<unsigned long long, false>(lb, blockIdx.x, btot, st);
+        const unsigned long long prefix = lookback_prefix<unsigned long long, false>(lb, b, btot, st);
         if (lane == 0) s_prefix = prefix;
     }
     __syncthreads();
     const unsigned long long off = s_prefix + wpre + winc - v;
     if (live & (s == 0)) op_off[i] = off;
     const uint64_t last_t = n_items - 1;
-    if ((uint64_t)blockIdx.x == last_t / BLOCK && threadIdx.x == 0) {  // the run's totals
+    if ((uint64_t)b == last_t / BLOCK && threadIdx.x == 0) {  // the run's totals
         const unsigned long long tot = s_prefix + btot;
         const uint32_t n_events = (uint32_t)(tot >> 32);
         st->op_total = tot;
@@ -2291,9 +2297,9 @@ static ft_status dev_alloc(Workspace &ws, T *&ptr, size_t n) {
     return FT_OK;
 }
 
-// Blocks of k_flatten the device holds at once (every block of its grid
-// takes a look-back prefix, so all must be resident): its occupancy per
-// compute unit, at most the four per CU that the scans assume, times the CUs.
+// Blocks of k_flatten the device holds at once (the one-pass flattening is
+// used for grids that fit: one wave of blocks, no block waiting for a slot):
+// its occupancy per compute unit, at most four, times the CUs.
 static ft_status flatten_resident(Workspace &ws) {
     int per_cu = 0, cus = 0;
     FT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_flatten, BLOCK, 0));
@@ -2386,11 +2392,17 @@ void ws_free(Workspace &ws) {
     ws = Workspace();
 }
 
-ft_status ws_pipe_attach(Workspace &ws, hipStream_t main) {
+ft_status ws_pipe_attach(Workspace &ws, hipStream_t main, hipStream_t front) {
     if (ws.arena || !main) return FT_EINVAL;
     graph_drop(ws);
     if (!ws.fstream) {
-        FT_HIP(hipStreamCreateWithFlags(&ws.fstream, hipStreamNonBlocking));
+        if (front) {
+            ws.fstream = front;
+            ws.fstream_owned = false;
+        } else {
+            FT_HIP(hipStreamCreateWithFlags(&ws.fstream, hipStreamNonBlocking));
+            ws.fstream_owned = true;
+        }
         FT_HIP(hipEventCreateWithFlags(&ws.ev_front, hipEventDisableTiming));
         FT_HIP(hipEventCreateWithFlags(&ws.ev_free, hipEventDisableTiming));
     }
@@ -2409,7 +2421,7 @@ ft_status ws_pipe_attach(Workspace &ws, hipStream_t main) {
 }
 
 void ws_pipe_detach(Workspace &ws) {
-    if (ws.fstream) (void)hipStreamDestroy(ws.fstream);
+    if (ws.fstream && ws.fstream_owned) (void)hipStreamDestroy(ws.fstream);
     if (ws.ev_front) (void)hipEventDestroy(ws.ev_front);
     if (ws.ev_free) (void)hipEventDestroy(ws.ev_free);
     ws.fstream = nullptr;
@@ -2456,7 +2468,8 @@ static ft_status ensure_scan_capacity(Workspace &ws) {
     if (ws.n_ops > n) n = ws.n_ops;
     if ((ws.n_bands << ws.band_shift) > n) n = ws.n_bands << ws.band_shift;
     size_t tiles = (n + SCAN_TILE_MIN - 1) / SCAN_TILE_MIN + 1;
-    if (tiles < SCAN_GRID_MAX) tiles = SCAN_GRID_MAX;  // one state per scan block: any scan grid fits
+    // look-back words for any scan grid
+    if (2 * tiles < lookback_words(SCAN_GRID_MAX)) tiles = lookback_words(SCAN_GRID_MAX) / 2;
     if (tiles <= ws.cap_tiles && ws.d_scan_state) return FT_OK;
     ft_status s;
     if ((s = dev_alloc(ws, ws.d_scan_state, 2 * tiles * SCAN_SLOTS))) return s;
@@ -2520,7 +2533,7 @@ static ft_status arena_split(Workspace &ws) {
     if (ws.n_ops > n) n = ws.n_ops;
     if ((ws.n_bands << ws.band_shift) > n) n = ws.n_bands << ws.band_shift;
     size_t tiles = (n + SCAN_TILE_MIN - 1) / SCAN_TILE_MIN + 1;
-    if (tiles < SCAN_GRID_MAX) tiles = SCAN_GRID_MAX;
+    if (2 * tiles < lookback_words(SCAN_GRID_MAX)) tiles = lookback_words(SCAN_GRID_MAX) / 2;
     const size_t scan_bytes = 2 * tiles * SCAN_SLOTS * sizeof(unsigned long long);
     if (left < scan_bytes + SLACK + 4096 * (EVENT_BYTES + 4)) return FT_ENOMEM;
     const size_t room = left - scan_bytes - SLACK;
@@ -3117,6 +3130,7 @@ ft_status ws_finish(Workspace &ws, bool read_pstate) {
                               ws.stream));
     ws.pstate_copied = false;
     FT_HIP(hipStreamSynchronize(ws.stream));
+    if (ws.fstream) FT_HIP(hipStreamSynchronize(ws.fstream));  // a pipelined run's status copy
     ws.uploads_pending = false;
     for (int k = 0; k < ws.n_pending; ++k) {
         float total = 0.0f, raster = 0.0f;

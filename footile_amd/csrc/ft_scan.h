@@ -119,9 +119,8 @@ __device__ inline T wave_incl_t(T v) {
     }
 }
 
-// Host side: blocks of a scan over n_cap items and the tiles each block owns.
-// The grid never exceeds the blocks resident at once, so every block's
-// predecessors are running when it waits on them.
+// Host side: blocks of a scan over n_cap items and the tiles each block owns
+// (a grid the device can hold at once: no block waits for a slot).
 struct ScanGrid {
     uint32_t blocks, tiles_per_block;
 };
@@ -139,14 +138,26 @@ inline ScanGrid scan_grid_for(size_t n_cap, uint32_t resident) {
     return r;
 }
 
-// Decoupled look-back for block b, called by one whole wavefront: publishes
-// the block's reduction btot, returns the reduction of blocks 0 .. b-1 (to
-// every lane) and publishes the block's inclusive prefix.  lb[2 j] is block
-// j's aggregate and lb[2 j + 1] its inclusive prefix, each LB_READY | value
-// (zero before the run); lane l reads block top - l, 64 blocks per round trip.
+// Look-back state of one scan (zero before the run): lb[0] hands out the
+// blocks' tickets, then two words per ticket.  A block works on the items of
+// the ticket it takes on arrival, not of its block index: every block it
+// waits on took its ticket earlier, is running or done, and publishes its
+// aggregate without waiting on anything.  So a scan completes however few of
+// its blocks the device holds at once and whatever runs beside it (kernels
+// of other streams included).  One thread takes the ticket.
+__device__ inline uint32_t lookback_ticket(unsigned long long *lb) { return (uint32_t)atomicAdd(lb, 1ull); }
+// Look-back words of a scan of `blocks` blocks.
+constexpr size_t lookback_words(size_t blocks) { return 2 * blocks + 2; }
+
+// Decoupled look-back for ticket b, called by one whole wavefront: publishes
+// the block's reduction btot, returns the reduction of tickets 0 .. b-1 (to
+// every lane) and publishes the block's inclusive prefix.  lb[2 j + 2] is
+// ticket j's aggregate and lb[2 j + 3] its inclusive prefix, each LB_READY |
+// value; lane l reads ticket top - l, 64 tickets per round trip.
 constexpr unsigned long long LB_READY = 1ull << 63;
 template <typename T, bool IS_MAX>
 __device__ inline T lookback_prefix(unsigned long long *lb, uint32_t b, T btot, DevStatus *st) {
+    lb += 2;  // past the ticket counter
     const int lane = threadIdx.x & 63;
     T prefix = 0;  // identity of + and of max over unsigned values
     if (lane == 0 && ((unsigned long long)btot & LB_READY)) raise_err(st, FT_ERANGE);  // 63-bit values only
@@ -212,10 +223,10 @@ struct ArraySrc {
 };
 
 // Prefix scan (sum or running maximum) of src's items [0, n) into out, n = *n_dev if
-// given, else n_host, at most n_cap.  Block b owns the tiles_per_block tiles
-// from b * tiles_per_block: it reduces them (coalesced loads), takes its
-// prefix by decoupled look-back over the blocks before it (lane l of the
-// first wavefront reads block top - l, 64 blocks per round trip), and then
+// given, else n_host, at most n_cap.  The block with ticket b owns the
+// tiles_per_block tiles from b * tiles_per_block: it reduces them (coalesced
+// loads), takes its prefix by decoupled look-back over the tickets before it
+// (lane l of the first wavefront reads ticket top - l, 64 per round trip), and then
 // scans the tiles again, transposing each through LDS so that both the loads
 // and the stores stay coalesced.  A single-tile block keeps its items in
 // registers between the passes.  `total` receives the whole reduction.
@@ -225,23 +236,25 @@ __global__ __launch_bounds__(BLOCK) void k_scan(Src src, T *out, const uint32_t 
                                                 T *total, DevStatus *st, bool check_failed, bool inclusive) {
     constexpr uint32_t ITEMS = Src::ITEMS, TILE = (uint32_t)BLOCK * ITEMS, ROW = ITEMS + 1;
     __shared__ T s_t[BLOCK * ROW];  // a tile, one padded row of ITEMS per thread
-    __shared__ uint32_t s_fail, s_n;
+    __shared__ uint32_t s_fail, s_n, s_tk;
     __shared__ T s_wave[BLOCK / 64];
     __shared__ T s_prefix;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) {  // status and length in one round trip
+    if (tid == 0) {  // status, length and the block's ticket in one round trip
         const uint32_t e = st->err, o = st->overflow;
         const uint32_t nn = n_dev ? *n_dev : n_host;
+        s_tk = lookback_ticket(lb);
         s_fail = (check_failed & ((e | o) != 0u)) ? 1u : 0u;
         s_n = nn;
     }
     __syncthreads();
     if (s_fail) return;
     src.bind(st);
+    const uint32_t b = s_tk;
     const uint64_t n = s_n < n_cap ? s_n : n_cap;
     const uint64_t n_tiles = (n + TILE - 1) / TILE;
-    const uint64_t t0 = (uint64_t)blockIdx.x * tiles_per_block;
+    const uint64_t t0 = (uint64_t)b * tiles_per_block;
     if (t0 >= n_tiles) return;  // no items: no later block has any either
     const uint64_t t1 = t0 + tiles_per_block < n_tiles ? t0 + tiles_per_block : n_tiles;
     const bool single = t1 - t0 == 1;
@@ -267,7 +280,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(Src src, T *out, const uint32_t 
     for (int w = 0; w < BLOCK / 64; ++w) btot = scan_op_t<T, IS_MAX>(btot, s_wave[w]);
     // the chunk's prefix
     if (wid == 0) {
-        const T prefix = lookback_prefix<T, IS_MAX>(lb, blockIdx.x, btot, st);
+        const T prefix = lookback_prefix<T, IS_MAX>(lb, b, btot, st);
         if (lane == 0) {
             s_prefix = prefix;
             if (t1 == n_tiles && total) *total = scan_op_t<T, IS_MAX>(prefix, btot);

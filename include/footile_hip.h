@@ -181,11 +181,11 @@ ft_status ft_batch_set_graph(ft_batch *b, int on);
  * at a time): with depth 2 to 4 the batch keeps depth workspaces of the same
  * layout, and consecutive ft_batch_launch calls take them in turn, so that
  * the front ends of later launches (flattening through binning, each on a
- * stream of its own) overlap the raster of an earlier one.  The rasters run
- * on the batch's stream in launch order, so every launch blends onto the
- * result of the one before, as without pipelining.  Depth 0 or 1: one
- * workspace, no overlap.  Each extra workspace costs a set of device buffers;
- * not combinable with graph mode (FT_EINVAL). */
+ * stream of its own) overlap each other and the raster of an earlier one.  The
+ * rasters run on the batch's stream in launch order, so every launch blends
+ * onto the result of the one before, as without pipelining.  Depth 0 or 1:
+ * one workspace, no overlap.  Each extra workspace costs a set of device
+ * buffers; not combinable with graph mode (FT_EINVAL). */
 ft_status ft_batch_set_pipeline(ft_batch *b, int depth);
 /* Work statistics of the raster: runs the batch once and returns the
  * (edge, row) pairs its cell pass walked and the nonzero signed-area cell

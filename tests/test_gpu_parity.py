@@ -444,6 +444,51 @@ def test_batch_pipelined_launches_match_serial(ft, oracle):
         assert np.array_equal(want_atlas.reshape(1024, 1024)[y:y + s, x:x + s].reshape(-1), o.pixels()), t
 
 
+def test_look_back_kernels_finish_on_eight_compute_units(ft, oracle):
+    """The scans and the one-pass flattening are sized for the whole device,
+    and their blocks wait on each other (decoupled look-back).  Run on a
+    stream that may use only 8 compute units, far fewer blocks than their
+    grids are resident at once; blocks take look-back tickets on arrival, so
+    none waits on a block that has not started and the run completes with
+    the bytes of an ordinary run."""
+    import ctypes
+    from footile_amd._lib import FillDescC, lib
+    from footile_amd.path import ops_array
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+    stream = ctypes.c_void_p()
+    mask = (ctypes.c_uint32 * 8)(0xFF, 0, 0, 0, 0, 0, 0, 0)
+    assert hip.hipExtStreamCreateWithCUMask(ctypes.byref(stream), 8, mask) == 0
+    try:
+        W = H = 4096
+        ops = workloads.random_walk(1)
+        both = np.concatenate([ops, ops])
+
+        def make():
+            d = DeviceBuffer(2 * W * H * 4)
+            d.zero()
+            return d, [FillDesc(0, len(ops), d.ptr, W, H, W * 4, RGBA8P, rule=0, clr=(200, 120, 40, 255)),
+                       FillDesc(len(ops), len(ops), d.ptr + W * H * 4, W, H, W * 4, RGBA8P, rule=1,
+                                clr=(30, 60, 90, 128))]
+        want_buf, descs = make()
+        batch = Batch(both, descs)
+        batch.run()
+        want = want_buf.download()
+        got_buf, descs = make()
+        a = ops_array(both)
+        arr = (FillDescC * 2)(*[d.c for d in descs])
+        off = np.zeros(2, np.uint64)
+        st = np.zeros(2, np.uint64)
+        rc = lib().ft_batch_run_noalloc(a.ctypes.data_as(ctypes.c_void_p), len(a), arr, 2, stream,
+                                        off.ctypes.data_as(ctypes.c_void_p), st.ctypes.data_as(ctypes.c_void_p))
+        assert rc == 0, rc
+        assert np.array_equal(got_buf.download(), want)
+        w_off, w_st, _ = batch.counts()
+        assert np.array_equal(off, np.asarray(w_off, np.uint64)) and np.array_equal(st, np.asarray(w_st, np.uint64))
+    finally:
+        hip.hipStreamDestroy(stream)
+
+
 def test_batch_rmw_formats_with_and_without_prefetch(ft, oracle):
     """Rgba8p / Graya8p batches over existing content: the destination
     prefetch on and off, through the bands and through the tile scheduler,
