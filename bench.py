@@ -20,7 +20,7 @@ Also reported under "extra": the batched atlas (16,384 tiles in a 16384^2
 Matte8 atlas, one launch) and the plain 256x256 fill/stroke latency through
 the Plotter API.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 3]
 """
 import argparse
 import json
@@ -391,7 +391,7 @@ def bench_diff(steps, warmup, check=True):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the atlas, plain and differentiable measurements")
@@ -480,7 +480,10 @@ def main():
     batch.set_timing(False)
     total_ms, raster_ms, runs = batch.timing()
     sec_per_step_serial = (t1 - t0) / args.steps
-    raster_serial_ms = raster_ms / max(runs, 1)
+    # the roofline's kernel duration: the raster alone, as the profiler's
+    # kernel trace of the same command sees it
+    raster_avg_ms = raster_ms / max(runs, 1)
+    achieved = bytes_per_step / (raster_avg_ms * 1e-3) / 1e9
 
     # The step: pipelined launches (two workspaces take turns; a launch's
     # front end overlaps the previous launch's raster, rasters keep launch
@@ -507,7 +510,7 @@ def main():
         batch.launch()
     batch.sync()
     # (the raster kernel's events, every TIMING_EVERY-th launch, time it
-    # beside the front ends it overlaps)
+    # beside the front ends it overlaps, which share its compute units)
     batch.timing()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -519,8 +522,6 @@ def main():
     total_p_ms, raster_p_ms, runs_p = batch.timing()
     batch.set_pipeline(False)
     sec_per_step = (t1 - t0) / args.steps
-    raster_avg_ms = raster_p_ms / max(runs_p, 1)
-    achieved = bytes_per_step / (raster_avg_ms * 1e-3) / 1e9
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_random_walk.json")
@@ -537,7 +538,10 @@ def main():
         "device_ms_per_step": total_ms / max(runs, 1),
         "device_ms_per_launch_pipelined": total_p_ms / max(runs_p, 1),
         "raster_kernel_ms": raster_avg_ms,
-        "raster_kernel_ms_one_at_a_time": raster_serial_ms,
+        "raster_kernel_ms_pipelined": raster_p_ms / max(runs_p, 1),
+        # the whole step against the same HBM roofline (bytes the raster must
+        # move per step over the pipelined step time)
+        "step_hbm_frac": bytes_per_step / sec_per_step / 1e9 / HBM_PEAK_GBS,
         "points_offered_per_fill": int(offered[0]),
         "points_stored_per_fill": int(stored[0]),
         "edges_per_fill": int(edges[0]),
@@ -547,13 +551,14 @@ def main():
     }
     if not args.no_extra:
         print("[bench] atlas", file=sys.stderr, flush=True)
-        extra["atlas"] = bench_atlas(ft, max(3, args.steps // 4), 1, args.atlas_tuning, min(4.0, args.cpu_seconds))
+        extra["atlas"] = bench_atlas(ft, min(25, max(3, args.steps // 4)), 1, args.atlas_tuning,
+                                     min(4.0, args.cpu_seconds))
         print("[bench] plain 256", file=sys.stderr, flush=True)
         extra["plain_256"] = bench_plain(ft)
         print("[bench] random walk through the Plotter API", file=sys.stderr, flush=True)
         extra["random_walk_plotter_api"] = bench_plotter_random_walk(ft, ops)
         print("[bench] differentiable fills", file=sys.stderr, flush=True)
-        extra["diff"] = bench_diff(max(4, args.steps // 2), 2, check=not args.no_diff_check)
+        extra["diff"] = bench_diff(min(10, max(4, args.steps // 2)), 2, check=not args.no_diff_check)
 
     cpu_step_px = 2 * W * (H - max(int(tops[0]), 0))
     cpu_med = float(np.median(cpu_times))
@@ -576,7 +581,9 @@ def main():
         "parity": parity and pipe_ok,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_raster", "algorithmic_bytes_per_launch": bytes_per_step},
+                     "kernel": "k_raster", "algorithmic_bytes_per_launch": bytes_per_step,
+                     "timing": "raster kernel alone (launches one at a time, every "
+                               f"{TIMING_EVERY}th timed by HIP events on its stream)"},
         "cpu_baseline": {"value": cpu_step_px / cpu_med / 1e6, "unit": "Mpixels/s", "cores": 1, "kind": "port",
                          "sample": f"{len(cpu_times)} step(s) of the same two fills (oracle, C, 1 thread, "
                                    f"SSSE3 accumulate), {cpu_times.sum():.1f} s; median step "

@@ -37,6 +37,10 @@ struct Workspace {
     // started.)
     hipStream_t fstream = nullptr;
     bool fstream_owned = false;
+    // look-back kernels take arrival tickets (pipelined and arena
+    // workspaces, whose runs may overlap other waiting kernels); otherwise
+    // block index order, over grids the device holds at once
+    bool tickets = false;
     hipEvent_t ev_front = nullptr, ev_free = nullptr;
     int device = 0;
     size_t lds_max = 65536;  // LDS bytes per block on the device

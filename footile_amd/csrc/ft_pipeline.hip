@@ -222,17 +222,19 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const ft_path_op *ops, const uin
 __global__ __launch_bounds__(BLOCK) void k_flatten(const ft_path_op *ops, const uint32_t *op_path,
                                                    const PathDev *paths, uint32_t n_ops,
                                                    unsigned long long *op_off, unsigned long long *lb,
-                                                   EventSink sink, DevStatus *st) {
+                                                   EventSink sink, DevStatus *st, bool tickets) {
     __shared__ unsigned long long s_wave[BLOCK / 64];
     __shared__ unsigned long long s_prefix;
     __shared__ uint32_t s_tk;
     __shared__ I2 s_leaf[LEAF_CACHE * BLOCK];  // point k of thread j at s_leaf[k * BLOCK + j]
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint64_t n_items = (uint64_t)n_ops * OP_LANES;
-    // the block's items are those of its look-back ticket
-    if (threadIdx.x == 0) s_tk = lookback_ticket(lb);
-    __syncthreads();
-    const uint32_t b = s_tk;
+    // the block's items are those of its look-back ticket (or of its index)
+    if (tickets) {
+        if (threadIdx.x == 0) s_tk = lookback_ticket(lb);
+        __syncthreads();
+    }
+    const uint32_t b = tickets ? s_tk : blockIdx.x;
     const uint64_t t = (uint64_t)b * BLOCK + threadIdx.x;
     const bool live = t < n_items;
     const uint32_t i = live ? (uint32_t)(t >> FLAT_SPLIT) : 0u, s = (uint32_t)t & (OP_LANES - 1);
@@ -2322,6 +2324,7 @@ ft_status ws_init_arena(Workspace &ws, void *arena, size_t bytes, hipStream_t st
     ws.arena_bytes = bytes;
     ws.arena_used = 0;
     ws.h_status = &ws.h_status_store;
+    ws.tickets = true;  // the caller's stream: other waiting kernels may run beside
     return FT_OK;
 }
 
@@ -2415,6 +2418,7 @@ ft_status ws_pipe_attach(Workspace &ws, hipStream_t main, hipStream_t front) {
         ws.stream = main;
         ws.stream_owned = false;
     }
+    ws.tickets = true;  // front ends of several workspaces may wait side by side
     // the first front end waits for everything on the raster stream so far
     FT_HIP(hipEventRecord(ws.ev_free, ws.stream));
     return FT_OK;
@@ -2426,6 +2430,7 @@ void ws_pipe_detach(Workspace &ws) {
     if (ws.ev_free) (void)hipEventDestroy(ws.ev_free);
     ws.fstream = nullptr;
     ws.ev_front = ws.ev_free = nullptr;
+    ws.tickets = ws.arena != nullptr;
 }
 
 static ft_status ensure_event_capacity(Workspace &ws, size_t need) {
@@ -2916,7 +2921,7 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
     if (mode != MODE_FILL && n_ops) {
         k_pen_width_mark<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, n_ops, ws.d_op_pw);
         scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_op_pw, ws.d_op_lastpw, nullptr, n_ops, n_ops,
-                                    lb + 10 * tiles, nullptr, st, false, false);
+                                    lb + 10 * tiles, nullptr, st, false, false, ws.tickets);
         lastpw = ws.d_op_lastpw;
     }
     if (n_ops) {
@@ -2936,13 +2941,13 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
         em.run_end = ws.d_run_end;
         const size_t n_items = (size_t)n_ops * OP_LANES;
         if (mode == MODE_FILL && !order && flatten_in_one_pass(ws, n_ops)) {
-            k_flatten<<<(uint32_t)((n_items + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths,
-                                                                                 n_ops, ws.d_op_off, lb, em, st);
+            k_flatten<<<(uint32_t)((n_items + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
+                ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, ws.d_op_off, lb, em, st, ws.tickets);
         } else {
             k_count<<<grid_for(n_items), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode, lastpw, order,
                                                         ws.d_op_cnt, ws.d_sub_off, leaf, ws.cap_ops * OP_LANES, st);
             scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_op_cnt, ws.d_op_off, nullptr, n_ops,
-                                                   n_ops, lb, &st->op_total, st, false, false);
+                                                   n_ops, lb, &st->op_total, st, false, false, ws.tickets);
             k_emit<<<grid_for(n_items), BLOCK, 0, s>>>(ws.d_ops, ws.d_op_path, ws.d_paths, n_ops, mode, lastpw, order,
                                                        ws.d_op_off, ws.d_sub_off, leaf, ws.cap_ops * OP_LANES, em, st);
         }
@@ -2957,9 +2962,9 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
         sp.tol_sq = ws.n_paths ? ws.stroke_tol_sq : 0.0f;
         k_st_flags<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_flags, ws.d_st_last, st);
         scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_st_flags, ws.d_st_off, &st->n_events, 0, cap,
-                                               lb + 12 * tiles, &st->st_total, st, true, false);
+                                               lb + 12 * tiles, &st->st_total, st, true, false, ws.tickets);
         scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_st_last, ws.d_st_last, &st->n_events, 0, cap,
-                                    lb + 14 * tiles, nullptr, st, true, true);
+                                    lb + 14 * tiles, nullptr, st, true, true, ws.tickets);
         k_st_compact<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_off, ws.d_spts, ws.d_sub_start, ws.d_sub_event, cap, st);
         k_st_subs<<<ge, BLOCK, 0, s>>>(ws.d_ev_f, ws.d_st_last, ws.d_sub_start, ws.d_sub_event, ws.d_sub_n,
                                        ws.d_sub_joined, cap, st);
@@ -2967,7 +2972,7 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
                                                          ws.d_it_cnt, st);
         scan_launch<unsigned long long, false>(s, ws.scan_resident, ws.d_it_cnt, ws.d_it_off, &st->st_items, 0,
                                                (uint32_t)cap_items, lb + 16 * tiles, &st->stroke_total, st, true,
-                                               false);
+                                               false, ws.tickets);
         EventSink em{};
         em.cap_events = ws.cap_events;
         em.cap_runs = cap;
@@ -3002,12 +3007,12 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
             if (in_scan) {
                 scan_launch_src<uint32_t, false>(s, ws.scan_resident, kl, ws.d_ev_idx, &st->n_events, 0,
                                                  (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true,
-                                                 false);
+                                                 false, ws.tickets);
             } else {
                 k_kept_local<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close,
                                                   ws.d_ev_kept, st);
                 scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_ev_kept, ws.d_ev_idx, &st->n_events, 0,
-                                             (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true, false);
+                                             (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true, false, ws.tickets);
             }
         } else {
             // pushed-point marks -> their running maximum (the last pushed
@@ -3015,12 +3020,12 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
             if (in_scan) {
                 const IngestSrc ing{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path, 0u};
                 scan_launch_src<uint32_t, true>(s, ws.scan_resident, ing, ws.d_ev_last, &st->n_events, 0,
-                                                (uint32_t)ws.cap_events, lb + 8 * tiles, nullptr, st, true, true);
+                                                (uint32_t)ws.cap_events, lb + 8 * tiles, nullptr, st, true, true, ws.tickets);
             } else {
                 k_ingest<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_path,
                                               ws.d_ev_last, st);
                 scan_launch<uint32_t, true>(s, ws.scan_resident, ws.d_ev_last, ws.d_ev_last, &st->n_events, 0,
-                                            (uint32_t)ws.cap_events, lb + 8 * tiles, nullptr, st, true, true);
+                                            (uint32_t)ws.cap_events, lb + 8 * tiles, nullptr, st, true, true, ws.tickets);
             }
             // kept flags -> point indices
             const KeptSrc kp{L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close, ws.d_ev_last,
@@ -3028,12 +3033,12 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
             if (in_scan) {
                 scan_launch_src<uint32_t, false>(s, ws.scan_resident, kp, ws.d_ev_idx, &st->n_events, 0,
                                                  (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true,
-                                                 false);
+                                                 false, ws.tickets);
             } else {
                 k_kept<<<ge, BLOCK, 0, s>>>(L0, ws.d_ev_meta, ws.d_ev_xy, ws.d_run_first, ws.d_run_close,
                                             ws.d_ev_last, ws.d_ev_kept, st);
                 scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_ev_kept, ws.d_ev_idx, &st->n_events, 0,
-                                             (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true, false);
+                                             (uint32_t)ws.cap_events, lb + 2 * tiles, &st->n_kept, st, true, false, ws.tickets);
             }
         }
         k_cap<<<grid_for(ws.n_paths * 64), BLOCK, 0, s>>>(L0, ws.d_paths, ws.d_op_off, ws.d_ev_meta, ws.d_ev_xy,
@@ -3049,7 +3054,7 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
             // pairs per (band, replica) -> list segment offsets and the pair total
             const uint32_t n_slots = (uint32_t)(ws.n_bands << ws.band_shift);
             scan_launch<uint32_t, false>(s, ws.scan_resident, ws.d_band_cnt, ws.d_band_off, nullptr, n_slots, n_slots,
-                                         lb + 4 * tiles, &st->n_pairs, st, true, false);
+                                         lb + 4 * tiles, &st->n_pairs, st, true, false, ws.tickets);
             (wide ? k_bin<4> : k_bin<1>)<<<ge, BLOCK, 0, s>>>(
                 L0, ws.band_shift, ws.d_pt_band, ws.d_band_off, ws.d_band_cur, ws.d_band_list, ws.cap_pairs, ws.d_pts,
                 ws.d_pt_run, ws.d_run_start, ws.d_run_end, ws.d_pstate, st);

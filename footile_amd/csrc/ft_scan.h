@@ -139,12 +139,15 @@ inline ScanGrid scan_grid_for(size_t n_cap, uint32_t resident) {
 }
 
 // Look-back state of one scan (zero before the run): lb[0] hands out the
-// blocks' tickets, then two words per ticket.  A block works on the items of
-// the ticket it takes on arrival, not of its block index: every block it
-// waits on took its ticket earlier, is running or done, and publishes its
-// aggregate without waiting on anything.  So a scan completes however few of
-// its blocks the device holds at once and whatever runs beside it (kernels
-// of other streams included).  One thread takes the ticket.
+// blocks' tickets, then two words per ticket.  With tickets a block works on
+// the items of the ticket it takes on arrival, not of its block index: every
+// block it waits on took its ticket earlier, is running or done, and
+// publishes its aggregate without waiting on anything.  So a scan completes
+// however few of its blocks the device holds at once and whatever runs
+// beside it (other streams' kernels that wait too).  One thread takes the
+// ticket; the atomics of a whole grid on one word cost a few microseconds.
+// Without tickets (block index = ticket) the grid must be resident at once
+// beside nothing else that waits: a workspace whose runs own the device.
 __device__ inline uint32_t lookback_ticket(unsigned long long *lb) { return (uint32_t)atomicAdd(lb, 1ull); }
 // Look-back words of a scan of `blocks` blocks.
 constexpr size_t lookback_words(size_t blocks) { return 2 * blocks + 2; }
@@ -233,7 +236,8 @@ struct ArraySrc {
 template <typename T, bool IS_MAX, class Src>
 __global__ __launch_bounds__(BLOCK) void k_scan(Src src, T *out, const uint32_t *n_dev, uint32_t n_host,
                                                 uint32_t n_cap, uint32_t tiles_per_block, unsigned long long *lb,
-                                                T *total, DevStatus *st, bool check_failed, bool inclusive) {
+                                                T *total, DevStatus *st, bool check_failed, bool inclusive,
+                                                bool tickets) {
     constexpr uint32_t ITEMS = Src::ITEMS, TILE = (uint32_t)BLOCK * ITEMS, ROW = ITEMS + 1;
     __shared__ T s_t[BLOCK * ROW];  // a tile, one padded row of ITEMS per thread
     __shared__ uint32_t s_fail, s_n, s_tk;
@@ -244,7 +248,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(Src src, T *out, const uint32_t 
     if (tid == 0) {  // status, length and the block's ticket in one round trip
         const uint32_t e = st->err, o = st->overflow;
         const uint32_t nn = n_dev ? *n_dev : n_host;
-        s_tk = lookback_ticket(lb);
+        s_tk = tickets ? lookback_ticket(lb) : blockIdx.x;
         s_fail = (check_failed & ((e | o) != 0u)) ? 1u : 0u;
         s_n = nn;
     }
@@ -338,18 +342,19 @@ __global__ __launch_bounds__(BLOCK) void k_scan(Src src, T *out, const uint32_t 
 template <typename T, bool IS_MAX, class Src>
 inline void scan_launch_src(hipStream_t s, uint32_t resident, const Src &src, T *out, const uint32_t *n_dev,
                             uint32_t n_host, uint32_t n_cap, unsigned long long *lb, T *total, DevStatus *st,
-                            bool check_failed, bool inclusive) {
+                            bool check_failed, bool inclusive, bool tickets = true) {
     const ScanGrid g = scan_grid_for<T, Src::ITEMS>(n_cap, resident);
     k_scan<T, IS_MAX, Src><<<g.blocks, BLOCK, 0, s>>>(src, out, n_dev, n_host, n_cap, g.tiles_per_block, lb, total,
-                                                      st, check_failed, inclusive);
+                                                      st, check_failed, inclusive, tickets);
 }
 
 template <typename T, bool IS_MAX>
 inline void scan_launch(hipStream_t s, uint32_t resident, const T *in, T *out, const uint32_t *n_dev,
                         uint32_t n_host, uint32_t n_cap, unsigned long long *lb, T *total, DevStatus *st,
-                        bool check_failed, bool inclusive) {
+                        bool check_failed, bool inclusive, bool tickets = true) {
     ArraySrc<T> src{in, 0u};
-    scan_launch_src<T, IS_MAX>(s, resident, src, out, n_dev, n_host, n_cap, lb, total, st, check_failed, inclusive);
+    scan_launch_src<T, IS_MAX>(s, resident, src, out, n_dev, n_host, n_cap, lb, total, st, check_failed, inclusive,
+                               tickets);
 }
 
 }  // namespace ft
