@@ -509,19 +509,24 @@ def main():
     for _ in range(args.warmup):
         batch.launch()
     batch.sync()
-    # (the raster kernel's events, every TIMING_EVERY-th launch, time it
-    # beside the front ends it overlaps, which share its compute units)
-    batch.timing()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        batch.set_timing(k % TIMING_EVERY == 0)
         batch.launch()
     batch.sync()
     t1 = time.perf_counter()
+    sec_per_step = (t1 - t0) / args.steps
+    # then, untimed, the raster kernel's events every TIMING_EVERY-th
+    # pipelined launch: the raster beside the front ends it overlaps, which
+    # share its compute units (the events cost queue time, so they stay out
+    # of the timed loop)
+    batch.timing()
+    for k in range(max(TIMING_EVERY * 8, args.steps // 2)):
+        batch.set_timing(k % TIMING_EVERY == 0)
+        batch.launch()
+    batch.sync()
     batch.set_timing(False)
     total_p_ms, raster_p_ms, runs_p = batch.timing()
     batch.set_pipeline(False)
-    sec_per_step = (t1 - t0) / args.steps
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_random_walk.json")
