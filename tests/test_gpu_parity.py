@@ -752,6 +752,42 @@ def test_batch_band_pair_overflow_grows_and_reruns(ft, oracle):
     assert (offered[0], stored[0]) == o.last_counts()
 
 
+def test_pipelined_overflow_then_run_grows_every_workspace(ft, oracle):
+    """Pipelined launches that outgrow the first capacities report FT_ENOMEM at
+    the sync; ft_batch_run then grows the main workspace and its twins, and
+    pipelined launches give the oracle's raster."""
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    w, h = 1024, 3000
+    ops = []
+    for k in range(6):
+        x = 40.0 + 160.0 * k
+        ops += [(MOVE, (x, 1.0)), (LINE, (x + 70.0, 2.0)), (LINE, (x + 90.0, 2999.0)), (LINE, (x + 5.0, 2990.0)),
+                (CLOSE, ())]
+    a = ops_array(ops)
+    buf = DeviceBuffer(w * h)
+    batch = Batch(a, [FillDesc(0, len(a), buf.ptr, w, h, w, MATTE8, rule=0, fresh=True)])
+    batch.set_tuning(1, 1024)
+    batch.set_pipeline(3)
+    buf.upload(np.full(w * h, 9, dtype=np.uint8))
+    for _ in range(3):
+        batch.launch()
+    with pytest.raises(ft.FootileError) as e:
+        batch.sync()
+    assert e.value.code == 2  # FT_ENOMEM: the band lists overflowed
+    batch.run()
+    o = oracle.Plotter(w, h)
+    assert o.fill(0, a) == 0
+    for n in (1, 4):
+        buf.upload(np.full(w * h, 9, dtype=np.uint8))
+        for _ in range(n):
+            batch.launch()
+        batch.sync()
+        assert np.array_equal(buf.download(), o.pixels()), n
+    offered, stored, _ = batch.counts()
+    assert (offered[0], stored[0]) == o.last_counts()
+    batch.set_pipeline(1)
+
+
 def _oracle_tiles(oracle, ops, tiles, idx, threads=16):
     """The oracle's fresh s x s Matte8 tile and (offered, stored) counts for
     each tile index, on a thread pool (the oracle's calls release the GIL)."""
