@@ -1147,12 +1147,23 @@ __device__ __forceinline__ void chunk_out(const uint32_t *v, uint32_t base, uint
         if constexpr (!FRESH) {
             // Outside the figure over a transparent destination SrcOver gives
             // 0 (S(0) = 0 and 0 * invw = 0): a wavefront whose every lane has
-            // such a chunk stores zeros without the table or the blend.
-            uint32_t any = q.x | q.y | q.z | q.w;
+            // such a chunk leaves it as it is.  Inside it with an opaque
+            // colour, coverage 255 gives the colour's entry whatever the
+            // destination (invw(255) = widen(255 - 255) = 0): a wavefront
+            // whose every pixel is covered stores that entry without the blend.
+            uint32_t any = q.x | q.y | q.z | q.w, all = 255u;
 #pragma unroll
-            for (int j = 0; j < CPT; ++j) any |= cov_of<RULE>(base + v[j]);
-            if (__all(any == 0u)) {
-                *reinterpret_cast<gu32x4 *>(dst) = (u32x4){0u, 0u, 0u, 0u};
+            for (int j = 0; j < CPT; ++j) {
+                const uint32_t a = cov_of<RULE>(base + v[j]);
+                any |= a;
+                all &= a;
+            }
+            if (__all(any == 0u)) return;
+            // the colour's alpha byte: 3 in Rgba8p, 1 in Graya8p (two pixels per word)
+            constexpr uint32_t ASH = FMT == FMT_RGBA8P ? 24u : 8u;
+            if (((clr >> ASH) & 0xFFu) == 255u && __all(all == 255u)) {
+                const uint32_t e = lut[255].w, c = FMT == FMT_RGBA8P ? e : (e | (e << 16));
+                *reinterpret_cast<gu32x4 *>(dst) = (u32x4){c, c, c, c};
                 return;
             }
         }

@@ -142,6 +142,34 @@ def test_random_curves_all_formats(ft, oracle):
         compare_fill(ft, oracle, w, h, fmt, [("fill", t % 2, ops, clr)], init=init, tag=f"curve{t}")
 
 
+def test_opaque_fills_over_existing_pixels(ft, oracle):
+    """Opaque colours over existing pixels, with figures wide enough that whole
+    wavefronts of pixels are fully covered (those store the colour's entry
+    without the blend) or uncovered (left as they are over transparent
+    pixels, blended over the others): Rgba8p and Graya8p, both rules."""
+    rng = np.random.default_rng(7)
+    w, h = 1400, 300
+    rect = [(MOVE, (20.5, 10.0)), (LINE, (1380.0, 14.0)), (LINE, (1375.5, 280.0)), (LINE, (30.0, 290.0)),
+            (CLOSE, ())]
+    star = [(MOVE, (700.0, 5.0))]
+    for k in range(1, 7):
+        star.append((LINE, (700.0 + 690.0 * np.sin(k * 6 * np.pi / 7), 150.0 - 145.0 * np.cos(k * 6 * np.pi / 7))))
+    star.append((CLOSE, ()))
+    blob = [(MOVE, (100.0, 150.0)), (CUBIC, (100.0, -60.0, 1300.0, -60.0, 1300.0, 150.0)),
+            (CUBIC, (1300.0, 360.0, 100.0, 360.0, 100.0, 150.0)), (CLOSE, ())]
+    for fmt in (RGBA8P, GRAYA8P):
+        for init_kind in ("random", "transparent_band"):
+            init = rng.integers(0, 256, size=w * h * BPP[fmt]).astype(np.uint8)
+            if init_kind == "transparent_band":
+                init.reshape(h, -1)[:, : 600 * BPP[fmt]] = 0  # transparent left part
+            clr = (200, 120, 40, 255) if fmt == RGBA8P else (90, 255, 0, 0)
+            for rule in (0, 1):
+                compare_fill(ft, oracle, w, h, fmt,
+                             [("fill", rule, ops_array(rect), clr), ("fill", rule, ops_array(star), clr),
+                              ("fill", rule, ops_array(blob), clr)],
+                             init=init, tag=f"opaque fmt {fmt} {init_kind} rule {rule}")
+
+
 def test_plain_256_fill_and_stroke(ft, oracle):
     compare_fill(ft, oracle, 256, 256, MATTE8, [("fill", 0, workloads.plain_256(0), (255,))],
                  tag="plain fill")
