@@ -138,7 +138,8 @@ struct Workspace {
     uint8_t *d_sp = nullptr, *h_sp = nullptr;
     size_t cap_sp_paths = 0;
     std::vector<PathState> h_pstate_store;
-    bool pstate_copied = false;  // the last eager run's final copy included the path states
+    bool pstate_copied = false;  // the status copy since the last launch included the path states
+    bool status_owed = false;    // eager launches left their status on the device: ws_finish copies it
     bool edges_counted = false;  // PathState::n_edges holds the last run's counts
     // band / LDS geometry of the next ws_set_batch
     Tuning tuning;

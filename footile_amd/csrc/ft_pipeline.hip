@@ -81,7 +81,8 @@ __device__ inline void block_reduce_keyed(bool has, uint32_t key, T v, T identit
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, unsigned long long *top_rep, uint32_t n_paths,
                                                  uint32_t *band_cnt, uint32_t *band_cur, uint32_t n_slots,
-                                                 unsigned long long *lb, uint32_t n_lb, DevStatus *st) {
+                                                 unsigned long long *lb, uint32_t n_lb, DevStatus *st,
+                                                 uint32_t keep_failed) {
     uint32_t n = n_paths * TOP_REPS > n_slots ? n_paths * TOP_REPS : n_slots;
     n = n > n_lb ? n : n_lb;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
@@ -104,7 +105,14 @@ __global__ __launch_bounds__(BLOCK) void k_clear(PathState *ps, unsigned long lo
         }
         if (i < n_lb) lb[i] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x < sizeof(DevStatus) / 4) reinterpret_cast<uint32_t *>(st)[threadIdx.x] = 0;
+    // The status (one wavefront's words): with keep_failed, a failed earlier
+    // launch whose status the host has not read yet stays failed, so this
+    // launch's kernels skip their work and the failure reaches the next read.
+    static_assert(sizeof(DevStatus) / 4 <= 64, "the status is one wavefront's words");
+    if (blockIdx.x == 0 && threadIdx.x < sizeof(DevStatus) / 4) {
+        const bool failed = (keep_failed != 0u) & ((st->err | st->overflow) != 0u);
+        if (!failed) reinterpret_cast<uint32_t *>(st)[threadIdx.x] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2843,7 +2851,8 @@ static inline uint32_t grid_for(size_t n) {
     return (uint32_t)(g > GRID_MAX ? GRID_MAX : g);
 }
 
-static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool with_pstate);
+static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool with_pstate, bool copy_status,
+                            bool keep_failed);
 
 // Enqueue one run.  In graph mode a fill run is captured once into a HIP
 // graph (the launch sequence has no memset nodes and no host waits) and then
@@ -2860,17 +2869,19 @@ ft_status ws_launch(Workspace &ws, uint32_t mode, bool read_pstate) {
         ev = ws.ev + 3 * ws.n_pending++;
     }
     if (!(ws.graph && mode == MODE_FILL) || ws.fstream) {
-        // an eager run brings the path states back with its status when they
-        // share the status block (not inside a graph: every replay would copy)
-        const bool with_ps = read_pstate && ws.d_sp && ws.n_paths;
-        const ft_status st = ws_enqueue(ws, mode, ev, with_ps);
-        ws.pstate_copied = with_ps && st == FT_OK;
+        // An eager run leaves its status (and path states) on the device:
+        // ws_finish copies them once for all the launches before it, and a
+        // failed launch stays failed until then (keep_failed).
+        (void)read_pstate;
+        const bool keep = ws.status_owed;
+        const ft_status st = ws_enqueue(ws, mode, ev, false, false, keep);
+        ws.status_owed = true;
         return st;
     }
     if (!ws.gexec) {
         hipGraph_t g = nullptr;
         FT_HIP(hipStreamBeginCapture(ws.stream, hipStreamCaptureModeThreadLocal));
-        const ft_status st = ws_enqueue(ws, mode, nullptr, false);
+        const ft_status st = ws_enqueue(ws, mode, nullptr, false, true, false);
         const hipError_t e = hipStreamEndCapture(ws.stream, &g);
         if (st) {
             if (g) (void)hipGraphDestroy(g);
@@ -2893,7 +2904,8 @@ ft_status ws_launch(Workspace &ws, uint32_t mode, bool read_pstate) {
     return FT_OK;
 }
 
-static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool with_pstate) {
+static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool with_pstate, bool copy_status,
+                            bool keep_failed) {
     if (mode == MODE_STROKE) {
         const ft_status e = ensure_stroke_capacity(ws);
         if (e) return e;
@@ -2927,7 +2939,8 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
     const size_t n_slots = ws.n_bands << ws.band_shift;
     hipExtLaunchKernelGGL(k_clear, dim3(grid_for(std::max(std::max(ws.n_paths * TOP_REPS, n_slots), (size_t)n_lb))),
                           dim3(BLOCK), 0, s, ev ? ev[0] : nullptr, nullptr, 0, ws.d_pstate, ws.d_top_rep,
-                          (uint32_t)ws.n_paths, ws.d_band_cnt, ws.d_band_cur, (uint32_t)n_slots, lb, n_lb, st);
+                          (uint32_t)ws.n_paths, ws.d_band_cnt, ws.d_band_cur, (uint32_t)n_slots, lb, n_lb, st,
+                          keep_failed ? 1u : 0u);
     const uint32_t *lastpw = nullptr;
     if (mode != MODE_FILL && n_ops) {
         k_pen_width_mark<<<grid_for(n_ops), BLOCK, 0, s>>>(ws.d_ops, n_ops, ws.d_op_pw);
@@ -3134,13 +3147,22 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
     }
     FT_HIP(hipGetLastError());
     FT_HIP(hand_over());
-    const size_t back = with_pstate ? STATUS_BYTES + ws.n_paths * sizeof(PathState) : sizeof(DevStatus);
-    FT_HIP(hipMemcpyAsync(ws.h_status, ws.d_status, back, hipMemcpyDeviceToHost, s));
+    if (copy_status) {
+        const size_t back = with_pstate ? STATUS_BYTES + ws.n_paths * sizeof(PathState) : sizeof(DevStatus);
+        FT_HIP(hipMemcpyAsync(ws.h_status, ws.d_status, back, hipMemcpyDeviceToHost, s));
+    }
     if (ws.fstream) FT_HIP(hipEventRecord(ws.ev_free, rs));  // this workspace's buffers are free again
     return FT_OK;
 }
 
 ft_status ws_finish(Workspace &ws, bool read_pstate) {
+    if (ws.status_owed) {  // the eager launches' status, with the path states when they share its block
+        const bool with_ps = read_pstate && ws.d_sp && ws.n_paths;
+        const size_t back = with_ps ? STATUS_BYTES + ws.n_paths * sizeof(PathState) : sizeof(DevStatus);
+        FT_HIP(hipMemcpyAsync(ws.h_status, ws.d_status, back, hipMemcpyDeviceToHost, ws.stream));
+        ws.pstate_copied = with_ps;
+        ws.status_owed = false;
+    }
     if (read_pstate && ws.n_paths && !ws.pstate_copied)
         FT_HIP(hipMemcpyAsync(ws.h_pstate, ws.d_pstate, ws.n_paths * sizeof(PathState), hipMemcpyDeviceToHost,
                               ws.stream));
