@@ -144,9 +144,12 @@ typedef struct ft_batch ft_batch;
 ft_status ft_batch_new(const ft_path_op *ops, size_t n_ops, const ft_fill_desc *paths,
                        size_t n_paths, ft_batch **out);
 void ft_batch_free(ft_batch *b);
-/* Enqueue the whole batch on the batch's stream (asynchronous). */
+/* Enqueue the whole batch on the batch's stream (asynchronous).  The
+ * launch's status stays on the device until the next ft_batch_sync; a launch
+ * that fails leaves the launches after it no-ops until then. */
 ft_status ft_batch_launch(ft_batch *b);
-/* Wait for the last launch; returns its status (FT_ERANGE etc.). */
+/* Wait for the launches since the last sync; returns their status (FT_ERANGE
+ * etc., FT_ENOMEM when a capacity ran out: ft_batch_run grows it). */
 ft_status ft_batch_sync(ft_batch *b);
 /* Launch + sync; grows workspace and relaunches once if a capacity ran out. */
 ft_status ft_batch_run(ft_batch *b);
