@@ -1155,18 +1155,35 @@ __device__ __forceinline__ void chunk_out(const uint32_t *v, uint32_t base, uint
         if constexpr (!FRESH) {
             // Outside the figure over a transparent destination SrcOver gives
             // 0 (S(0) = 0 and 0 * invw = 0): a wavefront whose every lane has
-            // such a chunk leaves it as it is.  Inside it with an opaque
+            // such a chunk leaves it as it is (over other pixels, the bytes
+            // the coverage-0 blend changes are changed).  Inside it with an opaque
             // colour, coverage 255 gives the colour's entry whatever the
             // destination (invw(255) = widen(255 - 255) = 0): a wavefront
             // whose every pixel is covered stores that entry without the blend.
-            uint32_t any = q.x | q.y | q.z | q.w, all = 255u;
+            uint32_t cany = 0u, all = 255u;
 #pragma unroll
             for (int j = 0; j < CPT; ++j) {
                 const uint32_t a = cov_of<RULE>(base + v[j]);
-                any |= a;
+                cany |= a;
                 all &= a;
             }
-            if (__all(any == 0u)) return;
+            if (__all(cany == 0u)) {
+                if (__all((q.x | q.y | q.z | q.w) == 0u)) return;
+                // Uncovered over other pixels: SrcOver with coverage 0 maps
+                // each byte d to ch8_mul(d, 255) = d - [1 <= d <= 15], so a
+                // lane subtracts 1 from such bytes and stores only if it did.
+                uint32_t dec[4], ch = 0u;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t lo = w[j] & 0x0F0F0F0Fu, hi = (w[j] >> 4) & 0x0F0F0F0Fu;
+                    dec[j] = (((lo + 0x0F0F0F0Fu) & ~(hi + 0x0F0F0F0Fu)) & 0x10101010u) >> 4;
+                    ch |= dec[j];
+                }
+                if (ch)
+                    *reinterpret_cast<gu32x4 *>(dst) =
+                        (u32x4){w[0] - dec[0], w[1] - dec[1], w[2] - dec[2], w[3] - dec[3]};
+                return;
+            }
             // the colour's alpha byte: 3 in Rgba8p, 1 in Graya8p (two pixels per word)
             constexpr uint32_t ASH = FMT == FMT_RGBA8P ? 24u : 8u;
             if (((clr >> ASH) & 0xFFu) == 255u && __all(all == 255u)) {
