@@ -122,6 +122,15 @@ ft_status ft_fig_fill(ft_fill_rule rule, const float *xy, const uint8_t *close_a
  * Batch -- many independent fills in one device-resident launch sequence.
  * No reference counterpart: it runs N Plotter::fill calls (src/plotter.rs:
  * 339-350), each into its own raster region, with ops uploaded once.
+ *
+ * Concurrency: the prefix scans of a batch or plotter run hand out their
+ * work in block order over a grid the device holds at once, which assumes no
+ * other such run's scans wait beside them.  Runs of different batches or
+ * plotters that may overlap on one device should take turns (or be one
+ * pipelined batch, ft_batch_set_pipeline); a run that still finds its scan
+ * blocks starved fails with FT_EDEVICE after a bounded wait instead of
+ * hanging.  Pipelined batches and ft_batch_run_noalloc take arrival tickets
+ * instead and carry no such assumption.
  * ------------------------------------------------------------------------- */
 typedef struct {
     uint64_t op_begin;   /* first op of this path in the batch op array */
