@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 namespace ft {
 
@@ -2282,6 +2283,28 @@ __global__ __launch_bounds__(BLOCK) void k_small_raster(PathDev P, const EdgeRec
         if ((call) != hipSuccess) return FT_EDEVICE;   \
     } while (0)
 
+// Eager runs whose look-back kernels take work in block-index order (no
+// tickets) assume that no other such kernel waits beside them.  Per device,
+// the workspace that launched such a run last is remembered; a run of
+// another workspace first waits on the host for that one's stream, so runs of
+// one workspace follow each other freely and runs of different workspaces
+// take turns.
+struct BlockOrderChain {
+    std::mutex mu;
+    const Workspace *last[64] = {};
+    hipStream_t stream[64] = {};
+};
+static BlockOrderChain g_block_order;
+
+static void block_order_forget(const Workspace &ws) {
+    std::lock_guard<std::mutex> lock(g_block_order.mu);
+    for (int d = 0; d < 64; ++d)
+        if (g_block_order.last[d] == &ws) {
+            g_block_order.last[d] = nullptr;
+            g_block_order.stream[d] = nullptr;
+        }
+}
+
 static void graph_drop(Workspace &ws) {
     if (ws.gexec) (void)hipGraphExecDestroy(ws.gexec);
     ws.gexec = nullptr;
@@ -2402,6 +2425,7 @@ ft_status ws_init(Workspace &ws) {
 }
 
 void ws_free(Workspace &ws) {
+    block_order_forget(ws);
     if (ws.arena) {  // nothing was allocated; the stream is the caller's
         graph_drop(ws);
         ws = Workspace();
@@ -2447,6 +2471,7 @@ ft_status ws_pipe_attach(Workspace &ws, hipStream_t main, hipStream_t front) {
     }
     if (ws.stream != main) {
         // the old stream's work (a layout upload) completes before it goes
+        block_order_forget(ws);
         if (ws.stream) {
             FT_HIP(hipStreamSynchronize(ws.stream));
             if (ws.stream_owned) FT_HIP(hipStreamDestroy(ws.stream));
@@ -2877,6 +2902,16 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
 ft_status ws_launch(Workspace &ws, uint32_t mode, bool read_pstate) {
     ws.edges_counted = false;
     ws.pstate_copied = false;
+    // block-index look-back: take turns with the other workspaces' runs
+    std::unique_lock<std::mutex> turn;
+    if (!ws.tickets && !ws.fstream) {
+        turn = std::unique_lock<std::mutex>(g_block_order.mu);
+        const int d = ws.device & 63;
+        if (g_block_order.last[d] && g_block_order.last[d] != &ws)
+            FT_HIP(hipStreamSynchronize(g_block_order.stream[d]));
+        g_block_order.last[d] = &ws;
+        g_block_order.stream[d] = ws.stream;
+    }
     hipEvent_t *ev = nullptr;
     if (ws.timing && mode == MODE_FILL && ws.n_paths && ws.n_pending < Workspace::TIMING_RING) {
         if (!ws.ev) {

@@ -125,12 +125,14 @@ ft_status ft_fig_fill(ft_fill_rule rule, const float *xy, const uint8_t *close_a
  *
  * Concurrency: the prefix scans of a batch or plotter run hand out their
  * work in block order over a grid the device holds at once, which assumes no
- * other such run's scans wait beside them.  Runs of different batches or
- * plotters that may overlap on one device should take turns (or be one
- * pipelined batch, ft_batch_set_pipeline); a run that still finds its scan
- * blocks starved fails with FT_EDEVICE after a bounded wait instead of
- * hanging.  Pipelined batches and ft_batch_run_noalloc take arrival tickets
- * instead and carry no such assumption.
+ * other such run's scans wait beside them.  Within a process the library
+ * makes such runs of different batches and plotters take turns on a device
+ * (a launch first waits on the host for the other one's stream), so they do
+ * not overlap; to overlap launches, pipeline one batch (ft_batch_set_pipeline).
+ * Pipelined batches and ft_batch_run_noalloc take arrival tickets instead and
+ * carry no such assumption.  Should another process's work starve a run's
+ * scan blocks anyway, the run fails with FT_EDEVICE after a bounded wait
+ * instead of hanging.
  * ------------------------------------------------------------------------- */
 typedef struct {
     uint64_t op_begin;   /* first op of this path in the batch op array */
