@@ -472,6 +472,42 @@ def test_batch_pipelined_launches_match_serial(ft, oracle):
         assert np.array_equal(want_atlas.reshape(1024, 1024)[y:y + s, x:x + s].reshape(-1), o.pixels()), t
 
 
+def test_two_batches_launched_alternately(ft, oracle):
+    """Two batches (each with its own workspace and stream) launched in turn
+    without syncs, the second also through a Plotter between them: runs of
+    different workspaces take turns, and every raster is what the batch gives
+    alone."""
+    from footile_amd.plotter import Batch, DeviceBuffer, FillDesc
+    W = H = 1024
+    ops = workloads.random_walk(1, n_cubics=2000, canvas=1024)
+    clr = (90, 50, 20, 96)
+    bufs = [DeviceBuffer(W * H * 4) for _ in range(2)]
+
+    def make(buf, rule):
+        return Batch(ops, [FillDesc(0, len(ops), buf.ptr, W, H, W * 4, RGBA8P, rule=rule, clr=clr)])
+
+    b1, b2 = make(bufs[0], 0), make(bufs[1], 1)
+    b1.run()
+    b2.run()
+    want = []
+    for b, buf in ((b1, bufs[0]), (b2, bufs[1])):
+        buf.zero()
+        for _ in range(3):
+            b.launch()
+        b.sync()
+        want.append(buf.download())
+    for buf in bufs:
+        buf.zero()
+    p = ft.Plotter(ft.Raster.with_clear(640, 480))
+    for k in range(3):
+        b1.launch()
+        p.fill(0, workloads.random_walk(2, n_cubics=300, canvas=480))
+        b2.launch()
+    b1.sync()
+    b2.sync()
+    assert np.array_equal(bufs[0].download(), want[0]) and np.array_equal(bufs[1].download(), want[1])
+
+
 def test_look_back_kernels_finish_on_eight_compute_units(ft, oracle):
     """The scans and the one-pass flattening are sized for the whole device,
     and their blocks wait on each other (decoupled look-back).  Run on a
