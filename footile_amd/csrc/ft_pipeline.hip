@@ -1919,8 +1919,14 @@ constexpr uint32_t SMALL_SPLIT = 5;  // a small figure's curves take 32 lanes wh
 // points and their order are the same for any split.  False (in every
 // thread, nothing emitted) when flattening fails or the events or closes
 // exceed their limits.
+// A small fill keeps each curve lane's first SMALL_LEAF points in LDS
+// (leaf[k * SMALL_ITEMS + t], with the lane's leaf count in n_leaf[t]), so
+// that its emit is a copy unless the lane has more.
+constexpr uint32_t SMALL_LEAF = 2;
+
 __device__ bool small_flatten(SmallLds &S, uint32_t n_ops, uint32_t mode, const uint32_t *lastpw, EventSink em,
-                              uint32_t max_runs, uint32_t &n_events, uint32_t &n_runs, uint32_t &bad) {
+                              uint32_t max_runs, uint32_t &n_events, uint32_t &n_runs, uint32_t &bad,
+                              I2 *leaf = nullptr, uint16_t *n_leaf = nullptr) {
     const uint32_t tid = threadIdx.x;
     static_assert(SMALL_OPS <= SBLOCK, "one op per thread");
     const uint32_t tag = tid < n_ops ? S.ops[tid].tag : TAG_CLOSE;
@@ -1944,8 +1950,10 @@ __device__ bool small_flatten(SmallLds &S, uint32_t n_ops, uint32_t mode, const 
         if (t < n_items) {
             uint32_t i, s, last;
             lane_of(t, i, s, last);
-            c = count_op(S.ops, S.op_path, &S.path, i, s, mode, lastpw, nullptr, 0, sp, last);
+            c = count_op(S.ops, S.op_path, &S.path, i, s, mode, lastpw, leaf ? leaf + t : nullptr, SMALL_ITEMS, sp,
+                         last, SMALL_LEAF);
         }
+        if (n_leaf) n_leaf[t] = (uint16_t)(c.leaves < 0xFFFFu ? c.leaves : 0xFFFFu);
         err |= c.err;
         S.b.cnt_e[t] = c.ne;
         S.c.cnt_c[t] = c.nc;
@@ -1959,7 +1967,8 @@ __device__ bool small_flatten(SmallLds &S, uint32_t n_ops, uint32_t mode, const 
         const unsigned long long off = ((unsigned long long)S.b.cnt_e[t] << 32) | S.c.cnt_c[t];
         uint32_t i, s, last;
         lane_of(t, i, s, last);
-        bad |= emit_op(S.ops, S.op_path, &S.path, i, s, mode, lastpw, off, em, nullptr, 0, 0, sp, last);
+        bad |= emit_op(S.ops, S.op_path, &S.path, i, s, mode, lastpw, off, em, leaf ? leaf + t : nullptr,
+                       n_leaf ? n_leaf[t] : 0u, SMALL_ITEMS, sp, last, SMALL_LEAF);
     }
     __syncthreads();
     return true;
@@ -2133,7 +2142,10 @@ __global__ __launch_bounds__(SBLOCK) void k_small_front(const ft_path_op *ops, S
         ok = small_flatten(S, n_ops, MODE_STROKE, T.lastpw, em, SMALL_EVENTS, n_sev, n_scl, bad) &&
              small_stroker(S, T, n_sev, sp, n_events, n_runs, bad);
     } else {
-        ok = small_flatten(S, n_ops, MODE_FILL, nullptr, small_fill_sink(S), SMALL_RUNS, n_events, n_runs, bad);
+        __shared__ I2 s_leaf[SMALL_LEAF * SMALL_ITEMS];
+        __shared__ uint16_t s_nleaf[SMALL_ITEMS];
+        ok = small_flatten(S, n_ops, MODE_FILL, nullptr, small_fill_sink(S), SMALL_RUNS, n_events, n_runs, bad,
+                           s_leaf, s_nleaf);
     }
     if (!ok) {
         if (tid == 0) atomicOr(&st->overflow, SMALL_NOT_HANDLED);

@@ -235,7 +235,8 @@ constexpr uint32_t LEAF_CACHE = 8;
 // pipeline gives every op OP_LANES lanes.
 FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, const PathDev *paths, uint32_t i,
                               uint32_t s, uint32_t mode, const uint32_t *lastpw, I2 *cache = nullptr,
-                              size_t stride = 0, uint32_t sp = FLAT_SPLIT, uint32_t last_lane = OP_LANES - 1) {
+                              size_t stride = 0, uint32_t sp = FLAT_SPLIT, uint32_t last_lane = OP_LANES - 1,
+                              uint32_t cap = LEAF_CACHE) {
     OpCount c = {0, 0, 0, 0, 0};
     const uint32_t p = op_path[i];
     if (p == NO_IDX) return c;
@@ -253,7 +254,7 @@ FT_HD inline OpCount count_op(const ft_path_op *ops, const uint32_t *op_path, co
         I2 *const keep = mode == MODE_FILL ? cache : nullptr;
         auto cnt = [&](WP q) {
             const uint32_t k = c.leaves - 1;  // flatten counts a leaf before handing it over
-            if (keep && k < LEAF_CACHE) {
+            if (keep && k < cap) {
                 I2 v;
                 v.x = fx_from_f32(q.x);
                 v.y = fx_from_f32(q.y);
@@ -358,11 +359,12 @@ struct EventSink {
 // before them).
 // `cache` / `cached`: the lane's points kept by count_op (point k at
 // cache[k * stride]) and their number (used in fill runs when it is at most
-// LEAF_CACHE).
+// the cache's capacity `cap`).
 FT_HD inline uint32_t emit_op(const ft_path_op *ops, const uint32_t *op_path, const PathDev *paths, uint32_t i,
                               uint32_t s, uint32_t mode, const uint32_t *lastpw, unsigned long long off,
                               EventSink em, const I2 *cache = nullptr, uint32_t cached = 0, size_t stride = 0,
-                              uint32_t sp = FLAT_SPLIT, uint32_t last_lane = OP_LANES - 1) {
+                              uint32_t sp = FLAT_SPLIT, uint32_t last_lane = OP_LANES - 1,
+                              uint32_t cap = LEAF_CACHE) {
     const uint32_t p = op_path[i];
     if (p == NO_IDX) return 0;
     const PathDev &P = paths[p];
@@ -386,7 +388,7 @@ FT_HD inline uint32_t emit_op(const ft_path_op *ops, const uint32_t *op_path, co
         break;
     case TAG_QUAD:
     case TAG_CUBIC: {
-        if (cache && mode == MODE_FILL && cached <= LEAF_CACHE) {
+        if (cache && mode == MODE_FILL && cached <= cap) {
             for (uint32_t k = 0; k < cached; ++k) em.point_fx(cache[k * stride]);
             break;
         }
