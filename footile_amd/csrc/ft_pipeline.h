@@ -186,10 +186,10 @@ ft_status ws_set_batch(Workspace &ws, const ft_path_op *ops, size_t n_ops, const
                        size_t n_paths, uint32_t extra_flags = 0);
 // Re-upload only the ops (same count and layout) -- for a plotter reusing ws.
 ft_status ws_upload_ops(Workspace &ws, const ft_path_op *ops, size_t n_ops);
-// Enqueue a run.  mode = MODE_FILL renders; MODE_STROKE_FLATTEN stops after
-// emitting (x, y, width, kind) events for the host stroker.
-// read_pstate: an eager run also copies the path states back with its status.
-ft_status ws_launch(Workspace &ws, uint32_t mode, bool read_pstate = false);
+// Enqueue a run (MODE_FILL: the fills; MODE_STROKE: the strokes, with the
+// workspace's join, miter and stroke tolerance).  An eager run's status and
+// path states stay on the device until ws_finish.
+ft_status ws_launch(Workspace &ws, uint32_t mode);
 // Wait and read the status block; returns the run's status.
 ft_status ws_finish(Workspace &ws, bool read_pstate);
 // launch + finish, growing capacities and relaunching on overflow.

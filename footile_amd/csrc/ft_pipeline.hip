@@ -2911,7 +2911,7 @@ static ft_status ws_enqueue(Workspace &ws, uint32_t mode, hipEvent_t *ev, bool w
 // Enqueue one run.  In graph mode a fill run is captured once into a HIP
 // graph (the launch sequence has no memset nodes and no host waits) and then
 // replayed; any change of buffers or layout drops the graph.
-ft_status ws_launch(Workspace &ws, uint32_t mode, bool read_pstate) {
+ft_status ws_launch(Workspace &ws, uint32_t mode) {
     ws.edges_counted = false;
     ws.pstate_copied = false;
     // block-index look-back: take turns with the other workspaces' runs
@@ -2936,7 +2936,6 @@ ft_status ws_launch(Workspace &ws, uint32_t mode, bool read_pstate) {
         // An eager run leaves its status (and path states) on the device:
         // ws_finish copies them once for all the launches before it, and a
         // failed launch stays failed until then (keep_failed).
-        (void)read_pstate;
         const bool keep = ws.status_owed;
         const ft_status st = ws_enqueue(ws, mode, ev, false, false, keep);
         ws.status_owed = true;
@@ -3386,7 +3385,7 @@ ft_status ws_small_fill(Workspace &ws, const ft_path_op *ops, size_t n_ops, cons
 
 ft_status ws_run(Workspace &ws, uint32_t mode, bool read_pstate) {
     for (int attempt = 0; attempt < 4; ++attempt) {
-        ft_status s = ws_launch(ws, mode, read_pstate);
+        ft_status s = ws_launch(ws, mode);
         if (s) return s;
         s = ws_finish(ws, read_pstate);
         if (s != (ft_status)0x100) return s;
